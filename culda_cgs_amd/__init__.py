@@ -1,0 +1,350 @@
+"""Python binding of libculda.so, the MI355X collapsed-Gibbs LDA sampler.
+
+Thin ctypes layer over the C ABI declared in include/culda.h.  The compute
+path is the gfx950 HIP code inside libculda.so; there is no Python or CPU
+fallback: if the library is missing, every entry point raises.
+
+Roles mirror the reference host classes: ``Corpus`` is Document/DocChunk
+(src_culda/model/doc.h:25, doc_chunk.h:19) plus the src_format writer;
+``Sampler`` owns the device model (ModelPhi / ModelTheta,
+model/model_phi.h:22, model/model_theta.h:22) and runs the iteration of
+train/SingleChunkSingleGPU.h:70-127.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "lib" / "libculda.so"
+CLI_PATH = _HERE / "bin" / "culda"
+
+_lib = None
+
+
+class CuldaError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"culda error {status}: {message}")
+        self.status = status
+
+
+class CorpusInfo(C.Structure):
+    _fields_ = [("num_words", C.c_int32), ("num_docs", C.c_int32), ("num_chunks", C.c_int32),
+                ("max_doc_len", C.c_int32), ("num_tokens", C.c_int64)]
+
+
+class ChunkView(C.Structure):
+    _fields_ = [("docid_start", C.c_int32), ("docid_end", C.c_int32), ("num_tokens", C.c_int64),
+                ("word_indices", C.POINTER(C.c_int64)), ("word_tokens", C.POINTER(C.c_int32))]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("num_words", C.c_int32), ("num_docs", C.c_int32), ("true_topics", C.c_int32),
+                ("num_chunks", C.c_int32), ("len_kind", C.c_int32), ("num_threads", C.c_int32),
+                ("len_mean", C.c_double), ("len_mu", C.c_double), ("len_sigma", C.c_double),
+                ("doc_topic_prior", C.c_double), ("topic_word_prior", C.c_double),
+                ("zipf_s", C.c_double), ("seed", C.c_uint64)]
+
+
+class Params(C.Structure):
+    _fields_ = [("k", C.c_int32), ("alpha", C.c_float), ("beta", C.c_float), ("seed", C.c_uint64),
+                ("device", C.c_int32), ("slot_tokens", C.c_int32), ("profile", C.c_int32),
+                ("fused_phi", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("num_tokens", C.c_int64), ("num_slots", C.c_int64), ("sum_doc_nnz", C.c_int64),
+                ("sum_token_nnz", C.c_int64), ("theta_capacity", C.c_int64), ("sampler_ms", C.c_double),
+                ("phi_ms", C.c_double), ("theta_ms", C.c_double), ("other_ms", C.c_double),
+                ("sampler_launches", C.c_int32), ("pad", C.c_int32)]
+
+
+# name -> (restype, argtypes); every int-returning call is checked.
+_P = C.c_void_p
+_SIGS = {
+    "culda_last_error": (C.c_char_p, []),
+    "culda_abi_version": (C.c_int, []),
+    "culda_corpus_load": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(_P)]),
+    "culda_corpus_from_arrays": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.c_int32,
+                                           C.POINTER(C.c_int32), C.POINTER(C.POINTER(C.c_int64)),
+                                           C.POINTER(C.POINTER(C.c_int32)), C.POINTER(_P)]),
+    "culda_corpus_write": (C.c_int, [_P, C.c_char_p]),
+    "culda_format_text": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32]),
+    "culda_corpus_rechunk": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
+    "culda_corpus_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
+    "culda_corpus_info_get": (C.c_int, [_P, C.POINTER(CorpusInfo)]),
+    "culda_corpus_doc_cum": (C.c_int, [_P, C.POINTER(C.POINTER(C.c_int64))]),
+    "culda_corpus_chunk": (C.c_int, [_P, C.c_int32, C.POINTER(ChunkView)]),
+    "culda_corpus_free": (None, [_P]),
+    "culda_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
+    "culda_destroy": (None, [_P]),
+    "culda_upload": (C.c_int, [_P, _P]),
+    "culda_init_z": (C.c_int, [_P]),
+    "culda_set_z": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint16), C.c_int64]),
+    "culda_get_z": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint16), C.c_int64]),
+    "culda_rebuild_phi": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "culda_phi_head": (C.c_int, [_P]),
+    "culda_rebuild_theta": (C.c_int, [_P, C.c_int32]),
+    "culda_sample": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "culda_loglik": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "culda_iterate": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "culda_rebuild_all": (C.c_int, [_P]),
+    "culda_loglik_joint": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "culda_sync": (C.c_int, [_P]),
+    "culda_get_phi": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "culda_get_phi_head": (C.c_int, [_P, C.POINTER(C.c_float)]),
+    "culda_get_theta": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int16)]),
+    "culda_save_phi": (C.c_int, [_P, C.c_char_p]),
+    "culda_save_theta": (C.c_int, [_P, C.c_char_p]),
+    "culda_save_checkpoint": (C.c_int, [_P, C.c_char_p, C.c_int32]),
+    "culda_load_checkpoint": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
+    "culda_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+}
+
+
+def lib():
+    """Load libculda.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c "
+                               "'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(st):
+    if st != 0:
+        raise CuldaError(st, lib().culda_last_error().decode(errors="replace"))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Corpus:
+    """Host corpus in the src_format data_chunk layout."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.culda_corpus_free(self._h)
+            self._h = C.c_void_p()
+
+    @classmethod
+    def load(cls, prefix, num_chunks):
+        h = C.c_void_p()
+        _check(lib().culda_corpus_load(str(prefix).encode(), num_chunks, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, num_words, doc_cum, chunk_doc_end, word_indices, word_tokens):
+        doc_cum = np.ascontiguousarray(doc_cum, dtype=np.int64)
+        ends = np.ascontiguousarray(chunk_doc_end, dtype=np.int32)
+        wi = [np.ascontiguousarray(x, dtype=np.int64) for x in word_indices]
+        wt = [np.ascontiguousarray(x, dtype=np.int32) for x in word_tokens]
+        nc = len(wi)
+        wi_p = (C.POINTER(C.c_int64) * nc)(*[_ptr(x, C.c_int64) for x in wi])
+        wt_p = (C.POINTER(C.c_int32) * nc)(*[_ptr(x, C.c_int32) for x in wt])
+        h = C.c_void_p()
+        _check(lib().culda_corpus_from_arrays(num_words, len(doc_cum), _ptr(doc_cum, C.c_int64), nc,
+                                              _ptr(ends, C.c_int32), wi_p, wt_p, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, num_words, num_docs, true_topics, num_chunks=1, *, len_kind=0, len_mean=100.0,
+                 len_mu=0.0, len_sigma=0.0, doc_topic_prior=0.1, topic_word_prior=0.05, zipf_s=1.0,
+                 seed=1, num_threads=0):
+        p = GenParams(num_words, num_docs, true_topics, num_chunks, len_kind, num_threads, len_mean, len_mu,
+                      len_sigma, doc_topic_prior, topic_word_prior, zipf_s, seed)
+        h = C.c_void_p()
+        _check(lib().culda_corpus_generate(C.byref(p), C.byref(h)))
+        return cls(h.value)
+
+    def write(self, prefix):
+        _check(lib().culda_corpus_write(self._h, str(prefix).encode()))
+
+    def rechunk(self, num_chunks):
+        h = C.c_void_p()
+        _check(lib().culda_corpus_rechunk(self._h, num_chunks, C.byref(h)))
+        return Corpus(h.value)
+
+    def info(self):
+        i = CorpusInfo()
+        _check(lib().culda_corpus_info_get(self._h, C.byref(i)))
+        return i
+
+    def doc_cum(self):
+        p = C.POINTER(C.c_int64)()
+        _check(lib().culda_corpus_doc_cum(self._h, C.byref(p)))
+        n = self.info().num_docs
+        return np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0, np.int64)
+
+    def chunk(self, c):
+        """(docid_start, docid_end, word_indices[V+1], word_tokens[N_c]) as copies."""
+        v = ChunkView()
+        _check(lib().culda_corpus_chunk(self._h, c, C.byref(v)))
+        V = self.info().num_words
+        wi = np.ctypeslib.as_array(v.word_indices, shape=(V + 1,)).copy()
+        wt = (np.ctypeslib.as_array(v.word_tokens, shape=(v.num_tokens,)).copy()
+              if v.num_tokens else np.zeros(0, np.int32))
+        return v.docid_start, v.docid_end, wi, wt
+
+
+def format_text(input_path, prefix, num_chunks=1):
+    """./format input output_prefix numChunks (src_format/format.cpp:19-168)."""
+    _check(lib().culda_format_text(str(input_path).encode(), str(prefix).encode(), num_chunks))
+
+
+class Sampler:
+    """Device model + iteration driver on one GPU."""
+
+    def __init__(self, k=1024, alpha=50.0 / 1024.0, beta=0.01, seed=1, device=0, slot_tokens=0,
+                 profile=False, fused_phi=True):
+        self.params = Params(k, alpha, beta, seed, device, slot_tokens, 1 if profile else 0,
+                             1 if fused_phi else 0)
+        h = C.c_void_p()
+        _check(lib().culda_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.k = k
+        self.corpus = None
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().culda_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- lifecycle -------------------------------------------------------
+    def upload(self, corpus: Corpus):
+        _check(lib().culda_upload(self._h, corpus._h))
+        self.corpus = corpus
+        self._info = corpus.info()
+        self._chunk_tokens = [corpus.chunk(c)[3].size for c in range(self._info.num_chunks)]
+
+    def init_z(self):
+        _check(lib().culda_init_z(self._h))
+
+    def rebuild_all(self):
+        _check(lib().culda_rebuild_all(self._h))
+
+    def rebuild_phi(self, chunk=-1, accumulate=False):
+        _check(lib().culda_rebuild_phi(self._h, chunk, 1 if accumulate else 0))
+
+    def phi_head(self):
+        _check(lib().culda_phi_head(self._h))
+
+    def rebuild_theta(self, chunk):
+        _check(lib().culda_rebuild_theta(self._h, chunk))
+
+    def sample(self, chunk, iteration):
+        _check(lib().culda_sample(self._h, chunk, iteration))
+
+    def loglik(self):
+        v = C.c_double()
+        _check(lib().culda_loglik(self._h, C.byref(v)))
+        return v.value
+
+    def iterate(self, iteration):
+        ll, sec = C.c_double(), C.c_double()
+        _check(lib().culda_iterate(self._h, iteration, C.byref(ll), C.byref(sec)))
+        return ll.value, sec.value
+
+    def sync(self):
+        _check(lib().culda_sync(self._h))
+
+    def loglik_joint(self):
+        v = C.c_double()
+        _check(lib().culda_loglik_joint(self._h, C.byref(v)))
+        return v.value
+
+    # -- state transfer ---------------------------------------------------
+    def get_z(self, chunk):
+        z = np.empty(self._chunk_tokens[chunk], np.uint16)
+        _check(lib().culda_get_z(self._h, chunk, _ptr(z, C.c_uint16), z.size))
+        return z
+
+    def set_z(self, chunk, z):
+        z = np.ascontiguousarray(z, dtype=np.uint16)
+        _check(lib().culda_set_z(self._h, chunk, _ptr(z, C.c_uint16), z.size))
+
+    def get_phi(self):
+        V = self._info.num_words
+        nwk = np.empty(V * self.k, np.int32)
+        nk = np.empty(self.k, np.int32)
+        _check(lib().culda_get_phi(self._h, _ptr(nwk, C.c_int32), _ptr(nk, C.c_int32)))
+        return nwk.reshape(V, self.k), nk
+
+    def get_phi_head(self):
+        V = self._info.num_words
+        h = np.empty(V * self.k, np.float32)
+        _check(lib().culda_get_phi_head(self._h, _ptr(h, C.c_float)))
+        return h.reshape(V, self.k)
+
+    def get_theta(self, chunk):
+        """Compact CSR over the chunk's docs: (offsets, counts, topics)."""
+        s, e, _, _ = self.corpus.chunk(chunk)
+        off = np.empty(e - s + 1, np.int64)
+        _check(lib().culda_get_theta(self._h, chunk, _ptr(off, C.c_int64), None, None))
+        n = int(off[-1])
+        cnt = np.empty(max(n, 1), np.int32)
+        top = np.empty(max(n, 1), np.int16)
+        _check(lib().culda_get_theta(self._h, chunk, _ptr(off, C.c_int64), _ptr(cnt, C.c_int32),
+                                     _ptr(top, C.c_int16)))
+        return off, cnt[:n], top[:n]
+
+    def save_phi(self, prefix):
+        _check(lib().culda_save_phi(self._h, str(prefix).encode()))
+
+    def save_theta(self, prefix):
+        _check(lib().culda_save_theta(self._h, str(prefix).encode()))
+
+    def save_checkpoint(self, path, next_iteration):
+        _check(lib().culda_save_checkpoint(self._h, str(path).encode(), next_iteration))
+
+    def load_checkpoint(self, path):
+        it = C.c_int32()
+        _check(lib().culda_load_checkpoint(self._h, str(path).encode(), C.byref(it)))
+        return it.value
+
+    def stats(self):
+        s = Stats()
+        _check(lib().culda_get_stats(self._h, C.byref(s)))
+        return s
+
+    def sampler_bytes(self):
+        v = C.c_double()
+        _check(lib().culda_sampler_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def stream_copy(self, n_bytes, repeats=10):
+        v = C.c_double()
+        _check(lib().culda_stream_copy(self._h, n_bytes, repeats, C.byref(v)))
+        return v.value
+
+
+__all__ = ["Corpus", "Sampler", "CuldaError", "format_text", "lib", "LIB_PATH", "CLI_PATH"]

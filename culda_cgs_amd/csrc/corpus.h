@@ -1,0 +1,58 @@
+// corpus.h -- host-side corpus in the src_format data_chunk layout.
+//
+// Mirrors the roles of the reference's Document (model/doc.h:25) and
+// DocChunk (model/doc_chunk.h:19) host data: per-doc cumulative lengths,
+// and per chunk the word-major doc-id stream with chunk-local word offsets.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace culda {
+
+struct Chunk {
+    int32_t doc_start = 0;             // first global doc id
+    int32_t doc_end = 0;               // one past the last
+    std::vector<int64_t> word_idx;     // V+1 chunk-local cumulative offsets
+    std::vector<int32_t> word_tokens;  // global doc ids, word-major
+    int64_t num_tokens() const { return (int64_t)word_tokens.size(); }
+};
+
+struct Corpus {
+    int32_t num_words = 0;
+    std::vector<int64_t> doc_cum;      // D entries: cumulative lengths
+    std::vector<Chunk> chunks;
+    std::vector<std::string> vocab;    // optional tokens (empty => "w<id>")
+
+    int32_t num_docs() const { return (int32_t)doc_cum.size(); }
+    int64_t num_tokens() const { return doc_cum.empty() ? 0 : doc_cum.back(); }
+    int64_t doc_len(int32_t d) const { return doc_cum[d] - (d ? doc_cum[d - 1] : 0); }
+    int32_t max_doc_len() const;
+
+    // Checks the invariants the loader relies on: offsets monotone,
+    // doc ids inside the chunk's range, per-doc token counts equal to the
+    // .doc.idx lengths, chunks contiguous and covering all docs.
+    void validate() const;
+};
+
+Corpus load_corpus(const std::string &prefix, int32_t num_chunks);
+void write_corpus(const Corpus &c, const std::string &prefix);
+void format_text(const std::string &input, const std::string &prefix, int32_t num_chunks);
+
+// Greedy doc -> chunk split of src_format/format.cpp:107-132: chunk c takes
+// docs until its token count reaches ceil(N / C).  Returns C doc-end ids.
+std::vector<int32_t> greedy_chunk_ends(const std::vector<int64_t> &doc_len, int32_t num_chunks);
+
+// Re-split (word order and doc order are preserved).
+Corpus rechunk(const Corpus &c, int32_t num_chunks);
+
+// Build word-major chunks from per-token (word, doc) pairs given in doc
+// order (docs ascending).  Used by the generator and the text formatter.
+void build_chunks_from_doc_major(Corpus &c, const std::vector<int32_t> &tok_word,
+                                 const std::vector<int32_t> &chunk_ends);
+
+Corpus generate_corpus(const culda_gen_params &p);
+
+}  // namespace culda

@@ -1,0 +1,510 @@
+// kernels.hip -- gfx950 kernels of one plain CGS iteration.
+//
+// Built with -ffp-contract=off: every FP32 operation below is a separate,
+// correctly rounded IEEE operation in the order written, so the CPU oracle
+// can replay it bit for bit.  The order is the normative one:
+//   phi_k   = fl(fl(fl(n_wk) + beta) / den_k),  den_k = fl(fl(n_k) + fl(beta * fl(V)))
+//   a_k     = fl(alpha * phi_k)                              (Q bucket)
+//   p_j     = fl(fl(count_j) * phi_{topic_j})                (S bucket)
+//   blocks of 16 values, inclusive Kogge-Stone scan inside a block
+//   (offsets 1, 2, 4, 8); S: block totals carried sequentially
+//   R_{b+1} = fl(R_b + T_b); Q: 64-lane Kogge-Stone scan of the block
+//   totals.  Search: first block whose inclusive prefix exceeds the
+//   target, then first element of that block whose
+//   fl(prefix_before_block + in-block prefix) exceeds it; fallbacks take
+//   the last element.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "philox.h"
+
+namespace culda {
+
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inclusive Kogge-Stone scan inside a 16-lane row.
+__device__ __forceinline__ float scan16(float x, int lr) {
+#pragma unroll
+    for (int d = 1; d < kBlock; d <<= 1) {
+        const float y = __shfl_up(x, d, kBlock);
+        if (lr >= d) x = x + y;
+    }
+    return x;
+}
+
+// Row-local ballot: bit m set when lane m of this row voted true.
+__device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
+    return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ z init
+// One wave per slot; z = mulhi64(Philox((g, 0, 0, 0), (seed, 0))[0], K).
+__global__ __launch_bounds__(256) void init_z_kernel(InitArgs a) {
+    const int slot = blockIdx.x * kWavesPerGroup + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (slot >= a.num_slots) return;
+    const SlotDesc s = a.slots[slot];
+    for (int64_t i = s.begin + lane; i < s.end; i += kWave) {
+        const uint64_t x = token_bits(a.seed, (uint64_t)(i + s.gdelta), 0, kPurposeInit);
+        a.z[i] = (uint16_t)bits_to_topic(x, (uint32_t)a.k);
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void zero_rows_kernel(int32_t *nwk, const int32_t *words, int32_t n, int32_t k) {
+    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+        int32_t *row = nwk + (int64_t)words[r] * k;
+        for (int j = threadIdx.x; j < k; j += blockDim.x) row[j] = 0;
+    }
+}
+
+// ------------------------------------------------------------ phi rebuild
+// LDS histogram per word (one wave per slot), whole rows written with
+// plain 16-byte stores, hot-word pieces added with atomics.  n_k is
+// accumulated per workgroup in LDS and flushed with one atomic per topic.
+// Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
+// chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
+__global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    __shared__ uint32_t nk_s[kMaxK];
+    const int K = a.k;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *h = hist_s[wave];
+    for (int si = blockIdx.x * kWavesPerGroup + wave; si < a.num_slots; si += gridDim.x * kWavesPerGroup) {
+        const PhiSlot sl = a.slots[si];
+        if (sl.chunk < 0 && a.accumulate) {
+            // nothing to add when the word is absent from the launch's chunks
+            int64_t n = 0;
+            for (int c = a.chunk_begin; c < a.chunk_end; ++c)
+                n += a.chunks[c].word_idx[sl.word + 1] - a.chunks[c].word_idx[sl.word];
+            if (n == 0) continue;
+        }
+        for (int k = lane; k < K; k += kWave) h[k] = 0;
+        wave_sync();
+        if (sl.chunk < 0) {
+            for (int c = a.chunk_begin; c < a.chunk_end; ++c) {
+                const uint16_t *z = a.chunks[c].z;
+                const int64_t b = a.chunks[c].word_idx[sl.word], e = a.chunks[c].word_idx[sl.word + 1];
+                for (int64_t i = b + lane; i < e; i += kWave) atomicAdd(&h[z[i]], 1u);
+            }
+        } else {
+            const uint16_t *z = a.chunks[sl.chunk].z;
+            for (int64_t i = sl.begin + lane; i < sl.end; i += kWave) atomicAdd(&h[z[i]], 1u);
+        }
+        wave_sync();
+        int32_t *row = a.nwk + (int64_t)sl.word * K;
+        if (sl.chunk < 0) {
+            for (int k = lane * 4; k < K; k += 4 * kWave) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(h + k);
+                int4 o = make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+                if (a.accumulate) {
+                    const int4 old = *reinterpret_cast<const int4 *>(row + k);
+                    o.x += old.x;
+                    o.y += old.y;
+                    o.z += old.z;
+                    o.w += old.w;
+                }
+                *reinterpret_cast<int4 *>(row + k) = o;
+            }
+        } else {
+            for (int k = lane; k < K; k += kWave) {
+                const uint32_t v = h[k];
+                if (v) atomicAdd(row + k, (int)v);
+            }
+        }
+        for (int k = lane; k < K; k += kWave) {
+            const uint32_t v = h[k];
+            if (v) atomicAdd(&nk_s[k], v);
+        }
+        wave_sync();
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const uint32_t v = nk_s[k];
+        if (v) atomicAdd(a.nk + k, (int)v);
+    }
+}
+
+// den_k = fl(fl(n_k) + fl(beta * fl(V)))   (kernel/lda_phi_kernel.cu:26)
+__global__ void den_kernel(const int32_t *nk, float *den, int32_t k, float beta, int32_t num_words) {
+    const float bv = beta * (float)num_words;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
+        den[i] = (float)nk[i] + bv;
+}
+
+// Materialised head, for downloads and save hooks only (the sampler forms
+// it on the fly while staging a word's row).
+__global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *head, int64_t n, int32_t k, float beta) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        head[i] = ((float)nwk[i] + beta) / den[i % k];
+}
+
+// ------------------------------------------------------------ theta rebuild
+// One wave per doc: LDS histogram over the doc's tokens (gathered through
+// rev_idx), then an ascending ballot compaction of the nonzero topics.
+// Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
+__global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    const int K = a.k;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *h = hist_s[wave];
+    unsigned long long s_nnz = 0, s_tok = 0;
+    for (int d = blockIdx.x * kWavesPerGroup + wave; d < a.num_docs; d += gridDim.x * kWavesPerGroup) {
+        for (int k = lane; k < K; k += kWave) h[k] = 0;
+        wave_sync();
+        const uint32_t b = a.rev_off[d], e = a.rev_off[d + 1];
+        for (uint32_t i = b + lane; i < e; i += kWave) atomicAdd(&h[a.z[a.rev_idx[i]]], 1u);
+        wave_sync();
+        const uint32_t start = a.meta[d].start;
+        uint32_t base = 0;
+        for (int k0 = 0; k0 < K; k0 += kWave) {
+            const int k = k0 + lane;
+            const uint32_t v = k < K ? h[k] : 0u;
+            const unsigned long long m = __ballot(v != 0);
+            const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (v) a.theta[start + pos] = (v << a.topic_bits) | (uint32_t)k;
+            base += (uint32_t)__popcll(m);
+        }
+        if (lane == 0) a.meta[d].nnz = base;
+        s_nnz += base;
+        s_tok += (unsigned long long)base * (e - b);
+        wave_sync();
+    }
+    if (lane == 0 && s_nnz) {
+        atomicAdd(&a.stats[0], s_nnz);
+        atomicAdd(&a.stats[1], s_tok);
+    }
+}
+
+// ------------------------------------------------------------ sampler
+// One wave per slot (one word's token range).  The wave stages the word's
+// head row phi_k in LDS (computed from n_wk), builds the Q-bucket block
+// prefix, then takes its tokens 64 at a time: lane l loads the doc id,
+// doc meta and Philox uniform of token l; the four 16-lane rows then
+// process four tokens at a time (S over the doc's theta nonzeros in blocks
+// of 16, two-level inverse CDF).  z and the FP64 LL term are written per
+// token lane, so stores stay coalesced.
+// Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
+__global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
+    __shared__ __attribute__((aligned(16))) float phi_s[kWavesPerGroup][kMaxK];
+    __shared__ float rq_s[kWavesPerGroup][kWave];
+    __shared__ float rs_s[kWavesPerGroup][4][kWave + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * kWavesPerGroup + wave;
+    if (slot >= a.num_slots) return;  // whole wave leaves; no block barrier below
+    const SlotDesc s = a.slots[slot];
+    const int K = a.k;
+    const float alpha = a.alpha, beta = a.beta;
+    const uint32_t tb = a.topic_bits, tmask = (1u << tb) - 1u;
+    float *ph = phi_s[wave];
+    float *rq = rq_s[wave];
+
+    // stage phi_k = fl(fl(n_wk) + beta) / den_k
+    const int32_t *row = a.nwk + (int64_t)s.word * K;
+    for (int k = lane * 4; k < K; k += 4 * kWave) {
+        const int4 c = *reinterpret_cast<const int4 *>(row + k);
+        const float4 dn = *reinterpret_cast<const float4 *>(a.den + k);
+        ph[k + 0] = ((float)c.x + beta) / dn.x;
+        ph[k + 1] = ((float)c.y + beta) / dn.y;
+        ph[k + 2] = ((float)c.z + beta) / dn.z;
+        ph[k + 3] = ((float)c.w + beta) / dn.w;
+    }
+    wave_sync();
+    // Q-bucket block totals (balanced tree of 16 = last element of the
+    // in-block scan), then a 64-lane Kogge-Stone scan of the totals
+    const int nbq = K >> 4;
+    float T = 0.0f;
+    if (lane < nbq) {
+        const float *v = ph + lane * kBlock;
+        float q[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) q[i] = alpha * v[i];
+        const float t01 = q[0] + q[1], t23 = q[2] + q[3], t45 = q[4] + q[5], t67 = q[6] + q[7];
+        const float t89 = q[8] + q[9], tab = q[10] + q[11], tcd = q[12] + q[13], tef = q[14] + q[15];
+        const float u03 = t01 + t23, u47 = t45 + t67, u8b = t89 + tab, ucf = tcd + tef;
+        T = (u03 + u47) + (u8b + ucf);
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float y = __shfl_up(T, d, kWave);
+        if (lane >= d) T = T + y;
+    }
+    rq[lane] = T;
+    const float Q = __shfl(T, nbq - 1, kWave);
+    wave_sync();
+
+    const int rrow = lane >> 4, lr = lane & 15;
+    float *rs = rs_s[wave][rrow];
+    double ll = 0.0;
+    for (int64_t base = s.begin; base < s.end; base += kWave) {
+        const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
+        const bool valid = lane < ntok;
+        const int64_t i = base + lane;
+        uint32_t mstart = 0, mnnz = 0, mnd = 0;
+        float u = 0.0f;
+        if (valid) {
+            const uint32_t doc = a.tokens[i];
+            const uint4 m = *reinterpret_cast<const uint4 *>(a.meta + doc);
+            mstart = m.x;
+            mnnz = m.y;
+            mnd = m.z;
+            u = bits_to_uniform(token_bits(a.seed, (uint64_t)(i + s.gdelta), a.sweep, kPurposeSweep));
+        }
+        uint32_t zout = 0;
+        float sqout = 1.0f;
+        for (int r = 0; r < ntok; r += 4) {
+            const int src = r + rrow;
+            const uint32_t start = __shfl(mstart, src, kWave);
+            const uint32_t nnz = __shfl(mnnz, src, kWave);
+            const float uu = __shfl(u, src, kWave);
+            uint32_t topic = 0;
+            float SQ = 1.0f;
+            if (src < ntok) {
+                // S = sum over the doc's nonzeros, 16 per block
+                const uint32_t nb = (nnz + 15u) >> 4;
+                float R = 0.0f;
+                for (uint32_t b = 0; b < nb; ++b) {
+                    const uint32_t j = (b << 4) + (uint32_t)lr;
+                    float p = 0.0f;
+                    if (j < nnz) {
+                        const uint32_t e = a.theta[start + j];
+                        p = (float)(e >> tb) * ph[e & tmask];
+                    }
+                    p = scan16(p, lr);
+                    const float Tb = __shfl(p, 15, kBlock);
+                    R = R + Tb;
+                    if (lr == 0) rs[b + 1] = R;
+                }
+                wave_sync();
+                const float S = R;
+                SQ = S + Q;
+                const float frac = S / SQ;
+                if (uu < frac) {
+                    const float t = uu * SQ;
+                    int bsel = -1;
+                    for (uint32_t q0 = 0; q0 < nb; q0 += 16) {
+                        const uint32_t b = q0 + (uint32_t)lr;
+                        const bool hit = (b < nb) && (t < rs[b + 1]);
+                        const uint32_t msk = row_ballot(hit, rrow);
+                        if (msk) {
+                            bsel = (int)q0 + __builtin_ctz(msk);
+                            break;
+                        }
+                    }
+                    if (bsel < 0) {
+                        topic = a.theta[start + nnz - 1] & tmask;
+                    } else {
+                        const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
+                        uint32_t e = 0;
+                        float p = 0.0f;
+                        if (j < nnz) {
+                            e = a.theta[start + j];
+                            p = (float)(e >> tb) * ph[e & tmask];
+                        }
+                        p = scan16(p, lr);
+                        const float C = bsel == 0 ? p : rs[bsel] + p;
+                        const uint32_t msk = row_ballot((j < nnz) && (t < C), rrow);
+                        const int nvalid = (int)nnz - (bsel << 4) < 16 ? (int)nnz - (bsel << 4) : 16;
+                        const int msel = msk ? __builtin_ctz(msk) : nvalid - 1;
+                        topic = __shfl(e & tmask, msel, kBlock);
+                    }
+                } else {
+                    const float t = (uu - frac) * SQ;
+                    int bsel = -1;
+                    for (int q0 = 0; q0 < nbq; q0 += 16) {
+                        const int b = q0 + lr;
+                        const bool hit = (b < nbq) && (t < rq[b]);
+                        const uint32_t msk = row_ballot(hit, rrow);
+                        if (msk) {
+                            bsel = q0 + __builtin_ctz(msk);
+                            break;
+                        }
+                    }
+                    if (bsel < 0) {
+                        topic = (uint32_t)(K - 1);
+                    } else {
+                        const float v = scan16(alpha * ph[(bsel << 4) + lr], lr);
+                        const float C = bsel == 0 ? v : rq[bsel - 1] + v;
+                        const uint32_t msk = row_ballot(t < C, rrow);
+                        const int msel = msk ? __builtin_ctz(msk) : 15;
+                        topic = (uint32_t)((bsel << 4) + msel);
+                    }
+                }
+            }
+            // hand token (r + row)'s result to lane r + row
+            const int from = ((lane - r) * kBlock) & 63;
+            const uint32_t tz = __shfl(topic, from, kWave);
+            const float tsq = __shfl(SQ, from, kWave);
+            if (lane >= r && lane < r + 4) {
+                zout = tz;
+                sqout = tsq;
+            }
+            wave_sync();
+        }
+        if (valid) {
+            a.z[i] = (uint16_t)zout;
+            ll += log((double)sqout / ((double)mnd + a.k_alpha));
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
+    if (lane == 0) a.ll_partial[slot] = ll;
+}
+
+// Fixed-order FP64 sum of the per-slot partials (one workgroup).
+// Reference: LDATrainPerplexityReduce1/2 (kernel/lda_train_kernel.cu:273-321),
+// which truncated partials to int.
+__global__ __launch_bounds__(256) void ll_reduce_kernel(const double *p, int64_t n, double *out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) acc += p[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// Joint log-likelihood pieces: sum over nonzero n_wk of
+// lgamma(n_wk + beta) - lgamma(beta), one partial per workgroup.
+__global__ __launch_bounds__(256) void joint_words_kernel(const int32_t *nwk, int64_t n, double lg_beta, float beta,
+                                                          double *partial) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int32_t c = nwk[i];
+        if (c) acc += lgamma((double)c + (double)beta) - lg_beta;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// sum over docs and nonzero n_dk of lgamma(n_dk + alpha) - lgamma(alpha)
+__global__ __launch_bounds__(256) void joint_docs_kernel(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
+                                                         uint32_t tb, float alpha, double *partial) {
+    __shared__ double s[256];
+    const double lg_alpha = lgamma((double)alpha);
+    double acc = 0.0;
+    for (int d = blockIdx.x * 4 + (threadIdx.x >> 6); d < num_docs; d += gridDim.x * 4) {
+        const DocMeta m = meta[d];
+        for (uint32_t j = threadIdx.x & 63; j < m.nnz; j += 64)
+            acc += lgamma((double)(theta[m.start + j] >> tb) + (double)alpha) - lg_alpha;
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ void copy_kernel(const float4 *src, float4 *dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ launchers
+static int32_t grid_for(int64_t items, int32_t per_block, int32_t cap) {
+    int64_t g = (items + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int32_t)g;
+}
+
+hipError_t launch_init_z(const InitArgs &a, hipStream_t s) {
+    if (a.num_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(init_z_kernel, dim3(grid_for(a.num_slots, kWavesPerGroup, INT32_MAX)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(int32_t *nwk, const int32_t *words, int32_t n, int32_t k, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(n, 1, 4096)), dim3(256), 0, s, nwk, words, n, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_phi_rebuild(const PhiArgs &a, int32_t grid, hipStream_t s) {
+    if (a.num_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(phi_rebuild_kernel, dim3(grid_for(a.num_slots, kWavesPerGroup, grid)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_den(const int32_t *nk, float *den, int32_t k, float beta, int32_t num_words, hipStream_t s) {
+    hipLaunchKernelGGL(den_kernel, dim3(grid_for(k, 256, 64)), dim3(256), 0, s, nk, den, k, beta, num_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, int32_t num_words, int32_t k,
+                           float beta, hipStream_t s) {
+    const int64_t n = (int64_t)num_words * k;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(phi_head_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, nwk, den, head, n, k, beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_theta_rebuild(const ThetaArgs &a, int32_t grid, hipStream_t s) {
+    if (a.num_docs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(theta_rebuild_kernel, dim3(grid_for(a.num_docs, kWavesPerGroup, grid)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample(const SampleArgs &a, hipStream_t s) {
+    if (a.num_slots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sample_kernel, dim3(grid_for(a.num_slots, kWavesPerGroup, INT32_MAX)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ll_reduce(const double *partial, int64_t n, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(ll_reduce_kernel, dim3(1), dim3(256), 0, s, partial, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
+                                 double *partial, int32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(joint_words_kernel, dim3(grid), dim3(256), 0, s, nwk, (int64_t)num_words * k, lg_beta, beta,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t topic_bits,
+                                float alpha, double *partial, int32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, topic_bits, alpha,
+                       partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const float4 *src, float4 *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+}  // namespace culda

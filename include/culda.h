@@ -1,0 +1,284 @@
+/*
+ * culda.h -- C ABI of the MI355X (gfx950) collapsed-Gibbs LDA sampler.
+ *
+ * Drop-in boundary for the CuLDA_CGS hot path: one plain CGS iteration
+ * (per-token sampler with fused log-likelihood, phi count rebuild with
+ * on-device accumulation over chunks, sparse theta rebuild) plus the
+ * data_chunk corpus layout and the ModelPhi / ModelTheta save hooks.
+ *
+ * Conventions
+ *   - Every call returns CULDA_OK (0) or a negative culda_status; the
+ *     message of the last failure on the calling thread is returned by
+ *     culda_last_error().  The library never calls exit().
+ *   - Plain pointers and sizes only.  Host arrays passed in are owned by
+ *     the caller and copied; arrays handed out by culda_corpus_* stay
+ *     owned by the corpus object.
+ *   - A culda_ctx owns all device memory and one HIP stream.  A context is
+ *     used by one thread at a time; different contexts are independent.
+ *
+ * Reference interfaces replaced are cited as file:line relative to the
+ * CuLDA_CGS source tree (src_culda/, src_format/).
+ */
+#ifndef CULDA_H_
+#define CULDA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CULDA_ABI_VERSION 1
+
+typedef enum culda_status {
+    CULDA_OK = 0,
+    CULDA_ERR_ARG = -1,      /* invalid argument                                 */
+    CULDA_ERR_IO = -2,       /* file missing, unreadable or malformed            */
+    CULDA_ERR_FORMAT = -3,   /* data_chunk files inconsistent with each other    */
+    CULDA_ERR_DEVICE = -4,   /* HIP runtime failure (no GPU, launch failure ...) */
+    CULDA_ERR_MEMORY = -5,   /* host or device allocation failed                 */
+    CULDA_ERR_STATE = -6,    /* call out of order (e.g. sample before upload)    */
+    CULDA_ERR_LIMIT = -7     /* size beyond a documented limit                   */
+} culda_status;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *culda_last_error(void);
+/* CULDA_ABI_VERSION of the loaded library. */
+int culda_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Host corpus: the src_format data_chunk layout.
+ *
+ * For a prefix P and C chunks the files are
+ *   P.vocab          text "token id" per word            (format.cpp:97-99)
+ *   P.doc.idx        text "docId cumTokens" per doc       (format.cpp:88)
+ *   P.chunk          text "chunkId docIdEnd" per chunk    (format.cpp:131)
+ *   P.word.idx[c]    text "wordId cumEnd" for all V words, chunk-local
+ *   P.word.data[c]   raw int32 global doc ids, word-major, ascending doc
+ *                    within a word, repeats allowed     (data_chunk.h:49-67)
+ * The [c] suffix is present only when C > 1 (data_chunk.h:37-43).
+ * ---------------------------------------------------------------------- */
+typedef struct culda_corpus culda_corpus;
+
+typedef struct culda_corpus_info {
+    int32_t num_words;    /* V                                             */
+    int32_t num_docs;     /* D                                             */
+    int32_t num_chunks;   /* C                                             */
+    int32_t max_doc_len;
+    int64_t num_tokens;   /* N                                             */
+} culda_corpus_info;
+
+typedef struct culda_chunk_view {
+    int32_t docid_start;          /* first global doc id of the chunk       */
+    int32_t docid_end;            /* one past the last                      */
+    int64_t num_tokens;           /* tokens of the chunk                    */
+    const int64_t *word_indices;  /* [V+1] chunk-local cumulative offsets   */
+    const int32_t *word_tokens;   /* [num_tokens] global doc ids, word-major*/
+} culda_chunk_view;
+
+/* Load P.doc.idx, P.chunk, P.word.idx[c], P.word.data[c]; num_chunks must
+ * match P.chunk.  Replaces Document::loadDocument (model/doc.cpp:62-156)
+ * and DocChunk::loadChunk (model/doc_chunk.cpp:106-256), which exit(0)
+ * on error. */
+int culda_corpus_load(const char *prefix, int32_t num_chunks, culda_corpus **out);
+
+/* Build a corpus from arrays already in the data_chunk layout.
+ * doc_cum[D] holds cumulative token counts (the second column of
+ * P.doc.idx); chunk_doc_end[C] the second column of P.chunk;
+ * word_indices[c] points to V+1 offsets (first 0), word_tokens[c] to the
+ * doc ids of chunk c. */
+int culda_corpus_from_arrays(int32_t num_words, int32_t num_docs, const int64_t *doc_cum,
+                             int32_t num_chunks, const int32_t *chunk_doc_end,
+                             const int64_t *const *word_indices,
+                             const int32_t *const *word_tokens, culda_corpus **out);
+
+/* Write the corpus as data_chunk files under prefix.  Word tokens in
+ * P.vocab are the loaded/formatted ones, or "w<id>" for generated
+ * corpora.  Replaces DataChunk::writeChunk (src_format/data_chunk.h:49-67)
+ * and the vocab/doc.idx/chunk writers of src_format/format.cpp:88-131. */
+int culda_corpus_write(const culda_corpus *corpus, const char *prefix);
+
+/* Text corpus ("title tok tok ..." per line) -> data_chunk files, the
+ * behaviour of ./format input output_prefix numChunks
+ * (src_format/format.cpp:19-168): word ids in first-seen order, lines
+ * without a title skipped, docs split greedily into chunks of
+ * ceil(N/C) tokens. */
+int culda_format_text(const char *input_path, const char *prefix, int32_t num_chunks);
+
+/* Re-split a corpus into num_chunks chunks with the same greedy rule. */
+int culda_corpus_rechunk(const culda_corpus *corpus, int32_t num_chunks, culda_corpus **out);
+
+/* Parameters of the synthetic LDA generator (seeded, deterministic for a
+ * given parameter set, independent of the thread count).
+ *   phi_k ~ Dirichlet(topic_word_prior) over V, multiplied by
+ *   (rank+1)^-zipf_s and renormalised; theta_d ~ Dirichlet(doc_topic_prior)
+ *   over true_topics; doc lengths Poisson(len_mean) (len_kind 0, zero
+ *   lengths redrawn) or round(exp(Normal(len_mu, len_sigma))) (len_kind 1,
+ *   at least 1). */
+typedef struct culda_gen_params {
+    int32_t num_words;
+    int32_t num_docs;
+    int32_t true_topics;
+    int32_t num_chunks;
+    int32_t len_kind;        /* 0 = Poisson, 1 = lognormal                 */
+    int32_t num_threads;     /* 0 = all hardware threads                   */
+    double  len_mean;
+    double  len_mu;
+    double  len_sigma;
+    double  doc_topic_prior;
+    double  topic_word_prior;
+    double  zipf_s;
+    uint64_t seed;
+} culda_gen_params;
+
+int culda_corpus_generate(const culda_gen_params *params, culda_corpus **out);
+
+int culda_corpus_info_get(const culda_corpus *corpus, culda_corpus_info *out);
+/* doc_cum[D]: cumulative doc lengths (P.doc.idx second column). */
+int culda_corpus_doc_cum(const culda_corpus *corpus, const int64_t **doc_cum);
+int culda_corpus_chunk(const culda_corpus *corpus, int32_t chunk, culda_chunk_view *out);
+void culda_corpus_free(culda_corpus *corpus);
+
+/* ------------------------------------------------------------------------
+ * Device model.
+ * ---------------------------------------------------------------------- */
+typedef struct culda_ctx culda_ctx;
+
+typedef struct culda_params {
+    int32_t  k;              /* topics; multiple of 16, 16 <= k <= 1024     */
+    float    alpha;          /* doc-topic smoothing (reference -a)          */
+    float    beta;           /* topic-word smoothing (reference -b)         */
+    uint64_t seed;           /* Philox key; z init and every sweep derive
+                                from (seed, global token id, iteration)    */
+    int32_t  device;         /* HIP device ordinal                          */
+    int32_t  slot_tokens;    /* sampler tokens per wave slot (0 = default)  */
+    int32_t  profile;        /* 1 = time each kernel with HIP events        */
+    int32_t  fused_phi;      /* 1 = rebuild phi over all chunks in one pass */
+} culda_params;
+
+/* Create a context on params->device.  Replaces the Argument plumbing and
+ * ModelPhi / ModelTheta constructors (train/lda_train.cu:44-50). */
+int culda_create(const culda_params *params, culda_ctx **out);
+void culda_destroy(culda_ctx *ctx);
+
+/* Copy the corpus to HBM and build the per-chunk layouts (word-major doc
+ * stream, doc-major token index, theta slabs, wave slots).  Replaces
+ * DocChunk::allocGPU/toGPU (model/doc_chunk.cpp:277-356), ModelPhi::InitData
+ * (model/model_phi.cpp:51-68) and ModelTheta::InitData
+ * (model/model_theta.cpp:44-86).  All chunks stay resident. */
+int culda_upload(culda_ctx *ctx, const culda_corpus *corpus);
+
+/* z_i = mulhi64(Philox4x64-10((g_i, 0, 0, 0), (seed, 0))[0], k) for every
+ * token; g_i is the token's index in the one-chunk word-major layout.
+ * Replaces DocChunk::generateTopics (model/doc_chunk.cpp:259-264). */
+int culda_init_z(culda_ctx *ctx);
+
+/* Replace / read z of one chunk (word-major order, chunk-local). */
+int culda_set_z(culda_ctx *ctx, int32_t chunk, const uint16_t *z, int64_t n);
+int culda_get_z(culda_ctx *ctx, int32_t chunk, uint16_t *z, int64_t n);
+
+/* Rebuild n_wk [V*k] and n_k [k] from z.  chunk = -1 rebuilds from all
+ * chunks in one pass; chunk >= 0 adds that chunk's counts, overwriting
+ * when accumulate == 0.  Replaces LDAUpdatePhiAPI
+ * (kernel/lda_phi_kernel.cu:136-159) and, for chunks in turn,
+ * LDAUpdatePhiReduceAPI (kernel/lda_phi_kernel.cu:185-207). */
+int culda_rebuild_phi(culda_ctx *ctx, int32_t chunk, int32_t accumulate);
+
+/* Prepare the head denominators den_k = (float)n_k + beta*(float)V used by
+ * every sampler launch.  Replaces LDAComputePhiHeadAPI
+ * (kernel/lda_phi_kernel.cu:49-72); the head itself is formed on the fly. */
+int culda_phi_head(culda_ctx *ctx);
+
+/* Rebuild the sparse doc-topic rows of one chunk (ascending topics).
+ * Replaces LDAUpdateThetaAPI (kernel/lda_theta_kernel.cu:89-125) and
+ * ModelThetaChunk::UpdateThetaGPU (model/model_theta_chunk.cpp:106-140). */
+int culda_rebuild_theta(culda_ctx *ctx, int32_t chunk);
+
+/* One sweep over the tokens of one chunk with frozen counts (sweep index
+ * iteration >= 1 keys the Philox stream).  Writes new z and the chunk's
+ * FP64 log-likelihood partials.  Replaces the LDAKernelTrain launch
+ * (train/SingleChunkSingleGPU.h:75-100, kernel/lda_train_kernel.cu:20-271). */
+int culda_sample(culda_ctx *ctx, int32_t chunk, int32_t iteration);
+
+/* Sum of the log-likelihood terms of the last sample() of every chunk,
+ * divided by N (FP64, deterministic order).  Replaces LDATrainPerplexity
+ * (kernel/lda_train_kernel.cu:323-386). */
+int culda_loglik(culda_ctx *ctx, double *ll_per_token);
+
+/* One full iteration: sample every chunk, LL, phi rebuild + head, theta
+ * rebuild, synchronise.  ll = predictive LL/token of the pre-sweep model,
+ * seconds = wall time of the whole iteration.  Replaces one pass of the
+ * loop body train/SingleChunkSingleGPU.h:70-127 (and, for C > 1, of
+ * train/MultiChunkMultiGPUequal.h:88-169 on one device). */
+int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds);
+
+/* Rebuild phi, head and theta from the current z (initial model state,
+ * train/SingleChunkSingleGPU.h:24-41). */
+int culda_rebuild_all(culda_ctx *ctx);
+
+/* Joint log p(w, z) in FP64 with lgamma over the current counts. */
+int culda_loglik_joint(culda_ctx *ctx, double *out);
+
+int culda_sync(culda_ctx *ctx);
+
+/* Counts to host.  n_wk [V*k] row-major by word, n_k [k].  Replaces
+ * ModelPhi::MasterGPUToCPU (model/model_phi.cpp:88-103). */
+int culda_get_phi(culda_ctx *ctx, int32_t *n_wk, int32_t *n_k);
+/* phi head (n_wk + beta)/(n_k + beta V) in FP32, [V*k]. */
+int culda_get_phi_head(culda_ctx *ctx, float *head);
+
+/* Theta rows of one chunk in compact CSR over the chunk's docs:
+ * offsets[chunk_docs+1], counts[nnz_total], topics[nnz_total].  Call with
+ * counts == topics == NULL to obtain offsets (and so the total) first.
+ * Replaces ModelThetaChunk::toCPU (model/model_theta_chunk.cpp:288-323). */
+int culda_get_theta(culda_ctx *ctx, int32_t chunk, int64_t *offsets, int32_t *counts,
+                    int16_t *topics);
+
+/* Save hooks (host side, after a download).
+ * save_phi: P.phi.bin  = int32 V, int32 k, int32 n_wk[V*k], int32 n_k[k]
+ *           P.phi.txt  = the text dump of ModelPhi::savePhi
+ *                        (model/model_phi.cpp:161-191) for every word.
+ * save_theta: P.theta.bin = int32 D, int32 k, int64 offsets[D+1],
+ *           int32 counts[nnz], int16 topics[nnz] (global doc order);
+ *           P.theta.txt = "doc | total (topic,count) ..." per doc, the
+ *           commented saveDoc of model/model_theta.cpp:207-241. */
+int culda_save_phi(culda_ctx *ctx, const char *prefix);
+int culda_save_theta(culda_ctx *ctx, const char *prefix);
+
+/* Checkpoint = (z of every chunk, next iteration, seed): all counts derive
+ * from z and the random stream is counter-based, so resume is exact. */
+int culda_save_checkpoint(culda_ctx *ctx, const char *path, int32_t next_iteration);
+int culda_load_checkpoint(culda_ctx *ctx, const char *path, int32_t *next_iteration);
+
+typedef struct culda_stats {
+    int64_t num_tokens;
+    int64_t num_slots;          /* sampler wave slots over all chunks          */
+    int64_t sum_doc_nnz;        /* sum_d nnz(d) after the last theta rebuild    */
+    int64_t sum_token_nnz;      /* sum_d n_d * nnz(d): theta entries gathered   */
+    int64_t theta_capacity;     /* theta slab entries over all chunks           */
+    double  sampler_ms;         /* last iteration, summed over chunks (profile) */
+    double  phi_ms;
+    double  theta_ms;
+    double  other_ms;
+    int32_t sampler_launches;   /* launches in the last iteration               */
+    int32_t pad;
+} culda_stats;
+
+int culda_get_stats(culda_ctx *ctx, culda_stats *out);
+
+/* Algorithmic bytes moved by the last sampler launches of the last
+ * iteration (doc id + meta + z write + theta entries gathered + count rows
+ * staged), summed over chunks. */
+int culda_sampler_bytes(culda_ctx *ctx, double *bytes);
+
+/* STREAM-style device copy of n_bytes (multiple of 16) between two device
+ * buffers owned by the context, timed with HIP events: achievable HBM
+ * bandwidth for the roofline.  gbps = 2*n_bytes/time. */
+int culda_stream_copy(culda_ctx *ctx, int64_t n_bytes, int32_t repeats, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CULDA_H_ */

@@ -1,0 +1,139 @@
+"""HIP path vs the CPU oracle, called through the C ABI (libculda.so).
+
+Bit-exact for z, n_wk, n_k and theta; LL/token within 1e-5 relative of the
+oracle's FP64 sum.
+"""
+import numpy as np
+import pytest
+
+from helpers import chunks_from_docs, oracle_model, product_corpus, random_docs
+
+pytestmark = pytest.mark.gpu
+
+LL_RTOL = 1e-5
+
+
+def _setup(seed, V=300, D=120, mean_len=50, chunks=1, K=64, empty_docs=0):
+    rng = np.random.default_rng(seed)
+    docs = random_docs(rng, V, D, mean_len, empty_docs=empty_docs)
+    arrays = chunks_from_docs(V, docs, chunks)
+    return V, K, arrays
+
+
+def _pair(V, K, arrays, alpha, beta, seed, **kw):
+    import culda_cgs_amd as cg
+    corpus = product_corpus(V, *arrays)
+    s = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=seed, **kw)
+    s.upload(corpus)
+    om = oracle_model(V, K, *arrays)
+    return s, om
+
+
+def _theta_equal(s, om, arrays):
+    ends = arrays[1]
+    start = 0
+    for c, e in enumerate(ends):
+        off, cnt, top = s.get_theta(c)
+        for d in range(start, e):
+            oc, ot = om.theta_rows(d)
+            lo, hi = off[d - start], off[d - start + 1]
+            assert np.array_equal(cnt[lo:hi], oc), f"theta counts differ at doc {d}"
+            assert np.array_equal(top[lo:hi], ot), f"theta topics differ at doc {d}"
+        start = e
+
+
+@pytest.mark.parametrize("chunks", [1, 3])
+def test_init_z_and_rebuild(chunks):
+    V, K, arrays = _setup(1, chunks=chunks, empty_docs=3)
+    s, om = _pair(V, K, arrays, 0.1, 0.01, 11)
+    s.init_z()
+    om.init_z(11)
+    for c in range(chunks):
+        assert np.array_equal(s.get_z(c), om.z[c]), f"init z differs in chunk {c}"
+    s.rebuild_all()
+    om.rebuild()
+    nwk, nk = s.get_phi()
+    assert np.array_equal(nwk.ravel(), om.nwk)
+    assert np.array_equal(nk, om.nk)
+    _theta_equal(s, om, arrays)
+    head = s.get_phi_head()
+    assert np.array_equal(head, om.phi_head(0.01)), "FP32 head differs"
+    s.close()
+
+
+@pytest.mark.parametrize("K", [16, 48, 64, 256, 1024])
+def test_sweeps_bit_exact(K):
+    V, _, arrays = _setup(2, V=400, D=150, mean_len=70, K=K)
+    alpha, beta, seed = 50.0 / K, 0.01, 5
+    s, om = _pair(V, K, arrays, alpha, beta, seed)
+    s.init_z()
+    s.rebuild_all()
+    om.init_z(seed)
+    om.rebuild()
+    for it in range(1, 6):
+        ll, _ = s.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+        z = s.get_z(0)
+        bad = np.flatnonzero(z != om.z[0])
+        assert bad.size == 0, f"K={K} sweep {it}: {bad.size} tokens differ (first {bad[:5]})"
+        ref = out.ll_sum / om.num_tokens
+        assert abs(ll - ref) <= LL_RTOL * abs(ref)
+    nwk, nk = s.get_phi()
+    assert np.array_equal(nwk.ravel(), om.nwk)
+    _theta_equal(s, om, arrays)
+    s.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_multichunk_matches_oracle_and_one_chunk(fused):
+    V, K, arrays4 = _setup(3, V=500, D=200, mean_len=60, chunks=4, K=128)
+    _, _, arrays1 = _setup(3, V=500, D=200, mean_len=60, chunks=1, K=128)
+    alpha, beta, seed = 0.2, 0.02, 9
+    s4, om = _pair(V, K, arrays4, alpha, beta, seed, fused_phi=fused)
+    s1, _ = _pair(V, K, arrays1, alpha, beta, seed)
+    for s in (s4, s1):
+        s.init_z()
+        s.rebuild_all()
+    om.init_z(seed)
+    om.rebuild()
+    for it in range(1, 4):
+        l4, _ = s4.iterate(it)
+        l1, _ = s1.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+        for c in range(4):
+            assert np.array_equal(s4.get_z(c), om.z[c])
+        ref = out.ll_sum / om.num_tokens
+        assert abs(l4 - ref) <= LL_RTOL * abs(ref)
+        assert abs(l1 - ref) <= LL_RTOL * abs(ref)
+    # 4 chunks == 1 chunk, bit for bit
+    n4, k4 = s4.get_phi()
+    n1, k1 = s1.get_phi()
+    assert np.array_equal(n4, n1) and np.array_equal(k4, k1)
+    s4.close()
+    s1.close()
+
+
+def test_single_sweep_from_common_state_and_boundaries():
+    """One sweep from an arbitrary (non-initial) z; report boundary draws."""
+    V, K, arrays = _setup(4, V=1000, D=400, mean_len=120, K=256)
+    alpha, beta, seed = 0.1, 0.05, 21
+    s, om = _pair(V, K, arrays, alpha, beta, seed)
+    rng = np.random.default_rng(4)
+    z0 = (rng.integers(0, 16, size=arrays[3][0].size) * 7 % K).astype(np.uint16)
+    s.set_z(0, z0)
+    s.rebuild_all()
+    om.set_z([z0])
+    om.rebuild()
+    s.sample(0, 3)
+    ll = s.loglik()
+    out, ex, fl = om.sweep(alpha, beta, seed, 3, want_exact=True)
+    z = s.get_z(0)
+    assert np.array_equal(z, om.z[0])
+    # faithful FP32 vs exact FP64 disagree only at flagged boundary draws
+    mis = np.flatnonzero(om.z[0] != ex[0])
+    assert np.all(fl[0][mis] == 1)
+    ref = out.ll_sum / om.num_tokens
+    assert abs(ll - ref) <= LL_RTOL * abs(ref)
+    s.close()
