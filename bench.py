@@ -1,0 +1,180 @@
+"""Benchmark: plain collapsed-Gibbs LDA iterations on one MI355X.
+
+A step is one full iteration of the hot path over the whole corpus: sample
+every chunk (in turn) with frozen counts + FP64 LL, rebuild n_wk/n_k on the
+device, rebuild every chunk's theta -- the window the reference times
+(src_culda/train/SingleChunkSingleGPU.h:72-127).  Inputs are resident in
+HBM before timing starts.
+
+Default workload (the at-scale plain-CGS configuration): K=1024,
+alpha=50/1024, beta=0.01; synthetic LDA corpus V=100,000 (Zipf 1.0
+frequencies), D=300,000 docs with lognormal(5.6, 0.6) lengths (~100M
+tokens), K_true=500, doc-topic Dirichlet(0.1), topic-word Dirichlet(0.01),
+generator seed 2; 4 data_chunks.
+
+Prints ONE JSON line.  --gpus > 1 is not supported (one device per run).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+CONFIGS = {
+    # name: generator + sampler settings
+    "c2": dict(V=100_000, D=300_000, KT=500, len_kind=1, len_mean=0.0, len_mu=5.6, len_sigma=0.6,
+               doc_prior=0.1, word_prior=0.01, zipf=1.0, chunks=4, gen_seed=2,
+               desc="plain CGS K=1024, V=100k Zipf(1.0), D=300k lognormal(5.6,0.6) docs, K_true=500, "
+                    "4 data_chunks processed in turn, on-device phi accumulation"),
+    "c1": dict(V=10_000, D=10_000, KT=100, len_kind=0, len_mean=100.0, len_mu=0.0, len_sigma=0.0,
+               doc_prior=0.1, word_prior=0.05, zipf=1.07, chunks=1, gen_seed=1,
+               desc="plain CGS K=1024, V=10k Zipf(1.07), D=10k Poisson(100) docs, K_true=100, 1 chunk"),
+}
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--k", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--slot-tokens", type=int, default=0)
+    ap.add_argument("--fused-phi", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py drives exactly one GPU (the corpus is processed chunk by chunk on one device); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import culda_cgs_amd as cg
+
+    cfg = CONFIGS[args.config]
+    K = args.k
+    alpha, beta = 50.0 / K, 0.01
+    t0 = time.time()
+    corpus = cg.Corpus.generate(cfg["V"], cfg["D"], cfg["KT"], cfg["chunks"], len_kind=cfg["len_kind"],
+                                len_mean=cfg["len_mean"], len_mu=cfg["len_mu"], len_sigma=cfg["len_sigma"],
+                                doc_topic_prior=cfg["doc_prior"], topic_word_prior=cfg["word_prior"],
+                                zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=min(16, os.cpu_count() or 1))
+    info = corpus.info()
+    N = info.num_tokens
+    log(f"corpus {args.config}: V={info.num_words} D={info.num_docs} N={N} chunks={info.num_chunks} "
+        f"({time.time() - t0:.1f}s)")
+
+    s = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=args.seed, profile=True, slot_tokens=args.slot_tokens,
+                   fused_phi=bool(args.fused_phi))
+    t0 = time.time()
+    s.upload(corpus)
+    s.init_z()
+    s.rebuild_all()
+    s.sync()
+    log(f"upload + init {time.time() - t0:.1f}s")
+    copy_gbps = s.stream_copy(1 << 30, 20)
+    log(f"stream copy {copy_gbps:.0f} GB/s")
+
+    it = 1
+    for _ in range(args.warmup):
+        s.iterate(it)
+        it += 1
+    s.sync()
+    samp_ms, samp_bytes, iter_s, lls = [], [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ll, sec = s.iterate(it)
+        st = s.stats()
+        samp_ms.append(st.sampler_ms)
+        samp_bytes.append(s.sampler_bytes())
+        iter_s.append(sec)
+        lls.append(ll)
+        it += 1
+    s.sync()
+    elapsed = time.perf_counter() - t_start
+    st = s.stats()
+    tokens_per_s = N * args.steps / elapsed
+    launches = st.sampler_launches
+    avg_launch_ms = float(np.sum(samp_ms)) / (len(samp_ms) * launches)
+    bytes_per_launch = float(np.sum(samp_bytes)) / (len(samp_bytes) * launches)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    log(f"{tokens_per_s / 1e6:.1f} M tokens/s, LL/token {lls[-1]:.6f}, sampler {avg_launch_ms:.3f} ms/launch, "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    traffic = None
+    prof = ROOT / "profiles" / f"sampler_{args.config}_traffic.json"
+    if prof.exists() and K == 1024:
+        try:
+            traffic = json.loads(prof.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s, corpus, info, K, alpha, beta, args.seed, it, args.cpu_threads)
+
+    line = {
+        "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
+        "value": tokens_per_s,
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded LDA generative model)",
+        "config": {"workload": cfg["desc"], "k": K, "alpha": alpha, "beta": beta, "num_tokens": N,
+                   "num_docs": info.num_docs, "num_words": info.num_words, "chunks": info.num_chunks,
+                   "iterations_timed": [it - args.steps, it - 1], "ll_per_token_last": lls[-1],
+                   "phi_rebuild": "fused over chunks" if args.fused_phi else "chunks in turn (+=)",
+                   "stream_copy_gbps": copy_gbps,
+                   "sum_token_nnz_last": st.sum_token_nnz},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "kernel": "sample_kernel", "avg_launch_ms": avg_launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    s.close()
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(s, corpus, info, K, alpha, beta, seed, next_it, threads):
+    """The C oracle (same schedule, OpenMP) timed on this host for one full
+    iteration from the GPU's current state."""
+    import oracle
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    C = info.num_chunks
+    parts = [corpus.chunk(c) for c in range(C)]
+    om = oracle.OracleModel(info.num_words, K, corpus.doc_cum(), [p[1] for p in parts], [p[2] for p in parts],
+                            [p[3] for p in parts], z=[s.get_z(c) for c in range(C)])
+    om.rebuild()
+    t0 = time.perf_counter()
+    om.sweep_fast(alpha, beta, seed, next_it)
+    om.rebuild()
+    dt = time.perf_counter() - t0
+    log(f"cpu oracle: 1 iteration in {dt:.1f}s on {threads} threads")
+    return {"value": info.num_tokens / dt, "unit": "tokens/s", "cores": threads, "kind": "port",
+            "sample": f"one full iteration (sweep {next_it} + phi/theta rebuild) of the same corpus "
+                      f"({info.num_tokens} tokens) from the GPU's state after the timed steps"}
+
+
+if __name__ == "__main__":
+    main()

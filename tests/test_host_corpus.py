@@ -1,0 +1,123 @@
+"""Host side of the boundary (no GPU): data_chunk formatter, writer, loader,
+re-chunking and the synthetic generator, through libculda.so's C ABI."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import culda_cgs_amd as cg
+from helpers import chunks_from_docs, random_docs
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "format_tiny.json").read_text())
+
+
+def _chunk_name(prefix, what, c, C):
+    return Path(f"{prefix}{what}{c if C > 1 else ''}")
+
+
+@pytest.mark.parametrize("C", [1, 2, 3])
+def test_format_text_matches_reference_layout(tmp_path, C):
+    inp = tmp_path / "in.txt"
+    inp.write_text(GOLD["input"])
+    prefix = tmp_path / "out"
+    cg.format_text(inp, prefix, C)
+    exp = GOLD["chunks"][str(C)]
+    assert Path(f"{prefix}.vocab").read_text() == GOLD["vocab"]
+    assert Path(f"{prefix}.doc.idx").read_text() == GOLD["doc_idx"]
+    assert Path(f"{prefix}.chunk").read_text() == exp["chunk"]
+    for c in range(C):
+        assert _chunk_name(prefix, ".word.idx", c, C).read_text() == exp["word_idx"][c]
+        data = np.fromfile(_chunk_name(prefix, ".word.data", c, C), dtype="<i4")
+        assert data.tolist() == exp["word_data"][c]
+    # the loader reads it back
+    corpus = cg.Corpus.load(prefix, C)
+    info = corpus.info()
+    assert (info.num_words, info.num_docs, info.num_tokens, info.num_chunks) == (4, 5, 10, C)
+    assert corpus.doc_cum().tolist() == [3, 4, 8, 8, 10]
+
+
+def test_loader_errors(tmp_path):
+    inp = tmp_path / "in.txt"
+    inp.write_text(GOLD["input"])
+    prefix = tmp_path / "out"
+    cg.format_text(inp, prefix, 2)
+    with pytest.raises(cg.CuldaError) as e:  # chunk count does not match .chunk
+        cg.Corpus.load(prefix, 3)
+    assert e.value.status == -3
+    with pytest.raises(cg.CuldaError) as e:  # missing files
+        cg.Corpus.load(tmp_path / "nothing", 1)
+    assert e.value.status == -2
+    # truncated data file
+    d1 = Path(f"{prefix}.word.data1")
+    d1.write_bytes(d1.read_bytes()[:-4])
+    with pytest.raises(cg.CuldaError) as e:
+        cg.Corpus.load(prefix, 2)
+    assert e.value.status == -3
+    # a doc id outside its chunk
+    cg.format_text(inp, prefix, 2)
+    d0 = Path(f"{prefix}.word.data0")
+    a = np.fromfile(d0, dtype="<i4")
+    a[0] = 4
+    a.tofile(d0)
+    with pytest.raises(cg.CuldaError):
+        cg.Corpus.load(prefix, 2)
+
+
+def test_write_load_roundtrip_and_rechunk(tmp_path):
+    rng = np.random.default_rng(0)
+    V = 50
+    docs = random_docs(rng, V, 40, 12, empty_docs=2)
+    doc_cum, ends, wi, wt = chunks_from_docs(V, docs, 1)
+    corpus = cg.Corpus.from_arrays(V, doc_cum, ends, wi, wt)
+    corpus.write(tmp_path / "c")
+    back = cg.Corpus.load(tmp_path / "c", 1)
+    _, e, wi2, wt2 = back.chunk(0)
+    assert e == len(docs) and np.array_equal(wi2, wi[0]) and np.array_equal(wt2, wt[0])
+    for C in (2, 3, 5):
+        r = corpus.rechunk(C)
+        dc, en, wic, wtc = chunks_from_docs(V, docs, C)
+        for c in range(C):
+            s, e, a, b = r.chunk(c)
+            assert e == en[c]
+            assert np.array_equal(a, wic[c]) and np.array_equal(b, wtc[c])
+        # concatenating the chunks' word lists gives the one-chunk lists
+        for w in range(V):
+            cat = np.concatenate([r.chunk(c)[3][r.chunk(c)[2][w]:r.chunk(c)[2][w + 1]] for c in range(C)])
+            assert np.array_equal(cat, wt[0][wi[0][w]:wi[0][w + 1]])
+
+
+def test_from_arrays_validation():
+    V = 3
+    with pytest.raises(cg.CuldaError):  # doc length disagrees with tokens
+        cg.Corpus.from_arrays(V, [2, 3], [2], [np.array([0, 1, 2, 2])], [np.array([0, 1], np.int32)])
+    with pytest.raises(cg.CuldaError):  # offsets decrease
+        cg.Corpus.from_arrays(V, [1], [1], [np.array([0, 1, 0, 1])], [np.array([0], np.int32)])
+
+
+def test_generator_deterministic_and_shaped():
+    kw = dict(len_mean=80.0, doc_topic_prior=0.1, topic_word_prior=0.05, zipf_s=1.07, seed=1)
+    a = cg.Corpus.generate(2000, 3000, 50, 1, num_threads=1, **kw)
+    b = cg.Corpus.generate(2000, 3000, 50, 1, num_threads=7, **kw)
+    assert np.array_equal(a.chunk(0)[3], b.chunk(0)[3])
+    assert np.array_equal(a.doc_cum(), b.doc_cum())
+    lens = np.diff(np.concatenate([[0], a.doc_cum()]))
+    assert lens.min() >= 1
+    assert abs(lens.mean() - 80.0) < 1.0  # Poisson(80): sd of the mean ~0.16
+    assert abs(lens.var() / 80.0 - 1.0) < 0.15
+    # word frequencies follow the Zipf reweighting: log-log slope near -1.07
+    wi = a.chunk(0)[2]
+    cnt = np.diff(wi).astype(float)
+    r = np.arange(1, 2001)
+    m = (r >= 5) & (r <= 500) & (cnt > 0)
+    slope = np.polyfit(np.log(r[m]), np.log(cnt[m]), 1)[0]
+    assert -1.6 < slope < -0.6
+    c4 = cg.Corpus.generate(2000, 3000, 50, 4, num_threads=3, **kw)
+    assert c4.info().num_chunks == 4 and c4.info().num_tokens == a.info().num_tokens
+
+
+def test_generator_lognormal_lengths():
+    c = cg.Corpus.generate(500, 4000, 20, 1, len_kind=1, len_mu=3.0, len_sigma=0.5, seed=5)
+    lens = np.diff(np.concatenate([[0], c.doc_cum()]))
+    ll = np.log(lens)
+    assert abs(ll.mean() - 3.0) < 0.05 and abs(ll.std() - 0.5) < 0.05
