@@ -28,15 +28,25 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive Kogge-Stone scan inside a 16-lane row.
-__device__ __forceinline__ float scan16(float x, int lr) {
-#pragma unroll
-    for (int d = 1; d < kBlock; d <<= 1) {
-        const float y = __shfl_up(x, d, kBlock);
-        if (lr >= d) x = x + y;
-    }
+// DPP move with zero for lanes whose source is outside the 16-lane row.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// Inclusive Kogge-Stone scan inside each 16-lane row (DPP row_shr:1,2,4,8):
+// x_m <- x_m + x_{m-d} for m >= d; lanes with m < d add 0, which leaves
+// the non-negative values unchanged.
+__device__ __forceinline__ float scan16(float x) {
+    x = x + dpp_f<0x111>(x);
+    x = x + dpp_f<0x112>(x);
+    x = x + dpp_f<0x114>(x);
+    x = x + dpp_f<0x118>(x);
     return x;
 }
+
+// Lane 15 of each row broadcast to the whole row (DPP row_newbcast:15).
+__device__ __forceinline__ float row_last(float x) { return dpp_f<0x15F>(x); }
 
 // Row-local ballot: bit m set when lane m of this row voted true.
 __device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
@@ -194,15 +204,19 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // One wave per slot (one word's token range).  The wave stages the word's
 // head row phi_k in LDS (computed from n_wk), builds the Q-bucket block
 // prefix, then takes its tokens 64 at a time: lane l loads the doc id,
-// doc meta and Philox uniform of token l; the four 16-lane rows then
-// process four tokens at a time (S over the doc's theta nonzeros in blocks
-// of 16, two-level inverse CDF).  z and the FP64 LL term are written per
-// token lane, so stores stay coalesced.
+// doc meta and Philox uniform of token l; the batch is sorted by nnz so
+// that the four 16-lane rows, which process four tokens side by side,
+// run S loops of similar length.  Theta blocks are prefetched one block
+// ahead; scans are DPP row operations.  z and the FP64 LL term are
+// written per token.  The processing order never changes a result: every
+// token depends only on the frozen counts and its own Philox draw.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
 __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
     __shared__ __attribute__((aligned(16))) float phi_s[kWavesPerGroup][kMaxK];
     __shared__ float rq_s[kWavesPerGroup][kWave];
-    __shared__ float rs_s[kWavesPerGroup][4][kWave + 1];
+    __shared__ float rs_s[kWavesPerGroup][4][kWave + 4];
+    __shared__ uint32_t zres_s[kWavesPerGroup][kWave];
+    __shared__ float sqres_s[kWavesPerGroup][kWave];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int slot = blockIdx.x * kWavesPerGroup + wave;
     if (slot >= a.num_slots) return;  // whole wave leaves; no block barrier below
@@ -212,6 +226,8 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
     const uint32_t tb = a.topic_bits, tmask = (1u << tb) - 1u;
     float *ph = phi_s[wave];
     float *rq = rq_s[wave];
+    uint32_t *zres = zres_s[wave];
+    float *sqres = sqres_s[wave];
 
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
     const int32_t *row = a.nwk + (int64_t)s.word * K;
@@ -253,10 +269,10 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
         const bool valid = lane < ntok;
-        const int64_t i = base + lane;
         uint32_t mstart = 0, mnnz = 0, mnd = 0;
         float u = 0.0f;
         if (valid) {
+            const int64_t i = base + lane;
             const uint32_t doc = a.tokens[i];
             const uint4 m = *reinterpret_cast<const uint4 *>(a.meta + doc);
             mstart = m.x;
@@ -264,101 +280,113 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
             mnd = m.z;
             u = bits_to_uniform(token_bits(a.seed, (uint64_t)(i + s.gdelta), a.sweep, kPurposeSweep));
         }
-        uint32_t zout = 0;
-        float sqout = 1.0f;
+        // bitonic sort of (valid, nnz, lane), descending: positions
+        // 0..ntok-1 hold the valid tokens, longest theta rows first
+        uint32_t key = (valid ? (0x80000000u | ((mnnz < 0xffffu ? mnnz : 0xffffu) << 6)) : 0u) | (uint32_t)lane;
+#pragma unroll
+        for (int kk = 2; kk <= kWave; kk <<= 1) {
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                const uint32_t other = __shfl_xor(key, j, kWave);
+                const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+                const uint32_t mx = key > other ? key : other, mn = key > other ? other : key;
+                key = (lower == up) ? mx : mn;
+            }
+        }
+        const int tok = (int)(key & 63u);
+        const uint32_t pstart = __shfl(mstart, tok, kWave);
+        const uint32_t pnnz = __shfl(mnnz, tok, kWave);
+        const uint32_t pnd = __shfl(mnd, tok, kWave);
+        const float pu = __shfl(u, tok, kWave);
         for (int r = 0; r < ntok; r += 4) {
-            const int src = r + rrow;
-            const uint32_t start = __shfl(mstart, src, kWave);
-            const uint32_t nnz = __shfl(mnnz, src, kWave);
-            const float uu = __shfl(u, src, kWave);
-            uint32_t topic = 0;
-            float SQ = 1.0f;
-            if (src < ntok) {
-                // S = sum over the doc's nonzeros, 16 per block
+            const int p = r + rrow;
+            const uint32_t start = __shfl(pstart, p, kWave);
+            const uint32_t nnz = __shfl(pnnz, p, kWave);
+            const float uu = __shfl(pu, p, kWave);
+            if (p < ntok) {
+                // S = sum over the doc's nonzeros, 16 per block; every block
+                // lies inside the doc's 16-aligned slab, entries past nnz
+                // are masked
+                // blocks are loaded four at a time, one group ahead, without
+                // branches (a block past nb re-reads block 0 of the slab);
+                // entries past nnz contribute exact zeros (R + 0 == R)
                 const uint32_t nb = (nnz + 15u) >> 4;
+                const uint32_t *tp = a.theta + start + lr;
+                uint32_t cur[4], nxt[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cur[q] = tp[(uint32_t)q < nb ? q << 4 : 0];
                 float R = 0.0f;
-                for (uint32_t b = 0; b < nb; ++b) {
-                    const uint32_t j = (b << 4) + (uint32_t)lr;
-                    float p = 0.0f;
-                    if (j < nnz) {
-                        const uint32_t e = a.theta[start + j];
-                        p = (float)(e >> tb) * ph[e & tmask];
+                for (uint32_t b0 = 0; b0 < nb; b0 += 4) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) nxt[q] = tp[b0 + 4 + q < nb ? (b0 + 4 + q) << 4 : 0u];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const uint32_t e = cur[q];
+                        const float pv = (float)(e >> tb) * ph[e & tmask];
+                        const float pm = (((b0 + q) << 4) + (uint32_t)lr) < nnz ? pv : 0.0f;
+                        R = R + row_last(scan16(pm));
+                        if (lr == 0) rs[b0 + q + 1] = R;
                     }
-                    p = scan16(p, lr);
-                    const float Tb = __shfl(p, 15, kBlock);
-                    R = R + Tb;
-                    if (lr == 0) rs[b + 1] = R;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
                 }
                 wave_sync();
                 const float S = R;
-                SQ = S + Q;
+                const float SQ = S + Q;
                 const float frac = S / SQ;
-                if (uu < frac) {
-                    const float t = uu * SQ;
-                    int bsel = -1;
-                    for (uint32_t q0 = 0; q0 < nb; q0 += 16) {
-                        const uint32_t b = q0 + (uint32_t)lr;
-                        const bool hit = (b < nb) && (t < rs[b + 1]);
-                        const uint32_t msk = row_ballot(hit, rrow);
-                        if (msk) {
-                            bsel = (int)q0 + __builtin_ctz(msk);
-                            break;
-                        }
-                    }
-                    if (bsel < 0) {
-                        topic = a.theta[start + nnz - 1] & tmask;
-                    } else {
-                        const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
-                        uint32_t e = 0;
-                        float p = 0.0f;
-                        if (j < nnz) {
-                            e = a.theta[start + j];
-                            p = (float)(e >> tb) * ph[e & tmask];
-                        }
-                        p = scan16(p, lr);
-                        const float C = bsel == 0 ? p : rs[bsel] + p;
-                        const uint32_t msk = row_ballot((j < nnz) && (t < C), rrow);
-                        const int nvalid = (int)nnz - (bsel << 4) < 16 ? (int)nnz - (bsel << 4) : 16;
-                        const int msel = msk ? __builtin_ctz(msk) : nvalid - 1;
-                        topic = __shfl(e & tmask, msel, kBlock);
-                    }
-                } else {
-                    const float t = (uu - frac) * SQ;
-                    int bsel = -1;
-                    for (int q0 = 0; q0 < nbq; q0 += 16) {
-                        const int b = q0 + lr;
-                        const bool hit = (b < nbq) && (t < rq[b]);
-                        const uint32_t msk = row_ballot(hit, rrow);
-                        if (msk) {
-                            bsel = q0 + __builtin_ctz(msk);
-                            break;
-                        }
-                    }
-                    if (bsel < 0) {
-                        topic = (uint32_t)(K - 1);
-                    } else {
-                        const float v = scan16(alpha * ph[(bsel << 4) + lr], lr);
-                        const float C = bsel == 0 ? v : rq[bsel - 1] + v;
-                        const uint32_t msk = row_ballot(t < C, rrow);
-                        const int msel = msk ? __builtin_ctz(msk) : 15;
-                        topic = (uint32_t)((bsel << 4) + msel);
+                const bool sb = uu < frac;
+                const float t = sb ? uu * SQ : (uu - frac) * SQ;
+                const float *rr = sb ? rs + 1 : rq;  // inclusive prefix after each block
+                const int nbb = sb ? (int)nb : nbq;
+                int bsel = -1;
+                for (int q0 = 0; q0 < nbb; q0 += 16) {
+                    const int b = q0 + lr;
+                    const uint32_t msk = row_ballot((b < nbb) && (t < rr[b]), rrow);
+                    if (msk) {
+                        bsel = q0 + __builtin_ctz(msk);
+                        break;
                     }
                 }
+                uint32_t topic;
+                if (bsel < 0) {
+                    topic = sb ? (a.theta[start + nnz - 1] & tmask) : (uint32_t)(K - 1);
+                } else {
+                    const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
+                    uint32_t e = 0;
+                    float v;
+                    bool ok = true;
+                    if (sb) {
+                        e = a.theta[start + j];
+                        ok = j < nnz;
+                        v = ok ? (float)(e >> tb) * ph[e & tmask] : 0.0f;
+                    } else {
+                        v = alpha * ph[j];
+                    }
+                    v = scan16(v);
+                    const float C = bsel == 0 ? v : rr[bsel - 1] + v;
+                    const uint32_t msk = row_ballot(ok && (t < C), rrow);
+                    int msel;
+                    if (msk) {
+                        msel = __builtin_ctz(msk);
+                    } else {
+                        const int nvalid = (int)nnz - (bsel << 4);
+                        msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
+                    }
+                    const uint32_t et = __shfl(e & tmask, msel, kBlock);
+                    topic = sb ? et : (uint32_t)((bsel << 4) + msel);
+                }
+                if (lr == 0) {
+                    zres[p] = topic;
+                    sqres[p] = SQ;
+                }
             }
-            // hand token (r + row)'s result to lane r + row
-            const int from = ((lane - r) * kBlock) & 63;
-            const uint32_t tz = __shfl(topic, from, kWave);
-            const float tsq = __shfl(SQ, from, kWave);
-            if (lane >= r && lane < r + 4) {
-                zout = tz;
-                sqout = tsq;
-            }
-            wave_sync();
         }
-        if (valid) {
-            a.z[i] = (uint16_t)zout;
-            ll += log((double)sqout / ((double)mnd + a.k_alpha));
+        wave_sync();
+        if (lane < ntok) {
+            a.z[base + tok] = (uint16_t)zres[lane];
+            ll += log((double)sqres[lane] / ((double)pnd + a.k_alpha));
         }
+        wave_sync();
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
