@@ -151,7 +151,7 @@ void Context::upload(const Corpus &c) {
             meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
             cs.meta_start_h[d] = (uint32_t)cap;
             cap += slab;
-            if (cap >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "chunk %d theta capacity exceeds 2^32 entries", k);
+            if (cap + 64 >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "chunk %d theta capacity exceeds 2^32 entries", k);
         }
         cs.theta_cap = cap;
         cs.word_idx_h = ch.word_idx;
@@ -163,7 +163,10 @@ void Context::upload(const Corpus &c) {
         cs.rev_off = upload_vec(*this, rev_off);
         cs.rev_idx = upload_vec(*this, rev_idx);
         cs.meta = upload_vec(*this, meta);
-        cs.theta = (uint32_t *)dalloc(std::max<int64_t>(cap, 1) * sizeof(uint32_t));
+        // 64 entries of tail padding: the sampler loads whole 64-entry
+        // groups and may read past the last slab (those values are unused)
+        cs.theta = (uint32_t *)dalloc((cap + 64) * sizeof(uint32_t));
+        hip_check(hipMemset(cs.theta, 0, (cap + 64) * sizeof(uint32_t)), "theta clear");
         cs.slots = upload_vec(*this, sl);
         // phi work list of this chunk alone
         std::vector<PhiSlot> ps;
@@ -198,6 +201,7 @@ void Context::upload(const Corpus &c) {
     ll_partial = (double *)dalloc(std::max<int64_t>(total_slots, 1) * sizeof(double));
     hip_check(hipMemset(ll_partial, 0, std::max<int64_t>(total_slots, 1) * sizeof(double)), "hipMemset");
     ll_out = (double *)dalloc(sizeof(double));
+    ll_scratch = (double *)dalloc(kLLGroups * sizeof(double));
     stats = (unsigned long long *)dalloc(2 * sizeof(unsigned long long));
     hip_check(hipMemset(stats, 0, 2 * sizeof(unsigned long long)), "hipMemset");
     hip_check(hipDeviceSynchronize(), "upload");
@@ -293,7 +297,7 @@ void Context::sample(int32_t chunk, int32_t sweep) {
 
 double Context::loglik() {
     if (!sampled) fail(CULDA_ERR_STATE, "loglik before any sweep");
-    hip_check(launch_ll_reduce(ll_partial, ll_partials, ll_out, stream), "ll reduce launch");
+    hip_check(launch_ll_reduce(ll_partial, ll_partials, ll_scratch, ll_out, stream), "ll reduce launch");
     double h = 0;
     hip_check(hipMemcpyAsync(&h, ll_out, sizeof(double), hipMemcpyDeviceToHost, stream), "ll copy");
     hip_check(hipStreamSynchronize(stream), "ll sync");
@@ -334,7 +338,7 @@ void Context::iterate(int32_t sweep, double *ll, double *seconds) {
     if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
     for (int32_t k = 0; k < C; ++k) sample(k, sweep);
     if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
-    hip_check(launch_ll_reduce(ll_partial, ll_partials, ll_out, stream), "ll reduce launch");
+    hip_check(launch_ll_reduce(ll_partial, ll_partials, ll_scratch, ll_out, stream), "ll reduce launch");
     if (prm.fused_phi || C == 1) {
         rebuild_phi(-1, false);
     } else {

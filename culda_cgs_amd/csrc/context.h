@@ -59,6 +59,7 @@ struct Context {
     double *ll_partial = nullptr;
     int64_t ll_partials = 0;
     double *ll_out = nullptr;
+    double *ll_scratch = nullptr;
     unsigned long long *stats = nullptr;
     ChunkPtrs *chunk_ptrs = nullptr;
     PhiSlot *phi_slots_all = nullptr;

@@ -15,7 +15,8 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
                            float beta, hipStream_t s);
 hipError_t launch_theta_rebuild(const ThetaArgs &a, int32_t grid, hipStream_t s);
 hipError_t launch_sample(const SampleArgs &a, hipStream_t s);
-hipError_t launch_ll_reduce(const double *partial, int64_t n, double *out, hipStream_t s);
+constexpr int kLLGroups = 256;
+hipError_t launch_ll_reduce(const double *partial, int64_t n, double *scratch, double *out, hipStream_t s);
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
                                  double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,

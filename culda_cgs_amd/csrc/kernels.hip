@@ -9,8 +9,9 @@
 //   blocks of 16 values, inclusive Kogge-Stone scan inside a block
 //   (offsets 1, 2, 4, 8); S: block totals carried sequentially
 //   R_{b+1} = fl(R_b + T_b); Q: 64-lane Kogge-Stone scan of the block
-//   totals.  Search: first block whose inclusive prefix exceeds the
-//   target, then first element of that block whose
+//   totals.  t = fl(u * fl(S + Q)); S bucket iff t < S (target t), else
+//   Q bucket (target fl(t - S)).  Search: first block whose inclusive
+//   prefix exceeds the target, then first element of that block whose
 //   fl(prefix_before_block + in-block prefix) exceeds it; fallbacks take
 //   the last element.
 #include <hip/hip_runtime.h>
@@ -177,7 +178,18 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         const uint32_t b = a.rev_off[d], e = a.rev_off[d + 1];
-        for (uint32_t i = b + lane; i < e; i += kWave) atomicAdd(&h[a.z[a.rev_idx[i]]], 1u);
+        // four index loads, then four z gathers, in flight per lane
+        for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
+            uint32_t t[4];
+            uint16_t zz[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? a.rev_idx[i0 + q * kWave] : 0xffffffffu;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? a.z[t[q]] : (uint16_t)0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
+        }
         wave_sync();
         const uint32_t start = a.meta[d].start;
         uint32_t base = 0;
@@ -189,6 +201,8 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
             if (v) a.theta[start + pos] = (v << a.topic_bits) | (uint32_t)k;
             base += (uint32_t)__popcll(m);
         }
+        // zero the slab tail up to the next multiple of 16 (read by the sampler)
+        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) a.theta[start + base + lane] = 0u;
         if (lane == 0) a.meta[d].nnz = base;
         s_nnz += base;
         s_tok += (unsigned long long)base * (e - b);
@@ -307,35 +321,60 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
                 // S = sum over the doc's nonzeros, 16 per block; every block
                 // lies inside the doc's 16-aligned slab, entries past nnz
                 // are masked
-                // blocks are loaded four at a time, one group ahead, without
-                // branches (a block past nb re-reads block 0 of the slab);
-                // entries past nnz contribute exact zeros (R + 0 == R)
+                // Blocks are loaded four at a time (a group of 64 entries),
+                // one group ahead.  Loads past the last block stay inside the
+                // theta array (64 entries of tail padding) and those blocks
+                // are skipped; inside the last block the slab is zero past
+                // nnz, so its extra products are exact zeros (R + 0 == R).
                 const uint32_t nb = (nnz + 15u) >> 4;
+                const uint32_t ng = (nb + 3u) >> 2;
                 const uint32_t *tp = a.theta + start + lr;
                 uint32_t cur[4], nxt[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) cur[q] = tp[(uint32_t)q < nb ? q << 4 : 0];
+                for (int q = 0; q < 4; ++q) cur[q] = tp[q << 4];
                 float R = 0.0f;
-                for (uint32_t b0 = 0; b0 < nb; b0 += 4) {
+                uint32_t g = 0;
+                for (; g + 1 < ng; ++g) {
+                    const uint32_t *gp = tp + ((g + 1) << 6);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) nxt[q] = tp[b0 + 4 + q < nb ? (b0 + 4 + q) << 4 : 0u];
+                    for (int q = 0; q < 4; ++q) nxt[q] = gp[q << 4];
+                    float x[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = (float)(cur[q] >> tb) * ph[cur[q] & tmask];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x111>(x[q]);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x112>(x[q]);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x114>(x[q]);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x118>(x[q]);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const uint32_t e = cur[q];
-                        const float pv = (float)(e >> tb) * ph[e & tmask];
-                        const float pm = (((b0 + q) << 4) + (uint32_t)lr) < nnz ? pv : 0.0f;
-                        R = R + row_last(scan16(pm));
-                        if (lr == 0) rs[b0 + q + 1] = R;
+                        R = R + row_last(x[q]);
+                        if (lr == 0) rs[(g << 2) + q + 1] = R;
                     }
 #pragma unroll
                     for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
                 }
+                // last group: blocks 4g .. nb-1
+                const uint32_t b0 = g << 2;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (b0 + q < nb) {
+                        const float x = scan16((float)(cur[q] >> tb) * ph[cur[q] & tmask]);
+                        R = R + row_last(x);
+                        if (lr == 0) rs[b0 + q + 1] = R;
+                    }
+                }
                 wave_sync();
+                // t = fl(u * (S + Q)); the S bucket takes t < S with target
+                // t, the Q bucket target fl(t - S)
                 const float S = R;
                 const float SQ = S + Q;
-                const float frac = S / SQ;
-                const bool sb = uu < frac;
-                const float t = sb ? uu * SQ : (uu - frac) * SQ;
+                const float tu = uu * SQ;
+                const bool sb = tu < S;
+                const float t = sb ? tu : tu - S;
                 const float *rr = sb ? rs + 1 : rq;  // inclusive prefix after each block
                 const int nbb = sb ? (int)nb : nbq;
                 int bsel = -1;
@@ -393,20 +432,25 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
     if (lane == 0) a.ll_partial[slot] = ll;
 }
 
-// Fixed-order FP64 sum of the per-slot partials (one workgroup).
+// Fixed-order FP64 sum of the per-slot partials in two launches: stage 1
+// gives workgroup g the contiguous range g*per .. (g+1)*per, stage 2 sums
+// the stage-1 results.  The order depends only on n, so every run adds the
+// same values in the same order.
 // Reference: LDATrainPerplexityReduce1/2 (kernel/lda_train_kernel.cu:273-321),
 // which truncated partials to int.
-__global__ __launch_bounds__(256) void ll_reduce_kernel(const double *p, int64_t n, double *out) {
+__global__ __launch_bounds__(256) void ll_reduce_kernel(const double *p, int64_t n, int64_t per, double *out) {
     __shared__ double s[256];
+    const int64_t b = (int64_t)blockIdx.x * per;
+    const int64_t e = b + per < n ? b + per : n;
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) acc += p[i];
+    for (int64_t i = b + threadIdx.x; i < e; i += 256) acc += p[i];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 // Joint log-likelihood pieces: sum over nonzero n_wk of
@@ -510,8 +554,12 @@ hipError_t launch_sample(const SampleArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_ll_reduce(const double *partial, int64_t n, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(ll_reduce_kernel, dim3(1), dim3(256), 0, s, partial, n, out);
+hipError_t launch_ll_reduce(const double *partial, int64_t n, double *scratch, double *out, hipStream_t s) {
+    // scratch holds kLLGroups doubles
+    const int64_t per = (n + kLLGroups - 1) / kLLGroups > 0 ? (n + kLLGroups - 1) / kLLGroups : 1;
+    hipLaunchKernelGGL(ll_reduce_kernel, dim3(kLLGroups), dim3(256), 0, s, partial, n, per, scratch);
+    hipLaunchKernelGGL(ll_reduce_kernel, dim3(1), dim3(256), 0, s, (const double *)scratch, (int64_t)kLLGroups,
+                       (int64_t)kLLGroups, out);
     return hipGetLastError();
 }
 

@@ -19,7 +19,10 @@
  *   - FP32 head (n_wk + beta) / (n_k + beta V) (kernel/lda_phi_kernel.cu:26);
  *   - FP32 bucket arithmetic in a fixed order: blocks of 16, Kogge-Stone
  *     inclusive scan inside a block, S block totals carried sequentially,
- *     Q block totals scanned Kogge-Stone over 64 lanes; two-level search
+ *     Q block totals scanned Kogge-Stone over 64 lanes; t = u * (S + Q)
+ *     picks the S bucket when t < S (target t) and otherwise the Q bucket
+ *     (target t - S) -- the reference's u < S/(S+Q) test and
+ *     (u - S/(S+Q)) * (S+Q) target without the division; two-level search
  *     (first block, then first element) with last-element fallbacks;
  *   - FP64 log-likelihood term log((S+Q) / (n_d + K alpha))
  *     (kernel/lda_train_kernel.cu:258, accumulated in FP64 instead of
@@ -306,11 +309,11 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                         R = R + p[15];
                         rs[b + 1] = R;
                     }
-                    const float S = R, SQ = S + Q, frac = S / SQ;
+                    const float S = R, SQ = S + Q, tu = u * SQ;
                     int32_t topic;
-                    if (u < frac) {
+                    if (tu < S) {
                         ns++;
-                        const float t = u * SQ;
+                        const float t = tu;
                         int64_t bsel = -1;
                         for (int64_t b = 0; b < nb; ++b)
                             if (t < rs[b + 1]) {
@@ -338,7 +341,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                             topic = th_top[o + bsel * 16 + msel];
                         }
                     } else {
-                        const float t = (u - frac) * SQ;
+                        const float t = tu - S;
                         int bsel = -1;
                         for (int b = 0; b < nbq; ++b)
                             if (t < rq[b]) {
@@ -481,10 +484,10 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
                         R = R + p[15];
                         rs[b + 1] = R;
                     }
-                    const float S = R, SQ = S + Q, frac = S / SQ;
+                    const float S = R, SQ = S + Q, tu = u * SQ;
                     int32_t topic;
-                    if (u < frac) {
-                        const float t = u * SQ;
+                    if (tu < S) {
+                        const float t = tu;
                         int64_t bsel = -1;
                         for (int64_t b = 0; b < nb; ++b)
                             if (t < rs[b + 1]) {
@@ -512,7 +515,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
                             topic = th_top[o + bsel * 16 + msel];
                         }
                     } else {
-                        const float t = (u - frac) * SQ;
+                        const float t = tu - S;
                         int bsel = -1;
                         for (int b = 0; b < nbq; ++b)
                             if (t < rq[b]) {
