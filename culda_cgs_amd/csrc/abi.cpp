@@ -62,12 +62,12 @@ void download_theta(Context &c, int32_t chunk, std::vector<int64_t> &off, std::v
     for (int32_t d = 0; d < Dc; ++d) off[d + 1] = off[d] + meta[d].nnz;
     cnt.resize(off[Dc]);
     top.resize(off[Dc]);
-    const uint32_t tmask = (1u << c.topic_bits) - 1u;
+    const uint32_t cmask = (1u << c.count_bits) - 1u;
     for (int32_t d = 0; d < Dc; ++d)
         for (uint32_t j = 0; j < meta[d].nnz; ++j) {
             const uint32_t e = th[meta[d].start + j];
-            cnt[off[d] + j] = (int32_t)(e >> c.topic_bits);
-            top[off[d] + j] = (int16_t)(e & tmask);
+            cnt[off[d] + j] = (int32_t)(e & cmask);
+            top[off[d] + j] = (int16_t)(e >> (c.count_bits + 2));
         }
 }
 
@@ -206,7 +206,7 @@ extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
         hip_check(launch_joint_ll_words(c.nwk, V, K, std::lgamma(beta), c.prm.beta, part, G, c.stream), "joint");
         for (size_t k = 0; k < c.chunks.size(); ++k) {
             ChunkState &cs = c.chunks[k];
-            hip_check(launch_joint_ll_docs(cs.meta, cs.theta, cs.doc_end - cs.doc_start, c.topic_bits, c.prm.alpha,
+            hip_check(launch_joint_ll_docs(cs.meta, cs.theta, cs.doc_end - cs.doc_start, c.count_bits, c.prm.alpha,
                                            part + G * (1 + k), G, c.stream),
                       "joint");
         }

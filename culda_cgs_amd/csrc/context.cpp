@@ -87,8 +87,8 @@ void Context::upload(const Corpus &c) {
     num_tokens = c.num_tokens();
     if (num_tokens >= (int64_t)INT32_MAX) fail(CULDA_ERR_LIMIT, "corpus has %lld tokens; counts are int32", (long long)num_tokens);
     if ((int64_t)num_words * K >= ((int64_t)1 << 40)) fail(CULDA_ERR_LIMIT, "V*K too large");
-    topic_bits = ceil_log2((uint32_t)K);
-    const int64_t max_count = ((int64_t)1 << (32 - topic_bits)) - 1;
+    count_bits = 30 - ceil_log2((uint32_t)K);
+    const int64_t max_count = ((int64_t)1 << count_bits) - 1;
     if (c.max_doc_len() > max_count)
         fail(CULDA_ERR_LIMIT, "doc length %d exceeds %lld, the largest count a packed theta entry holds at K=%d",
              c.max_doc_len(), (long long)max_count, K);
@@ -262,7 +262,7 @@ void Context::rebuild_theta(int32_t chunk) {
     if (!uploaded) fail(CULDA_ERR_STATE, "rebuild_theta before upload");
     if (chunk < 0 || chunk >= (int32_t)chunks.size()) fail(CULDA_ERR_ARG, "chunk %d out of range", chunk);
     ChunkState &cs = chunks[chunk];
-    ThetaArgs a{cs.rev_off, cs.rev_idx, cs.z, cs.meta, cs.theta, stats, cs.doc_end - cs.doc_start, prm.k, topic_bits};
+    ThetaArgs a{cs.rev_off, cs.rev_idx, cs.z, cs.meta, cs.theta, stats, cs.doc_end - cs.doc_start, prm.k, count_bits};
     hip_check(launch_theta_rebuild(a, std::max(1, cu_count) * 8, stream), "theta rebuild launch");
     theta_ready = true;
 }
@@ -288,7 +288,7 @@ void Context::sample(int32_t chunk, int32_t sweep) {
     a.beta = prm.beta;
     a.num_slots = cs.num_slots;
     a.k = prm.k;
-    a.topic_bits = topic_bits;
+    a.count_bits = count_bits;
     a.sweep = (uint32_t)sweep;
     hip_check(launch_sample(a, stream), "sampler launch");
     sampled = true;

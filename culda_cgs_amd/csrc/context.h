@@ -48,7 +48,7 @@ struct Context {
     hipStream_t stream = nullptr;
     int32_t num_words = 0, num_docs = 0;
     int64_t num_tokens = 0;
-    uint32_t topic_bits = 0;
+    uint32_t count_bits = 0;  // packed theta entry: count field width
     bool uploaded = false, theta_ready = false, phi_ready = false, sampled = false;
     std::vector<int64_t> doc_len;  // global
     std::vector<ChunkState> chunks;

@@ -20,7 +20,7 @@ hipError_t launch_ll_reduce(const double *partial, int64_t n, double *scratch, d
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
                                  double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
-                                uint32_t topic_bits, float alpha, double *partial, int32_t grid, hipStream_t s);
+                                uint32_t count_bits, float alpha, double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_copy(const float4 *src, float4 *dst, int64_t n, hipStream_t s);
 
 }  // namespace culda

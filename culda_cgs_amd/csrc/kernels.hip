@@ -198,7 +198,7 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
             const uint32_t v = k < K ? h[k] : 0u;
             const unsigned long long m = __ballot(v != 0);
             const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (v) a.theta[start + pos] = (v << a.topic_bits) | (uint32_t)k;
+            if (v) a.theta[start + pos] = ((uint32_t)k << (a.count_bits + 2)) | v;
             base += (uint32_t)__popcll(m);
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
@@ -225,23 +225,23 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // written per token.  The processing order never changes a result: every
 // token depends only on the frozen counts and its own Philox draw.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
-__global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
-    __shared__ __attribute__((aligned(16))) float phi_s[kWavesPerGroup][kMaxK];
-    __shared__ float rq_s[kWavesPerGroup][kWave];
-    __shared__ float rs_s[kWavesPerGroup][4][kWave + 4];
-    __shared__ uint32_t zres_s[kWavesPerGroup][kWave];
-    __shared__ float sqres_s[kWavesPerGroup][kWave];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int slot = blockIdx.x * kWavesPerGroup + wave;
-    if (slot >= a.num_slots) return;  // whole wave leaves; no block barrier below
+__global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
+    // one wave per workgroup: every LDS array sits at a fixed address, so
+    // the phi lookup is a shift of the packed entry plus a constant offset
+    __shared__ __attribute__((aligned(16))) float ph[kMaxK];
+    __shared__ float rq[kWave];
+    __shared__ float rs_s[4][kWave + 4];
+    __shared__ uint32_t zres[kWave];
+    __shared__ float sqres[kWave];
+    const int lane = threadIdx.x;
+    const int slot = blockIdx.x;
     const SlotDesc s = a.slots[slot];
     const int K = a.k;
     const float alpha = a.alpha, beta = a.beta;
-    const uint32_t tb = a.topic_bits, tmask = (1u << tb) - 1u;
-    float *ph = phi_s[wave];
-    float *rq = rq_s[wave];
-    uint32_t *zres = zres_s[wave];
-    float *sqres = sqres_s[wave];
+    const uint32_t cb = a.count_bits, cmask = (1u << cb) - 1u;
+    // phi of a packed theta entry e (byte offset e >> cb) and its count
+#define PHI_OF(e) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + ((e) >> cb)))
+#define COUNT_OF(e) ((float)((e)&cmask))
 
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
     const int32_t *row = a.nwk + (int64_t)s.word * K;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
     wave_sync();
 
     const int rrow = lane >> 4, lr = lane & 15;
-    float *rs = rs_s[wave][rrow];
+    float *rs = rs_s[rrow];
     double ll = 0.0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
                     for (int q = 0; q < 4; ++q) nxt[q] = gp[q << 4];
                     float x[4];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = (float)(cur[q] >> tb) * ph[cur[q] & tmask];
+                    for (int q = 0; q < 4; ++q) x[q] = COUNT_OF(cur[q]) * PHI_OF(cur[q]);
 #pragma unroll
                     for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x111>(x[q]);
 #pragma unroll
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     if (b0 + q < nb) {
-                        const float x = scan16((float)(cur[q] >> tb) * ph[cur[q] & tmask]);
+                        const float x = scan16(COUNT_OF(cur[q]) * PHI_OF(cur[q]));
                         R = R + row_last(x);
                         if (lr == 0) rs[b0 + q + 1] = R;
                     }
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
                 }
                 uint32_t topic;
                 if (bsel < 0) {
-                    topic = sb ? (a.theta[start + nnz - 1] & tmask) : (uint32_t)(K - 1);
+                    topic = sb ? (a.theta[start + nnz - 1] >> (cb + 2)) : (uint32_t)(K - 1);
                 } else {
                     const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
                     uint32_t e = 0;
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
                     if (sb) {
                         e = a.theta[start + j];
                         ok = j < nnz;
-                        v = ok ? (float)(e >> tb) * ph[e & tmask] : 0.0f;
+                        v = ok ? COUNT_OF(e) * PHI_OF(e) : 0.0f;
                     } else {
                         v = alpha * ph[j];
                     }
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
                         const int nvalid = (int)nnz - (bsel << 4);
                         msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
                     }
-                    const uint32_t et = __shfl(e & tmask, msel, kBlock);
+                    const uint32_t et = __shfl(e >> (cb + 2), msel, kBlock);
                     topic = sb ? et : (uint32_t)((bsel << 4) + msel);
                 }
                 if (lr == 0) {
@@ -430,6 +430,8 @@ __global__ __launch_bounds__(256) void sample_kernel(SampleArgs a) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
     if (lane == 0) a.ll_partial[slot] = ll;
+#undef PHI_OF
+#undef COUNT_OF
 }
 
 // Fixed-order FP64 sum of the per-slot partials in two launches: stage 1
@@ -474,14 +476,14 @@ __global__ __launch_bounds__(256) void joint_words_kernel(const int32_t *nwk, in
 
 // sum over docs and nonzero n_dk of lgamma(n_dk + alpha) - lgamma(alpha)
 __global__ __launch_bounds__(256) void joint_docs_kernel(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
-                                                         uint32_t tb, float alpha, double *partial) {
+                                                         uint32_t cb, float alpha, double *partial) {
     __shared__ double s[256];
     const double lg_alpha = lgamma((double)alpha);
     double acc = 0.0;
     for (int d = blockIdx.x * 4 + (threadIdx.x >> 6); d < num_docs; d += gridDim.x * 4) {
         const DocMeta m = meta[d];
         for (uint32_t j = threadIdx.x & 63; j < m.nnz; j += 64)
-            acc += lgamma((double)(theta[m.start + j] >> tb) + (double)alpha) - lg_alpha;
+            acc += lgamma((double)(theta[m.start + j] & ((1u << cb) - 1u)) + (double)alpha) - lg_alpha;
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -550,7 +552,7 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, int32_t grid, hipStream_t s)
 
 hipError_t launch_sample(const SampleArgs &a, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sample_kernel, dim3(grid_for(a.num_slots, kWavesPerGroup, INT32_MAX)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(sample_kernel, dim3(a.num_slots), dim3(kWave * kSampleWaves), 0, s, a);
     return hipGetLastError();
 }
 
@@ -570,9 +572,9 @@ hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t topic_bits,
+hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
                                 float alpha, double *partial, int32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, topic_bits, alpha,
+    hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, count_bits, alpha,
                        partial);
     return hipGetLastError();
 }

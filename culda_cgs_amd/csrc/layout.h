@@ -9,8 +9,10 @@
 //   rev_idx[N_c]  u32  word-major position of each token, doc-major
 //                      (docRevIndices / docRevIdx, doc_chunk.cpp:225-255)
 //   meta[D_c]     DocMeta: theta slab start, nnz, n_d
-//   theta[cap_c]  u32  packed (count << topic_bits) | topic, ascending
-//                      topic within a doc; slab capacity of doc d is
+//   theta[cap_c]  u32  packed ((4 * topic) << count_bits) | count,
+//                      count_bits = 30 - ceil(log2 K) (20 at K=1024), so
+//                      e >> count_bits is the byte offset of phi_topic in
+//                      LDS; ascending topic within a doc; slab capacity of doc d is
 //                      roundup16(min(n_d, K)) (model_theta.cpp:49-56 uses
 //                      roundup32 for 32-lane warps)
 // Global: n_wk[V*K] i32 (row per word), n_k[K] i32, den[K] f32.
@@ -22,7 +24,8 @@ namespace culda {
 constexpr int kWave = 64;       // CDNA wavefront
 constexpr int kBlock = 16;      // FP32 prefix block = one DPP row
 constexpr int kMaxK = 1024;     // Q block totals scanned by one wave
-constexpr int kWavesPerGroup = 4;
+constexpr int kWavesPerGroup = 4;   // rebuild kernels
+constexpr int kSampleWaves = 1;     // sampler: one wave per workgroup
 
 // Sampler work unit: tokens [begin, end) of one word, one wavefront.
 // gdelta turns a chunk-local token index into the chunking-invariant
@@ -72,7 +75,7 @@ struct SampleArgs {
     float beta;
     int32_t num_slots;
     int32_t k;
-    uint32_t topic_bits;
+    uint32_t count_bits;
     uint32_t sweep;
 };
 
@@ -97,7 +100,7 @@ struct ThetaArgs {
     unsigned long long *stats;  // [0] += sum nnz, [1] += sum n_d * nnz
     int32_t num_docs;
     int32_t k;
-    uint32_t topic_bits;
+    uint32_t count_bits;
 };
 
 struct InitArgs {
