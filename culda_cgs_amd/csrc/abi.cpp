@@ -97,6 +97,7 @@ extern "C" int culda_create(const culda_params *params, culda_ctx **out) {
         auto *h = new culda_ctx;
         h->c.prm = p;
         e = hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->c.stream2, hipStreamNonBlocking);
         if (e != hipSuccess) {
             delete h;
             fail(CULDA_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));

@@ -26,9 +26,14 @@ void hip_check(hipError_t e, const char *what) {
 Context::~Context() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *p : allocations) (void)hipFree(p);
+    if (stream2) (void)hipStreamSynchronize(stream2);
     for (hipEvent_t &e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : ev_sampled)
+        if (e) (void)hipEventDestroy(e);
+    if (ev_theta_done) (void)hipEventDestroy(ev_theta_done);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
 }
 
 void *Context::dalloc(size_t bytes) {
@@ -204,6 +209,9 @@ void Context::upload(const Corpus &c) {
     ll_scratch = (double *)dalloc(kLLGroups * sizeof(double));
     stats = (unsigned long long *)dalloc(2 * sizeof(unsigned long long));
     hip_check(hipMemset(stats, 0, 2 * sizeof(unsigned long long)), "hipMemset");
+    ev_sampled.resize(C, nullptr);
+    for (hipEvent_t &e : ev_sampled) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    if (!ev_theta_done) hip_check(hipEventCreateWithFlags(&ev_theta_done, hipEventDisableTiming), "event");
     hip_check(hipDeviceSynchronize(), "upload");
     uploaded = true;
     theta_ready = phi_ready = sampled = false;
@@ -258,12 +266,12 @@ void Context::phi_head() {
     phi_ready = true;
 }
 
-void Context::rebuild_theta(int32_t chunk) {
+void Context::rebuild_theta(int32_t chunk, hipStream_t s) {
     if (!uploaded) fail(CULDA_ERR_STATE, "rebuild_theta before upload");
     if (chunk < 0 || chunk >= (int32_t)chunks.size()) fail(CULDA_ERR_ARG, "chunk %d out of range", chunk);
     ChunkState &cs = chunks[chunk];
     ThetaArgs a{cs.rev_off, cs.rev_idx, cs.z, cs.meta, cs.theta, stats, cs.doc_end - cs.doc_start, prm.k, count_bits};
-    hip_check(launch_theta_rebuild(a, std::max(1, cu_count) * 8, stream), "theta rebuild launch");
+    hip_check(launch_theta_rebuild(a, std::max(1, cu_count) * 8, s ? s : stream), "theta rebuild launch");
     theta_ready = true;
 }
 
@@ -335,8 +343,19 @@ void Context::iterate(int32_t sweep, double *ll, double *seconds) {
     int64_t slots_total = 0;
     for (auto &cs : chunks) slots_total += cs.num_slots;
     auto t0 = std::chrono::steady_clock::now();
+    // stream: sweeps of all chunks, LL, phi + head denominators.
+    // stream2: theta of chunk c as soon as chunk c's sweep is done, so the
+    // (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta
+    // of chunk c is read only by chunk c's sweep, which has finished.
     if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
-    for (int32_t k = 0; k < C; ++k) sample(k, sweep);
+    hip_check(hipStreamWaitEvent(stream2, ev_theta_done, 0), "wait");  // previous stats read
+    hip_check(launch_fill_u32((uint32_t *)stats, 4, 0, stream2), "stats clear");
+    for (int32_t k = 0; k < C; ++k) {
+        sample(k, sweep);
+        hip_check(hipEventRecord(ev_sampled[k], stream), "event");
+        hip_check(hipStreamWaitEvent(stream2, ev_sampled[k], 0), "wait");
+        rebuild_theta(k, stream2);
+    }
     if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
     hip_check(launch_ll_reduce(ll_partial, ll_partials, ll_scratch, ll_out, stream), "ll reduce launch");
     if (prm.fused_phi || C == 1) {
@@ -346,8 +365,8 @@ void Context::iterate(int32_t sweep, double *ll, double *seconds) {
     }
     phi_head();
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
-    hip_check(launch_fill_u32((uint32_t *)stats, 4, 0, stream), "stats clear");
-    for (int32_t k = 0; k < C; ++k) rebuild_theta(k);
+    hip_check(hipEventRecord(ev_theta_done, stream2), "event");
+    hip_check(hipStreamWaitEvent(stream, ev_theta_done, 0), "wait");
     if (prof) hip_check(hipEventRecord(ev[3], stream), "event");
     double h_ll = 0;
     unsigned long long h_st[2] = {0, 0};

@@ -45,7 +45,10 @@ struct ChunkState {
 
 struct Context {
     culda_params prm{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // sampler, LL, phi
+    hipStream_t stream2 = nullptr;  // theta rebuilds, overlapped with the next chunk's sweep
+    std::vector<hipEvent_t> ev_sampled;  // per chunk: its sweep finished
+    hipEvent_t ev_theta_done = nullptr;
     int32_t num_words = 0, num_docs = 0;
     int64_t num_tokens = 0;
     uint32_t count_bits = 0;  // packed theta entry: count field width
@@ -83,7 +86,7 @@ struct Context {
     void init_z();
     void rebuild_phi(int32_t chunk, bool accumulate);
     void phi_head();
-    void rebuild_theta(int32_t chunk);
+    void rebuild_theta(int32_t chunk, hipStream_t s = nullptr);
     void sample(int32_t chunk, int32_t sweep);
     double loglik();
     void iterate(int32_t sweep, double *ll, double *seconds);

@@ -257,9 +257,10 @@ typedef struct culda_stats {
     int64_t sum_doc_nnz;        /* sum_d nnz(d) after the last theta rebuild    */
     int64_t sum_token_nnz;      /* sum_d n_d * nnz(d): theta entries gathered   */
     int64_t theta_capacity;     /* theta slab entries over all chunks           */
-    double  sampler_ms;         /* last iteration, summed over chunks (profile) */
-    double  phi_ms;
-    double  theta_ms;
+    double  sampler_ms;         /* last iteration: the chunks' sweeps (profile=1) */
+    double  phi_ms;             /* LL reduce + phi rebuild + head denominators    */
+    double  theta_ms;           /* wait for the theta rebuilds, which run on a
+                                   second stream overlapped with the sweeps     */
     double  other_ms;
     int32_t sampler_launches;   /* launches in the last iteration               */
     int32_t pad;
