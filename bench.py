@@ -36,6 +36,14 @@ CONFIGS = {
                doc_prior=0.1, word_prior=0.05, zipf=1.07, chunks=1, gen_seed=1,
                desc="plain CGS K=1024, V=10k Zipf(1.07), D=10k Poisson(100) docs, K_true=100, 1 chunk"),
 }
+# Batched workload: 64 independent corpora, each K=1024, Poisson doc lengths
+# with mean ~U[30, 400], V ~U[5k, 50k], Zipf(1.0); D ~U[1000, 5000] so the
+# total is ~40M tokens (D ~U[1000, 50000] with those lengths would be ~350M).
+C3 = dict(num=64, D=(1000, 5000), V=(5000, 50000), mean=(30.0, 400.0), KT=100, doc_prior=0.1,
+          word_prior=0.01, zipf=1.0, gen_seed=3,
+          desc="batched plain CGS: 64 independent LDA corpora, K=1024, D~U[1000,5000], V~U[5k,50k] Zipf(1.0), "
+               "Poisson doc lengths with mean~U[30,400], K_true=100; one persistent sampler launch per "
+               "iteration, graph-replayed, slot size and grid autotuned")
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -48,7 +56,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
     ap.add_argument("--k", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--slot-tokens", type=int, default=0)
@@ -63,6 +72,9 @@ def main():
 
     import culda_cgs_amd as cg
 
+    if args.config == "c3":
+        run_batch(cg, args)
+        return
     cfg = CONFIGS[args.config]
     K = args.k
     alpha, beta = 50.0 / K, 0.01
@@ -152,6 +164,111 @@ def main():
     }
     s.close()
     print(json.dumps(line), flush=True)
+
+
+def run_batch(cg, args):
+    """64 corpora sampled together: one graph-replayed iteration per step."""
+    K = args.k
+    alpha, beta = 50.0 / K, 0.01
+    rng = np.random.default_rng(C3["gen_seed"])
+    t0 = time.time()
+    corpora = []
+    for i in range(C3["num"]):
+        V = int(rng.integers(C3["V"][0], C3["V"][1] + 1))
+        D = int(rng.integers(C3["D"][0], C3["D"][1] + 1))
+        mean = float(rng.uniform(*C3["mean"]))
+        corpora.append(cg.Corpus.generate(V, D, C3["KT"], 1, len_kind=0, len_mean=mean,
+                                          doc_topic_prior=C3["doc_prior"], topic_word_prior=C3["word_prior"],
+                                          zipf_s=C3["zipf"], seed=1000 + i,
+                                          num_threads=min(16, os.cpu_count() or 1)))
+    N = sum(c.info().num_tokens for c in corpora)
+    log(f"c3: {len(corpora)} corpora, N={N} ({time.time() - t0:.1f}s)")
+    b = cg.Batch(k=K, profile=True, slot_tokens=args.slot_tokens)
+    for i, c in enumerate(corpora):
+        b.add(c, alpha, beta, args.seed + i)
+    t0 = time.time()
+    b.upload()
+    b.init()
+    log(f"upload + init {time.time() - t0:.1f}s")
+    tune = None
+    if not args.no_tune:
+        tune = b.autotune()
+        log(f"autotune: slot_tokens={tune.slot_tokens} groups={tune.persistent_groups} "
+            f"({tune.best_ms:.2f} ms/sweep)")
+    it = 1
+    if args.warmup:
+        b.iterate(it, count=args.warmup, use_graph=True)
+        it += args.warmup
+    t_start = time.perf_counter()
+    lls, _ = b.iterate(it, count=args.steps, use_graph=True)
+    elapsed = time.perf_counter() - t_start
+    it += args.steps
+    tokens_per_s = N * args.steps / elapsed
+    # the sampler launch alone, timed with events on its stream in one
+    # eager iteration after the timed region
+    b.iterate(it, count=1, use_graph=False)
+    it += 1
+    st = b.stats()
+    launch_ms = st.sampler_ms
+    bytes_per_launch = b.sampler_bytes()
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    log(f"{tokens_per_s / 1e6:.1f} M tokens/s, {elapsed / args.steps * 1e3:.2f} ms/iteration, sampler "
+        f"{launch_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(b, corpora, K, alpha, beta, args.seed, it, args.cpu_threads)
+    line = {
+        "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
+        "value": tokens_per_s,
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded LDA generative model)",
+        "config": {"workload": C3["desc"], "k": K, "alpha": alpha, "beta": beta, "num_tokens": N,
+                   "num_corpora": len(corpora), "num_slots": st.num_slots,
+                   "slot_tokens": tune.slot_tokens if tune else args.slot_tokens,
+                   "persistent_groups": tune.persistent_groups if tune else 0,
+                   "autotune_tried_ms": tune.tried() if tune else None,
+                   "iterations_timed": [it - 1 - args.steps, it - 2],
+                   "ll_per_token_last_mean": float(np.mean(lls[-1]))},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "sample_kernel", "avg_launch_ms": launch_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_batch(b, corpora, K, alpha, beta, seed, next_it, threads):
+    """The C oracle timed on the first 4 corpora, one iteration each, from
+    the GPU's current state."""
+    import oracle
+
+    threads = threads or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    n, dt = 0, 0.0
+    for m in range(4):
+        c = corpora[m]
+        info = c.info()
+        _, end, wi, wt = c.chunk(0)
+        om = oracle.OracleModel(info.num_words, K, c.doc_cum(), [end], [wi], [wt], z=[b.get_z(m, 0)])
+        om.rebuild()
+        t0 = time.perf_counter()
+        om.sweep_fast(alpha, beta, seed + m, next_it)
+        om.rebuild()
+        dt += time.perf_counter() - t0
+        n += info.num_tokens
+    log(f"cpu oracle: {n} tokens in {dt:.1f}s on {threads} threads")
+    return {"value": n / dt, "unit": "tokens/s", "cores": threads, "kind": "port",
+            "sample": f"one full iteration of the first 4 corpora ({n} tokens) from the GPU's state"}
 
 
 def cpu_baseline(s, corpus, info, K, alpha, beta, seed, next_it, threads):

@@ -62,6 +62,15 @@ class Stats(C.Structure):
                 ("sampler_launches", C.c_int32), ("pad", C.c_int32)]
 
 
+class Tune(C.Structure):
+    _fields_ = [("slot_tokens", C.c_int32), ("persistent_groups", C.c_int32), ("num_tried", C.c_int32),
+                ("pad", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 16),
+                ("tried_ms", C.c_double * 16)]
+
+    def tried(self):
+        return [(self.tried_slot_tokens[i], self.tried_ms[i]) for i in range(self.num_tried)]
+
+
 # name -> (restype, argtypes); every int-returning call is checked.
 _P = C.c_void_p
 _SIGS = {
@@ -105,6 +114,21 @@ _SIGS = {
     "culda_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
+    "culda_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
+    "culda_batch_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
+    "culda_batch_destroy": (None, [_P]),
+    "culda_batch_add": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_uint64, C.POINTER(C.c_int32)]),
+    "culda_batch_upload": (C.c_int, [_P]),
+    "culda_batch_init": (C.c_int, [_P]),
+    "culda_batch_set_persistent": (C.c_int, [_P, C.c_int32]),
+    "culda_batch_iterate": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]),
+    "culda_batch_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
+    "culda_batch_get_z": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_uint16), C.c_int64]),
+    "culda_batch_get_phi": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "culda_batch_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "culda_batch_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
+    "culda_batch_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
 }
 
 
@@ -346,5 +370,101 @@ class Sampler:
         _check(lib().culda_stream_copy(self._h, n_bytes, repeats, C.byref(v)))
         return v.value
 
+    def autotune(self):
+        """Pick the sampler's tokens per slot by timing one sweep per size."""
+        t = Tune()
+        _check(lib().culda_autotune(self._h, C.byref(t)))
+        return t
 
-__all__ = ["Corpus", "Sampler", "CuldaError", "format_text", "lib", "LIB_PATH", "CLI_PATH"]
+
+class Batch:
+    """Many independent corpora sampled together, one launch per step.
+
+    Each added corpus is its own model with its own alpha, beta and seed;
+    its chain is bit-identical to that corpus run alone in a ``Sampler``.
+    """
+
+    def __init__(self, k=1024, device=0, slot_tokens=0, profile=False):
+        self.params = Params(k, 0.0, 0.0, 0, device, slot_tokens, 1 if profile else 0, 1)
+        h = C.c_void_p()
+        _check(lib().culda_batch_create(C.byref(self.params), C.byref(h)))
+        self._h = h
+        self.k = k
+        self.corpora = []
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().culda_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, corpus: Corpus, alpha, beta, seed):
+        m = C.c_int32()
+        _check(lib().culda_batch_add(self._h, corpus._h, alpha, beta, seed, C.byref(m)))
+        self.corpora.append(corpus)
+        return m.value
+
+    def upload(self):
+        _check(lib().culda_batch_upload(self._h))
+
+    def init(self):
+        _check(lib().culda_batch_init(self._h))
+
+    def set_persistent(self, groups):
+        _check(lib().culda_batch_set_persistent(self._h, groups))
+
+    def iterate(self, first_iteration, count=1, use_graph=False):
+        """Run ``count`` iterations; returns (ll[count, M], seconds)."""
+        M = len(self.corpora)
+        ll = np.empty(count * M, np.float64)
+        sec = C.c_double()
+        _check(lib().culda_batch_iterate(self._h, first_iteration, count, 1 if use_graph else 0,
+                                         _ptr(ll, C.c_double), C.byref(sec)))
+        return ll.reshape(count, M), sec.value
+
+    def autotune(self):
+        t = Tune()
+        _check(lib().culda_batch_autotune(self._h, C.byref(t)))
+        return t
+
+    def info(self):
+        m, n, s = C.c_int32(), C.c_int64(), C.c_int64()
+        _check(lib().culda_batch_info(self._h, C.byref(m), C.byref(n), C.byref(s)))
+        return m.value, n.value, s.value
+
+    def stats(self):
+        s = Stats()
+        _check(lib().culda_batch_get_stats(self._h, C.byref(s)))
+        return s
+
+    def sampler_bytes(self):
+        v = C.c_double()
+        _check(lib().culda_batch_sampler_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def get_z(self, model, chunk):
+        n = self.corpora[model].chunk(chunk)[3].size
+        z = np.empty(n, np.uint16)
+        _check(lib().culda_batch_get_z(self._h, model, chunk, _ptr(z, C.c_uint16), z.size))
+        return z
+
+    def get_phi(self, model):
+        V = self.corpora[model].info().num_words
+        nwk = np.empty(V * self.k, np.int32)
+        nk = np.empty(self.k, np.int32)
+        _check(lib().culda_batch_get_phi(self._h, model, _ptr(nwk, C.c_int32), _ptr(nk, C.c_int32)))
+        return nwk.reshape(V, self.k), nk
+
+
+__all__ = ["Corpus", "Sampler", "Batch", "Tune", "CuldaError", "format_text", "lib", "LIB_PATH", "CLI_PATH"]

@@ -1,73 +1,93 @@
-// abi.cpp -- extern "C" entry points over culda::Context, plus the host
-// side of the ModelPhi / ModelTheta save hooks and checkpoints.
+// abi.cpp -- extern "C" entry points over culda::Engine: the single-corpus
+// context (culda_ctx), the multi-corpus batch (culda_batch), the host side
+// of the ModelPhi / ModelTheta save hooks and checkpoints.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
-#include "context.h"
+#include "engine.h"
 #include "kernels.h"
 
 using namespace culda;
 
 struct culda_ctx {
-    Context c;
+    Engine e;
+    float alpha = 0, beta = 0;
+    uint64_t seed = 0;
+    uint32_t last_sweep = 0;
+};
+
+struct culda_batch {
+    Engine e;
 };
 
 const culda::Corpus &corpus_ref(const culda_corpus *p);
 
 namespace {
 
-Context &ctx_of(culda_ctx *p) {
+Engine &eng(culda_ctx *p) {
     if (!p) fail(CULDA_ERR_ARG, "null context");
-    return p->c;
+    hip_check(hipSetDevice(p->e.prm.device), "hipSetDevice");
+    return p->e;
 }
 
-void check_chunk(Context &c, int32_t chunk) {
-    if (!c.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
-    if (chunk < 0 || chunk >= (int32_t)c.chunks.size()) fail(CULDA_ERR_ARG, "chunk %d out of range", chunk);
+Engine &eng(culda_batch *p) {
+    if (!p) fail(CULDA_ERR_ARG, "null batch");
+    hip_check(hipSetDevice(p->e.prm.device), "hipSetDevice");
+    return p->e;
 }
 
-// Host mirror of the phi counts (ModelPhi::hostPhiTopicWordShort / hostPhiTopic,
-// model/model_phi.h:32-33).
+void check_unit(Engine &e, int32_t unit) {
+    if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+    if (unit < 0 || unit >= (int32_t)e.units.size()) fail(CULDA_ERR_ARG, "chunk %d out of range", unit);
+}
+
+void check_model(Engine &e, int32_t m) {
+    if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+    if (m < 0 || m >= (int32_t)e.models.size()) fail(CULDA_ERR_ARG, "model %d out of range", m);
+}
+
+// Host mirror of one model's counts (ModelPhi::hostPhiTopicWordShort /
+// hostPhiTopic, model/model_phi.h:32-33).
 struct HostPhi {
     std::vector<int32_t> nwk, nk;
 };
 
-HostPhi download_phi(Context &c) {
-    if (!c.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+HostPhi download_phi(Engine &e, int32_t m) {
+    check_model(e, m);
+    const ModelHost &mh = e.models[m];
     HostPhi h;
-    h.nwk.resize((size_t)c.num_words * c.prm.k);
-    h.nk.resize(c.prm.k);
-    hip_check(hipStreamSynchronize(c.stream), "sync");
-    if (!h.nwk.empty())
-        hip_check(hipMemcpy(h.nwk.data(), c.nwk, h.nwk.size() * 4, hipMemcpyDeviceToHost), "phi copy");
-    hip_check(hipMemcpy(h.nk.data(), c.nk, h.nk.size() * 4, hipMemcpyDeviceToHost), "n_k copy");
+    h.nwk.resize((size_t)mh.V * mh.K);
+    h.nk.resize(mh.K);
+    e.sync();
+    if (!h.nwk.empty()) hip_check(hipMemcpy(h.nwk.data(), mh.nwk, h.nwk.size() * 4, hipMemcpyDeviceToHost), "phi copy");
+    hip_check(hipMemcpy(h.nk.data(), mh.nk, h.nk.size() * 4, hipMemcpyDeviceToHost), "n_k copy");
     return h;
 }
 
-// Compact CSR of one chunk's theta rows.
-void download_theta(Context &c, int32_t chunk, std::vector<int64_t> &off, std::vector<int32_t> &cnt,
+// Compact CSR of one unit's theta rows.
+void download_theta(Engine &e, int32_t unit, std::vector<int64_t> &off, std::vector<int32_t> &cnt,
                     std::vector<int16_t> &top) {
-    check_chunk(c, chunk);
-    ChunkState &cs = c.chunks[chunk];
-    const int32_t Dc = cs.doc_end - cs.doc_start;
+    check_unit(e, unit);
+    UnitHost &u = e.units[unit];
+    const uint32_t cb = e.models[u.model].count_bits;
+    const int32_t Dc = u.doc_end - u.doc_start;
     std::vector<DocMeta> meta(Dc);
-    std::vector<uint32_t> th(cs.theta_cap);
-    hip_check(hipStreamSynchronize(c.stream), "sync");
-    if (Dc) hip_check(hipMemcpy(meta.data(), cs.meta, Dc * sizeof(DocMeta), hipMemcpyDeviceToHost), "meta copy");
-    if (cs.theta_cap)
-        hip_check(hipMemcpy(th.data(), cs.theta, cs.theta_cap * 4, hipMemcpyDeviceToHost), "theta copy");
+    std::vector<uint32_t> th(u.theta_cap);
+    e.sync();
+    if (Dc) hip_check(hipMemcpy(meta.data(), u.meta, Dc * sizeof(DocMeta), hipMemcpyDeviceToHost), "meta copy");
+    if (u.theta_cap) hip_check(hipMemcpy(th.data(), u.theta, u.theta_cap * 4, hipMemcpyDeviceToHost), "theta copy");
     off.assign(Dc + 1, 0);
     for (int32_t d = 0; d < Dc; ++d) off[d + 1] = off[d] + meta[d].nnz;
     cnt.resize(off[Dc]);
     top.resize(off[Dc]);
-    const uint32_t cmask = (1u << c.count_bits) - 1u;
+    const uint32_t cmask = (1u << cb) - 1u;
     for (int32_t d = 0; d < Dc; ++d)
         for (uint32_t j = 0; j < meta[d].nnz; ++j) {
-            const uint32_t e = th[meta[d].start + j];
-            cnt[off[d] + j] = (int32_t)(e & cmask);
-            top[off[d] + j] = (int16_t)(e >> (c.count_bits + 2));
+            const uint32_t v = th[meta[d].start + j];
+            cnt[off[d] + j] = (int32_t)(v & cmask);
+            top[off[d] + j] = (int16_t)(v >> (cb + 2));
         }
 }
 
@@ -77,34 +97,67 @@ FILE *open_or_fail(const std::string &path, const char *mode) {
     return f;
 }
 
+void copy_z(Engine &e, int32_t unit, uint16_t *z, int64_t n, bool to_device) {
+    check_unit(e, unit);
+    UnitHost &u = e.units[unit];
+    if (n != u.num_tokens)
+        fail(CULDA_ERR_ARG, "chunk %d has %lld tokens, got %lld", unit, (long long)u.num_tokens, (long long)n);
+    if (n && !z) fail(CULDA_ERR_ARG, "null z");
+    e.sync();
+    if (!n) return;
+    if (to_device) {
+        const uint32_t K = (uint32_t)e.prm.k;
+        for (int64_t i = 0; i < n; ++i)
+            if (z[i] >= K) fail(CULDA_ERR_ARG, "z[%lld] = %u is not below k", (long long)i, z[i]);
+        hip_check(hipMemcpy(u.z, z, n * 2, hipMemcpyHostToDevice), "z copy");
+        e.theta_ready = e.phi_ready = false;
+    } else {
+        hip_check(hipMemcpy(z, u.z, n * 2, hipMemcpyDeviceToHost), "z copy");
+    }
+}
+
+void fill_tune(const TuneResult &r, culda_tune *out) {
+    std::memset(out, 0, sizeof *out);
+    out->slot_tokens = r.slot_tokens;
+    out->best_ms = r.best_ms;
+    out->num_tried = (int32_t)std::min<size_t>(r.tried.size(), 16);
+    for (int32_t i = 0; i < out->num_tried; ++i) {
+        out->tried_slot_tokens[i] = r.tried[i].first;
+        out->tried_ms[i] = r.tried[i].second;
+    }
+}
+
+const std::vector<int32_t> kSlotCandidates = {256, 512, 1024, 2048, 4096};
+
 }  // namespace
+
+#define CTX_CALL(...)         \
+    return guarded([&] {      \
+        Engine &e = eng(ctx); \
+        (void)e;              \
+        __VA_ARGS__;          \
+        return CULDA_OK;      \
+    })
+
+// ====================================================== single corpus
 
 extern "C" int culda_create(const culda_params *params, culda_ctx **out) {
     return guarded([&] {
         if (!params || !out) fail(CULDA_ERR_ARG, "null argument");
         *out = nullptr;
         const culda_params &p = *params;
-        if (p.k < 16 || p.k > kMaxK || p.k % 16)
-            fail(CULDA_ERR_ARG, "k = %d: the sampler needs a multiple of 16 in [16, %d]", p.k, kMaxK);
         if (!(p.alpha > 0.0f) || !std::isfinite(p.alpha)) fail(CULDA_ERR_ARG, "alpha must be positive and finite");
         if (!(p.beta > 0.0f) || !std::isfinite(p.beta)) fail(CULDA_ERR_ARG, "beta must be positive and finite");
-        if (p.slot_tokens < 0) fail(CULDA_ERR_ARG, "slot_tokens must be >= 0");
-        int n = 0;
-        hipError_t e = hipGetDeviceCount(&n);
-        if (e != hipSuccess || n == 0) fail(CULDA_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
-        if (p.device < 0 || p.device >= n) fail(CULDA_ERR_ARG, "device %d out of range (%d devices)", p.device, n);
-        hip_check(hipSetDevice(p.device), "hipSetDevice");
         auto *h = new culda_ctx;
-        h->c.prm = p;
-        e = hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking);
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->c.stream2, hipStreamNonBlocking);
-        if (e != hipSuccess) {
+        try {
+            h->e.init_device(p);
+        } catch (...) {
             delete h;
-            fail(CULDA_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+            throw;
         }
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device) != hipSuccess) cus = 0;
-        h->c.cu_count = cus > 0 ? cus : 256;
+        h->alpha = p.alpha;
+        h->beta = p.beta;
+        h->seed = p.seed;
         *out = h;
         return CULDA_OK;
     });
@@ -112,126 +165,133 @@ extern "C" int culda_create(const culda_params *params, culda_ctx **out) {
 
 extern "C" void culda_destroy(culda_ctx *ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->c.prm.device);
+    (void)hipSetDevice(ctx->e.prm.device);
     delete ctx;
 }
-
-#define CTX_CALL(...)                                 \
-    return guarded([&] {                              \
-        Context &c = ctx_of(ctx);                     \
-        hip_check(hipSetDevice(c.prm.device), "dev"); \
-        __VA_ARGS__;                                  \
-        return CULDA_OK;                              \
-    })
 
 extern "C" int culda_upload(culda_ctx *ctx, const culda_corpus *corpus) {
     CTX_CALL({
         if (!corpus) fail(CULDA_ERR_ARG, "null corpus");
-        if (c.uploaded) fail(CULDA_ERR_STATE, "a corpus is already uploaded to this context");
-        c.upload(corpus_ref(corpus));
+        if (e.uploaded) fail(CULDA_ERR_STATE, "a corpus is already uploaded to this context");
+        e.add_corpus(corpus_ref(corpus), ctx->alpha, ctx->beta, ctx->seed);
+        e.upload();
     });
 }
 
-extern "C" int culda_init_z(culda_ctx *ctx) { CTX_CALL(c.init_z()); }
+extern "C" int culda_init_z(culda_ctx *ctx) { CTX_CALL(e.init_z()); }
 
 extern "C" int culda_set_z(culda_ctx *ctx, int32_t chunk, const uint16_t *z, int64_t n) {
-    CTX_CALL({
-        check_chunk(c, chunk);
-        ChunkState &cs = c.chunks[chunk];
-        if (n != cs.num_tokens) fail(CULDA_ERR_ARG, "chunk %d has %lld tokens, got %lld", chunk,
-                                     (long long)cs.num_tokens, (long long)n);
-        if (n && !z) fail(CULDA_ERR_ARG, "null z");
-        for (int64_t i = 0; i < n; ++i)
-            if (z[i] >= (uint32_t)c.prm.k) fail(CULDA_ERR_ARG, "z[%lld] = %u is not below k", (long long)i, z[i]);
-        hip_check(hipStreamSynchronize(c.stream), "sync");
-        if (n) hip_check(hipMemcpy(cs.z, z, n * 2, hipMemcpyHostToDevice), "z copy");
-        c.theta_ready = c.phi_ready = false;
-    });
+    CTX_CALL(copy_z(e, chunk, const_cast<uint16_t *>(z), n, true));
 }
 
 extern "C" int culda_get_z(culda_ctx *ctx, int32_t chunk, uint16_t *z, int64_t n) {
-    CTX_CALL({
-        check_chunk(c, chunk);
-        ChunkState &cs = c.chunks[chunk];
-        if (n != cs.num_tokens) fail(CULDA_ERR_ARG, "chunk %d has %lld tokens, got %lld", chunk,
-                                     (long long)cs.num_tokens, (long long)n);
-        if (n && !z) fail(CULDA_ERR_ARG, "null z");
-        hip_check(hipStreamSynchronize(c.stream), "sync");
-        if (n) hip_check(hipMemcpy(z, cs.z, n * 2, hipMemcpyDeviceToHost), "z copy");
-    });
+    CTX_CALL(copy_z(e, chunk, z, n, false));
 }
 
 extern "C" int culda_rebuild_phi(culda_ctx *ctx, int32_t chunk, int32_t accumulate) {
-    CTX_CALL(c.rebuild_phi(chunk, accumulate != 0));
+    CTX_CALL({
+        if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+        if (chunk < -1 || chunk >= (int32_t)e.units.size()) fail(CULDA_ERR_ARG, "chunk %d out of range", chunk);
+        hip_check(hipStreamSynchronize(e.stream2), "sync");
+        e.rebuild_phi_model(0, chunk, accumulate != 0, e.stream);
+    });
 }
 
-extern "C" int culda_phi_head(culda_ctx *ctx) { CTX_CALL(c.phi_head()); }
+extern "C" int culda_phi_head(culda_ctx *ctx) {
+    CTX_CALL({
+        if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+        e.phi_head(e.stream);
+    });
+}
 
 extern "C" int culda_rebuild_theta(culda_ctx *ctx, int32_t chunk) {
     CTX_CALL({
-        check_chunk(c, chunk);
-        c.rebuild_theta(chunk);
+        check_unit(e, chunk);
+        e.rebuild_theta_unit(chunk, e.stream);
+        e.theta_ready = true;
     });
 }
 
 extern "C" int culda_sample(culda_ctx *ctx, int32_t chunk, int32_t iteration) {
-    CTX_CALL(c.sample(chunk, iteration));
+    CTX_CALL({
+        check_unit(e, chunk);
+        if (!e.theta_ready || !e.phi_ready) fail(CULDA_ERR_STATE, "sample before the phi/theta rebuild");
+        if (iteration < 1) fail(CULDA_ERR_ARG, "sweep index must be >= 1 (0 keys the z initialisation)");
+        hip_check(hipStreamSynchronize(e.stream2), "sync");
+        e.set_sweep((uint32_t)iteration, e.stream);
+        e.sample_unit(chunk, e.stream);
+        e.sampled = true;
+        ctx->last_sweep = (uint32_t)iteration;
+    });
 }
 
 extern "C" int culda_loglik(culda_ctx *ctx, double *ll_per_token) {
     CTX_CALL({
         if (!ll_per_token) fail(CULDA_ERR_ARG, "null output");
-        *ll_per_token = c.loglik();
+        if (!e.sampled) fail(CULDA_ERR_STATE, "loglik before any sweep");
+        e.set_sweep(ctx->last_sweep, e.stream);
+        e.ll_reduce(e.stream);
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+        *ll_per_token = e.model_ll(0, ctx->last_sweep);
     });
 }
 
 extern "C" int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds) {
     CTX_CALL({
         if (iteration < 1) fail(CULDA_ERR_ARG, "iteration must be >= 1");
-        c.iterate(iteration, ll, seconds);
+        e.iterate_units((uint32_t)iteration, ll, seconds);
+        ctx->last_sweep = (uint32_t)iteration;
     });
 }
 
-extern "C" int culda_rebuild_all(culda_ctx *ctx) { CTX_CALL(c.rebuild_all()); }
+extern "C" int culda_rebuild_all(culda_ctx *ctx) { CTX_CALL(e.rebuild_all()); }
 
-extern "C" int culda_sync(culda_ctx *ctx) { CTX_CALL(c.sync()); }
+extern "C" int culda_sync(culda_ctx *ctx) { CTX_CALL(e.sync()); }
+
+extern "C" int culda_autotune(culda_ctx *ctx, culda_tune *out) {
+    CTX_CALL({
+        if (!out) fail(CULDA_ERR_ARG, "null output");
+        fill_tune(e.autotune(kSlotCandidates, false), out);
+    });
+}
 
 extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
     CTX_CALL({
         if (!out) fail(CULDA_ERR_ARG, "null output");
-        if (!c.theta_ready) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
-        const int32_t K = c.prm.k, V = c.num_words;
-        const double alpha = c.prm.alpha, beta = c.prm.beta;
+        if (!e.theta_ready) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
+        e.sync();
+        const ModelHost &m = e.models[0];
+        const int32_t K = m.K, V = m.V;
+        const double alpha = m.alpha, beta = m.beta;
         const int32_t G = 1024;
-        double *part = (double *)c.dalloc(G * sizeof(double) * (1 + c.chunks.size()));
-        hip_check(launch_joint_ll_words(c.nwk, V, K, std::lgamma(beta), c.prm.beta, part, G, c.stream), "joint");
-        for (size_t k = 0; k < c.chunks.size(); ++k) {
-            ChunkState &cs = c.chunks[k];
-            hip_check(launch_joint_ll_docs(cs.meta, cs.theta, cs.doc_end - cs.doc_start, c.count_bits, c.prm.alpha,
-                                           part + G * (1 + k), G, c.stream),
+        const size_t U = e.units.size();
+        double *part = (double *)e.dalloc(G * sizeof(double) * (1 + U));
+        hip_check(launch_joint_ll_words(m.nwk, V, K, std::lgamma(beta), m.beta, part, G, e.stream), "joint");
+        for (size_t k = 0; k < U; ++k) {
+            UnitHost &u = e.units[k];
+            hip_check(launch_joint_ll_docs(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits, m.alpha,
+                                           part + G * (1 + k), G, e.stream),
                       "joint");
         }
-        std::vector<double> h(G * (1 + c.chunks.size()));
-        hip_check(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, c.stream), "copy");
+        std::vector<double> h(G * (1 + U));
+        hip_check(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, e.stream), "copy");
         std::vector<int32_t> nk(K);
-        hip_check(hipMemcpyAsync(nk.data(), c.nk, K * 4, hipMemcpyDeviceToHost, c.stream), "copy");
-        hip_check(hipStreamSynchronize(c.stream), "sync");
+        hip_check(hipMemcpyAsync(nk.data(), m.nk, K * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+        e.dfree(part);
         double s = 0;
         for (double v : h) s += v;
         const double vb = (double)V * beta, ka = (double)K * alpha;
         for (int32_t k = 0; k < K; ++k) s += std::lgamma(vb) - std::lgamma((double)nk[k] + vb);
-        for (int64_t nd : c.doc_len) s += std::lgamma(ka) - std::lgamma((double)nd + ka);
+        for (int64_t nd : m.doc_len) s += std::lgamma(ka) - std::lgamma((double)nd + ka);
         *out = s;
-        // release the scratch partials
-        (void)hipFree(part);
-        c.allocations.erase(std::find(c.allocations.begin(), c.allocations.end(), (void *)part));
     });
 }
 
 extern "C" int culda_get_phi(culda_ctx *ctx, int32_t *n_wk, int32_t *n_k) {
     CTX_CALL({
         if (!n_wk || !n_k) fail(CULDA_ERR_ARG, "null output");
-        HostPhi h = download_phi(c);
+        HostPhi h = download_phi(e, 0);
         std::memcpy(n_wk, h.nwk.data(), h.nwk.size() * 4);
         std::memcpy(n_k, h.nk.data(), h.nk.size() * 4);
     });
@@ -240,14 +300,14 @@ extern "C" int culda_get_phi(culda_ctx *ctx, int32_t *n_wk, int32_t *n_k) {
 extern "C" int culda_get_phi_head(culda_ctx *ctx, float *head) {
     CTX_CALL({
         if (!head) fail(CULDA_ERR_ARG, "null output");
-        if (!c.phi_ready) fail(CULDA_ERR_STATE, "phi head not prepared");
-        const size_t n = (size_t)c.num_words * c.prm.k;
-        float *d = (float *)c.dalloc(n * 4);
-        hip_check(launch_phi_head(c.nwk, c.den, d, c.num_words, c.prm.k, c.prm.beta, c.stream), "head");
-        hip_check(hipStreamSynchronize(c.stream), "sync");
+        if (!e.uploaded || !e.phi_ready) fail(CULDA_ERR_STATE, "phi head not prepared");
+        const ModelHost &m = e.models[0];
+        const size_t n = (size_t)m.V * m.K;
+        float *d = (float *)e.dalloc(n * 4);
+        hip_check(launch_phi_head(m.nwk, m.den, d, m.V, m.K, m.beta, e.stream), "head");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
         hip_check(hipMemcpy(head, d, n * 4, hipMemcpyDeviceToHost), "head copy");
-        (void)hipFree(d);
-        c.allocations.erase(std::find(c.allocations.begin(), c.allocations.end(), (void *)d));
+        e.dfree(d);
     });
 }
 
@@ -257,7 +317,7 @@ extern "C" int culda_get_theta(culda_ctx *ctx, int32_t chunk, int64_t *offsets, 
         std::vector<int64_t> off;
         std::vector<int32_t> cnt;
         std::vector<int16_t> top;
-        download_theta(c, chunk, off, cnt, top);
+        download_theta(e, chunk, off, cnt, top);
         std::memcpy(offsets, off.data(), off.size() * 8);
         if (counts) std::memcpy(counts, cnt.data(), cnt.size() * 4);
         if (topics) std::memcpy(topics, top.data(), top.size() * 2);
@@ -265,13 +325,12 @@ extern "C" int culda_get_theta(culda_ctx *ctx, int32_t chunk, int64_t *offsets, 
 }
 
 // ModelPhi::savePhi (model/model_phi.cpp:161-191) wrote a text dump of the
-// first 1000 words only (and nothing else); this writes every word, plus a
-// binary file a program can load.
+// first 1000 words only; this writes every word, plus a binary file.
 extern "C" int culda_save_phi(culda_ctx *ctx, const char *prefix) {
     CTX_CALL({
         if (!prefix) fail(CULDA_ERR_ARG, "null prefix");
-        HostPhi h = download_phi(c);
-        const int32_t V = c.num_words, K = c.prm.k;
+        HostPhi h = download_phi(e, 0);
+        const int32_t V = e.models[0].V, K = e.models[0].K;
         FILE *f = open_or_fail(std::string(prefix) + ".phi.bin", "wb");
         fwrite(&V, 4, 1, f);
         fwrite(&K, 4, 1, f);
@@ -301,17 +360,17 @@ extern "C" int culda_save_phi(culda_ctx *ctx, const char *prefix) {
 extern "C" int culda_save_theta(culda_ctx *ctx, const char *prefix) {
     CTX_CALL({
         if (!prefix) fail(CULDA_ERR_ARG, "null prefix");
-        if (!c.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
-        const int32_t D = c.num_docs, K = c.prm.k;
+        if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+        const int32_t D = e.models[0].num_docs, K = e.models[0].K;
         std::vector<int64_t> goff(D + 1, 0);
         std::vector<int32_t> gcnt;
         std::vector<int16_t> gtop;
-        for (size_t k = 0; k < c.chunks.size(); ++k) {
+        for (size_t k = 0; k < e.units.size(); ++k) {
             std::vector<int64_t> off;
             std::vector<int32_t> cnt;
             std::vector<int16_t> top;
-            download_theta(c, (int32_t)k, off, cnt, top);
-            const int32_t ds = c.chunks[k].doc_start;
+            download_theta(e, (int32_t)k, off, cnt, top);
+            const int32_t ds = e.units[k].doc_start;
             for (size_t d = 0; d + 1 < off.size(); ++d) goff[ds + d + 1] = off[d + 1] - off[d];
             gcnt.insert(gcnt.end(), cnt.begin(), cnt.end());
             gtop.insert(gtop.end(), top.begin(), top.end());
@@ -347,19 +406,19 @@ static const char kCkptMagic[8] = {'C', 'U', 'L', 'D', 'A', 'C', 'K', '1'};
 extern "C" int culda_save_checkpoint(culda_ctx *ctx, const char *path, int32_t next_iteration) {
     CTX_CALL({
         if (!path) fail(CULDA_ERR_ARG, "null path");
-        if (!c.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
-        hip_check(hipStreamSynchronize(c.stream), "sync");
+        if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
+        e.sync();
         FILE *f = open_or_fail(path, "wb");
-        const int32_t C = (int32_t)c.chunks.size(), K = c.prm.k;
+        const int32_t C = (int32_t)e.units.size(), K = e.prm.k;
         fwrite(kCkptMagic, 1, 8, f);
         fwrite(&C, 4, 1, f);
         fwrite(&K, 4, 1, f);
-        fwrite(&c.prm.seed, 8, 1, f);
+        fwrite(&ctx->seed, 8, 1, f);
         fwrite(&next_iteration, 4, 1, f);
-        for (auto &cs : c.chunks) {
-            std::vector<uint16_t> z(cs.num_tokens);
-            if (cs.num_tokens) hip_check(hipMemcpy(z.data(), cs.z, cs.num_tokens * 2, hipMemcpyDeviceToHost), "z");
-            fwrite(&cs.num_tokens, 8, 1, f);
+        for (auto &u : e.units) {
+            std::vector<uint16_t> z(u.num_tokens);
+            if (u.num_tokens) hip_check(hipMemcpy(z.data(), u.z, u.num_tokens * 2, hipMemcpyDeviceToHost), "z");
+            fwrite(&u.num_tokens, 8, 1, f);
             if (!z.empty()) fwrite(z.data(), 2, z.size(), f);
         }
         fclose(f);
@@ -369,21 +428,21 @@ extern "C" int culda_save_checkpoint(culda_ctx *ctx, const char *path, int32_t n
 extern "C" int culda_load_checkpoint(culda_ctx *ctx, const char *path, int32_t *next_iteration) {
     CTX_CALL({
         if (!path || !next_iteration) fail(CULDA_ERR_ARG, "null argument");
-        if (!c.uploaded) fail(CULDA_ERR_STATE, "upload the corpus before loading a checkpoint");
+        if (!e.uploaded) fail(CULDA_ERR_STATE, "upload the corpus before loading a checkpoint");
         FILE *f = open_or_fail(path, "rb");
         char magic[8];
         int32_t C = 0, K = 0, it = 0;
         uint64_t seed = 0;
         bool ok = fread(magic, 1, 8, f) == 8 && !std::memcmp(magic, kCkptMagic, 8) && fread(&C, 4, 1, f) == 1 &&
                   fread(&K, 4, 1, f) == 1 && fread(&seed, 8, 1, f) == 1 && fread(&it, 4, 1, f) == 1;
-        if (!ok || C != (int32_t)c.chunks.size() || K != c.prm.k || seed != c.prm.seed) {
+        if (!ok || C != (int32_t)e.units.size() || K != e.prm.k || seed != ctx->seed) {
             fclose(f);
             fail(CULDA_ERR_FORMAT, "%s is not a checkpoint of this corpus / k / seed", path);
         }
         std::vector<std::vector<uint16_t>> zs(C);
         for (int32_t k = 0; k < C; ++k) {
             int64_t n = 0;
-            if (fread(&n, 8, 1, f) != 1 || n != c.chunks[k].num_tokens) {
+            if (fread(&n, 8, 1, f) != 1 || n != e.units[k].num_tokens) {
                 fclose(f);
                 fail(CULDA_ERR_FORMAT, "checkpoint chunk %d size mismatch", k);
             }
@@ -399,11 +458,11 @@ extern "C" int culda_load_checkpoint(culda_ctx *ctx, const char *path, int32_t *
                 }
         }
         fclose(f);
-        hip_check(hipStreamSynchronize(c.stream), "sync");
+        e.sync();
         for (int32_t k = 0; k < C; ++k)
             if (!zs[k].empty())
-                hip_check(hipMemcpy(c.chunks[k].z, zs[k].data(), zs[k].size() * 2, hipMemcpyHostToDevice), "z");
-        c.theta_ready = c.phi_ready = false;
+                hip_check(hipMemcpy(e.units[k].z, zs[k].data(), zs[k].size() * 2, hipMemcpyHostToDevice), "z");
+        e.theta_ready = e.phi_ready = false;
         *next_iteration = it;
     });
 }
@@ -412,49 +471,187 @@ extern "C" int culda_get_stats(culda_ctx *ctx, culda_stats *out) {
     CTX_CALL({
         if (!out) fail(CULDA_ERR_ARG, "null output");
         std::memset(out, 0, sizeof *out);
-        out->num_tokens = c.num_tokens;
-        for (auto &cs : c.chunks) {
-            out->num_slots += cs.num_slots;
-            out->theta_capacity += cs.theta_cap;
-        }
-        out->sum_doc_nnz = c.sum_doc_nnz;
-        out->sum_token_nnz = c.sum_token_nnz;
-        out->sampler_ms = c.sampler_ms;
-        out->phi_ms = c.phi_ms;
-        out->theta_ms = c.theta_ms;
-        out->other_ms = c.other_ms;
-        out->sampler_launches = c.sampler_launches;
+        for (auto &m : e.models) out->num_tokens += m.num_tokens;
+        out->num_slots = e.num_slots;
+        for (auto &u : e.units) out->theta_capacity += u.theta_cap;
+        for (int64_t v : e.unit_doc_nnz) out->sum_doc_nnz += v;
+        for (int64_t v : e.unit_token_nnz) out->sum_token_nnz += v;
+        out->sampler_ms = e.sampler_ms;
+        out->phi_ms = e.phi_ms;
+        out->theta_ms = e.theta_ms;
+        out->sampler_launches = e.sampler_launches;
     });
 }
 
 extern "C" int culda_sampler_bytes(culda_ctx *ctx, double *bytes) {
     CTX_CALL({
         if (!bytes) fail(CULDA_ERR_ARG, "null output");
-        *bytes = c.last_sampler_bytes;
+        *bytes = e.last_sampler_bytes;
     });
 }
 
 extern "C" int culda_stream_copy(culda_ctx *ctx, int64_t n_bytes, int32_t repeats, double *gbps) {
     CTX_CALL({
         if (!gbps || n_bytes <= 0 || n_bytes % 16 || repeats < 1) fail(CULDA_ERR_ARG, "bad stream copy arguments");
-        float4 *a = (float4 *)c.dalloc(n_bytes), *b = (float4 *)c.dalloc(n_bytes);
-        hip_check(launch_fill_u32((uint32_t *)a, n_bytes / 4, 0x3f800000u, c.stream), "fill");
+        float4 *a = (float4 *)e.dalloc(n_bytes), *b = (float4 *)e.dalloc(n_bytes);
+        hip_check(launch_fill_u32((uint32_t *)a, n_bytes / 4, 0x3f800000u, e.stream), "fill");
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        hip_check(launch_copy(a, b, n_bytes / 16, c.stream), "copy warmup");
-        hip_check(hipEventRecord(e0, c.stream), "event");
-        for (int32_t r = 0; r < repeats; ++r) hip_check(launch_copy(a, b, n_bytes / 16, c.stream), "copy");
-        hip_check(hipEventRecord(e1, c.stream), "event");
+        hip_check(launch_copy(a, b, n_bytes / 16, e.stream), "copy warmup");
+        hip_check(hipEventRecord(e0, e.stream), "event");
+        for (int32_t r = 0; r < repeats; ++r) hip_check(launch_copy(a, b, n_bytes / 16, e.stream), "copy");
+        hip_check(hipEventRecord(e1, e.stream), "event");
         hip_check(hipEventSynchronize(e1), "event sync");
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         *gbps = 2.0 * (double)n_bytes * repeats / (ms * 1e-3) / 1e9;
-        (void)hipFree(a);
-        (void)hipFree(b);
-        c.allocations.erase(std::find(c.allocations.begin(), c.allocations.end(), (void *)a));
-        c.allocations.erase(std::find(c.allocations.begin(), c.allocations.end(), (void *)b));
+        e.dfree(a);
+        e.dfree(b);
+    });
+}
+
+// ====================================================== batch of corpora
+
+#define BATCH_CALL(...)         \
+    return guarded([&] {        \
+        Engine &e = eng(batch); \
+        (void)e;                \
+        __VA_ARGS__;            \
+        return CULDA_OK;        \
+    })
+
+extern "C" int culda_batch_create(const culda_params *params, culda_batch **out) {
+    return guarded([&] {
+        if (!params || !out) fail(CULDA_ERR_ARG, "null argument");
+        *out = nullptr;
+        auto *h = new culda_batch;
+        try {
+            h->e.init_device(*params);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        h->e.prm.fused_phi = 1;
+        *out = h;
+        return CULDA_OK;
+    });
+}
+
+extern "C" void culda_batch_destroy(culda_batch *batch) {
+    if (!batch) return;
+    (void)hipSetDevice(batch->e.prm.device);
+    delete batch;
+}
+
+extern "C" int culda_batch_add(culda_batch *batch, const culda_corpus *corpus, float alpha, float beta, uint64_t seed,
+                               int32_t *model) {
+    BATCH_CALL({
+        if (!corpus) fail(CULDA_ERR_ARG, "null corpus");
+        const int32_t m = e.add_corpus(corpus_ref(corpus), alpha, beta, seed);
+        if (model) *model = m;
+    });
+}
+
+extern "C" int culda_batch_upload(culda_batch *batch) { BATCH_CALL(e.upload()); }
+
+extern "C" int culda_batch_init(culda_batch *batch) {
+    BATCH_CALL({
+        e.init_z();
+        e.rebuild_all();
+    });
+}
+
+extern "C" int culda_batch_set_persistent(culda_batch *batch, int32_t groups) {
+    BATCH_CALL({
+        if (groups < 0) fail(CULDA_ERR_ARG, "groups must be >= 0");
+        if (e.graph) {
+            (void)hipGraphExecDestroy(e.graph);
+            e.graph = nullptr;
+        }
+        e.persistent_groups = groups;
+    });
+}
+
+extern "C" int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, int32_t count, int32_t use_graph,
+                                   double *ll, double *seconds) {
+    BATCH_CALL(e.iterate_batch((uint32_t)std::max(first_iteration, 0), count, use_graph != 0, ll, seconds));
+}
+
+// Tries each slot size with one workgroup per slot and with a persistent
+// grid of 4 and 8 workgroups per CU; keeps the fastest pair.
+extern "C" int culda_batch_autotune(culda_batch *batch, culda_tune *out) {
+    BATCH_CALL({
+        if (!out) fail(CULDA_ERR_ARG, "null output");
+        TuneResult best;
+        best.best_ms = 1e30;
+        int32_t best_groups = 0;
+        std::vector<std::pair<int32_t, double>> all;
+        for (int32_t groups : {0, 4 * e.cu_count, 8 * e.cu_count}) {
+            e.persistent_groups = groups;
+            TuneResult r = e.autotune(kSlotCandidates, true);
+            for (auto &t : r.tried) all.push_back(t);
+            if (r.best_ms < best.best_ms) {
+                best = r;
+                best_groups = groups;
+            }
+        }
+        e.persistent_groups = best_groups;
+        e.rebuild_slots(best.slot_tokens);
+        best.tried = all;
+        fill_tune(best, out);
+        out->persistent_groups = best_groups;
+    });
+}
+
+extern "C" int culda_batch_get_z(culda_batch *batch, int32_t model, int32_t chunk, uint16_t *z, int64_t n) {
+    BATCH_CALL({
+        check_model(e, model);
+        const ModelHost &m = e.models[model];
+        if (chunk < 0 || chunk >= m.unit_end - m.unit_begin) fail(CULDA_ERR_ARG, "chunk %d out of range", chunk);
+        copy_z(e, m.unit_begin + chunk, z, n, false);
+    });
+}
+
+extern "C" int culda_batch_get_phi(culda_batch *batch, int32_t model, int32_t *n_wk, int32_t *n_k) {
+    BATCH_CALL({
+        if (!n_wk || !n_k) fail(CULDA_ERR_ARG, "null output");
+        HostPhi h = download_phi(e, model);
+        std::memcpy(n_wk, h.nwk.data(), h.nwk.size() * 4);
+        std::memcpy(n_k, h.nk.data(), h.nk.size() * 4);
+    });
+}
+
+extern "C" int culda_batch_get_stats(culda_batch *batch, culda_stats *out) {
+    BATCH_CALL({
+        if (!out) fail(CULDA_ERR_ARG, "null output");
+        std::memset(out, 0, sizeof *out);
+        for (auto &m : e.models) out->num_tokens += m.num_tokens;
+        out->num_slots = e.num_slots;
+        for (auto &u : e.units) out->theta_capacity += u.theta_cap;
+        for (int64_t v : e.unit_doc_nnz) out->sum_doc_nnz += v;
+        for (int64_t v : e.unit_token_nnz) out->sum_token_nnz += v;
+        out->sampler_ms = e.sampler_ms;
+        out->other_ms = e.iter_ms;
+        out->sampler_launches = e.sampler_launches;
+    });
+}
+
+extern "C" int culda_batch_sampler_bytes(culda_batch *batch, double *bytes) {
+    BATCH_CALL({
+        if (!bytes) fail(CULDA_ERR_ARG, "null output");
+        *bytes = e.last_sampler_bytes;
+    });
+}
+
+extern "C" int culda_batch_info(culda_batch *batch, int32_t *num_models, int64_t *num_tokens, int64_t *num_slots) {
+    BATCH_CALL({
+        int64_t n = 0;
+        for (auto &m : e.models) n += m.num_tokens;
+        if (num_models) *num_models = (int32_t)(e.uploaded ? e.models.size() : e.pending.size());
+        if (num_tokens) *num_tokens = n;
+        if (num_slots) *num_slots = e.num_slots;
     });
 }

@@ -1,0 +1,679 @@
+// engine.cpp -- layouts, work lists and iteration schedules on the device.
+//
+// Iteration (train/SingleChunkSingleGPU.h:70-127 for one chunk,
+// train/MultiChunkMultiGPUequal.h:88-169 for several, here on one device):
+// sample every unit with the frozen counts of the previous iteration,
+// reduce the FP64 log-likelihood, rebuild n_wk / n_k from z on the device
+// (summed over a corpus's chunks), prepare the head denominators, rebuild
+// every unit's theta.  One host synchronisation per iteration, or none
+// inside a replayed graph.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "kernels.h"
+
+namespace culda {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) fail(CULDA_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+Engine::~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    for (void *p : allocations) (void)hipFree(p);
+    for (hipEvent_t &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t &e : ev_sampled)
+        if (e) (void)hipEventDestroy(e);
+    if (ev_theta_done) (void)hipEventDestroy(ev_theta_done);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
+}
+
+void Engine::init_device(const culda_params &p) {
+    if (p.k < 16 || p.k > kMaxK || p.k % 16)
+        fail(CULDA_ERR_ARG, "k = %d: the sampler needs a multiple of 16 in [16, %d]", p.k, kMaxK);
+    if (p.slot_tokens < 0) fail(CULDA_ERR_ARG, "slot_tokens must be >= 0");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) fail(CULDA_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (p.device < 0 || p.device >= n) fail(CULDA_ERR_ARG, "device %d out of range (%d devices)", p.device, n);
+    hip_check(hipSetDevice(p.device), "hipSetDevice");
+    prm = p;
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking), "hipStreamCreate");
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device) != hipSuccess) cus = 0;
+    cu_count = cus > 0 ? cus : 256;
+}
+
+void *Engine::dalloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) fail(CULDA_ERR_MEMORY, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    allocations.push_back(p);
+    return p;
+}
+
+void Engine::dfree(void *p) {
+    if (!p) return;
+    auto it = std::find(allocations.begin(), allocations.end(), p);
+    if (it != allocations.end()) {
+        (void)hipFree(p);
+        allocations.erase(it);
+    }
+}
+
+template <class T>
+static T *upload_vec(Engine &e, const std::vector<T> &v) {
+    T *d = (T *)e.dalloc(v.size() * sizeof(T));
+    if (!v.empty()) hip_check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return d;
+}
+
+static uint32_t ceil_log2(uint32_t k) {
+    uint32_t b = 0;
+    while ((1u << b) < k) ++b;
+    return b ? b : 1;
+}
+
+static constexpr int64_t kPhiSplit = 8192;  // hot words are rebuilt in pieces
+
+// Packets of at most `per` consecutive items of [first, last), all owned by `owner`.
+static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last, int32_t owner, int32_t per) {
+    for (int32_t x = first; x < last; x += per) out.push_back(Packet{x, std::min(last, x + per), owner, 0});
+}
+
+// Phi list of model m over units [u0, u1): whole words, hot words as pieces
+// of at most kPhiSplit tokens per unit (rows zeroed first, atomics).
+static void build_phi_list(const std::vector<UnitHost> &units, int32_t m, int32_t u0, int32_t u1, int32_t V,
+                           std::vector<PhiSlot> &slots, std::vector<int2> &zero) {
+    std::vector<std::pair<int64_t, int32_t>> whole;
+    whole.reserve(V);
+    for (int32_t w = 0; w < V; ++w) {
+        int64_t tot = 0;
+        for (int32_t u = u0; u < u1; ++u) tot += units[u].word_idx_h[w + 1] - units[u].word_idx_h[w];
+        if (tot > kPhiSplit) {
+            zero.push_back(make_int2(m, w));
+            for (int32_t u = u0; u < u1; ++u) {
+                const int64_t b = units[u].word_idx_h[w], e = units[u].word_idx_h[w + 1];
+                for (int64_t x = b; x < e; x += kPhiSplit)
+                    slots.push_back(PhiSlot{w, u, u + 1, 1, x, std::min(e, x + kPhiSplit)});
+            }
+        } else {
+            whole.emplace_back(tot, w);
+        }
+    }
+    std::stable_sort(whole.begin(), whole.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    for (auto &pr : whole) slots.push_back(PhiSlot{pr.second, u0, u1, 0, 0, 0});
+}
+
+int32_t Engine::add_corpus(const Corpus &c, float alpha, float beta, uint64_t seed) {
+    if (uploaded) fail(CULDA_ERR_STATE, "corpora must be added before the upload");
+    if (!(alpha > 0.0f) || !std::isfinite(alpha)) fail(CULDA_ERR_ARG, "alpha must be positive and finite");
+    if (!(beta > 0.0f) || !std::isfinite(beta)) fail(CULDA_ERR_ARG, "beta must be positive and finite");
+    c.validate();
+    pending.push_back(&c);
+    pending_alpha.push_back(alpha);
+    pending_beta.push_back(beta);
+    pending_seed.push_back(seed);
+    return (int32_t)pending.size() - 1;
+}
+
+void Engine::upload() {
+    if (uploaded) fail(CULDA_ERR_STATE, "already uploaded");
+    if (pending.empty()) fail(CULDA_ERR_STATE, "no corpus to upload");
+    const int32_t K = prm.k;
+    const uint32_t cbits = 30 - ceil_log2((uint32_t)K);
+    const int64_t max_count = ((int64_t)1 << cbits) - 1;
+    int64_t total_docs = 0;
+    for (size_t mi = 0; mi < pending.size(); ++mi) {
+        const Corpus &c = *pending[mi];
+        ModelHost m;
+        m.V = c.num_words;
+        m.K = K;
+        m.num_docs = c.num_docs();
+        m.alpha = pending_alpha[mi];
+        m.beta = pending_beta[mi];
+        m.seed = pending_seed[mi];
+        m.count_bits = cbits;
+        m.num_tokens = c.num_tokens();
+        if (m.num_tokens >= (int64_t)INT32_MAX)
+            fail(CULDA_ERR_LIMIT, "corpus %zu has %lld tokens; counts are int32", mi, (long long)m.num_tokens);
+        if (c.max_doc_len() > max_count)
+            fail(CULDA_ERR_LIMIT, "doc length %d exceeds %lld, the largest count a packed theta entry holds at K=%d",
+                 c.max_doc_len(), (long long)max_count, K);
+        m.doc_len.resize(m.num_docs);
+        for (int32_t d = 0; d < m.num_docs; ++d) m.doc_len[d] = c.doc_len(d);
+        total_docs += m.num_docs;
+        m.unit_begin = (int32_t)units.size();
+        for (size_t k = 0; k < c.chunks.size(); ++k) {
+            const Chunk &ch = c.chunks[k];
+            UnitHost u;
+            u.model = (int32_t)mi;
+            u.doc_start = ch.doc_start;
+            u.doc_end = ch.doc_end;
+            u.num_tokens = ch.num_tokens();
+            if (u.num_tokens >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk has >= 2^32 tokens");
+            const int32_t Dc = ch.doc_end - ch.doc_start;
+            std::vector<uint32_t> tok(u.num_tokens), rev_off(Dc + 1, 0), rev_idx(u.num_tokens);
+            for (int64_t i = 0; i < u.num_tokens; ++i) tok[i] = (uint32_t)(ch.word_tokens[i] - ch.doc_start);
+            for (int32_t d = 0; d < Dc; ++d) rev_off[d + 1] = rev_off[d] + (uint32_t)m.doc_len[ch.doc_start + d];
+            {
+                std::vector<uint32_t> nxt(rev_off.begin(), rev_off.end() - 1);
+                for (int64_t i = 0; i < u.num_tokens; ++i) rev_idx[nxt[tok[i]]++] = (uint32_t)i;
+            }
+            std::vector<DocMeta> meta(Dc);
+            int64_t cap = 0;
+            for (int32_t d = 0; d < Dc; ++d) {
+                const int64_t nd = m.doc_len[ch.doc_start + d];
+                meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
+                cap += (std::min<int64_t>(nd, K) + 15) / 16 * 16;
+                if (cap + 64 >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^32 entries");
+            }
+            u.theta_cap = cap;
+            u.word_idx_h = ch.word_idx;
+            u.tokens = upload_vec(*this, tok);
+            u.z = (uint16_t *)dalloc(std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t));
+            hip_check(hipMemset(u.z, 0, std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t)), "z clear");
+            u.word_idx = upload_vec(*this, ch.word_idx);
+            u.rev_off = upload_vec(*this, rev_off);
+            u.rev_idx = upload_vec(*this, rev_idx);
+            u.meta = upload_vec(*this, meta);
+            // 64 entries of tail padding: the sampler loads whole 64-entry
+            // groups and may read past the last slab (those values are unused)
+            u.theta = (uint32_t *)dalloc((cap + 64) * sizeof(uint32_t));
+            hip_check(hipMemset(u.theta, 0, (cap + 64) * sizeof(uint32_t)), "theta clear");
+            units.push_back(std::move(u));
+        }
+        m.unit_end = (int32_t)units.size();
+        m.nwk = (int32_t *)dalloc((size_t)m.V * K * sizeof(int32_t));
+        m.den = (float *)dalloc(K * sizeof(float));
+        models.push_back(std::move(m));
+    }
+    const int32_t M = (int32_t)models.size(), U = (int32_t)units.size();
+    d_nk_all = (int32_t *)dalloc((size_t)M * K * sizeof(int32_t));
+    for (int32_t mi = 0; mi < M; ++mi) models[mi].nk = d_nk_all + (size_t)mi * K;
+    d_stats = (unsigned long long *)dalloc(2 * sizeof(unsigned long long) * U);
+    hip_check(hipMemset(d_stats, 0, 2 * sizeof(unsigned long long) * U), "stats clear");
+    unit_doc_nnz.assign(U, 0);
+    unit_token_nnz.assign(U, 0);
+    // device tables
+    std::vector<ModelDev> md(M);
+    for (int32_t mi = 0; mi < M; ++mi) {
+        const ModelHost &m = models[mi];
+        md[mi] = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K, m.V,
+                          m.count_bits, 0};
+    }
+    d_models = upload_vec(*this, md);
+    std::vector<UnitDev> ud(U);
+    for (int32_t ui = 0; ui < U; ++ui) {
+        const UnitHost &u = units[ui];
+        ud[ui] = UnitDev{u.tokens, u.z, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+                         u.doc_end - u.doc_start};
+    }
+    d_units = upload_vec(*this, ud);
+    // phi work lists: fused per model (and all models concatenated), and per unit
+    {
+        std::vector<PhiSlot> all_slots;
+        std::vector<Packet> all_packets;
+        std::vector<int2> all_zero;
+        for (int32_t mi = 0; mi < M; ++mi) {
+            ModelHost &m = models[mi];
+            std::vector<PhiSlot> sl;
+            std::vector<int2> zr;
+            build_phi_list(units, mi, m.unit_begin, m.unit_end, m.V, sl, zr);
+            const int32_t per = std::max<int32_t>(4, (int32_t)((sl.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
+            std::vector<Packet> pk;
+            append_packets(pk, 0, (int32_t)sl.size(), mi, per);
+            m.phi_fused.slots = upload_vec(*this, sl);
+            m.phi_fused.packets = upload_vec(*this, pk);
+            m.phi_fused.zero_rows = upload_vec(*this, zr);
+            m.phi_fused.num_slots = (int32_t)sl.size();
+            m.phi_fused.num_packets = (int32_t)pk.size();
+            m.phi_fused.num_zero_rows = (int32_t)zr.size();
+            const int32_t off = (int32_t)all_slots.size();
+            for (Packet p : pk) all_packets.push_back(Packet{p.first + off, p.last + off, p.owner, 0});
+            all_slots.insert(all_slots.end(), sl.begin(), sl.end());
+            all_zero.insert(all_zero.end(), zr.begin(), zr.end());
+            for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+                std::vector<PhiSlot> s1;
+                std::vector<int2> z1;
+                build_phi_list(units, mi, ui, ui + 1, m.V, s1, z1);
+                const int32_t per1 =
+                    std::max<int32_t>(4, (int32_t)((s1.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
+                std::vector<Packet> p1;
+                append_packets(p1, 0, (int32_t)s1.size(), mi, per1);
+                UnitHost &u = units[ui];
+                u.phi_single.slots = upload_vec(*this, s1);
+                u.phi_single.packets = upload_vec(*this, p1);
+                u.phi_single.zero_rows = upload_vec(*this, z1);
+                u.phi_single.num_slots = (int32_t)s1.size();
+                u.phi_single.num_packets = (int32_t)p1.size();
+                u.phi_single.num_zero_rows = (int32_t)z1.size();
+            }
+        }
+        phi_all.slots = upload_vec(*this, all_slots);
+        phi_all.packets = upload_vec(*this, all_packets);
+        phi_all.zero_rows = upload_vec(*this, all_zero);
+        phi_all.num_slots = (int32_t)all_slots.size();
+        phi_all.num_packets = (int32_t)all_packets.size();
+        phi_all.num_zero_rows = (int32_t)all_zero.size();
+    }
+    // theta packets per unit (and all units concatenated)
+    {
+        const int64_t per_docs = std::max<int64_t>(16, (total_docs + (int64_t)cu_count * 8 - 1) / ((int64_t)cu_count * 8));
+        std::vector<Packet> all;
+        for (int32_t ui = 0; ui < U; ++ui) {
+            UnitHost &u = units[ui];
+            std::vector<Packet> pk;
+            append_packets(pk, 0, u.doc_end - u.doc_start, ui, (int32_t)per_docs);
+            u.theta_packets = upload_vec(*this, pk);
+            u.num_theta_packets = (int32_t)pk.size();
+            all.insert(all.end(), pk.begin(), pk.end());
+        }
+        d_theta_packets_all = upload_vec(*this, all);
+        num_theta_packets_all = (int32_t)all.size();
+    }
+    d_sweep = (uint32_t *)dalloc(16);
+    d_queue = (uint32_t *)dalloc(16);
+    hip_check(hipMemset(d_sweep, 0, 16), "sweep clear");
+    d_ll_hist = (double *)dalloc(sizeof(double) * kLLHistory * M);
+    hip_check(hipMemset(d_ll_hist, 0, sizeof(double) * kLLHistory * M), "ll clear");
+    ev_sampled.assign(U, nullptr);
+    for (hipEvent_t &e : ev_sampled) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    hip_check(hipEventCreateWithFlags(&ev_theta_done, hipEventDisableTiming), "event");
+    uploaded = true;
+    rebuild_slots(prm.slot_tokens);
+    hip_check(hipDeviceSynchronize(), "upload");
+    pending.clear();
+    theta_ready = phi_ready = sampled = false;
+}
+
+// Sampler slots: each unit's word ranges cut into pieces of at most
+// slot_tokens, longest first within the unit; units in order, so a model's
+// slots are contiguous (its LL partials too).  d_order lists every slot
+// longest first, for launches over all units.
+void Engine::rebuild_slots(int32_t slot_tokens_in) {
+    if (graph) {
+        (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+    dfree(d_slots);
+    dfree(d_order);
+    dfree(d_ll_partial);
+    dfree(d_ll_packets);
+    dfree(d_model_packets);
+    dfree(d_ll_scratch);
+    const int64_t slot_tokens = slot_tokens_in > 0 ? slot_tokens_in : 1024;
+    std::vector<SlotDesc> all;
+    std::vector<Packet> llp;
+    std::vector<int32_t> mp(2 * models.size());
+    for (size_t mi = 0; mi < models.size(); ++mi) {
+        ModelHost &m = models[mi];
+        m.slot_begin = (int32_t)all.size();
+        // g = sum_{w'<w} cnt(w') + sum_{c'<c} cnt_c'(w) + local
+        std::vector<int64_t> pre(m.V + 1, 0);
+        for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui)
+            for (int32_t w = 0; w < m.V; ++w) pre[w + 1] += units[ui].word_idx_h[w + 1] - units[ui].word_idx_h[w];
+        for (int32_t w = 0; w < m.V; ++w) pre[w + 1] += pre[w];
+        std::vector<int64_t> run(pre.begin(), pre.end() - 1);
+        for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+            UnitHost &u = units[ui];
+            std::vector<SlotDesc> sl;
+            for (int32_t w = 0; w < m.V; ++w) {
+                const int64_t b = u.word_idx_h[w], e = u.word_idx_h[w + 1], n = e - b;
+                if (n > 0) {
+                    const int64_t pieces = (n + slot_tokens - 1) / slot_tokens;
+                    const int64_t step = (n + pieces - 1) / pieces;
+                    for (int64_t x = b; x < e; x += step) sl.push_back(SlotDesc{w, ui, x, std::min(e, x + step), run[w] - b});
+                }
+                run[w] += n;
+            }
+            std::stable_sort(sl.begin(), sl.end(),
+                             [](const SlotDesc &a, const SlotDesc &b) { return a.end - a.begin > b.end - b.begin; });
+            u.slot_begin = (int32_t)all.size();
+            all.insert(all.end(), sl.begin(), sl.end());
+            u.slot_end = (int32_t)all.size();
+        }
+        m.slot_end = (int32_t)all.size();
+        mp[2 * mi] = (int32_t)llp.size();
+        append_packets(llp, m.slot_begin, m.slot_end, (int32_t)mi, 2048);
+        mp[2 * mi + 1] = (int32_t)llp.size();
+    }
+    if (all.size() >= (size_t)INT32_MAX) fail(CULDA_ERR_LIMIT, "too many sampler slots");
+    num_slots = (int32_t)all.size();
+    std::vector<int32_t> order(all.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return all[a].end - all[a].begin > all[b].end - all[b].begin;
+    });
+    d_slots = upload_vec(*this, all);
+    d_order = upload_vec(*this, order);
+    d_ll_partial = (double *)dalloc(sizeof(double) * std::max<size_t>(all.size(), 1));
+    hip_check(hipMemset(d_ll_partial, 0, sizeof(double) * std::max<size_t>(all.size(), 1)), "ll clear");
+    d_ll_packets = upload_vec(*this, llp);
+    d_model_packets = upload_vec(*this, mp);
+    d_ll_scratch = (double *)dalloc(sizeof(double) * std::max<size_t>(llp.size(), 1));
+    num_ll_packets = (int32_t)llp.size();
+    prm.slot_tokens = (int32_t)slot_tokens;
+    // the copies and the memset above run on the null stream, which does not
+    // order against the engine's non-blocking streams
+    hip_check(hipDeviceSynchronize(), "slot upload");
+}
+
+void Engine::set_sweep(uint32_t t, hipStream_t s) {
+    hip_check(launch_fill_u32(d_sweep, 1, t, s), "sweep set");
+}
+
+void Engine::init_z() {
+    if (!uploaded) fail(CULDA_ERR_STATE, "init_z before upload");
+    InitArgs a{d_slots, d_units, d_models, num_slots, 0};
+    hip_check(launch_init_z(a, stream), "init_z launch");
+    theta_ready = phi_ready = false;
+}
+
+// unit < 0: model's chunks in one pass (overwrite).  unit >= 0: that unit
+// alone, added to the counts when accumulate (chunks in turn).
+void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStream_t s) {
+    ModelHost &m = models[mi];
+    const PhiList &L = unit < 0 ? m.phi_fused : units[unit].phi_single;
+    if (unit < 0) accumulate = false;
+    if (!accumulate) {
+        hip_check(launch_fill_u32((uint32_t *)m.nk, m.K, 0, s), "fill n_k");
+        hip_check(launch_zero_rows(d_models, L.zero_rows, L.num_zero_rows, s), "zero rows");
+    }
+    PhiArgs a{L.slots, L.packets, d_units, d_models, L.num_packets, accumulate ? 1 : 0};
+    hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
+    phi_ready = false;
+}
+
+void Engine::rebuild_phi_all(hipStream_t s) {
+    hip_check(launch_fill_u32((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, 0, s), "fill n_k");
+    hip_check(launch_zero_rows(d_models, phi_all.zero_rows, phi_all.num_zero_rows, s), "zero rows");
+    PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 0};
+    hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
+    phi_ready = false;
+}
+
+void Engine::phi_head(hipStream_t s) {
+    hip_check(launch_den(d_models, (int32_t)models.size(), s), "den launch");
+    phi_ready = true;
+}
+
+void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
+    hip_check(launch_fill_u32((uint32_t *)(d_stats + 2 * ui), 4, 0, s), "stats clear");
+    ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
+    hip_check(launch_theta_rebuild(a, s), "theta rebuild launch");
+}
+
+void Engine::rebuild_theta_all(hipStream_t s) {
+    hip_check(launch_fill_u32((uint32_t *)d_stats, 4 * (int64_t)units.size(), 0, s), "stats clear");
+    ThetaArgs a{d_theta_packets_all, d_units, d_models, num_theta_packets_all, 0};
+    hip_check(launch_theta_rebuild(a, s), "theta rebuild launch");
+    theta_ready = true;
+}
+
+void Engine::sample_unit(int32_t ui, hipStream_t s) {
+    UnitHost &u = units[ui];
+    SampleArgs a{d_slots + u.slot_begin, nullptr, d_units, d_models, d_ll_partial + u.slot_begin, d_sweep, nullptr,
+                 u.slot_end - u.slot_begin, 0};
+    hip_check(launch_sample(a, 0, s), "sampler launch");
+}
+
+void Engine::sample_all(hipStream_t s) {
+    SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_sweep, nullptr, num_slots, 0};
+    int32_t grid = 0;
+    if (persistent_groups > 0) {
+        hip_check(launch_fill_u32(d_queue, 1, 0, s), "queue reset");
+        a.queue = d_queue;
+        grid = std::min<int32_t>(num_slots, persistent_groups);
+    }
+    hip_check(launch_sample(a, grid, s), "sampler launch");
+}
+
+void Engine::ll_reduce(hipStream_t s) {
+    LLArgs a{d_ll_partial, d_ll_packets, d_model_packets, d_ll_scratch, d_ll_hist, d_sweep, num_ll_packets,
+             (int32_t)models.size()};
+    hip_check(launch_ll_reduce(a, s), "ll reduce launch");
+}
+
+void Engine::read_stats() {
+    std::vector<unsigned long long> h(2 * units.size());
+    hip_check(hipMemcpyAsync(h.data(), d_stats, h.size() * 8, hipMemcpyDeviceToHost, stream), "stats copy");
+    hip_check(hipStreamSynchronize(stream), "stats sync");
+    for (size_t u = 0; u < units.size(); ++u) {
+        unit_doc_nnz[u] = (int64_t)h[2 * u];
+        unit_token_nnz[u] = (int64_t)h[2 * u + 1];
+    }
+}
+
+void Engine::rebuild_all() {
+    if (!uploaded) fail(CULDA_ERR_STATE, "rebuild before upload");
+    hip_check(hipStreamSynchronize(stream2), "sync");
+    if (prm.fused_phi) {
+        rebuild_phi_all(stream);
+    } else {
+        for (size_t mi = 0; mi < models.size(); ++mi)
+            for (int32_t ui = models[mi].unit_begin; ui < models[mi].unit_end; ++ui)
+                rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, stream);
+    }
+    phi_head(stream);
+    rebuild_theta_all(stream);
+    read_stats();
+}
+
+double Engine::model_ll(int32_t mi, uint32_t sweep) {
+    double h = 0;
+    hip_check(hipMemcpy(&h, d_ll_hist + (size_t)(sweep % kLLHistory) * models.size() + mi, sizeof(double),
+                        hipMemcpyDeviceToHost),
+              "ll copy");
+    return h / (double)models[mi].num_tokens;
+}
+
+void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
+    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
+    const int32_t U = (int32_t)units.size();
+    const bool prof = prm.profile != 0;
+    if (prof && !ev[0])
+        for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    const double bytes = sweep_bytes();  // theta entries read by this sweep: the last rebuild's
+    auto t0 = std::chrono::steady_clock::now();
+    set_sweep(sweep, stream);
+    // stream: sweeps of all chunks, LL, phi + head denominators.
+    // stream2: theta of chunk c as soon as chunk c's sweep is done, so the
+    // (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta
+    // of chunk c is read only by chunk c's sweep, which has finished.
+    if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
+    hip_check(hipStreamWaitEvent(stream2, ev_theta_done, 0), "wait");
+    for (int32_t u = 0; u < U; ++u) {
+        sample_unit(u, stream);
+        hip_check(hipEventRecord(ev_sampled[u], stream), "event");
+        hip_check(hipStreamWaitEvent(stream2, ev_sampled[u], 0), "wait");
+        rebuild_theta_unit(u, stream2);
+    }
+    if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
+    ll_reduce(stream);
+    if (prm.fused_phi || U == 1) {
+        rebuild_phi_model(0, -1, false, stream);
+    } else {
+        for (int32_t u = 0; u < U; ++u) rebuild_phi_model(0, u, u > 0, stream);
+    }
+    phi_head(stream);
+    if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
+    hip_check(hipEventRecord(ev_theta_done, stream2), "event");
+    hip_check(hipStreamWaitEvent(stream, ev_theta_done, 0), "wait");
+    if (prof) hip_check(hipEventRecord(ev[3], stream), "event");
+    double h_ll = 0;
+    std::vector<unsigned long long> h_st(2 * U);
+    hip_check(hipMemcpyAsync(&h_ll, d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
+                             stream),
+              "ll copy");
+    hip_check(hipMemcpyAsync(h_st.data(), d_stats, h_st.size() * 8, hipMemcpyDeviceToHost, stream), "stats copy");
+    hip_check(hipStreamSynchronize(stream), "iteration sync");
+    auto t1 = std::chrono::steady_clock::now();
+    for (int32_t u = 0; u < U; ++u) {
+        unit_doc_nnz[u] = (int64_t)h_st[2 * u];
+        unit_token_nnz[u] = (int64_t)h_st[2 * u + 1];
+    }
+    if (prof) {
+        float a = 0, b = 0, c = 0;
+        hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
+        hip_check(hipEventElapsedTime(&b, ev[1], ev[2]), "elapsed");
+        hip_check(hipEventElapsedTime(&c, ev[2], ev[3]), "elapsed");
+        sampler_ms = a;
+        phi_ms = b;
+        theta_ms = c;
+    }
+    sampler_launches = U;
+    last_sampler_bytes = bytes;
+    sampled = true;
+    if (ll) *ll = h_ll / (double)models[0].num_tokens;
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// One iteration of every model: a single sampler launch over all slots
+// (persistent when persistent_groups > 0), LL, phi, head, theta, then the
+// sweep counter advances -- everything reads the sweep index on the device,
+// so the sequence can be captured once and replayed.  No host memset and
+// no host synchronisation inside.
+void Engine::enqueue_batch_iteration(hipStream_t s, hipEvent_t before, hipEvent_t after) {
+    if (before) hip_check(hipEventRecord(before, s), "event");
+    sample_all(s);
+    if (after) hip_check(hipEventRecord(after, s), "event");
+    ll_reduce(s);
+    rebuild_phi_all(s);
+    phi_head(s);
+    rebuild_theta_all(s);
+    hip_check(launch_add_u32(d_sweep, 1, s), "sweep advance");
+}
+
+void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double *ll, double *seconds) {
+    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (first < 1 || count < 1) fail(CULDA_ERR_ARG, "first iteration must be >= 1 and count >= 1");
+    if (count > kLLHistory) fail(CULDA_ERR_LIMIT, "at most %d iterations per call", kLLHistory);
+    hip_check(hipStreamSynchronize(stream2), "sync");
+    const bool prof = prm.profile != 0;
+    if (prof && !ev[0])
+        for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    if (use_graph && !graph) {
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            enqueue_batch_iteration(stream);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "end capture");
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "graph instantiate");
+    }
+    set_sweep(first, stream);
+    auto t0 = std::chrono::steady_clock::now();
+    if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
+    // the last eager iteration's sampler launch is timed on its own
+    const bool time_sampler = prof && !use_graph;
+    if (time_sampler) last_sampler_bytes = sweep_bytes();
+    for (int32_t i = 0; i < count; ++i) {
+        if (use_graph)
+            hip_check(hipGraphLaunch(graph, stream), "graph launch");
+        else if (time_sampler && i == count - 1)
+            enqueue_batch_iteration(stream, ev[2], ev[3]);
+        else
+            enqueue_batch_iteration(stream);
+    }
+    if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
+    hip_check(hipStreamSynchronize(stream), "iteration sync");
+    auto t1 = std::chrono::steady_clock::now();
+    if (prof) {
+        float a = 0, b = 0;
+        hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
+        iter_ms = a / count;
+        if (time_sampler) hip_check(hipEventElapsedTime(&b, ev[2], ev[3]), "elapsed");
+        sampler_ms = b;
+        sampler_launches = time_sampler ? 1 : 0;
+    }
+    sampled = true;
+    if (ll) {
+        const size_t M = models.size();
+        std::vector<double> hist((size_t)kLLHistory * M);
+        hip_check(hipMemcpy(hist.data(), d_ll_hist, hist.size() * 8, hipMemcpyDeviceToHost), "ll copy");
+        for (int32_t i = 0; i < count; ++i)
+            for (size_t mi = 0; mi < M; ++mi)
+                ll[(size_t)i * M + mi] = hist[(size_t)((first + i) % kLLHistory) * M + mi] / (double)models[mi].num_tokens;
+    }
+    read_stats();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+double Engine::sweep_bytes() const {
+    int64_t n = 0, nnz = 0;
+    for (auto &m : models) n += m.num_tokens;
+    for (int64_t v : unit_token_nnz) nnz += v;
+    return (double)n * (4.0 + 16.0 + 2.0) + 4.0 * (double)nnz + 4.0 * (double)prm.k * (double)num_slots;
+}
+
+// Times one sweep per candidate slot size from the same state (z is saved
+// and restored, counts rebuilt), then keeps the fastest.  Results never
+// depend on the slot size, so tuning leaves the chain unchanged.
+TuneResult Engine::autotune(const std::vector<int32_t> &candidates, bool batch) {
+    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "autotune needs the initial rebuild");
+    hip_check(hipStreamSynchronize(stream2), "sync");
+    std::vector<uint16_t *> saved(units.size());
+    for (size_t u = 0; u < units.size(); ++u) {
+        saved[u] = (uint16_t *)dalloc(std::max<int64_t>(units[u].num_tokens, 1) * 2);
+        hip_check(hipMemcpyAsync(saved[u], units[u].z, units[u].num_tokens * 2, hipMemcpyDeviceToDevice, stream), "z save");
+    }
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    TuneResult r;
+    r.best_ms = 1e30;
+    for (int32_t cand : candidates) {
+        rebuild_slots(cand);
+        set_sweep(1, stream);
+        hip_check(hipEventRecord(e0, stream), "event");
+        if (batch) {
+            sample_all(stream);
+        } else {
+            for (size_t u = 0; u < units.size(); ++u) sample_unit((int32_t)u, stream);
+        }
+        hip_check(hipEventRecord(e1, stream), "event");
+        hip_check(hipEventSynchronize(e1), "event sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        r.tried.emplace_back(prm.slot_tokens, ms);
+        if (ms < r.best_ms) {
+            r.best_ms = ms;
+            r.slot_tokens = prm.slot_tokens;
+        }
+        for (size_t u = 0; u < units.size(); ++u)
+            hip_check(hipMemcpyAsync(units[u].z, saved[u], units[u].num_tokens * 2, hipMemcpyDeviceToDevice, stream),
+                      "z restore");
+    }
+    rebuild_slots(r.slot_tokens);
+    hip_check(hipStreamSynchronize(stream), "sync");
+    for (uint16_t *p : saved) dfree(p);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+}
+
+void Engine::sync() {
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipStreamSynchronize(stream2), "sync");
+}
+
+}  // namespace culda

@@ -1,0 +1,145 @@
+// engine.h -- device-resident models, units and work lists behind the C ABI.
+//
+// Plays the roles of the reference's DocChunk device buffers
+// (model/doc_chunk.h:43-57), ModelPhiGPU (model/model_phi_gpu.h:17-44) and
+// ModelThetaChunk (model/model_theta_chunk.h:20-59), owned by one engine
+// instead of by model objects that each call cudaMalloc.  One engine holds
+// one corpus (culda_ctx: a model with one unit per data_chunk) or a batch
+// of independent corpora (culda_batch: one model per corpus).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <vector>
+
+#include "corpus.h"
+#include "layout.h"
+
+namespace culda {
+
+void hip_check(hipError_t e, const char *what);
+
+// A phi work list with its workgroup packets and the hot-word rows to zero.
+struct PhiList {
+    PhiSlot *slots = nullptr;
+    Packet *packets = nullptr;
+    int2 *zero_rows = nullptr;  // (model, word)
+    int32_t num_slots = 0, num_packets = 0, num_zero_rows = 0;
+};
+
+struct ModelHost {
+    int32_t V = 0, K = 0, num_docs = 0;
+    float alpha = 0, beta = 0;
+    uint64_t seed = 0;
+    uint32_t count_bits = 0;
+    int32_t unit_begin = 0, unit_end = 0;
+    int32_t slot_begin = 0, slot_end = 0;  // contiguous in the slot array
+    int64_t num_tokens = 0;
+    std::vector<int64_t> doc_len;
+    int32_t *nwk = nullptr, *nk = nullptr;
+    float *den = nullptr;
+    PhiList phi_fused;  // all units of the model in one pass
+};
+
+struct UnitHost {
+    int32_t model = 0;
+    int32_t doc_start = 0, doc_end = 0;
+    int64_t num_tokens = 0, theta_cap = 0;
+    int32_t slot_begin = 0, slot_end = 0;
+    uint32_t *tokens = nullptr;
+    uint16_t *z = nullptr;
+    int64_t *word_idx = nullptr;
+    uint32_t *rev_off = nullptr, *rev_idx = nullptr;
+    DocMeta *meta = nullptr;
+    uint32_t *theta = nullptr;
+    PhiList phi_single;  // this unit alone (chunks-in-turn accumulation)
+    Packet *theta_packets = nullptr;
+    int32_t num_theta_packets = 0;
+    std::vector<int64_t> word_idx_h;
+};
+
+struct TuneResult {
+    int32_t slot_tokens = 0;
+    double best_ms = 0;
+    std::vector<std::pair<int32_t, double>> tried;  // (slot tokens, ms per sweep)
+};
+
+class Engine {
+public:
+    culda_params prm{};  // k, device, slot_tokens, profile, fused_phi (alpha/beta/seed per model)
+    hipStream_t stream = nullptr;   // sweeps, LL, phi
+    hipStream_t stream2 = nullptr;  // theta rebuilds overlapped with later sweeps
+    int32_t cu_count = 256;
+    std::vector<ModelHost> models;
+    std::vector<UnitHost> units;
+    std::vector<const Corpus *> pending;  // corpora added, not yet uploaded
+    std::vector<float> pending_alpha, pending_beta;
+    std::vector<uint64_t> pending_seed;
+    bool uploaded = false, theta_ready = false, phi_ready = false, sampled = false;
+
+    // device tables and work lists
+    ModelDev *d_models = nullptr;
+    UnitDev *d_units = nullptr;
+    SlotDesc *d_slots = nullptr;
+    int32_t *d_order = nullptr;  // slot indices, longest first over all units
+    int32_t num_slots = 0;
+    double *d_ll_partial = nullptr;
+    Packet *d_ll_packets = nullptr;
+    int32_t *d_model_packets = nullptr;
+    double *d_ll_scratch = nullptr;
+    double *d_ll_hist = nullptr;
+    int32_t num_ll_packets = 0;
+    PhiList phi_all;  // every model, fused over its units
+    Packet *d_theta_packets_all = nullptr;
+    int32_t num_theta_packets_all = 0;
+    int32_t *d_nk_all = nullptr;  // M*K
+    unsigned long long *d_stats = nullptr;  // 2 per unit
+    uint32_t *d_sweep = nullptr;
+    uint32_t *d_queue = nullptr;
+    int32_t persistent_groups = 0;  // 0 = per-slot grid
+    hipGraphExec_t graph = nullptr;
+    std::vector<void *> allocations;
+
+    // profiling / stats
+    std::vector<hipEvent_t> ev_sampled;
+    hipEvent_t ev_theta_done = nullptr;
+    hipEvent_t ev[4] = {};
+    double sampler_ms = 0, phi_ms = 0, theta_ms = 0, iter_ms = 0;
+    int32_t sampler_launches = 0;
+    double last_sampler_bytes = 0;
+    std::vector<int64_t> unit_doc_nnz, unit_token_nnz;  // after the last theta rebuild
+
+    ~Engine();
+    void init_device(const culda_params &p);
+    void *dalloc(size_t bytes);
+    void dfree(void *p);
+    int32_t add_corpus(const Corpus &c, float alpha, float beta, uint64_t seed);
+    void upload();
+    void rebuild_slots(int32_t slot_tokens);
+
+    void set_sweep(uint32_t t, hipStream_t s);
+    void init_z();
+    void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s);
+    void rebuild_phi_all(hipStream_t s);
+    void phi_head(hipStream_t s);
+    void rebuild_theta_unit(int32_t unit, hipStream_t s);
+    void rebuild_theta_all(hipStream_t s);
+    void sample_unit(int32_t unit, hipStream_t s);
+    void sample_all(hipStream_t s);
+    void ll_reduce(hipStream_t s);
+    void read_stats();
+    void rebuild_all();
+
+    // culda_ctx iteration: per-chunk sweeps, theta of chunk c overlapped
+    // with the sweep of chunk c + 1
+    void iterate_units(uint32_t sweep, double *ll, double *seconds);
+    // culda_batch iteration: one launch per step for every model
+    void enqueue_batch_iteration(hipStream_t s, hipEvent_t before = nullptr, hipEvent_t after = nullptr);
+    // algorithmic bytes of one sweep over every unit with the current theta
+    double sweep_bytes() const;
+    void iterate_batch(uint32_t first, int32_t count, bool use_graph, double *ll, double *seconds);
+    double model_ll(int32_t model, uint32_t sweep);  // per token, from the history
+    TuneResult autotune(const std::vector<int32_t> &candidates, bool batch);
+    void sync();
+};
+
+}  // namespace culda

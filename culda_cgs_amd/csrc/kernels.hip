@@ -49,6 +49,21 @@ __device__ __forceinline__ float scan16(float x) {
 // Lane 15 of each row broadcast to the whole row (DPP row_newbcast:15).
 __device__ __forceinline__ float row_last(float x) { return dpp_f<0x15F>(x); }
 
+// Copy of a wave-uniform struct held in scalar registers (each dword
+// through readfirstlane), so descriptors read inside a loop that also
+// stores do not occupy vector registers.
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized struct expected");
+    T v = *p;
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // Row-local ballot: bit m set when lane m of this row voted true.
 __device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
     return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
@@ -63,9 +78,11 @@ __global__ __launch_bounds__(256) void init_z_kernel(InitArgs a) {
     const int lane = threadIdx.x & 63;
     if (slot >= a.num_slots) return;
     const SlotDesc s = a.slots[slot];
+    const UnitDev &u = a.units[s.unit];
+    const ModelDev &m = a.models[u.model];
     for (int64_t i = s.begin + lane; i < s.end; i += kWave) {
-        const uint64_t x = token_bits(a.seed, (uint64_t)(i + s.gdelta), 0, kPurposeInit);
-        a.z[i] = (uint16_t)bits_to_topic(x, (uint32_t)a.k);
+        const uint64_t x = token_bits(m.seed, (uint64_t)(i + s.gdelta), 0, kPurposeInit);
+        u.z[i] = (uint16_t)bits_to_topic(x, (uint32_t)m.k);
     }
 }
 
@@ -74,51 +91,62 @@ __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
         p[i] = v;
 }
 
-__global__ void zero_rows_kernel(int32_t *nwk, const int32_t *words, int32_t n, int32_t k) {
+__global__ void add_u32_kernel(uint32_t *p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
+}
+
+// rows[r] = (model, word): zero that model's n_wk row (hot words, before
+// their pieces are added with atomics)
+__global__ void zero_rows_kernel(const ModelDev *models, const int2 *rows, int32_t n) {
     for (int r = blockIdx.x; r < n; r += gridDim.x) {
-        int32_t *row = nwk + (int64_t)words[r] * k;
-        for (int j = threadIdx.x; j < k; j += blockDim.x) row[j] = 0;
+        const int2 mw = rows[r];
+        const ModelDev &m = models[mw.x];
+        int32_t *row = m.nwk + (int64_t)mw.y * m.k;
+        for (int j = threadIdx.x; j < m.k; j += blockDim.x) row[j] = 0;
     }
 }
 
 // ------------------------------------------------------------ phi rebuild
-// LDS histogram per word (one wave per slot), whole rows written with
-// plain 16-byte stores, hot-word pieces added with atomics.  n_k is
-// accumulated per workgroup in LDS and flushed with one atomic per topic.
+// One workgroup per packet (a run of phi slots of one model), one wave per
+// slot: LDS histogram of the word's topics, whole rows written with plain
+// 16-byte stores, hot-word pieces added with atomics.  n_k is accumulated
+// per workgroup in LDS and flushed with one atomic per topic.
 // Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
 // chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
 __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
     __shared__ uint32_t nk_s[kMaxK];
-    const int K = a.k;
+    const Packet pk = a.packets[blockIdx.x];
+    const ModelDev &m = a.models[pk.owner];
+    const int K = m.k;
     for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
-    for (int si = blockIdx.x * kWavesPerGroup + wave; si < a.num_slots; si += gridDim.x * kWavesPerGroup) {
+    for (int si = pk.first + wave; si < pk.last; si += kWavesPerGroup) {
         const PhiSlot sl = a.slots[si];
-        if (sl.chunk < 0 && a.accumulate) {
-            // nothing to add when the word is absent from the launch's chunks
+        if (!sl.split && a.accumulate) {
+            // nothing to add when the word is absent from these units
             int64_t n = 0;
-            for (int c = a.chunk_begin; c < a.chunk_end; ++c)
-                n += a.chunks[c].word_idx[sl.word + 1] - a.chunks[c].word_idx[sl.word];
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u)
+                n += a.units[u].word_idx[sl.word + 1] - a.units[u].word_idx[sl.word];
             if (n == 0) continue;
         }
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
-        if (sl.chunk < 0) {
-            for (int c = a.chunk_begin; c < a.chunk_end; ++c) {
-                const uint16_t *z = a.chunks[c].z;
-                const int64_t b = a.chunks[c].word_idx[sl.word], e = a.chunks[c].word_idx[sl.word + 1];
+        if (!sl.split) {
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+                const uint16_t *z = a.units[u].z;
+                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
                 for (int64_t i = b + lane; i < e; i += kWave) atomicAdd(&h[z[i]], 1u);
             }
         } else {
-            const uint16_t *z = a.chunks[sl.chunk].z;
+            const uint16_t *z = a.units[sl.unit_lo].z;
             for (int64_t i = sl.begin + lane; i < sl.end; i += kWave) atomicAdd(&h[z[i]], 1u);
         }
         wave_sync();
-        int32_t *row = a.nwk + (int64_t)sl.word * K;
-        if (sl.chunk < 0) {
+        int32_t *row = m.nwk + (int64_t)sl.word * K;
+        if (!sl.split) {
             for (int k = lane * 4; k < K; k += 4 * kWave) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(h + k);
                 int4 o = make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
@@ -146,15 +174,18 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         const uint32_t v = nk_s[k];
-        if (v) atomicAdd(a.nk + k, (int)v);
+        if (v) atomicAdd(m.nk + k, (int)v);
     }
 }
 
-// den_k = fl(fl(n_k) + fl(beta * fl(V)))   (kernel/lda_phi_kernel.cu:26)
-__global__ void den_kernel(const int32_t *nk, float *den, int32_t k, float beta, int32_t num_words) {
-    const float bv = beta * (float)num_words;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
-        den[i] = (float)nk[i] + bv;
+// den_k = fl(fl(n_k) + fl(beta * fl(V))) for every model of the table
+// (kernel/lda_phi_kernel.cu:26); one workgroup per model.
+__global__ void den_kernel(const ModelDev *models, int32_t num_models) {
+    for (int mi = blockIdx.x; mi < num_models; mi += gridDim.x) {
+        const ModelDev &m = models[mi];
+        const float bv = m.beta * (float)m.num_words;
+        for (int i = threadIdx.x; i < m.k; i += blockDim.x) m.den[i] = (float)m.nk[i] + bv;
+    }
 }
 
 // Materialised head, for downloads and save hooks only (the sampler forms
@@ -165,66 +196,75 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 }
 
 // ------------------------------------------------------------ theta rebuild
-// One wave per doc: LDS histogram over the doc's tokens (gathered through
-// rev_idx), then an ascending ballot compaction of the nonzero topics.
+// One workgroup per packet (a run of docs of one unit), one wave per doc:
+// LDS histogram over the doc's tokens (gathered through rev_idx), then an
+// ascending ballot compaction of the nonzero topics.
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
-    const int K = a.k;
+    const Packet pk = a.packets[blockIdx.x];
+    const UnitDev &un = a.units[pk.owner];
+    const ModelDev &m = a.models[un.model];
+    const int K = m.k;
+    const uint32_t cb = m.count_bits;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
     unsigned long long s_nnz = 0, s_tok = 0;
-    for (int d = blockIdx.x * kWavesPerGroup + wave; d < a.num_docs; d += gridDim.x * kWavesPerGroup) {
+    for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
-        const uint32_t b = a.rev_off[d], e = a.rev_off[d + 1];
+        const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
         // four index loads, then four z gathers, in flight per lane
         for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
             uint32_t t[4];
             uint16_t zz[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? a.rev_idx[i0 + q * kWave] : 0xffffffffu;
+            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? un.rev_idx[i0 + q * kWave] : 0xffffffffu;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? a.z[t[q]] : (uint16_t)0;
+            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? un.z[t[q]] : (uint16_t)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
         }
         wave_sync();
-        const uint32_t start = a.meta[d].start;
+        const uint32_t start = un.meta[d].start;
         uint32_t base = 0;
         for (int k0 = 0; k0 < K; k0 += kWave) {
             const int k = k0 + lane;
             const uint32_t v = k < K ? h[k] : 0u;
-            const unsigned long long m = __ballot(v != 0);
-            const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (v) a.theta[start + pos] = ((uint32_t)k << (a.count_bits + 2)) | v;
-            base += (uint32_t)__popcll(m);
+            const unsigned long long msk = __ballot(v != 0);
+            const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+            if (v) un.theta[start + pos] = ((uint32_t)k << (cb + 2)) | v;
+            base += (uint32_t)__popcll(msk);
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
-        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) a.theta[start + base + lane] = 0u;
-        if (lane == 0) a.meta[d].nnz = base;
+        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
+        if (lane == 0) un.meta[d].nnz = base;
         s_nnz += base;
         s_tok += (unsigned long long)base * (e - b);
         wave_sync();
     }
     if (lane == 0 && s_nnz) {
-        atomicAdd(&a.stats[0], s_nnz);
-        atomicAdd(&a.stats[1], s_tok);
+        atomicAdd(&un.stats[0], s_nnz);
+        atomicAdd(&un.stats[1], s_tok);
     }
 }
 
 // ------------------------------------------------------------ sampler
-// One wave per slot (one word's token range).  The wave stages the word's
-// head row phi_k in LDS (computed from n_wk), builds the Q-bucket block
-// prefix, then takes its tokens 64 at a time: lane l loads the doc id,
-// doc meta and Philox uniform of token l; the batch is sorted by nnz so
-// that the four 16-lane rows, which process four tokens side by side,
-// run S loops of similar length.  Theta blocks are prefetched one block
-// ahead; scans are DPP row operations.  z and the FP64 LL term are
+// One wave per slot (one word's token range of one unit).  The wave stages
+// the word's head row phi_k in LDS (computed from n_wk), builds the
+// Q-bucket block prefix, then takes its tokens 64 at a time: lane l loads
+// the doc id, doc meta and Philox uniform of token l; the batch is sorted
+// by nnz so that the four 16-lane rows, which process four tokens side by
+// side, run S loops of similar length.  Theta blocks are prefetched a
+// group ahead; scans are DPP row operations.  z and the FP64 LL term are
 // written per token.  The processing order never changes a result: every
 // token depends only on the frozen counts and its own Philox draw.
+// Persistent mode (queue != null): each workgroup takes list positions from
+// an atomic counter until the list is exhausted; otherwise workgroup b
+// takes position b.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
+template <bool kPersistent>
 __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     // one wave per workgroup: every LDS array sits at a fixed address, so
     // the phi lookup is a shift of the packed entry plus a constant offset
@@ -234,20 +274,32 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     __shared__ uint32_t zres[kWave];
     __shared__ float sqres[kWave];
     const int lane = threadIdx.x;
-    const int slot = blockIdx.x;
-    const SlotDesc s = a.slots[slot];
-    const int K = a.k;
-    const float alpha = a.alpha, beta = a.beta;
-    const uint32_t cb = a.count_bits, cmask = (1u << cb) - 1u;
+    const uint32_t sweep = *a.sweep;
+    int pos = blockIdx.x;
+    while (true) {
+    if (kPersistent) {
+        uint32_t q = 0;
+        if (lane == 0) q = atomicAdd(a.queue, 1u);
+        pos = (int)__builtin_amdgcn_readfirstlane(q);  // wave-uniform: keeps descriptors in SGPRs
+    }
+    if (pos >= a.num_slots) break;
+    const int slot = a.order ? (int)__builtin_amdgcn_readfirstlane(a.order[pos]) : pos;
+    const SlotDesc s = uniform_copy(a.slots + slot);
+    const UnitDev un = uniform_copy(a.units + s.unit);
+    const ModelDev md = uniform_copy(a.models + un.model);
+    const int K = md.k;
+    const float alpha = md.alpha, beta = md.beta;
+    const uint32_t cb = md.count_bits, cmask = (1u << cb) - 1u;
+    const uint32_t *theta = un.theta;
     // phi of a packed theta entry e (byte offset e >> cb) and its count
 #define PHI_OF(e) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + ((e) >> cb)))
 #define COUNT_OF(e) ((float)((e)&cmask))
 
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
-    const int32_t *row = a.nwk + (int64_t)s.word * K;
+    const int32_t *row = md.nwk + (int64_t)s.word * K;
     for (int k = lane * 4; k < K; k += 4 * kWave) {
         const int4 c = *reinterpret_cast<const int4 *>(row + k);
-        const float4 dn = *reinterpret_cast<const float4 *>(a.den + k);
+        const float4 dn = *reinterpret_cast<const float4 *>(md.den + k);
         ph[k + 0] = ((float)c.x + beta) / dn.x;
         ph[k + 1] = ((float)c.y + beta) / dn.y;
         ph[k + 2] = ((float)c.z + beta) / dn.z;
@@ -287,12 +339,12 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
         float u = 0.0f;
         if (valid) {
             const int64_t i = base + lane;
-            const uint32_t doc = a.tokens[i];
-            const uint4 m = *reinterpret_cast<const uint4 *>(a.meta + doc);
+            const uint32_t doc = un.tokens[i];
+            const uint4 m = *reinterpret_cast<const uint4 *>(un.meta + doc);
             mstart = m.x;
             mnnz = m.y;
             mnd = m.z;
-            u = bits_to_uniform(token_bits(a.seed, (uint64_t)(i + s.gdelta), a.sweep, kPurposeSweep));
+            u = bits_to_uniform(token_bits(md.seed, (uint64_t)(i + s.gdelta), sweep, kPurposeSweep));
         }
         // bitonic sort of (valid, nnz, lane), descending: positions
         // 0..ntok-1 hold the valid tokens, longest theta rows first
@@ -328,7 +380,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 // nnz, so its extra products are exact zeros (R + 0 == R).
                 const uint32_t nb = (nnz + 15u) >> 4;
                 const uint32_t ng = (nb + 3u) >> 2;
-                const uint32_t *tp = a.theta + start + lr;
+                const uint32_t *tp = theta + start + lr;
                 uint32_t cur[4], nxt[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) cur[q] = tp[q << 4];
@@ -388,14 +440,14 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 }
                 uint32_t topic;
                 if (bsel < 0) {
-                    topic = sb ? (a.theta[start + nnz - 1] >> (cb + 2)) : (uint32_t)(K - 1);
+                    topic = sb ? (theta[start + nnz - 1] >> (cb + 2)) : (uint32_t)(K - 1);
                 } else {
                     const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
                     uint32_t e = 0;
                     float v;
                     bool ok = true;
                     if (sb) {
-                        e = a.theta[start + j];
+                        e = theta[start + j];
                         ok = j < nnz;
                         v = ok ? COUNT_OF(e) * PHI_OF(e) : 0.0f;
                     } else {
@@ -422,8 +474,8 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
         }
         wave_sync();
         if (lane < ntok) {
-            a.z[base + tok] = (uint16_t)zres[lane];
-            ll += log((double)sqres[lane] / ((double)pnd + a.k_alpha));
+            un.z[base + tok] = (uint16_t)zres[lane];
+            ll += log((double)sqres[lane] / ((double)pnd + md.k_alpha));
         }
         wave_sync();
     }
@@ -432,27 +484,38 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     if (lane == 0) a.ll_partial[slot] = ll;
 #undef PHI_OF
 #undef COUNT_OF
+    if (!kPersistent) break;
+    wave_sync();  // LDS is reused by the next slot
+    }
 }
 
-// Fixed-order FP64 sum of the per-slot partials in two launches: stage 1
-// gives workgroup g the contiguous range g*per .. (g+1)*per, stage 2 sums
-// the stage-1 results.  The order depends only on n, so every run adds the
-// same values in the same order.
+// Fixed-order FP64 sums of the per-slot LL partials, per model, in two
+// launches: stage 1 sums each packet (a contiguous slot range of one
+// model), stage 2 sums a model's packets in order and stores the total at
+// history[sweep % kLLHistory][model].  The order depends only on the slot
+// lists, so every run adds the same values in the same order.
 // Reference: LDATrainPerplexityReduce1/2 (kernel/lda_train_kernel.cu:273-321),
 // which truncated partials to int.
-__global__ __launch_bounds__(256) void ll_reduce_kernel(const double *p, int64_t n, int64_t per, double *out) {
+__global__ __launch_bounds__(256) void ll_stage1_kernel(LLArgs a) {
     __shared__ double s[256];
-    const int64_t b = (int64_t)blockIdx.x * per;
-    const int64_t e = b + per < n ? b + per : n;
+    const Packet pk = a.packets[blockIdx.x];
     double acc = 0.0;
-    for (int64_t i = b + threadIdx.x; i < e; i += 256) acc += p[i];
+    for (int i = pk.first + threadIdx.x; i < pk.last; i += 256) acc += a.partial[i];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) a.scratch[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(64) void ll_stage2_kernel(LLArgs a) {
+    const int mi = blockIdx.x;
+    if (threadIdx.x != 0 || mi >= a.num_models) return;
+    double acc = 0.0;
+    for (int p = a.model_packets[2 * mi]; p < a.model_packets[2 * mi + 1]; ++p) acc += a.scratch[p];
+    a.history[(size_t)(*a.sweep % kLLHistory) * a.num_models + mi] = acc;
 }
 
 // Joint log-likelihood pieces: sum over nonzero n_wk of
@@ -519,20 +582,26 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero_rows(int32_t *nwk, const int32_t *words, int32_t n, int32_t k, hipStream_t s) {
+hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(add_u32_kernel, dim3(1), dim3(64), 0, s, p, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(const ModelDev *models, const int2 *rows, int32_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(n, 1, 4096)), dim3(256), 0, s, nwk, words, n, k);
+    hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(n, 1, 4096)), dim3(256), 0, s, models, rows, n);
     return hipGetLastError();
 }
 
-hipError_t launch_phi_rebuild(const PhiArgs &a, int32_t grid, hipStream_t s) {
-    if (a.num_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(phi_rebuild_kernel, dim3(grid_for(a.num_slots, kWavesPerGroup, grid)), dim3(256), 0, s, a);
+hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s) {
+    if (a.num_packets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(phi_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_den(const int32_t *nk, float *den, int32_t k, float beta, int32_t num_words, hipStream_t s) {
-    hipLaunchKernelGGL(den_kernel, dim3(grid_for(k, 256, 64)), dim3(256), 0, s, nk, den, k, beta, num_words);
+hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s) {
+    if (num_models <= 0) return hipSuccess;
+    hipLaunchKernelGGL(den_kernel, dim3(grid_for(num_models, 1, 4096)), dim3(256), 0, s, models, num_models);
     return hipGetLastError();
 }
 
@@ -544,24 +613,25 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
     return hipGetLastError();
 }
 
-hipError_t launch_theta_rebuild(const ThetaArgs &a, int32_t grid, hipStream_t s) {
-    if (a.num_docs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(theta_rebuild_kernel, dim3(grid_for(a.num_docs, kWavesPerGroup, grid)), dim3(256), 0, s, a);
+hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s) {
+    if (a.num_packets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(theta_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_sample(const SampleArgs &a, hipStream_t s) {
+hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sample_kernel, dim3(a.num_slots), dim3(kWave * kSampleWaves), 0, s, a);
+    if (a.queue)
+        hipLaunchKernelGGL(sample_kernel<true>, dim3(grid), dim3(kWave), 0, s, a);
+    else
+        hipLaunchKernelGGL(sample_kernel<false>, dim3(a.num_slots), dim3(kWave), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_ll_reduce(const double *partial, int64_t n, double *scratch, double *out, hipStream_t s) {
-    // scratch holds kLLGroups doubles
-    const int64_t per = (n + kLLGroups - 1) / kLLGroups > 0 ? (n + kLLGroups - 1) / kLLGroups : 1;
-    hipLaunchKernelGGL(ll_reduce_kernel, dim3(kLLGroups), dim3(256), 0, s, partial, n, per, scratch);
-    hipLaunchKernelGGL(ll_reduce_kernel, dim3(1), dim3(256), 0, s, (const double *)scratch, (int64_t)kLLGroups,
-                       (int64_t)kLLGroups, out);
+hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s) {
+    if (a.num_packets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ll_stage1_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ll_stage2_kernel, dim3(a.num_models), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

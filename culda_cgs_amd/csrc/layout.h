@@ -1,42 +1,39 @@
-// layout.h -- HBM layout shared by the host context and the gfx950 kernels.
+// layout.h -- HBM layout shared by the host engine and the gfx950 kernels.
 //
-// Per chunk (all chunks stay resident):
-//   tokens[N_c]   u32  chunk-local doc id of each token, word-major (the
+// Device state is a set of *models* (one per corpus: n_wk, n_k, head
+// denominators, hyper-parameters, Philox seed) and *units* (one per
+// data_chunk of a corpus: its tokens, z, doc-major index and theta slabs),
+// described by two device tables.  A single corpus with C chunks is one
+// model with C units; a batch of corpora is several models.  Work lists
+// name a unit (and through it a model), so one launch can serve a single
+// chunk, all chunks of a corpus, or every corpus of a batch.
+//
+// Per unit (all units stay resident):
+//   tokens[N_u]   u32  unit-local doc id of each token, word-major (the
 //                      reference's wordTokens, doc_chunk.h:39, rebased)
-//   z[N_c]        u16  topic of each token, word-major (wordTopics)
-//   word_idx[V+1] i64  chunk-local token offsets of each word (wordIndices)
-//   rev_off[D_c+1] u32 doc-major offsets = cumulative doc lengths
-//   rev_idx[N_c]  u32  word-major position of each token, doc-major
+//   z[N_u]        u16  topic of each token, word-major (wordTopics)
+//   word_idx[V+1] i64  unit-local token offsets of each word (wordIndices)
+//   rev_off[D_u+1] u32 doc-major offsets = cumulative doc lengths
+//   rev_idx[N_u]  u32  word-major position of each token, doc-major
 //                      (docRevIndices / docRevIdx, doc_chunk.cpp:225-255)
-//   meta[D_c]     DocMeta: theta slab start, nnz, n_d
-//   theta[cap_c]  u32  packed ((4 * topic) << count_bits) | count,
+//   meta[D_u]     DocMeta: theta slab start, nnz, n_d
+//   theta[cap_u + 64] u32 packed ((4 * topic) << count_bits) | count,
 //                      count_bits = 30 - ceil(log2 K) (20 at K=1024), so
 //                      e >> count_bits is the byte offset of phi_topic in
-//                      LDS; ascending topic within a doc; slab capacity of doc d is
-//                      roundup16(min(n_d, K)) (model_theta.cpp:49-56 uses
-//                      roundup32 for 32-lane warps)
-// Global: n_wk[V*K] i32 (row per word), n_k[K] i32, den[K] f32.
+//                      LDS; ascending topic within a doc; slab capacity of
+//                      doc d is roundup16(min(n_d, K)) (model_theta.cpp:49-56
+//                      uses roundup32 for 32-lane warps); 64 padding entries
+// Per model: n_wk[V*K] i32 (row per word), n_k[K] i32, den[K] f32.
 #pragma once
 #include <stdint.h>
 
 namespace culda {
 
-constexpr int kWave = 64;       // CDNA wavefront
-constexpr int kBlock = 16;      // FP32 prefix block = one DPP row
-constexpr int kMaxK = 1024;     // Q block totals scanned by one wave
-constexpr int kWavesPerGroup = 4;   // rebuild kernels
-constexpr int kSampleWaves = 1;     // sampler: one wave per workgroup
-
-// Sampler work unit: tokens [begin, end) of one word, one wavefront.
-// gdelta turns a chunk-local token index into the chunking-invariant
-// global id: g = i + gdelta.
-struct SlotDesc {
-    int32_t word;
-    int32_t pad;
-    int64_t begin;
-    int64_t end;
-    int64_t gdelta;
-};
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kBlock = 16;         // FP32 prefix block = one DPP row
+constexpr int kMaxK = 1024;        // Q block totals scanned by one wave
+constexpr int kWavesPerGroup = 4;  // rebuild kernels
+constexpr int kLLHistory = 1024;   // LL kept per model for this many sweeps
 
 struct DocMeta {
     uint32_t start;  // theta slab start (entries)
@@ -45,70 +42,112 @@ struct DocMeta {
     uint32_t pad;
 };
 
-// Phi rebuild work unit.  chunk < 0: every token of `word` in the launch's
-// chunk range, written as a whole row.  chunk >= 0: tokens [begin, end) of
-// chunk `chunk`, added to a pre-zeroed row with atomics (hot words).
-struct PhiSlot {
-    int32_t word;
-    int32_t chunk;
-    int64_t begin;
-    int64_t end;
-};
-
-struct ChunkPtrs {
-    const uint16_t *z;
-    const int64_t *word_idx;
-};
-
-struct SampleArgs {
-    const SlotDesc *slots;
-    const uint32_t *tokens;
-    uint16_t *z;
-    const DocMeta *meta;
-    const uint32_t *theta;
-    const int32_t *nwk;
-    const float *den;
-    double *ll_partial;
+struct ModelDev {
+    int32_t *nwk;
+    int32_t *nk;
+    float *den;
     uint64_t seed;
     double k_alpha;  // (double)K * (double)alpha
     float alpha;
     float beta;
-    int32_t num_slots;
     int32_t k;
+    int32_t num_words;
     uint32_t count_bits;
-    uint32_t sweep;
+    int32_t pad;
 };
 
-struct PhiArgs {
-    const PhiSlot *slots;
-    const ChunkPtrs *chunks;
-    int32_t *nwk;
-    int32_t *nk;
-    int32_t num_slots;
-    int32_t k;
-    int32_t chunk_begin;
-    int32_t chunk_end;
-    int32_t accumulate;
-};
-
-struct ThetaArgs {
+struct UnitDev {
+    const uint32_t *tokens;
+    uint16_t *z;
+    const int64_t *word_idx;
     const uint32_t *rev_off;
     const uint32_t *rev_idx;
-    const uint16_t *z;
     DocMeta *meta;
     uint32_t *theta;
-    unsigned long long *stats;  // [0] += sum nnz, [1] += sum n_d * nnz
+    unsigned long long *stats;  // [0] += sum nnz, [1] += sum n_d * nnz (last rebuild)
+    int32_t model;
     int32_t num_docs;
-    int32_t k;
-    uint32_t count_bits;
+};
+
+// Sampler / z-init work item: tokens [begin, end) of one word of one unit.
+// gdelta turns a unit-local token index into the chunking-invariant global
+// id g = i + gdelta (the token's index in the one-chunk word-major layout).
+struct SlotDesc {
+    int32_t word;
+    int32_t unit;
+    int64_t begin;
+    int64_t end;
+    int64_t gdelta;
+};
+
+// Phi work item.  split == 0: every token of `word` in units
+// [unit_lo, unit_hi) of the model, written as a whole row (or added to it
+// when accumulating).  split == 1: tokens [begin, end) of unit unit_lo,
+// added with atomics to a row zeroed beforehand (hot words).
+struct PhiSlot {
+    int32_t word;
+    int32_t unit_lo;
+    int32_t unit_hi;
+    int32_t split;
+    int64_t begin;
+    int64_t end;
+};
+
+// Items [first, last) of a work list, handled by one workgroup; all of them
+// belong to `owner` (a model for phi and LL, a unit for theta).
+struct Packet {
+    int32_t first;
+    int32_t last;
+    int32_t owner;
+    int32_t pad;
+};
+
+struct SampleArgs {
+    const SlotDesc *slots;
+    const int32_t *order;   // optional permutation of slot indices (balance)
+    const UnitDev *units;
+    const ModelDev *models;
+    double *ll_partial;     // one per slot index
+    const uint32_t *sweep;  // sweep index t >= 1, read on the device
+    uint32_t *queue;        // persistent mode: next list position; else null
+    int32_t num_slots;
+    int32_t pad;
 };
 
 struct InitArgs {
     const SlotDesc *slots;
-    uint16_t *z;
-    uint64_t seed;
+    const UnitDev *units;
+    const ModelDev *models;
     int32_t num_slots;
-    int32_t k;
+    int32_t pad;
+};
+
+struct PhiArgs {
+    const PhiSlot *slots;
+    const Packet *packets;
+    const UnitDev *units;
+    const ModelDev *models;
+    int32_t num_packets;
+    int32_t accumulate;
+};
+
+struct ThetaArgs {
+    const Packet *packets;  // owner = unit, [first, last) = its docs
+    const UnitDev *units;
+    const ModelDev *models;
+    int32_t num_packets;
+    int32_t pad;
+};
+
+struct LLArgs {
+    const double *partial;  // per slot index
+    const Packet *packets;  // owner = model, [first, last) = slot indices
+    const int32_t *model_packets;  // per model: [begin, end) into packets (2 ints)
+    double *scratch;        // one per packet
+    double *history;        // [kLLHistory][num_models]
+    const uint32_t *sweep;
+    int32_t num_packets;
+    int32_t num_models;
 };
 
 }  // namespace culda

@@ -13,8 +13,9 @@
  *   - Plain pointers and sizes only.  Host arrays passed in are owned by
  *     the caller and copied; arrays handed out by culda_corpus_* stay
  *     owned by the corpus object.
- *   - A culda_ctx owns all device memory and one HIP stream.  A context is
- *     used by one thread at a time; different contexts are independent.
+ *   - A culda_ctx (one corpus) or culda_batch (many corpora) owns all its
+ *     device memory and two HIP streams.  A handle is used by one thread
+ *     at a time; different handles are independent.
  *
  * Reference interfaces replaced are cited as file:line relative to the
  * CuLDA_CGS source tree (src_culda/, src_format/).
@@ -273,10 +274,69 @@ int culda_get_stats(culda_ctx *ctx, culda_stats *out);
  * staged), summed over chunks. */
 int culda_sampler_bytes(culda_ctx *ctx, double *bytes);
 
+/* Autotuner result: the sampler's tokens per wave slot (and, for batches,
+ * the persistent grid) chosen by timing one sweep per candidate from the
+ * same state.  z, counts and every later result are unchanged by tuning:
+ * the sampler's output does not depend on the launch shape. */
+typedef struct culda_tune {
+    int32_t slot_tokens;          /* chosen tokens per wave slot             */
+    int32_t persistent_groups;    /* batches: chosen workgroups (0 = one per slot) */
+    int32_t num_tried;
+    int32_t pad;
+    double  best_ms;              /* sampler time of the chosen candidate    */
+    int32_t tried_slot_tokens[16];
+    double  tried_ms[16];
+} culda_tune;
+
+/* Time slot sizes 256..4096 for this corpus and keep the fastest.  Needs
+ * the initial rebuild (culda_rebuild_all). */
+int culda_autotune(culda_ctx *ctx, culda_tune *out);
+
 /* STREAM-style device copy of n_bytes (multiple of 16) between two device
  * buffers owned by the context, timed with HIP events: achievable HBM
  * bandwidth for the roofline.  gbps = 2*n_bytes/time. */
 int culda_stream_copy(culda_ctx *ctx, int64_t n_bytes, int32_t repeats, double *gbps);
+
+/* ------------------------------------------------------------------------
+ * Batch: many independent corpora sampled together (no counterpart in the
+ * reference, which runs one corpus per process).  Each corpus is its own
+ * model (counts, alpha, beta, seed); one iteration of the whole batch is a
+ * single sampler launch over every corpus's work (a persistent launch with
+ * an atomic work queue when set), one LL, one phi, one head and one theta
+ * launch, and may be captured once as a HIP graph and replayed.  Every
+ * corpus's chain is bit-identical to the same corpus run alone with
+ * culda_ctx and the same k, alpha, beta and seed.
+ * ---------------------------------------------------------------------- */
+typedef struct culda_batch culda_batch;
+
+/* params->k, device, slot_tokens, profile are used; alpha, beta and seed
+ * are given per corpus. */
+int culda_batch_create(const culda_params *params, culda_batch **out);
+void culda_batch_destroy(culda_batch *batch);
+/* The corpus must stay alive until culda_batch_upload returns. */
+int culda_batch_add(culda_batch *batch, const culda_corpus *corpus, float alpha, float beta, uint64_t seed,
+                    int32_t *model);
+int culda_batch_upload(culda_batch *batch);
+/* z init and the initial rebuild of every model. */
+int culda_batch_init(culda_batch *batch);
+/* groups > 0: persistent sampler with that many workgroups taking work
+ * from an atomic queue; 0: one workgroup per work item. */
+int culda_batch_set_persistent(culda_batch *batch, int32_t groups);
+/* Iterations first .. first+count-1 (count <= 1024).  ll[i * M + m] =
+ * LL/token of model m at iteration first+i; use_graph = 1 captures one
+ * iteration as a HIP graph (once) and replays it. */
+int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, int32_t count, int32_t use_graph,
+                        double *ll, double *seconds);
+int culda_batch_autotune(culda_batch *batch, culda_tune *out);
+int culda_batch_get_z(culda_batch *batch, int32_t model, int32_t chunk, uint16_t *z, int64_t n);
+int culda_batch_get_phi(culda_batch *batch, int32_t model, int32_t *n_wk, int32_t *n_k);
+int culda_batch_info(culda_batch *batch, int32_t *num_models, int64_t *num_tokens, int64_t *num_slots);
+/* With params->profile: sampler_ms = the batched sampler launch of the last
+ * eager (use_graph = 0) iteration, other_ms = mean ms per iteration of the
+ * last culda_batch_iterate call; counts as for culda_get_stats. */
+int culda_batch_get_stats(culda_batch *batch, culda_stats *out);
+/* Algorithmic bytes of that timed sampler launch (as culda_sampler_bytes). */
+int culda_batch_sampler_bytes(culda_batch *batch, double *bytes);
 
 #ifdef __cplusplus
 }
