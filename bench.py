@@ -248,14 +248,14 @@ def run_batch(cg, args):
 
 
 def cpu_baseline_batch(b, corpora, K, alpha, beta, seed, next_it, threads):
-    """The C oracle timed on the first 4 corpora, one iteration each, from
-    the GPU's current state."""
+    """The C oracle timed on every corpus of the batch, one iteration each,
+    from the GPU's current state."""
     import oracle
 
     threads = threads or min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
     n, dt = 0, 0.0
-    for m in range(4):
+    for m in range(len(corpora)):
         c = corpora[m]
         info = c.info()
         _, end, wi, wt = c.chunk(0)
@@ -268,7 +268,7 @@ def cpu_baseline_batch(b, corpora, K, alpha, beta, seed, next_it, threads):
         n += info.num_tokens
     log(f"cpu oracle: {n} tokens in {dt:.1f}s on {threads} threads")
     return {"value": n / dt, "unit": "tokens/s", "cores": threads, "kind": "port",
-            "sample": f"one full iteration of the first 4 corpora ({n} tokens) from the GPU's state"}
+            "sample": f"one full iteration of all {len(corpora)} corpora ({n} tokens) from the GPU's state"}
 
 
 def cpu_baseline(s, corpus, info, K, alpha, beta, seed, next_it, threads):

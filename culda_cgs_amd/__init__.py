@@ -115,6 +115,9 @@ _SIGS = {
     "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "culda_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
+    "culda_corpus_heldout_split": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
+    "culda_heldout_set_eval": (C.c_int, [_P, _P]),
+    "culda_heldout_ll": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_double)]),
     "culda_batch_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
     "culda_batch_destroy": (None, [_P]),
     "culda_batch_add": (C.c_int, [_P, _P, C.c_float, C.c_float, C.c_uint64, C.POINTER(C.c_int32)]),
@@ -197,6 +200,14 @@ class Corpus:
         h = C.c_void_p()
         _check(lib().culda_corpus_generate(C.byref(p), C.byref(h)))
         return cls(h.value)
+
+    def heldout_split(self, every=10, train_chunks=1):
+        """(train, observed, eval) corpora for document completion: docs
+        d % every == every - 1 held out, their tokens alternating observed /
+        evaluated in ascending word order."""
+        t, o, e = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().culda_corpus_heldout_split(self._h, every, train_chunks, C.byref(t), C.byref(o), C.byref(e)))
+        return Corpus(t.value), Corpus(o.value), Corpus(e.value)
 
     def write(self, prefix):
         _check(lib().culda_corpus_write(self._h, str(prefix).encode()))
@@ -368,6 +379,19 @@ class Sampler:
     def stream_copy(self, n_bytes, repeats=10):
         v = C.c_double()
         _check(lib().culda_stream_copy(self._h, n_bytes, repeats, C.byref(v)))
+        return v.value
+
+    def set_eval(self, eval_corpus: Corpus):
+        """This sampler holds held-out docs' observed halves; give it the
+        evaluated halves."""
+        _check(lib().culda_heldout_set_eval(self._h, eval_corpus._h))
+        self.eval_corpus = eval_corpus
+
+    def heldout_ll(self, train: "Sampler", fold_in_sweeps=10):
+        """Document-completion LL/token of the evaluated tokens under the
+        training sampler's current counts."""
+        v = C.c_double()
+        _check(lib().culda_heldout_ll(train._h, self._h, fold_in_sweeps, C.byref(v)))
         return v.value
 
     def autotune(self):

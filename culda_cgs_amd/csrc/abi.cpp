@@ -580,8 +580,31 @@ extern "C" int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, 
     BATCH_CALL(e.iterate_batch((uint32_t)std::max(first_iteration, 0), count, use_graph != 0, ll, seconds));
 }
 
+// ====================================================== held-out LL
+
+extern "C" int culda_heldout_set_eval(culda_ctx *fold, const culda_corpus *eval) {
+    return guarded([&] {
+        if (!eval) fail(CULDA_ERR_ARG, "null corpus");
+        eng(fold).set_eval(corpus_ref(eval));
+        return CULDA_OK;
+    });
+}
+
+extern "C" int culda_heldout_ll(culda_ctx *train, culda_ctx *fold, int32_t fold_in_sweeps, double *ll_per_token) {
+    return guarded([&] {
+        if (!ll_per_token) fail(CULDA_ERR_ARG, "null output");
+        if (train == fold) fail(CULDA_ERR_ARG, "training and held-out contexts must differ");
+        Engine &t = eng(train);
+        Engine &f = eng(fold);
+        if (t.prm.device != f.prm.device) fail(CULDA_ERR_ARG, "contexts on different devices");
+        *ll_per_token = f.heldout_ll(t, fold_in_sweeps);
+        return CULDA_OK;
+    });
+}
+
 // Tries each slot size with one workgroup per slot and with a persistent
-// grid of 4 and 8 workgroups per CU; keeps the fastest pair.
+// grid of all (and half) the sampler workgroups the device holds at once;
+// keeps the fastest pair.
 extern "C" int culda_batch_autotune(culda_batch *batch, culda_tune *out) {
     BATCH_CALL({
         if (!out) fail(CULDA_ERR_ARG, "null output");
@@ -589,7 +612,9 @@ extern "C" int culda_batch_autotune(culda_batch *batch, culda_tune *out) {
         best.best_ms = 1e30;
         int32_t best_groups = 0;
         std::vector<std::pair<int32_t, double>> all;
-        for (int32_t groups : {0, 4 * e.cu_count, 8 * e.cu_count}) {
+        const int32_t per_cu = std::max(1, sample_blocks_per_cu());
+        const int32_t resident = per_cu * e.cu_count;
+        for (int32_t groups : {0, resident, std::max(1, resident / 2)}) {
             e.persistent_groups = groups;
             TuneResult r = e.autotune(kSlotCandidates, true);
             for (auto &t : r.tried) all.push_back(t);

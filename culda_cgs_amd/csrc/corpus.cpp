@@ -11,6 +11,7 @@
 #include <cerrno>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <unordered_map>
 
@@ -277,6 +278,61 @@ void format_text(const std::string &input, const std::string &prefix, int32_t nu
     write_corpus(c, prefix);
 }
 
+// Doc-major word ids of every token: the tokens of doc d, in ascending
+// word order, at [doc_cum[d-1], doc_cum[d]).
+static std::vector<int32_t> doc_major_words(const Corpus &c) {
+    std::vector<int32_t> out(c.num_tokens());
+    std::vector<int64_t> cur(c.num_docs());
+    for (int32_t d = 0; d < c.num_docs(); ++d) cur[d] = d ? c.doc_cum[d - 1] : 0;
+    for (const Chunk &ch : c.chunks)
+        for (int32_t w = 0; w < c.num_words; ++w)
+            for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i) out[cur[ch.word_tokens[i]]++] = w;
+    return out;
+}
+
+void heldout_split(const Corpus &c, int32_t every, int32_t train_chunks, Corpus &train, Corpus &observed,
+                   Corpus &eval) {
+    if (every < 2) fail(CULDA_ERR_ARG, "held-out period must be >= 2 (every %d-th doc)", every);
+    if (train_chunks < 1) fail(CULDA_ERR_ARG, "number of chunks must be >= 1");
+    const std::vector<int32_t> words = doc_major_words(c);
+    std::vector<int32_t> tw, ow, ew;
+    std::vector<int64_t> tl, ol, el;
+    for (int32_t d = 0; d < c.num_docs(); ++d) {
+        const int64_t b = d ? c.doc_cum[d - 1] : 0, e = c.doc_cum[d];
+        if (d % every == every - 1) {
+            int64_t no = 0, ne = 0;
+            for (int64_t t = b; t < e; ++t) {
+                if ((t - b) % 2 == 0) {
+                    ow.push_back(words[t]);
+                    ++no;
+                } else {
+                    ew.push_back(words[t]);
+                    ++ne;
+                }
+            }
+            ol.push_back(no);
+            el.push_back(ne);
+        } else {
+            tw.insert(tw.end(), words.begin() + b, words.begin() + e);
+            tl.push_back(e - b);
+        }
+    }
+    auto make = [&](Corpus &out, const std::vector<int32_t> &w, const std::vector<int64_t> &lens, int32_t chunks) {
+        out = Corpus();
+        out.num_words = c.num_words;
+        out.vocab = c.vocab;
+        out.doc_cum.resize(lens.size());
+        int64_t acc = 0;
+        for (size_t d = 0; d < lens.size(); ++d) out.doc_cum[d] = acc += lens[d];
+        if (lens.empty()) fail(CULDA_ERR_ARG, "held-out split leaves an empty corpus");
+        build_chunks_from_doc_major(out, w, greedy_chunk_ends(lens, chunks));
+        out.validate();
+    };
+    make(train, tw, tl, train_chunks);
+    make(observed, ow, ol, 1);
+    make(eval, ew, el, 1);
+}
+
 Corpus rechunk(const Corpus &src, int32_t num_chunks) {
     if (num_chunks < 1) fail(CULDA_ERR_ARG, "number of chunks must be >= 1");
     const int32_t V = src.num_words, D = src.num_docs();
@@ -407,6 +463,21 @@ extern "C" int culda_corpus_rechunk(const culda_corpus *corpus, int32_t num_chun
         if (!corpus || !out) fail(CULDA_ERR_ARG, "null argument");
         *out = nullptr;
         *out = new culda_corpus{rechunk(corpus->c, num_chunks)};
+        return CULDA_OK;
+    });
+}
+
+extern "C" int culda_corpus_heldout_split(const culda_corpus *corpus, int32_t every, int32_t train_chunks,
+                                          culda_corpus **train, culda_corpus **observed, culda_corpus **eval) {
+    return guarded([&] {
+        if (!corpus || !train || !observed || !eval) fail(CULDA_ERR_ARG, "null argument");
+        *train = *observed = *eval = nullptr;
+        auto t = std::make_unique<culda_corpus>(), o = std::make_unique<culda_corpus>(),
+             e = std::make_unique<culda_corpus>();
+        heldout_split(corpus->c, every, train_chunks, t->c, o->c, e->c);
+        *train = t.release();
+        *observed = o.release();
+        *eval = e.release();
         return CULDA_OK;
     });
 }

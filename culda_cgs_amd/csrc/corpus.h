@@ -45,6 +45,14 @@ void format_text(const std::string &input, const std::string &prefix, int32_t nu
 // docs until its token count reaches ceil(N / C).  Returns C doc-end ids.
 std::vector<int32_t> greedy_chunk_ends(const std::vector<int64_t> &doc_len, int32_t num_chunks);
 
+// Document-completion split: docs d with d % every == every - 1 are held
+// out.  train = the other docs (renumbered in order, train_chunks chunks);
+// observed / eval = the held-out docs (renumbered in order, one chunk),
+// whose tokens, taken in ascending word order, alternate observed (even
+// positions) and evaluated (odd positions).
+void heldout_split(const Corpus &c, int32_t every, int32_t train_chunks, Corpus &train, Corpus &observed,
+                   Corpus &eval);
+
 // Re-split (word order and doc order are preserved).
 Corpus rechunk(const Corpus &c, int32_t num_chunks);
 

@@ -1,0 +1,72 @@
+// device.h -- wave-level helpers shared by the gfx950 kernels (64-lane
+// wavefronts, DPP row operations, scalar-register copies).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "layout.h"
+
+namespace culda {
+
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// DPP move with zero for lanes whose source is outside the 16-lane row.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
+}
+
+// Inclusive Kogge-Stone scan inside each 16-lane row (DPP row_shr:1,2,4,8):
+// x_m <- x_m + x_{m-d} for m >= d; lanes with m < d add 0, which leaves
+// the non-negative values unchanged.
+__device__ __forceinline__ float scan16(float x) {
+    x = x + dpp_f<0x111>(x);
+    x = x + dpp_f<0x112>(x);
+    x = x + dpp_f<0x114>(x);
+    x = x + dpp_f<0x118>(x);
+    return x;
+}
+
+// Lane 15 of each row broadcast to the whole row (DPP row_newbcast:15).
+__device__ __forceinline__ float row_last(float x) { return dpp_f<0x15F>(x); }
+
+// Copy of a wave-uniform struct held in scalar registers (each dword
+// through readfirstlane), so descriptors read inside a loop that also
+// stores do not occupy vector registers.
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized struct expected");
+    T v = *p;
+    uint32_t w[sizeof(T) / 4];
+    __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
+// Row-local ballot: bit m set when lane m of this row voted true.
+__device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
+    return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
+}
+
+}  // namespace
+
+namespace {
+
+// Sum over the 64 lanes in a fixed butterfly order (same result in every
+// lane, same order in every run).
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+}  // namespace
+
+}  // namespace culda

@@ -309,6 +309,7 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     }
     dfree(d_slots);
     dfree(d_order);
+    dfree(d_order_claims);
     dfree(d_ll_partial);
     dfree(d_ll_packets);
     dfree(d_model_packets);
@@ -356,8 +357,19 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
         return all[a].end - all[a].begin > all[b].end - all[b].begin;
     });
+    // persistent claims: run j of kQueueClaim positions holds the slots of
+    // rank j, j + R, j + 2R, ... (R runs), so every claim mixes long and
+    // short slots and the longest are still taken first
+    std::vector<int32_t> claims;
+    claims.reserve(order.size());
+    {
+        const size_t R = (order.size() + kQueueClaim - 1) / kQueueClaim;
+        for (size_t j = 0; j < R; ++j)
+            for (size_t i = j; i < order.size(); i += R) claims.push_back(order[i]);
+    }
     d_slots = upload_vec(*this, all);
     d_order = upload_vec(*this, order);
+    d_order_claims = upload_vec(*this, claims);
     d_ll_partial = (double *)dalloc(sizeof(double) * std::max<size_t>(all.size(), 1));
     hip_check(hipMemset(d_ll_partial, 0, sizeof(double) * std::max<size_t>(all.size(), 1)), "ll clear");
     d_ll_packets = upload_vec(*this, llp);
@@ -435,6 +447,7 @@ void Engine::sample_all(hipStream_t s) {
     if (persistent_groups > 0) {
         hip_check(launch_fill_u32(d_queue, 1, 0, s), "queue reset");
         a.queue = d_queue;
+        a.order = d_order_claims;
         grid = std::min<int32_t>(num_slots, persistent_groups);
     }
     hip_check(launch_sample(a, grid, s), "sampler launch");

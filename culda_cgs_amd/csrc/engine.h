@@ -57,6 +57,16 @@ struct UnitHost {
     std::vector<int64_t> word_idx_h;
 };
 
+// Evaluated tokens of the held-out docs (document completion), kept by the
+// engine that holds their observed halves.
+struct EvalHost {
+    uint32_t *tokens = nullptr;  // held-out doc id per evaluated token, word-major
+    EvalSlot *slots = nullptr;
+    double *partial = nullptr;
+    int32_t num_slots = 0;
+    int64_t num_tokens = 0;
+};
+
 struct TuneResult {
     int32_t slot_tokens = 0;
     double best_ms = 0;
@@ -81,6 +91,7 @@ public:
     UnitDev *d_units = nullptr;
     SlotDesc *d_slots = nullptr;
     int32_t *d_order = nullptr;  // slot indices, longest first over all units
+    int32_t *d_order_claims = nullptr;  // the same, interleaved so each kQueueClaim run spans all lengths
     int32_t num_slots = 0;
     double *d_ll_partial = nullptr;
     Packet *d_ll_packets = nullptr;
@@ -140,6 +151,13 @@ public:
     double model_ll(int32_t model, uint32_t sweep);  // per token, from the history
     TuneResult autotune(const std::vector<int32_t> &candidates, bool batch);
     void sync();
+
+    // held-out log-likelihood (mh_engine.cpp): this engine holds the
+    // observed halves of the held-out docs (one corpus, one chunk) and the
+    // evaluated halves; counts come from the training engine
+    EvalHost eval;
+    void set_eval(const Corpus &ev);
+    double heldout_ll(Engine &train, int32_t sweeps);
 };
 
 }  // namespace culda

@@ -17,6 +17,11 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
 hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s);
 // grid is used in persistent mode (a.queue != null) only
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
+// workgroups of the persistent sampler resident per CU (0 if unknown)
+int32_t sample_blocks_per_cu();
+
+// mh.hip
+hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s);
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
                                  double *partial, int32_t grid, hipStream_t s);

@@ -16,60 +16,12 @@
 //   the last element.
 #include <hip/hip_runtime.h>
 
+#include "device.h"
 #include "kernels.h"
 #include "philox.h"
 
 namespace culda {
 
-namespace {
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// DPP move with zero for lanes whose source is outside the 16-lane row.
-template <int CTRL>
-__device__ __forceinline__ float dpp_f(float x) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
-}
-
-// Inclusive Kogge-Stone scan inside each 16-lane row (DPP row_shr:1,2,4,8):
-// x_m <- x_m + x_{m-d} for m >= d; lanes with m < d add 0, which leaves
-// the non-negative values unchanged.
-__device__ __forceinline__ float scan16(float x) {
-    x = x + dpp_f<0x111>(x);
-    x = x + dpp_f<0x112>(x);
-    x = x + dpp_f<0x114>(x);
-    x = x + dpp_f<0x118>(x);
-    return x;
-}
-
-// Lane 15 of each row broadcast to the whole row (DPP row_newbcast:15).
-__device__ __forceinline__ float row_last(float x) { return dpp_f<0x15F>(x); }
-
-// Copy of a wave-uniform struct held in scalar registers (each dword
-// through readfirstlane), so descriptors read inside a loop that also
-// stores do not occupy vector registers.
-template <class T>
-__device__ __forceinline__ T uniform_copy(const T *p) {
-    static_assert(sizeof(T) % 4 == 0, "dword-sized struct expected");
-    T v = *p;
-    uint32_t w[sizeof(T) / 4];
-    __builtin_memcpy(w, &v, sizeof(T));
-#pragma unroll
-    for (unsigned i = 0; i < sizeof(T) / 4; ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
-    __builtin_memcpy(&v, w, sizeof(T));
-    return v;
-}
-
-// Row-local ballot: bit m set when lane m of this row voted true.
-__device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
-    return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
-}
-
-}  // namespace
 
 // ------------------------------------------------------------ z init
 // One wave per slot; z = mulhi64(Philox((g, 0, 0, 0), (seed, 0))[0], K).
@@ -260,9 +212,10 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // group ahead; scans are DPP row operations.  z and the FP64 LL term are
 // written per token.  The processing order never changes a result: every
 // token depends only on the frozen counts and its own Philox draw.
-// Persistent mode (queue != null): each workgroup takes list positions from
-// an atomic counter until the list is exhausted; otherwise workgroup b
-// takes position b.
+// Persistent mode (queue != null): each workgroup claims kQueueClaim
+// consecutive list positions per atomic on a shared counter until the list
+// is exhausted (the host interleaves the order so that every claim holds
+// slots of all lengths); otherwise workgroup b takes position b.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
 template <bool kPersistent>
 __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
@@ -276,11 +229,16 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     const int lane = threadIdx.x;
     const uint32_t sweep = *a.sweep;
     int pos = blockIdx.x;
+    uint32_t next = 0, claim_end = 0;  // persistent: claimed positions [next, claim_end)
     while (true) {
     if (kPersistent) {
-        uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(a.queue, 1u);
-        pos = (int)__builtin_amdgcn_readfirstlane(q);  // wave-uniform: keeps descriptors in SGPRs
+        if (next >= claim_end) {
+            uint32_t q = 0;
+            if (lane == 0) q = atomicAdd(a.queue, (uint32_t)kQueueClaim);
+            next = __builtin_amdgcn_readfirstlane(q);  // wave-uniform: keeps descriptors in SGPRs
+            claim_end = next + kQueueClaim;
+        }
+        pos = (int)next++;
     }
     if (pos >= a.num_slots) break;
     const int slot = a.order ? (int)__builtin_amdgcn_readfirstlane(a.order[pos]) : pos;
@@ -626,6 +584,12 @@ hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
     else
         hipLaunchKernelGGL(sample_kernel<false>, dim3(a.num_slots), dim3(kWave), 0, s, a);
     return hipGetLastError();
+}
+
+int32_t sample_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave, 0) != hipSuccess) n = 0;
+    return n;
 }
 
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s) {

@@ -34,6 +34,7 @@ constexpr int kBlock = 16;         // FP32 prefix block = one DPP row
 constexpr int kMaxK = 1024;        // Q block totals scanned by one wave
 constexpr int kWavesPerGroup = 4;  // rebuild kernels
 constexpr int kLLHistory = 1024;   // LL kept per model for this many sweeps
+constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 
 struct DocMeta {
     uint32_t start;  // theta slab start (entries)
@@ -148,6 +149,33 @@ struct LLArgs {
     const uint32_t *sweep;
     int32_t num_packets;
     int32_t num_models;
+};
+
+// Held-out evaluation work item: evaluated tokens [begin, end) of `word`.
+struct EvalSlot {
+    int32_t word;
+    int32_t pad;
+    int64_t begin;
+    int64_t end;
+};
+
+// Document-completion log-likelihood of evaluated tokens (FP64): per token
+// log(sum_k (n_dk + alpha) / (n_d + K alpha) * (n_wk + beta) / (n_k + V beta))
+// with n_dk, n_d from the fold-in theta of the held-out doc.
+struct HeldoutArgs {
+    const EvalSlot *slots;
+    const uint32_t *tokens;  // held-out doc id of each evaluated token, word-major
+    const DocMeta *meta;     // fold-in theta of the held-out docs
+    const uint32_t *theta;
+    const int32_t *nwk;
+    const int32_t *nk;
+    double *partial;         // one per slot
+    double alpha;
+    double beta;
+    int32_t num_slots;
+    int32_t k;
+    int32_t num_words;
+    uint32_t count_bits;
 };
 
 }  // namespace culda

@@ -110,6 +110,17 @@ int culda_format_text(const char *input_path, const char *prefix, int32_t num_ch
 /* Re-split a corpus into num_chunks chunks with the same greedy rule. */
 int culda_corpus_rechunk(const culda_corpus *corpus, int32_t num_chunks, culda_corpus **out);
 
+/* Document-completion split for held-out log-likelihood (no counterpart in
+ * the reference, which reports training LL only).  Docs d with
+ * d % every == every - 1 are held out: *train gets the other docs
+ * (renumbered in order, train_chunks chunks); *observed and *eval get the
+ * held-out docs (renumbered in order, one chunk each), whose tokens, taken
+ * in ascending word order, alternate between observed (even positions) and
+ * evaluated (odd positions).  All three keep the vocabulary size.  Free
+ * each with culda_corpus_free. */
+int culda_corpus_heldout_split(const culda_corpus *corpus, int32_t every, int32_t train_chunks,
+                               culda_corpus **train, culda_corpus **observed, culda_corpus **eval);
+
 /* Parameters of the synthetic LDA generator (seeded, deterministic for a
  * given parameter set, independent of the thread count).
  *   phi_k ~ Dirichlet(topic_word_prior) over V, multiplied by
@@ -296,6 +307,22 @@ int culda_autotune(culda_ctx *ctx, culda_tune *out);
  * buffers owned by the context, timed with HIP events: achievable HBM
  * bandwidth for the roofline.  gbps = 2*n_bytes/time. */
 int culda_stream_copy(culda_ctx *ctx, int64_t n_bytes, int32_t repeats, double *gbps);
+
+/* ------------------------------------------------------------------------
+ * Held-out log-likelihood by document completion (no counterpart in the
+ * reference).  Split with culda_corpus_heldout_split; train one context on
+ * *train; create a second context ("fold") with the same k, alpha, beta
+ * (its own seed), upload *observed into it and give it *eval here.
+ * ---------------------------------------------------------------------- */
+int culda_heldout_set_eval(culda_ctx *fold, const culda_corpus *eval);
+/* Copies the training context's n_wk / n_k into the fold context (phi
+ * fixed), draws the observed tokens' z from the fold context's seed, runs
+ * fold_in_sweeps plain sweeps (each followed by a theta rebuild), then
+ * returns the mean over evaluated tokens of
+ *   log sum_k (n_dk + alpha) / (n_d + K alpha) * (n_wk + beta) / (n_k + V beta)
+ * with n_dk, n_d the fold-in counts of the token's doc, in FP64.  The same
+ * inputs give the same value on every call. */
+int culda_heldout_ll(culda_ctx *train, culda_ctx *fold, int32_t fold_in_sweeps, double *ll_per_token);
 
 /* ------------------------------------------------------------------------
  * Batch: many independent corpora sampled together (no counterpart in the

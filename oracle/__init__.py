@@ -137,6 +137,11 @@ class OracleModel:
         self.token_nnz = L.oc_rebuild_theta(C.byref(self._c), _p(self.th_off, C.c_int64),
                                             _p(self.th_cnt, C.c_int32), _p(self.th_top, C.c_int16))
 
+    def rebuild_theta(self):
+        """Theta only (n_wk / n_k left as they are: fold-in with phi fixed)."""
+        self.token_nnz = lib().oc_rebuild_theta(C.byref(self._c), _p(self.th_off, C.c_int64),
+                                                _p(self.th_cnt, C.c_int32), _p(self.th_top, C.c_int16))
+
     def phi_head(self, beta):
         h = np.zeros(self.V * self.K, np.float32)
         lib().oc_phi_head(self.V, self.K, beta, _p(self.nwk, C.c_int32), _p(self.nk, C.c_int32),

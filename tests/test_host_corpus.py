@@ -121,3 +121,47 @@ def test_generator_lognormal_lengths():
     lens = np.diff(np.concatenate([[0], c.doc_cum()]))
     ll = np.log(lens)
     assert abs(ll.mean() - 3.0) < 0.05 and abs(ll.std() - 0.5) < 0.05
+
+
+def _doc_words(corpus):
+    """Per doc: its word ids in ascending order, from the word-major chunks."""
+    info = corpus.info()
+    docs = [[] for _ in range(info.num_docs)]
+    for c in range(info.num_chunks):
+        _, _, wi, wt = corpus.chunk(c)
+        for w in range(info.num_words):
+            for d in wt[wi[w]:wi[w + 1]]:
+                docs[d].append(w)
+    return [np.array(sorted(x), np.int64) for x in docs]
+
+
+@pytest.mark.parametrize("every,chunks", [(10, 1), (3, 2), (2, 3)])
+def test_heldout_split(every, chunks):
+    rng = np.random.default_rng(7)
+    V = 60
+    docs = random_docs(rng, V, 47, 9, empty_docs=2)
+    full = cg.Corpus.from_arrays(V, *chunks_from_docs(V, docs, 2))
+    train, obs, ev = full.heldout_split(every, chunks)
+    assert train.info().num_chunks == chunks and obs.info().num_chunks == 1 and ev.info().num_chunks == 1
+    assert train.info().num_words == V and obs.info().num_words == V and ev.info().num_words == V
+    fw, tw, ow, ew = _doc_words(full), _doc_words(train), _doc_words(obs), _doc_words(ev)
+    held = [d for d in range(len(fw)) if d % every == every - 1]
+    kept = [d for d in range(len(fw)) if d % every != every - 1]
+    assert len(tw) == len(kept) and len(ow) == len(held) == len(ev.doc_cum())
+    for i, d in enumerate(kept):
+        assert np.array_equal(tw[i], fw[d])
+    for i, d in enumerate(held):
+        assert np.array_equal(ow[i], fw[d][0::2]), f"observed half of doc {d}"
+        assert np.array_equal(ew[i], fw[d][1::2]), f"evaluated half of doc {d}"
+    n = full.info().num_tokens
+    assert train.info().num_tokens + obs.info().num_tokens + ev.info().num_tokens == n
+
+
+def test_heldout_split_errors():
+    V = 10
+    docs = [np.array([1, 2, 3]), np.array([4])]
+    full = cg.Corpus.from_arrays(V, *chunks_from_docs(V, docs, 1))
+    with pytest.raises(cg.CuldaError):
+        full.heldout_split(1)
+    with pytest.raises(cg.CuldaError):
+        full.heldout_split(10)  # no doc held out: the held-out corpora would be empty
