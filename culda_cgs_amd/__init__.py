@@ -62,6 +62,12 @@ class Stats(C.Structure):
                 ("sampler_launches", C.c_int32), ("pad", C.c_int32)]
 
 
+class MHParams(C.Structure):
+    _fields_ = [("batch", C.c_int32), ("infer_batch", C.c_int32), ("max_snapshots", C.c_int32), ("pad", C.c_int32),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("init_seed", C.c_uint64)]
+
+
 class Tune(C.Structure):
     _fields_ = [("slot_tokens", C.c_int32), ("persistent_groups", C.c_int32), ("num_tried", C.c_int32),
                 ("pad", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 16),
@@ -115,6 +121,19 @@ _SIGS = {
     "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "culda_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
+    "culda_mh_enable": (C.c_int, [_P, C.POINTER(MHParams)]),
+    "culda_mh_collect": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "culda_mh_train": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "culda_mh_iterate": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "culda_mh_get_weights": (C.c_int, [_P] + [C.POINTER(C.c_float)] * 5),
+    "culda_mh_set_weights": (C.c_int, [_P] + [C.POINTER(C.c_float)] * 5),
+    "culda_mh_logits": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
+    "culda_mh_gradients": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_double)]
+                           + [C.POINTER(C.c_float)] * 5),
+    "culda_mh_tables": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32)]),
+    "culda_mh_sketch": (C.c_int, [_P, C.POINTER(C.c_uint16)]),
     "culda_corpus_heldout_split": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)]),
     "culda_heldout_set_eval": (C.c_int, [_P, _P]),
     "culda_heldout_ll": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_double)]),
@@ -394,6 +413,73 @@ class Sampler:
         _check(lib().culda_heldout_ll(train._h, self._h, fold_in_sweeps, C.byref(v)))
         return v.value
 
+    # -- learned-MH proposal ---------------------------------------------
+    MH_IN, MH_HIDDEN, MH_EMB = 384, 512, 128
+
+    def mh_enable(self, batch=0, infer_batch=0, max_snapshots=0, lr=0.0, beta1=0.0, beta2=0.0, eps=0.0,
+                  init_seed=1):
+        self.mh_params = MHParams(batch, infer_batch, max_snapshots, 0, lr, beta1, beta2, eps, init_seed)
+        _check(lib().culda_mh_enable(self._h, C.byref(self.mh_params)))
+
+    def mh_collect(self):
+        n = C.c_int32()
+        _check(lib().culda_mh_collect(self._h, C.byref(n)))
+        return n.value
+
+    def mh_train(self, epochs=1):
+        loss, sec = C.c_double(), C.c_double()
+        _check(lib().culda_mh_train(self._h, epochs, C.byref(loss), C.byref(sec)))
+        return loss.value, sec.value
+
+    def mh_iterate(self, iteration, proposals=2):
+        rate, sec = C.c_double(), C.c_double()
+        _check(lib().culda_mh_iterate(self._h, iteration, proposals, C.byref(rate), C.byref(sec)))
+        return rate.value, sec.value
+
+    def _mh_shapes(self):
+        K, V = self.k, self._info.num_words
+        return {"w1": (self.MH_IN, self.MH_HIDDEN), "b1": (self.MH_HIDDEN,), "w2": (self.MH_HIDDEN, K), "b2": (K,),
+                "emb": (V, self.MH_EMB)}
+
+    def mh_get_weights(self):
+        out = {k: np.empty(v, np.float32) for k, v in self._mh_shapes().items()}
+        _check(lib().culda_mh_get_weights(self._h, *[_ptr(out[k], C.c_float) for k in ("w1", "b1", "w2", "b2", "emb")]))
+        return out
+
+    def mh_set_weights(self, w):
+        arr = {k: np.ascontiguousarray(w[k], np.float32).reshape(v) for k, v in self._mh_shapes().items()}
+        _check(lib().culda_mh_set_weights(self._h, *[_ptr(arr[k], C.c_float) for k in ("w1", "b1", "w2", "b2", "emb")]))
+
+    def mh_logits(self, docs, words):
+        d = np.ascontiguousarray(docs, np.int32)
+        w = np.ascontiguousarray(words, np.int32)
+        out = np.empty((d.size, self.k), np.float32)
+        _check(lib().culda_mh_logits(self._h, d.size, _ptr(d, C.c_int32), _ptr(w, C.c_int32), _ptr(out, C.c_float)))
+        return out
+
+    def mh_gradients(self, docs, words, targets, train_step=False):
+        d = np.ascontiguousarray(docs, np.int32)
+        w = np.ascontiguousarray(words, np.int32)
+        t = np.ascontiguousarray(targets, np.int32)
+        g = {k: np.empty(v, np.float32) for k, v in self._mh_shapes().items()}
+        loss = C.c_double()
+        _check(lib().culda_mh_gradients(self._h, d.size, _ptr(d, C.c_int32), _ptr(w, C.c_int32), _ptr(t, C.c_int32),
+                                        1 if train_step else 0, C.byref(loss),
+                                        *[_ptr(g[k], C.c_float) for k in ("w1", "b1", "w2", "b2", "emb")]))
+        return loss.value, g
+
+    def mh_tables(self, first, n):
+        W = np.empty((n, self.k), np.int32)
+        F = np.empty((n, self.k), np.uint32)
+        A = np.empty((n, self.k), np.uint32)
+        _check(lib().culda_mh_tables(self._h, first, n, _ptr(W, C.c_int32), _ptr(F, C.c_uint32), _ptr(A, C.c_uint32)))
+        return W, F, A
+
+    def mh_sketch(self):
+        out = np.empty((self._info.num_docs, 256), np.uint16)
+        _check(lib().culda_mh_sketch(self._h, _ptr(out, C.c_uint16)))
+        return out
+
     def autotune(self):
         """Pick the sampler's tokens per slot by timing one sweep per size."""
         t = Tune()
@@ -491,4 +577,4 @@ class Batch:
         return nwk.reshape(V, self.k), nk
 
 
-__all__ = ["Corpus", "Sampler", "Batch", "Tune", "CuldaError", "format_text", "lib", "LIB_PATH", "CLI_PATH"]
+__all__ = ["Corpus", "Sampler", "Batch", "Tune", "MHParams", "CuldaError", "format_text", "lib", "LIB_PATH", "CLI_PATH"]

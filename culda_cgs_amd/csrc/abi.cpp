@@ -2,11 +2,13 @@
 // context (culda_ctx), the multi-corpus batch (culda_batch), the host side
 // of the ModelPhi / ModelTheta save hooks and checkpoints.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "engine.h"
+#include "mh_state.h"
 #include "kernels.h"
 
 using namespace culda;
@@ -578,6 +580,215 @@ extern "C" int culda_batch_set_persistent(culda_batch *batch, int32_t groups) {
 extern "C" int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, int32_t count, int32_t use_graph,
                                    double *ll, double *seconds) {
     BATCH_CALL(e.iterate_batch((uint32_t)std::max(first_iteration, 0), count, use_graph != 0, ll, seconds));
+}
+
+// ====================================================== learned MH
+
+extern "C" int culda_mh_enable(culda_ctx *ctx, const culda_mh_params *params) {
+    CTX_CALL({
+        if (!params) fail(CULDA_ERR_ARG, "null params");
+        if (e.units.size() != 1) fail(CULDA_ERR_ARG, "learned MH needs a one-chunk corpus");
+        e.mh_enable(*params);
+    });
+}
+
+extern "C" int culda_mh_collect(culda_ctx *ctx, int32_t *num_snapshots) {
+    CTX_CALL({
+        const int32_t n = e.mh_collect();
+        if (num_snapshots) *num_snapshots = n;
+    });
+}
+
+extern "C" int culda_mh_train(culda_ctx *ctx, int32_t epochs, double *mean_loss, double *seconds) {
+    CTX_CALL({
+        auto t0 = std::chrono::steady_clock::now();
+        const double l = e.mh_train(epochs);
+        if (mean_loss) *mean_loss = l;
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+extern "C" int culda_mh_iterate(culda_ctx *ctx, int32_t iteration, int32_t proposals, double *accept_rate,
+                                double *seconds) {
+    CTX_CALL({
+        if (iteration < 1) fail(CULDA_ERR_ARG, "iteration must be >= 1");
+        auto t0 = std::chrono::steady_clock::now();
+        const double r = e.mh_iterate((uint32_t)iteration, proposals);
+        if (accept_rate) *accept_rate = r;
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        ctx->last_sweep = (uint32_t)iteration;
+    });
+}
+
+namespace {
+struct MHSizes {
+    size_t w1, b1, w2, b2, emb;
+};
+MHSizes mh_sizes(const MHState &s) {
+    return MHSizes{(size_t)kIn * kHidden, (size_t)kHidden, (size_t)kHidden * s.K, (size_t)s.K, (size_t)s.V * kEmb};
+}
+}  // namespace
+
+extern "C" int culda_mh_get_weights(culda_ctx *ctx, float *w1, float *b1, float *w2, float *b2, float *emb) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        const MHSizes z = mh_sizes(s);
+        e.sync();
+        const std::pair<float *, std::pair<const float *, size_t>> parts[] = {
+            {w1, {s.w1, z.w1}}, {b1, {s.b1, z.b1}}, {w2, {s.w2, z.w2}}, {b2, {s.b2, z.b2}}, {emb, {s.emb, z.emb}}};
+        for (auto &p : parts)
+            if (p.first) hip_check(hipMemcpy(p.first, p.second.first, p.second.second * 4, hipMemcpyDeviceToHost), "copy");
+    });
+}
+
+extern "C" int culda_mh_set_weights(culda_ctx *ctx, const float *w1, const float *b1, const float *w2, const float *b2,
+                                    const float *emb) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        const MHSizes z = mh_sizes(s);
+        if (!w1 || !b1 || !w2 || !b2 || !emb) fail(CULDA_ERR_ARG, "null weights");
+        e.sync();
+        hip_check(hipMemcpy(s.w1, w1, z.w1 * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(s.b1, b1, z.b1 * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(s.w2, w2, z.w2 * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(s.b2, b2, z.b2 * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(s.emb, emb, z.emb * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(launch_to_bf16(s.w1, s.w1h, z.w1, e.stream), "bf16");
+        hip_check(launch_to_bf16(s.w2, s.w2h, z.w2, e.stream), "bf16");
+        hip_check(launch_to_bf16(s.emb, s.embh, z.emb, e.stream), "bf16");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+    });
+}
+
+static void upload_queries(Engine &e, MHState &s, int64_t n, const int32_t *docs, const int32_t *words) {
+    if (!docs || !words) fail(CULDA_ERR_ARG, "null queries");
+    for (int64_t i = 0; i < n; ++i) {
+        if (docs[i] < 0 || docs[i] >= s.D) fail(CULDA_ERR_ARG, "query %lld: doc %d out of range", (long long)i, docs[i]);
+        if (words[i] < 0 || words[i] >= s.V)
+            fail(CULDA_ERR_ARG, "query %lld: word %d out of range", (long long)i, words[i]);
+    }
+    hip_check(hipMemcpy(s.q_doc, docs, n * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(s.q_word, words, n * 4, hipMemcpyHostToDevice), "copy");
+    (void)e;
+}
+
+extern "C" int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, const int32_t *words, float *logits) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        if (!logits || n < 0 || n > s.prm.infer_batch) fail(CULDA_ERR_ARG, "n must be in [0, infer_batch]");
+        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        e.sync();
+        upload_queries(e, s, n, docs, words);
+        e.mh_sketch_current(e.stream);
+        e.mh_query_logits(s.q_doc, s.q_word, n, s.Li, e.stream);
+        hip_check(hipMemcpyAsync(logits, s.Li, (size_t)n * s.K * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+    });
+}
+
+extern "C" int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs, const int32_t *words,
+                                  const int32_t *targets, int32_t train_step, double *loss, float *dw1, float *db1,
+                                  float *dw2, float *db2, float *demb) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        if (n < 1 || n > s.prm.batch) fail(CULDA_ERR_ARG, "n must be in [1, batch]");
+        if (!targets) fail(CULDA_ERR_ARG, "null targets");
+        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        for (int32_t i = 0; i < n; ++i)
+            if (targets[i] < 0 || targets[i] >= s.K) fail(CULDA_ERR_ARG, "target %d out of range", targets[i]);
+        e.sync();
+        upload_queries(e, s, n, docs, words);
+        e.mh_sketch_current(e.stream);
+        const int32_t B = s.prm.batch;
+        hip_check(launch_gather_query(s.sk_cur, s.q_doc, s.q_word, s.embh, n, B, s.X, e.stream), "gather");
+        std::vector<int32_t> tg(B, -1), wr(B, -1);
+        for (int32_t i = 0; i < n; ++i) {
+            tg[i] = targets[i];
+            wr[i] = words[i];
+        }
+        hip_check(hipMemcpy(s.tgt, tg.data(), B * 4, hipMemcpyHostToDevice), "copy");
+        hip_check(hipMemcpy(s.wrow, wr.data(), B * 4, hipMemcpyHostToDevice), "copy");
+        float *lsum = (float *)e.dalloc(16);
+        mh_batch_step(e, s, n, false, lsum, e.stream);
+        const MHSizes z = mh_sizes(s);
+        float hl = 0;
+        hip_check(hipMemcpyAsync(&hl, lsum, 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        const std::pair<float *, std::pair<const float *, size_t>> parts[] = {
+            {dw1, {s.g_w1, z.w1}}, {db1, {s.g_b1, z.b1}}, {dw2, {s.g_w2, z.w2}}, {db2, {s.g_b2, z.b2}}};
+        for (auto &p : parts)
+            if (p.first)
+                hip_check(hipMemcpyAsync(p.first, p.second.first, p.second.second * 4, hipMemcpyDeviceToHost, e.stream),
+                          "copy");
+        std::vector<long long> ge(z.emb);
+        hip_check(hipMemcpyAsync(ge.data(), s.g_emb, z.emb * 8, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+        e.dfree(lsum);
+        if (demb)
+            for (size_t i = 0; i < z.emb; ++i) demb[i] = (float)((double)ge[i] * (1.0 / 4294967296.0));
+        if (loss) *loss = (double)hl / (double)n;
+        if (train_step) {
+            mh_apply_adam(e, s, e.stream);  // consumes (and clears) the embedding accumulator
+        } else {
+            hip_check(launch_fill_u32((uint32_t *)s.g_emb, (int64_t)z.emb * 2, 0, e.stream), "clear");
+        }
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+    });
+}
+
+extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t *weights, uint32_t *thresholds,
+                               uint32_t *aliases) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        if (!weights || !thresholds || !aliases) fail(CULDA_ERR_ARG, "null output");
+        const UnitHost &u = e.units[0];
+        if (n < 0 || n > s.prm.infer_batch || first < 0 || first + n > u.num_tokens)
+            fail(CULDA_ERR_ARG, "token range out of bounds");
+        if (!e.theta_ready || !e.phi_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        e.sync();
+        const size_t cells = (size_t)n * s.K;
+        int32_t *dw = (int32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
+        uint32_t *df = (uint32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
+        uint32_t *da = (uint32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
+        e.mh_sketch_current(e.stream);
+        e.mh_query_logits(s.tok_doc + first, s.tok_word + first, n, s.Li, e.stream);
+        const ModelHost &m = e.models[0];
+        MHArgs a{};
+        a.logits = s.Li;
+        a.word = s.tok_word;
+        a.gdelta = s.gdelta[0];
+        a.unit = UnitDev{u.tokens, u.z, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, e.d_stats, u.model,
+                         u.doc_end - u.doc_start};
+        a.model = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
+                           m.count_bits, 0};
+        a.accepted = s.accepted;
+        a.w_out = dw;
+        a.f_out = df;
+        a.a_out = da;
+        a.start = first;
+        a.n = n;
+        a.proposals = 0;
+        a.sweep = 0;
+        hip_check(launch_mh(a, e.stream), "mh tables");
+        hip_check(hipMemcpyAsync(weights, dw, cells * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipMemcpyAsync(thresholds, df, cells * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipMemcpyAsync(aliases, da, cells * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+        e.dfree(dw);
+        e.dfree(df);
+        e.dfree(da);
+    });
+}
+
+extern "C" int culda_mh_sketch(culda_ctx *ctx, uint16_t *out) {
+    CTX_CALL({
+        MHState &s = e.mh_state();
+        if (!out) fail(CULDA_ERR_ARG, "null output");
+        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        e.sync();
+        e.mh_sketch_current(e.stream);
+        hip_check(hipMemcpyAsync(out, s.sk_cur, (size_t)s.D * kSketch * 2, hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+    });
 }
 
 // ====================================================== held-out LL

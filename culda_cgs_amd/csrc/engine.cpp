@@ -16,6 +16,7 @@
 #include <numeric>
 
 #include "kernels.h"
+#include "mh_state.h"
 
 namespace culda {
 

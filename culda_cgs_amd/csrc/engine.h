@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime_api.h>
 
+#include <memory>
 #include <vector>
 
 #include "corpus.h"
@@ -66,6 +67,8 @@ struct EvalHost {
     int32_t num_slots = 0;
     int64_t num_tokens = 0;
 };
+
+struct MHState;  // learned-MH proposal (mh_engine.cpp)
 
 struct TuneResult {
     int32_t slot_tokens = 0;
@@ -158,6 +161,16 @@ public:
     EvalHost eval;
     void set_eval(const Corpus &ev);
     double heldout_ll(Engine &train, int32_t sweeps);
+
+    // learned-MH proposal (mh_engine.cpp); one corpus (model 0)
+    std::unique_ptr<MHState> mh;
+    void mh_enable(const culda_mh_params &p);
+    int32_t mh_collect();
+    double mh_train(int32_t epochs);
+    double mh_iterate(uint32_t sweep, int32_t proposals);
+    void mh_sketch_current(hipStream_t s);
+    void mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, float *d_logits, hipStream_t s);
+    MHState &mh_state();
 };
 
 }  // namespace culda

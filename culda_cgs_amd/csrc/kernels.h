@@ -22,6 +22,25 @@ int32_t sample_blocks_per_cu();
 
 // mh.hip
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);
+// ta: A stored [k][m]; tb: B stored [n][k]
+hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
+hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
+                         uint16_t *out, hipStream_t s);
+hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
+                               const uint32_t *tok_word, const uint16_t *emb16, uint64_t mul, uint64_t add, uint64_t T,
+                               uint64_t N, int64_t D, int64_t p0, int32_t rows, int32_t rows_pad, uint16_t *X,
+                               int32_t *tgt, int32_t *wrow, hipStream_t s);
+hipError_t launch_gather_query(const uint16_t *sketch, const uint32_t *docs, const uint32_t *words,
+                               const uint16_t *emb16, int64_t n, int64_t rows_pad, uint16_t *X, hipStream_t s);
+hipError_t launch_softmax_ce(const float *logits, int32_t K, const int32_t *tgt, float inv_rows, uint16_t *G,
+                             float *loss, int32_t rows_pad, hipStream_t s);
+hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out, hipStream_t s);
+hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc, hipStream_t s);
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
+                       float step, float b1, float b2, float eps, float sqrt_bc2, hipStream_t s);
+hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
+hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s);
+hipError_t launch_mh(const MHArgs &a, hipStream_t s);
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s);
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
                                  double *partial, int32_t grid, hipStream_t s);

@@ -178,4 +178,64 @@ struct HeldoutArgs {
     uint32_t count_bits;
 };
 
+// ------------------------------------------------------------ learned MH
+// MLP proposal q(k | d, w) = softmax(W2^T GELU(W1^T [sketch(d); emb(w)] +
+// b1) + b2): a 256-wide signed count sketch of the doc's topic counts, a
+// 128-wide word embedding, 512 hidden units (tanh-GELU), K logits.  BF16
+// operands on MFMA, FP32 accumulation, FP32 master weights (Adam).
+constexpr int kSketch = 256;
+constexpr int kEmb = 128;
+constexpr int kIn = kSketch + kEmb;  // 384
+constexpr int kHidden = 512;
+constexpr int kAliasBits = 20;       // proposal weights are integers summing to K << kAliasBits
+constexpr int kGemmTile = 64;        // C tile (rows and columns) of the BF16 GEMM
+constexpr int kGemmK = 32;           // reduction step of the BF16 GEMM
+
+// C[m x n] = sum_k A(i, k) B(k, j), A and B in BF16 (raw bits).  A is
+// stored [m][k] (lda), or [k][m] when transposed; B is stored [k][n]
+// (ldb), or [n][k] when transposed.  m, n multiples of kGemmTile, k a
+// multiple of kGemmK, every row start 16-byte aligned.
+enum GemmEpilogue : int32_t {
+    kEpiStore = 0,     // c = acc (+ bias[j])
+    kEpiBiasGelu = 1,  // c = acc + bias[j] (optional), c16 = bf16(gelu(acc + bias[j]))
+    kEpiGeluGrad = 2,  // c16 = bf16(acc * gelu'(aux[i][j]))
+};
+
+struct GemmArgs {
+    const uint16_t *a;
+    const uint16_t *b;
+    float *c;              // FP32 output [m][n] (ldc) or null
+    uint16_t *c16;         // BF16 output [m][n] (ldc16) or null
+    const float *bias;     // per column, or null
+    const float *aux;      // kEpiGeluGrad: pre-activations [m][n] (ldaux)
+    int64_t lda, ldb, ldc, ldc16, ldaux;
+    int32_t m, n, k;
+    int32_t epilogue;
+};
+
+// Per-token query of the proposal: doc (global id) and word of every token
+// of the corpus, in unit order (tokens of unit u at [unit_base_u, ...)).
+struct MHTokenInfo {
+    const uint32_t *doc;   // global doc id
+    const uint32_t *word;
+};
+
+// Inputs of the MH step for a run of tokens of one unit.
+struct MHArgs {
+    const float *logits;        // [n][K] proposal logits (rows = tokens start .. start+n-1)
+    const uint32_t *word;       // word id per token of the unit
+    const int64_t *gdelta;      // per word: global id = unit-local index + gdelta[w]
+    UnitDev unit;               // tokens (doc ids), z, meta, theta
+    ModelDev model;
+    unsigned long long *accepted;  // += accepted proposals
+    int32_t *w_out;             // debug: [n][K] integer weights, or null
+    uint32_t *f_out;            // debug: [n][K] bucket thresholds, or null
+    uint32_t *a_out;            // debug: [n][K] aliases, or null
+    int64_t start;              // first token (unit-local index)
+    int32_t n;
+    int32_t proposals;
+    uint32_t sweep;
+    int32_t pad;
+};
+
 }  // namespace culda

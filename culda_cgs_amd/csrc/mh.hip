@@ -4,6 +4,7 @@
 
 #include "device.h"
 #include "kernels.h"
+#include "philox.h"
 
 namespace culda {
 
@@ -48,6 +49,585 @@ __global__ __launch_bounds__(64) void heldout_kernel(HeldoutArgs a) {
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
     hipLaunchKernelGGL(heldout_kernel, dim3(a.num_slots), dim3(kWave), 0, s, a);
+    return hipGetLastError();
+}
+
+
+// ======================================================= learned MH proposal
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// tanh-approximated GELU and its derivative.
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+constexpr float kGeluA = 0.044715f;
+
+__device__ __forceinline__ float gelu(float x) {
+    const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
+    return 0.5f * x * (1.0f + t);
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
+}
+
+__device__ __forceinline__ float wave_sum_f(float x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+    return x;
+}
+
+__device__ __forceinline__ float wave_max_f(float x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = fmaxf(x, __shfl_xor(x, d, kWave));
+    return x;
+}
+
+// Exclusive prefix over the 64 lanes (integer: any order gives the same).
+__device__ __forceinline__ uint32_t wave_exclusive_u32(uint32_t x, uint32_t *total) {
+    uint32_t inc = x;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t y = __shfl_up(inc, d, kWave);
+        if ((int)(threadIdx.x & 63) >= d) inc += y;
+    }
+    *total = __shfl(inc, kWave - 1, kWave);
+    return inc - x;
+}
+
+// Signed count sketch of a doc's topic counts: topic k adds +-n_dk to
+// bucket h(k); h and the sign are fixed multiplicative hashes of k.
+__device__ __host__ __forceinline__ uint32_t sketch_bucket(uint32_t k) { return (k * 2654435761u) >> 24; }
+__device__ __host__ __forceinline__ bool sketch_negative(uint32_t k) { return ((k * 0x85EBCA6Bu) >> 31) != 0; }
+
+}  // namespace
+
+// ------------------------------------------------------------ BF16 GEMM
+// 64 x 64 C tile per workgroup of four waves (2 x 2, each 32 x 32 = 2 x 2
+// MFMA 16x16x32 tiles); A and B staged through LDS 32 deep as [row][k]
+// (B as [col][k]) so that every lane reads its 8 consecutive k as one
+// 16-byte LDS load.  MFMA lane layout (checked on gfx950): lane l feeds
+// A[l % 16][8 (l / 16) .. +7] and B[8 (l / 16) .. +7][l % 16] and receives
+// C[4 (l / 16) + r][l % 16], r = 0..3.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[kGemmTile][kGemmK + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kGemmTile][kGemmK + 8];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * kGemmTile, n0 = (int64_t)blockIdx.x * kGemmTile;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.k; k0 += kGemmK) {
+        if (!TA) {
+            const int r = t >> 2, c8 = (t & 3) * 8;
+            *reinterpret_cast<uint4 *>(&As[r][c8]) =
+                *reinterpret_cast<const uint4 *>(g.a + (m0 + r) * g.lda + k0 + c8);
+        } else {
+            const int kk = t >> 3, c8 = (t & 7) * 8;
+            const uint4 v = *reinterpret_cast<const uint4 *>(g.a + (int64_t)(k0 + kk) * g.lda + m0 + c8);
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) As[c8 + j][kk] = h[j];
+        }
+        if (!TB) {
+            const int kk = t >> 3, c8 = (t & 7) * 8;
+            const uint4 v = *reinterpret_cast<const uint4 *>(g.b + (int64_t)(k0 + kk) * g.ldb + n0 + c8);
+            const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Bs[c8 + j][kk] = h[j];
+        } else {
+            const int r = t >> 2, c8 = (t & 3) * 8;
+            *reinterpret_cast<uint4 *>(&Bs[r][c8]) =
+                *reinterpret_cast<const uint4 *>(g.b + (n0 + r) * g.ldb + k0 + c8);
+        }
+        __syncthreads();
+        bf16x8 af[2], bfr[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            af[i] = *reinterpret_cast<const bf16x8 *>(&As[wm * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn * 32 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int64_t col = n0 + wn * 32 + j * 16 + (lane & 15);
+                float x = acc[i][j][r];
+                if (g.epilogue == kEpiGeluGrad) {
+                    g.c16[row * g.ldc16 + col] = f2bf(x * gelu_grad(g.aux[row * g.ldaux + col]));
+                } else {
+                    if (g.bias) x += g.bias[col];
+                    if (g.c) g.c[row * g.ldc + col] = x;
+                    if (g.c16) g.c16[row * g.ldc16 + col] = f2bf(g.epilogue == kEpiBiasGelu ? gelu(x) : x);
+                }
+            }
+}
+
+// ------------------------------------------------------------ features
+// One wave per doc of a unit: integer bucket sums in LDS (exact, so the
+// result does not depend on the order), then bucket / n_d rounded to BF16.
+__global__ __launch_bounds__(64) void sketch_kernel(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
+                                                    uint32_t count_bits, uint16_t *out) {
+    __shared__ int32_t acc[kSketch];
+    const int lane = threadIdx.x;
+    const int d = blockIdx.x;
+    if (d >= num_docs) return;
+    for (int b = lane; b < kSketch; b += kWave) acc[b] = 0;
+    wave_sync();
+    const DocMeta m = meta[d];
+    const uint32_t cmask = (1u << count_bits) - 1u;
+    for (uint32_t j = lane; j < m.nnz; j += kWave) {
+        const uint32_t e = theta[m.start + j];
+        const uint32_t k = (e >> count_bits) >> 2;
+        const int32_t c = (int32_t)(e & cmask);
+        atomicAdd(&acc[sketch_bucket(k)], sketch_negative(k) ? -c : c);
+    }
+    wave_sync();
+    const float nd = (float)m.nd;
+    for (int b = lane; b < kSketch; b += kWave)
+        out[(int64_t)d * kSketch + b] = m.nd ? f2bf((float)acc[b] / nd) : (uint16_t)0;
+}
+
+// Training rows: position p of the epoch's order maps to triple
+// tau = (mul * p + add) mod T (a bijection: mul is coprime to T), i.e.
+// snapshot tau / N, token tau % N.  X row = [sketch; embedding] (BF16);
+// rows past `rows` are zero with target -1.  One thread per 16-byte piece.
+__global__ void gather_train_kernel(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
+                                    const uint32_t *tok_word, const uint16_t *emb16, uint64_t mul, uint64_t add,
+                                    uint64_t T, uint64_t N, int64_t D, int64_t p0, int32_t rows, int32_t rows_pad,
+                                    uint16_t *X, int32_t *tgt, int32_t *wrow) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kPieces = kIn / 8;
+    const int64_t r = idx / kPieces;
+    const int pc = (int)(idx % kPieces);
+    if (r >= rows_pad) return;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r < rows) {
+        const uint64_t tau = (mul * (uint64_t)(p0 + r) + add) % T;
+        const uint64_t slot = tau / N, i = tau % N;
+        const uint32_t d = tok_doc[i], w = tok_word[i];
+        if (pc < kSketch / 8)
+            v = *reinterpret_cast<const uint4 *>(sketches + ((int64_t)slot * D + d) * kSketch + pc * 8);
+        else
+            v = *reinterpret_cast<const uint4 *>(emb16 + (int64_t)w * kEmb + (pc - kSketch / 8) * 8);
+        if (pc == 0) {
+            tgt[r] = zs[slot * N + i];
+            wrow[r] = (int32_t)w;
+        }
+    } else if (pc == 0) {
+        tgt[r] = -1;
+        wrow[r] = -1;
+    }
+    *reinterpret_cast<uint4 *>(X + r * kIn + pc * 8) = v;
+}
+
+// Query rows for explicit (global doc, word) pairs; rows past n are zero.
+__global__ void gather_query_kernel(const uint16_t *sketch, const uint32_t *docs, const uint32_t *words,
+                                    const uint16_t *emb16, int64_t n, int64_t rows_pad, uint16_t *X) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int kPieces = kIn / 8;
+    const int64_t r = idx / kPieces;
+    const int pc = (int)(idx % kPieces);
+    if (r >= rows_pad) return;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r < n) {
+        if (pc < kSketch / 8)
+            v = *reinterpret_cast<const uint4 *>(sketch + (int64_t)docs[r] * kSketch + pc * 8);
+        else
+            v = *reinterpret_cast<const uint4 *>(emb16 + (int64_t)words[r] * kEmb + (pc - kSketch / 8) * 8);
+    }
+    *reinterpret_cast<uint4 *>(X + r * kIn + pc * 8) = v;
+}
+
+// ------------------------------------------------------------ training
+// One wave per row: softmax over K logits, loss = log sum exp - logit[z],
+// G = (softmax - onehot(z)) * inv_rows in BF16 (zero for padding rows).
+__global__ __launch_bounds__(64) void softmax_ce_kernel(const float *logits, int32_t K, const int32_t *tgt,
+                                                        float inv_rows, uint16_t *G, float *loss) {
+    const int lane = threadIdx.x;
+    const int64_t r = blockIdx.x;
+    const float *l = logits + r * K;
+    uint16_t *gr = G + r * K;
+    const int32_t z = tgt[r];
+    if (z < 0) {
+        for (int k = lane; k < K; k += kWave) gr[k] = 0;
+        if (lane == 0) loss[r] = 0.f;
+        return;
+    }
+    float mx = -INFINITY;
+    for (int k = lane; k < K; k += kWave) mx = fmaxf(mx, l[k]);
+    mx = wave_max_f(mx);
+    float s = 0.f;
+    for (int k = lane; k < K; k += kWave) s += expf(l[k] - mx);
+    s = wave_sum_f(s);
+    const float inv_s = 1.0f / s;
+    for (int k = lane; k < K; k += kWave) {
+        const float p = expf(l[k] - mx) * inv_s;
+        gr[k] = f2bf((p - (k == z ? 1.0f : 0.0f)) * inv_rows);
+    }
+    if (lane == 0) loss[r] = logf(s) + mx - l[z];
+}
+
+// Column sums of a BF16 matrix, rows in order (bias gradients).
+__global__ void colsum_bf16_kernel(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r) s += __uint_as_float((uint32_t)x[(int64_t)r * ld + j] << 16);
+    out[j] = s;
+}
+
+// Embedding gradient: row r adds dX_emb[r] to word wrow[r]'s row, in
+// fixed point (2^-32 units, int64 atomics), so the sum is exact and the
+// same in every run whatever the order of the additions.
+__global__ void emb_grad_kernel(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)rows * kEmb) return;
+    const int64_t r = idx / kEmb;
+    const int c = (int)(idx % kEmb);
+    const int32_t w = wrow[r];
+    if (w < 0) return;
+    const long long q = (long long)rint((double)dxe[idx] * 4294967296.0);
+    if (q) atomicAdd(reinterpret_cast<unsigned long long *>(acc + (int64_t)w * kEmb + c), (unsigned long long)q);
+}
+
+// Adam (torch.optim.Adam's update order): m = b1 m + (1 - b1) g;
+// v = b2 v + (1 - b2) g^2; p += -step * m / (sqrt(v) / sqrt_bc2 + eps).
+// The gradient is FP32 (g) or fixed point (g64, cleared here).  Also
+// refreshes the BF16 copy.
+__global__ void adam_kernel(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
+                            float step, float b1, float b2, float eps, float sqrt_bc2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi;
+    if (g) {
+        gi = g[i];
+    } else {
+        gi = (float)((double)g64[i] * (1.0 / 4294967296.0));
+        g64[i] = 0;
+    }
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] + (-step) * (mi / (sqrtf(vi) / sqrt_bc2 + eps));
+    p[i] = pi;
+    if (p16) p16[i] = f2bf(pi);
+}
+
+__global__ void to_bf16_kernel(const float *x, uint16_t *y, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// Fixed-order sum of n floats (one wave), stored at out[0].
+__global__ __launch_bounds__(64) void sum_f32_kernel(const float *x, int32_t n, float *out) {
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += kWave) s += x[i];
+    s = wave_sum_f(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// ------------------------------------------------------------ MH step
+// One wave per token.  Proposal weights: W_k = max(1, floor(softmax_k *
+// K * 2^20)), the remainder added to the largest weight so that
+// sum W = K * 2^20 exactly; the proposal is then exactly W_k / (K 2^20).
+// Alias table (integer sweep, built in parallel): lights (W < M, M = 2^20)
+// and heavies in index order with inclusive prefix deficits D_a =
+// sum (M - W) and excesses E_b = sum (W - M).  Light a keeps W and aliases
+// the first heavy b with E_b > D_{a-1}; heavy b (not the last) is emptied
+// after a* = min{a : D_a >= E_b} lights, keeps M + E_b - D_{a*} and
+// aliases heavy b + 1; the last heavy keeps M.  Every weight's mass is
+// preserved exactly.  Then `proposals` independence-MH steps on the
+// token's collapsed conditional with frozen counts (own assignment
+// included, as in the plain sampler), p(k) = (n_dk + alpha)(n_wk + beta) /
+// (n_k + V beta) in FP64: propose k' from the table (bucket j = K u0,
+// threshold test on 20 more bits), accept iff
+// u (p(k) W_k') < p(k') W_k.  Draws: Philox((g, sweep, 2, r), (seed, 0)).
+__global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
+    __shared__ uint32_t sW[kMaxK], sF[kMaxK], sD[kMaxK], sE[kMaxK];
+    __shared__ uint16_t sA[kMaxK], sHI[kMaxK];
+    const int lane = threadIdx.x;
+    const int r = blockIdx.x;
+    if (r >= a.n) return;
+    const int K = a.model.k, KL = K / kWave;
+    constexpr uint32_t M = 1u << kAliasBits;
+    const float *lg = a.logits + (int64_t)r * K;
+    float e[16];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        e[j] = j < KL ? lg[lane * KL + j] : -INFINITY;
+        mx = fmaxf(mx, e[j]);
+    }
+    mx = wave_max_f(mx);
+    float s = 0.f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        e[j] = j < KL ? expf(e[j] - mx) : 0.f;
+        s += e[j];
+    }
+    s = wave_sum_f(s);
+    const uint32_t KM = (uint32_t)K << kAliasBits;
+    const float scale = (float)KM / s;
+    uint32_t W[16];
+    unsigned long long lsum = 0, best = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j < KL) {
+            const float x = e[j] * scale;
+            uint32_t wv = x >= 1.0f ? (uint32_t)x : 1u;
+            if (wv > KM) wv = KM;
+            W[j] = wv;
+            lsum += wv;
+            const uint32_t k = lane * KL + j;
+            const unsigned long long key = ((unsigned long long)wv << 32) | (0xffffffffu - k);
+            best = key > best ? key : best;
+        } else {
+            W[j] = 0;
+        }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        lsum += __shfl_xor(lsum, d, kWave);
+        const unsigned long long o = __shfl_xor(best, d, kWave);
+        best = o > best ? o : best;
+    }
+    const long long rem = (long long)KM - (long long)lsum;
+    const uint32_t kmax = 0xffffffffu - (uint32_t)(best & 0xffffffffu);
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (j < KL && (uint32_t)(lane * KL + j) == kmax) W[j] = (uint32_t)((long long)W[j] + rem);
+    // lane-local light / heavy counts and sums, then wave prefixes
+    uint32_t cL = 0, sDef = 0, cH = 0, sExc = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (j < KL) {
+            if (W[j] < M) {
+                ++cL;
+                sDef += M - W[j];
+            } else {
+                ++cH;
+                sExc += W[j] - M;
+            }
+        }
+    uint32_t nL, nH, totDef, totExc;
+    uint32_t aL = wave_exclusive_u32(cL, &nL), dAcc = wave_exclusive_u32(sDef, &totDef);
+    uint32_t bH = wave_exclusive_u32(cH, &nH), eAcc = wave_exclusive_u32(sExc, &totExc);
+    uint16_t rank[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (j < KL) {
+            const uint32_t k = lane * KL + j;
+            sW[k] = W[j];
+            if (W[j] < M) {
+                dAcc += M - W[j];
+                sD[aL] = dAcc;
+                rank[j] = (uint16_t)aL++;
+            } else {
+                eAcc += W[j] - M;
+                sHI[bH] = (uint16_t)k;
+                sE[bH] = eAcc;
+                rank[j] = (uint16_t)bH++;
+            }
+        }
+    wave_sync();
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (j < KL) {
+            const uint32_t k = lane * KL + j;
+            const uint32_t rk = rank[j];
+            if (W[j] < M) {
+                const uint32_t dprev = rk ? sD[rk - 1] : 0u;
+                uint32_t lo = 0, hi = nH;  // first b with E_b > dprev
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (sE[mid] > dprev) hi = mid; else lo = mid + 1;
+                }
+                sF[k] = W[j];
+                sA[k] = sHI[lo < nH ? lo : nH - 1];
+            } else if (rk + 1 == nH) {
+                sF[k] = M;
+                sA[k] = (uint16_t)k;
+            } else {
+                const uint32_t eb = sE[rk];
+                uint32_t astar = 0;
+                if (eb) {
+                    uint32_t lo = 0, hi = nL;  // first i with D_i >= eb
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (sD[mid] >= eb) hi = mid; else lo = mid + 1;
+                    }
+                    astar = lo + 1;
+                }
+                const uint32_t dstar = astar ? sD[astar - 1] : 0u;
+                sF[k] = M + eb - dstar;
+                sA[k] = sHI[rk + 1];
+            }
+        }
+    wave_sync();
+    if (a.w_out)
+        for (int k = lane; k < K; k += kWave) {
+            a.w_out[(int64_t)r * K + k] = (int32_t)sW[k];
+            a.f_out[(int64_t)r * K + k] = sF[k];
+            a.a_out[(int64_t)r * K + k] = sA[k];
+        }
+    // MH steps (wave-uniform)
+    const int64_t i = a.start + r;
+    const uint32_t d = a.unit.tokens[i];
+    const uint32_t w = a.word[i];
+    const uint64_t gid = (uint64_t)(i + a.gdelta[w]);
+    const DocMeta dm = a.unit.meta[d];
+    const uint32_t *trow = a.unit.theta + dm.start;
+    const uint32_t cb = a.model.count_bits, cmask = (1u << cb) - 1u;
+    const int32_t *nrow = a.model.nwk + (int64_t)w * K;
+    const double alpha = a.model.alpha, beta = a.model.beta;
+    const double vbeta = (double)a.model.num_words * beta;
+    auto target = [&](uint32_t k) {
+        uint32_t lo = 0, hi = dm.nnz;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (((trow[mid] >> cb) >> 2) < k) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t ndk = (lo < dm.nnz && ((trow[lo] >> cb) >> 2) == k) ? (trow[lo] & cmask) : 0u;
+        return (((double)ndk + alpha) * ((double)nrow[k] + beta)) / ((double)a.model.nk[k] + vbeta);
+    };
+    uint32_t cur = a.unit.z[i];
+    double pc = target(cur);
+    uint32_t acc = 0;
+    for (int p = 0; p < a.proposals; ++p) {
+        const u64x4 x = philox4x64_10(gid, a.sweep, kPurposeMH, (uint64_t)p, a.model.seed, 0);
+        const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
+        const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
+        const double u = (double)(x.v[2] >> 11) * 0x1.0p-53;
+        const uint32_t kp = rr < sF[j] ? j : (uint32_t)sA[j];
+        if (kp != cur) {
+            const double pp = target(kp);
+            if (u * (pc * (double)sW[kp]) < pp * (double)sW[cur]) {
+                cur = kp;
+                pc = pp;
+                ++acc;
+            }
+        }
+    }
+    if (lane == 0) {
+        a.unit.z[i] = (uint16_t)cur;
+        if (acc) atomicAdd(a.accepted, (unsigned long long)acc);
+    }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s) {
+    if (g.m <= 0 || g.n <= 0 || g.k <= 0) return hipSuccess;
+    if (g.m % kGemmTile || g.n % kGemmTile || g.k % kGemmK || g.lda % 8 || g.ldb % 8 ||
+        ((uintptr_t)g.a & 15) || ((uintptr_t)g.b & 15) || g.m / kGemmTile > 65535)
+        return hipErrorInvalidValue;
+    if (g.epilogue == kEpiGeluGrad && (!g.c16 || !g.aux)) return hipErrorInvalidValue;
+    const dim3 grid(g.n / kGemmTile, g.m / kGemmTile);
+    if (!ta && !tb)
+        hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, g);
+    else if (ta && !tb)
+        hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, g);
+    else if (!ta && tb)
+        hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, g);
+    else
+        hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
+                         uint16_t *out, hipStream_t s) {
+    if (num_docs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sketch_kernel, dim3(num_docs), dim3(kWave), 0, s, meta, theta, num_docs, count_bits, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
+                               const uint32_t *tok_word, const uint16_t *emb16, uint64_t mul, uint64_t add, uint64_t T,
+                               uint64_t N, int64_t D, int64_t p0, int32_t rows, int32_t rows_pad, uint16_t *X,
+                               int32_t *tgt, int32_t *wrow, hipStream_t s) {
+    const int64_t n = (int64_t)rows_pad * (kIn / 8);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_train_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sketches, zs, tok_doc,
+                       tok_word, emb16, mul, add, T, N, D, p0, rows, rows_pad, X, tgt, wrow);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_query(const uint16_t *sketch, const uint32_t *docs, const uint32_t *words,
+                               const uint16_t *emb16, int64_t n, int64_t rows_pad, uint16_t *X, hipStream_t s) {
+    const int64_t m = rows_pad * (kIn / 8);
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_query_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, sketch, docs, words,
+                       emb16, n, rows_pad, X);
+    return hipGetLastError();
+}
+
+hipError_t launch_softmax_ce(const float *logits, int32_t K, const int32_t *tgt, float inv_rows, uint16_t *G,
+                             float *loss, int32_t rows_pad, hipStream_t s) {
+    if (rows_pad <= 0) return hipSuccess;
+    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows_pad), dim3(kWave), 0, s, logits, K, tgt, inv_rows, G, loss);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out, hipStream_t s) {
+    if (cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, x, ld, rows, cols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc, hipStream_t s) {
+    const int64_t n = (int64_t)rows * kEmb;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(emb_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dxe, wrow, rows, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
+                       float step, float b1, float b2, float eps, float sqrt_bc2, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, g64, p16, n, step,
+                       b1, b2, eps, sqrt_bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_f32_kernel, dim3(1), dim3(kWave), 0, s, x, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mh(const MHArgs &a, hipStream_t s) {
+    if (a.n <= 0) return hipSuccess;
+    if (a.model.k % kWave || a.model.k > kMaxK) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mh_kernel, dim3(a.n), dim3(kWave), 0, s, a);
     return hipGetLastError();
 }
 

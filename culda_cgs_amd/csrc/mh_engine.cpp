@@ -11,6 +11,7 @@
 
 #include "engine.h"
 #include "kernels.h"
+#include "mh_state.h"
 
 namespace culda {
 
@@ -77,6 +78,334 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     phi_ready = true;
     theta_ready = true;
     return sum / (double)eval.num_tokens;
+}
+
+}  // namespace culda
+
+// ====================================================== learned MH proposal
+namespace culda {
+
+
+MHState &Engine::mh_state() {
+    if (!mh) fail(CULDA_ERR_STATE, "learned MH not enabled (culda_mh_enable)");
+    return *mh;
+}
+
+static uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static void fill_uniform(std::vector<float> &v, float bound, uint64_t &s) {
+    for (float &x : v) x = (float)(((double)(splitmix64(s) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * bound);
+}
+
+static int64_t round64(int64_t n) { return (n + 63) / 64 * 64; }
+
+void Engine::mh_enable(const culda_mh_params &p_in) {
+    if (!uploaded || models.size() != 1) fail(CULDA_ERR_STATE, "learned MH needs one uploaded corpus");
+    const ModelHost &m = models[0];
+    if (m.K % 64) fail(CULDA_ERR_ARG, "learned MH needs k %% 64 == 0 (k = %d)", m.K);
+    culda_mh_params p = p_in;
+    if (!p.batch) p.batch = 8192;
+    if (!p.infer_batch) p.infer_batch = 65536;
+    if (!p.max_snapshots) p.max_snapshots = 21;
+    if (!(p.lr > 0)) p.lr = 1e-3f;
+    if (!(p.beta1 > 0)) p.beta1 = 0.9f;
+    if (!(p.beta2 > 0)) p.beta2 = 0.999f;
+    if (!(p.eps > 0)) p.eps = 1e-8f;
+    if (p.batch < 64 || p.batch % 64 || p.infer_batch < 64 || p.infer_batch % 64 || p.max_snapshots < 1)
+        fail(CULDA_ERR_ARG, "batch and infer_batch must be positive multiples of 64, max_snapshots >= 1");
+    if ((int64_t)p.infer_batch / 64 > 65535) fail(CULDA_ERR_ARG, "infer_batch too large");
+    mh.reset(new MHState);
+    MHState &s = *mh;
+    s.prm = p;
+    s.K = m.K;
+    s.V = m.V;
+    s.D = m.num_docs;
+    s.N = m.num_tokens;
+    if ((uint64_t)s.N * (uint64_t)p.max_snapshots >= (1ULL << 32))
+        fail(CULDA_ERR_LIMIT, "training triples (tokens x snapshots) must stay below 2^32");
+    const int64_t K = s.K, V = s.V;
+    auto falloc = [&](int64_t n) { return (float *)dalloc(std::max<int64_t>(n, 1) * 4); };
+    auto hzero = [&](void *ptr, int64_t bytes) {
+        if (bytes > 0) hip_check(hipMemset(ptr, 0, bytes), "clear");
+    };
+    // weights (torch Linear-style uniform init, embeddings U(-0.1, 0.1))
+    uint64_t seed = p.init_seed ^ 0x6d68u;
+    std::vector<float> hw1((size_t)kIn * kHidden), hb1(kHidden), hw2((size_t)kHidden * K), hb2(K),
+        he((size_t)V * kEmb);
+    fill_uniform(hw1, 1.0f / std::sqrt((float)kIn), seed);
+    fill_uniform(hb1, 1.0f / std::sqrt((float)kIn), seed);
+    fill_uniform(hw2, 1.0f / std::sqrt((float)kHidden), seed);
+    fill_uniform(hb2, 1.0f / std::sqrt((float)kHidden), seed);
+    fill_uniform(he, 0.1f, seed);
+    s.w1 = falloc(hw1.size());
+    s.b1 = falloc(hb1.size());
+    s.w2 = falloc(hw2.size());
+    s.b2 = falloc(hb2.size());
+    s.emb = falloc(he.size());
+    float **moments[] = {&s.m_w1, &s.v_w1, &s.m_b1, &s.v_b1, &s.m_w2, &s.v_w2, &s.m_b2, &s.v_b2, &s.m_emb, &s.v_emb};
+    const int64_t sizes[] = {kIn * kHidden, kHidden, kHidden * K, K, V * kEmb};
+    for (int i = 0; i < 10; ++i) {
+        *moments[i] = falloc(sizes[i / 2]);
+        hzero(*moments[i], sizes[i / 2] * 4);
+    }
+    s.w1h = (uint16_t *)dalloc(hw1.size() * 2);
+    s.w2h = (uint16_t *)dalloc(hw2.size() * 2);
+    s.embh = (uint16_t *)dalloc(he.size() * 2);
+    s.g_w1 = falloc(hw1.size());
+    s.g_b1 = falloc(kHidden);
+    s.g_w2 = falloc(hw2.size());
+    s.g_b2 = falloc(K);
+    s.g_emb = (long long *)dalloc(he.size() * 8);
+    hzero(s.g_emb, he.size() * 8);
+    // per-token doc / word (global token order = units in order)
+    s.tok_doc = (uint32_t *)dalloc(std::max<int64_t>(s.N, 1) * 4);
+    s.tok_word = (uint32_t *)dalloc(std::max<int64_t>(s.N, 1) * 4);
+    {
+        std::vector<int64_t> pre(V + 1, 0);
+        for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui)
+            for (int32_t w = 0; w < V; ++w) pre[w + 1] += units[ui].word_idx_h[w + 1] - units[ui].word_idx_h[w];
+        for (int32_t w = 0; w < V; ++w) pre[w + 1] += pre[w];
+        std::vector<int64_t> run(pre.begin(), pre.end() - 1);
+        int64_t base = 0;
+        for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+            const UnitHost &u = units[ui];
+            std::vector<uint32_t> loc(u.num_tokens), doc(u.num_tokens), word(u.num_tokens);
+            if (u.num_tokens)
+                hip_check(hipMemcpy(loc.data(), u.tokens, u.num_tokens * 4, hipMemcpyDeviceToHost), "tokens copy");
+            std::vector<int64_t> gd(V);
+            for (int32_t w = 0; w < V; ++w) {
+                gd[w] = run[w] - u.word_idx_h[w];
+                for (int64_t i = u.word_idx_h[w]; i < u.word_idx_h[w + 1]; ++i) {
+                    word[i] = (uint32_t)w;
+                    doc[i] = loc[i] + (uint32_t)u.doc_start;
+                }
+                run[w] += u.word_idx_h[w + 1] - u.word_idx_h[w];
+            }
+            if (u.num_tokens) {
+                hip_check(hipMemcpy(s.tok_doc + base, doc.data(), u.num_tokens * 4, hipMemcpyHostToDevice), "copy");
+                hip_check(hipMemcpy(s.tok_word + base, word.data(), u.num_tokens * 4, hipMemcpyHostToDevice), "copy");
+            }
+            int64_t *dg = (int64_t *)dalloc(V * 8);
+            hip_check(hipMemcpy(dg, gd.data(), V * 8, hipMemcpyHostToDevice), "copy");
+            s.gdelta.push_back(dg);
+            s.unit_base.push_back(base);
+            base += u.num_tokens;
+        }
+    }
+    s.zs = (uint16_t *)dalloc(std::max<int64_t>(s.N, 1) * 2 * p.max_snapshots);
+    s.sks = (uint16_t *)dalloc((int64_t)std::max(s.D, 1) * kSketch * 2 * p.max_snapshots);
+    s.sk_cur = (uint16_t *)dalloc((int64_t)std::max(s.D, 1) * kSketch * 2);
+    const int64_t B = p.batch;
+    s.X = (uint16_t *)dalloc(B * kIn * 2);
+    s.H = (uint16_t *)dalloc(B * kHidden * 2);
+    s.G = (uint16_t *)dalloc(B * K * 2);
+    s.dP = (uint16_t *)dalloc(B * kHidden * 2);
+    s.P = falloc(B * kHidden);
+    s.L = falloc(B * K);
+    s.dXe = falloc(B * kEmb);
+    s.loss = falloc(B);
+    s.tgt = (int32_t *)dalloc(B * 4);
+    s.wrow = (int32_t *)dalloc(B * 4);
+    s.q_doc = (uint32_t *)dalloc(std::max<int64_t>(B, p.infer_batch) * 4);
+    s.q_word = (uint32_t *)dalloc(std::max<int64_t>(B, p.infer_batch) * 4);
+    const int64_t IB = p.infer_batch;
+    s.Xi = (uint16_t *)dalloc(IB * kIn * 2);
+    s.Hi = (uint16_t *)dalloc(IB * kHidden * 2);
+    s.Li = falloc(IB * K);
+    s.accepted = (unsigned long long *)dalloc(8);
+    hip_check(hipDeviceSynchronize(), "mh buffers");
+    // upload the initial weights (and their BF16 copies)
+    hip_check(hipMemcpy(s.w1, hw1.data(), hw1.size() * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(s.b1, hb1.data(), hb1.size() * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(s.w2, hw2.data(), hw2.size() * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(s.b2, hb2.data(), hb2.size() * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(hipMemcpy(s.emb, he.data(), he.size() * 4, hipMemcpyHostToDevice), "copy");
+    hip_check(launch_to_bf16(s.w1, s.w1h, hw1.size(), stream), "bf16");
+    hip_check(launch_to_bf16(s.w2, s.w2h, hw2.size(), stream), "bf16");
+    hip_check(launch_to_bf16(s.emb, s.embh, he.size(), stream), "bf16");
+    hip_check(hipStreamSynchronize(stream), "mh init");
+}
+
+void Engine::mh_sketch_current(hipStream_t st) {
+    MHState &s = mh_state();
+    const ModelHost &m = models[0];
+    for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+        const UnitHost &u = units[ui];
+        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits,
+                                s.sk_cur + (int64_t)u.doc_start * kSketch, st),
+                  "sketch");
+    }
+}
+
+int32_t Engine::mh_collect() {
+    MHState &s = mh_state();
+    if (!theta_ready) fail(CULDA_ERR_STATE, "collect needs rebuilt counts");
+    if (s.nsnap >= s.prm.max_snapshots) fail(CULDA_ERR_LIMIT, "all %d snapshots taken", s.prm.max_snapshots);
+    hip_check(hipStreamSynchronize(stream2), "sync");
+    const ModelHost &m = models[0];
+    for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+        const UnitHost &u = units[ui];
+        if (u.num_tokens)
+            hip_check(hipMemcpyAsync(s.zs + (int64_t)s.nsnap * s.N + s.unit_base[ui - m.unit_begin], u.z,
+                                     u.num_tokens * 2, hipMemcpyDeviceToDevice, stream),
+                      "z snapshot");
+        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits,
+                                s.sks + ((int64_t)s.nsnap * s.D + u.doc_start) * kSketch, stream),
+                  "sketch");
+    }
+    hip_check(hipStreamSynchronize(stream), "collect");
+    return ++s.nsnap;
+}
+
+// Forward + backward of the rows already gathered into s.X / s.tgt /
+// s.wrow (rows_pad = batch rows), gradients into s.g_*; per-row loss sum
+// into *loss_sum (device).  Then optionally one Adam step.
+void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, float *loss_sum, hipStream_t st) {
+    const int32_t B = s.prm.batch, K = s.K;
+    auto gemm = [&](const GemmArgs &g, bool ta, bool tb, const char *what) {
+        hip_check(launch_gemm_bf16(g, ta, tb, st), what);
+    };
+    GemmArgs f1{s.X, s.w1h, s.P, s.H, s.b1, nullptr, kIn, kHidden, kHidden, kHidden, 0, B, kHidden, kIn, kEpiBiasGelu};
+    gemm(f1, false, false, "gemm x.w1");
+    GemmArgs f2{s.H, s.w2h, s.L, nullptr, s.b2, nullptr, kHidden, K, K, 0, 0, B, K, kHidden, kEpiStore};
+    gemm(f2, false, false, "gemm h.w2");
+    hip_check(launch_softmax_ce(s.L, K, s.tgt, 1.0f / (float)std::max(rows, 1), s.G, s.loss, B, st), "softmax");
+    if (loss_sum) hip_check(launch_sum_f32(s.loss, B, loss_sum, st), "loss sum");
+    GemmArgs gw2{s.H, s.G, s.g_w2, nullptr, nullptr, nullptr, kHidden, K, K, 0, 0, kHidden, K, B, kEpiStore};
+    gemm(gw2, true, false, "gemm h^T.g");
+    hip_check(launch_colsum_bf16(s.G, K, B, K, s.g_b2, st), "colsum");
+    GemmArgs dp{s.G, s.w2h, nullptr, s.dP, nullptr, s.P, K, K, 0, kHidden, kHidden, B, kHidden, K, kEpiGeluGrad};
+    gemm(dp, false, true, "gemm g.w2^T");
+    hip_check(launch_colsum_bf16(s.dP, kHidden, B, kHidden, s.g_b1, st), "colsum");
+    GemmArgs gw1{s.X, s.dP, s.g_w1, nullptr, nullptr, nullptr, kIn, kHidden, kHidden, 0, 0, kIn, kHidden, B, kEpiStore};
+    gemm(gw1, true, false, "gemm x^T.dp");
+    GemmArgs dx{s.dP, s.w1h + (int64_t)kSketch * kHidden, s.dXe, nullptr, nullptr, nullptr, kHidden, kHidden, kEmb,
+                0, 0, B, kEmb, kHidden, kEpiStore};
+    gemm(dx, false, true, "gemm dp.w1e^T");
+    hip_check(launch_emb_grad(s.dXe, s.wrow, B, s.g_emb, st), "emb grad");
+    if (update) mh_apply_adam(e, s, st);
+}
+
+// One Adam step on every weight from the gradients in s.g_* (the
+// embedding accumulator is cleared by the update).
+void mh_apply_adam(Engine &e, MHState &s, hipStream_t st) {
+    const int32_t K = s.K;
+    ++s.adam_t;
+    const double bc1 = 1.0 - std::pow((double)s.prm.beta1, (double)s.adam_t);
+    const double bc2 = 1.0 - std::pow((double)s.prm.beta2, (double)s.adam_t);
+    const float step = (float)(s.prm.lr / bc1), sq = (float)std::sqrt(bc2);
+    const float b1 = s.prm.beta1, b2 = s.prm.beta2, eps = s.prm.eps;
+    hip_check(launch_adam(s.w1, s.m_w1, s.v_w1, s.g_w1, nullptr, s.w1h, (int64_t)kIn * kHidden, step, b1, b2, eps, sq, st), "adam");
+    hip_check(launch_adam(s.b1, s.m_b1, s.v_b1, s.g_b1, nullptr, nullptr, kHidden, step, b1, b2, eps, sq, st), "adam");
+    hip_check(launch_adam(s.w2, s.m_w2, s.v_w2, s.g_w2, nullptr, s.w2h, (int64_t)kHidden * K, step, b1, b2, eps, sq, st), "adam");
+    hip_check(launch_adam(s.b2, s.m_b2, s.v_b2, s.g_b2, nullptr, nullptr, K, step, b1, b2, eps, sq, st), "adam");
+    hip_check(launch_adam(s.emb, s.m_emb, s.v_emb, nullptr, s.g_emb, s.embh, (int64_t)s.V * kEmb, step, b1, b2, eps, sq, st),
+              "adam");
+    (void)e;
+}
+
+static uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+double Engine::mh_train(int32_t epochs) {
+    MHState &s = mh_state();
+    if (epochs < 1) fail(CULDA_ERR_ARG, "epochs must be >= 1");
+    if (!s.nsnap) fail(CULDA_ERR_STATE, "no training snapshots (culda_mh_collect)");
+    const uint64_t T = (uint64_t)s.nsnap * (uint64_t)s.N;
+    const int64_t B = s.prm.batch;
+    const int64_t steps = (int64_t)((T + B - 1) / B);
+    float *step_loss = (float *)dalloc(std::max<int64_t>(steps, 1) * 4);
+    uint64_t rs = s.prm.init_seed ^ 0x747261696eULL;
+    for (int32_t ep = 0; ep < epochs; ++ep) {
+        uint64_t mul = T > 1 ? splitmix64(rs) % (T - 1) + 1 : 1;
+        while (gcd64(mul, T) != 1) mul = mul % (T - 1) + 1;
+        const uint64_t add = T > 1 ? splitmix64(rs) % T : 0;
+        for (int64_t st = 0; st < steps; ++st) {
+            const int64_t p0 = st * B;
+            const int32_t rows = (int32_t)std::min<int64_t>(B, (int64_t)T - p0);
+            hip_check(launch_gather_train(s.sks, s.zs, s.tok_doc, s.tok_word, s.embh, mul, add, T, (uint64_t)s.N, s.D, p0,
+                                          rows, (int32_t)B, s.X, s.tgt, s.wrow, stream),
+                      "gather");
+            mh_batch_step(*this, s, rows, true, step_loss + st, stream);
+        }
+    }
+    std::vector<float> hl(steps);
+    hip_check(hipMemcpyAsync(hl.data(), step_loss, steps * 4, hipMemcpyDeviceToHost, stream), "loss copy");
+    hip_check(hipStreamSynchronize(stream), "train");
+    dfree(step_loss);
+    double sum = 0;
+    for (float v : hl) sum += v;
+    return sum / (double)T;
+}
+
+// Logits [n][K] (rows padded to 64 inside) for explicit (global doc, word)
+// queries against s.sk_cur.
+void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, float *d_logits,
+                             hipStream_t st) {
+    MHState &s = mh_state();
+    const int64_t np = round64(n);
+    hip_check(launch_gather_query(s.sk_cur, d_docs, d_words, s.embh, n, np, s.Xi, st), "gather");
+    GemmArgs f1{s.Xi, s.w1h, nullptr, s.Hi, s.b1, nullptr, kIn, kHidden, 0, kHidden, 0, (int32_t)np, kHidden, kIn,
+                kEpiBiasGelu};
+    hip_check(launch_gemm_bf16(f1, false, false, st), "gemm x.w1");
+    GemmArgs f2{s.Hi, s.w2h, d_logits, nullptr, s.b2, nullptr, kHidden, s.K, s.K, 0, 0, (int32_t)np, s.K, kHidden,
+                kEpiStore};
+    hip_check(launch_gemm_bf16(f2, false, false, st), "gemm h.w2");
+}
+
+double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
+    MHState &s = mh_state();
+    if (!theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "MH iteration needs rebuilt counts");
+    if (proposals < 0) fail(CULDA_ERR_ARG, "proposals must be >= 0");
+    hip_check(hipStreamSynchronize(stream2), "sync");
+    const ModelHost &m = models[0];
+    hip_check(launch_fill_u32((uint32_t *)s.accepted, 2, 0, stream), "clear");
+    mh_sketch_current(stream);
+    const int64_t IB = s.prm.infer_batch;
+    for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
+        const UnitHost &u = units[ui];
+        const int64_t base = s.unit_base[ui - m.unit_begin];
+        for (int64_t start = 0; start < u.num_tokens; start += IB) {
+            const int64_t n = std::min(IB, u.num_tokens - start);
+            mh_query_logits(s.tok_doc + base + start, s.tok_word + base + start, n, s.Li, stream);
+            MHArgs a{};
+            a.logits = s.Li;
+            a.word = s.tok_word + base;
+            a.gdelta = s.gdelta[ui - m.unit_begin];
+            a.unit = UnitDev{u.tokens, u.z, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+                             u.doc_end - u.doc_start};
+            a.model = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
+                               m.count_bits, 0};
+            a.accepted = s.accepted;
+            a.start = start;
+            a.n = (int32_t)n;
+            a.proposals = proposals;
+            a.sweep = sweep;
+            hip_check(launch_mh(a, stream), "mh launch");
+        }
+    }
+    // the count rebuilds of a plain iteration
+    rebuild_phi_all(stream);
+    phi_head(stream);
+    rebuild_theta_all(stream);
+    unsigned long long acc = 0;
+    hip_check(hipMemcpyAsync(&acc, s.accepted, 8, hipMemcpyDeviceToHost, stream), "copy");
+    hip_check(hipStreamSynchronize(stream), "mh iteration");
+    read_stats();
+    sampled = true;
+    const double tried = (double)s.N * (double)proposals;
+    return tried > 0 ? (double)acc / tried : 0.0;
 }
 
 }  // namespace culda

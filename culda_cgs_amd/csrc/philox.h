@@ -62,7 +62,9 @@ CULDA_HD u64x4 philox4x64_10(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
 }
 
 // Purposes (third counter word).
-enum : uint64_t { kPurposeInit = 0, kPurposeSweep = 1 };
+// Counter word 2 separates the streams: z init, plain sweeps, and the
+// learned-MH proposals (word 3 = proposal index within the sweep).
+enum : uint64_t { kPurposeInit = 0, kPurposeSweep = 1, kPurposeMH = 2 };
 
 // First output word for (global token id, sweep, purpose).
 CULDA_HD uint64_t token_bits(uint64_t seed, uint64_t gid, uint64_t sweep, uint64_t purpose) {

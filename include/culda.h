@@ -325,6 +325,58 @@ int culda_heldout_set_eval(culda_ctx *fold, const culda_corpus *eval);
 int culda_heldout_ll(culda_ctx *train, culda_ctx *fold, int32_t fold_in_sweeps, double *ll_per_token);
 
 /* ------------------------------------------------------------------------
+ * Learned Metropolis-Hastings proposal (no counterpart in the reference).
+ * An MLP maps [256-wide signed sketch of the doc's topic counts / n_d;
+ * 128-wide learned word embedding] through 512 tanh-GELU units to K
+ * logits (BF16 operands on the matrix cores, FP32 accumulation, FP32
+ * master weights trained with Adam on (doc, word, z) triples collected from
+ * the plain sampler's own chain).  A sweep then proposes, for every token,
+ * from the integer-weight alias table of its softmax and accepts with the
+ * exact MH ratio of the token's collapsed conditional (the plain sampler's
+ * frozen-count conditional), so the plain sampler's per-token target is
+ * left invariant.  Requires k % 64 == 0 and one data_chunk.
+ * ---------------------------------------------------------------------- */
+typedef struct culda_mh_params {
+    int32_t batch;          /* training rows per Adam step, multiple of 64 (0: 8192)   */
+    int32_t infer_batch;    /* tokens per proposal launch, multiple of 64 (0: 65536)  */
+    int32_t max_snapshots;  /* training snapshots kept (0: 21)                         */
+    int32_t pad;
+    float lr, beta1, beta2, eps;  /* Adam (0: 1e-3, 0.9, 0.999, 1e-8)                 */
+    uint64_t init_seed;     /* weight initialisation and training order                */
+} culda_mh_params;
+
+/* Allocate the MLP, its optimiser state and buffers for the uploaded corpus. */
+int culda_mh_enable(culda_ctx *ctx, const culda_mh_params *params);
+/* Keep the current state (z and every doc's sketch, from the last rebuild)
+ * as one snapshot of training triples; returns the number of snapshots. */
+int culda_mh_collect(culda_ctx *ctx, int32_t *num_snapshots);
+/* epochs passes over every (snapshot, token) triple in a seeded order;
+ * mean cross-entropy of the last epoch. */
+int culda_mh_train(culda_ctx *ctx, int32_t epochs, double *mean_loss, double *seconds);
+/* One learned-MH iteration: `proposals` MH steps per token from the
+ * current state, then the count rebuilds of a plain iteration. */
+int culda_mh_iterate(culda_ctx *ctx, int32_t iteration, int32_t proposals, double *accept_rate, double *seconds);
+/* Weights (row-major, input x output): w1 384x512, b1 512, w2 512xK, b2 K,
+ * emb Vx128; get copies the FP32 masters out, set replaces them. */
+int culda_mh_get_weights(culda_ctx *ctx, float *w1, float *b1, float *w2, float *b2, float *emb);
+int culda_mh_set_weights(culda_ctx *ctx, const float *w1, const float *b1, const float *w2, const float *b2,
+                         const float *emb);
+/* Logits [n][K] for (doc, word) queries, doc sketches from the current
+ * counts. */
+int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, const int32_t *words, float *logits);
+/* Gradients of the mean cross-entropy over n rows (n <= batch) of
+ * (doc, word, target) at the current weights, without updating them;
+ * train_step = 1 also applies one Adam step with them. */
+int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs, const int32_t *words, const int32_t *targets,
+                       int32_t train_step, double *loss, float *dw1, float *db1, float *dw2, float *db2, float *demb);
+/* Proposal tables of tokens [first, first + n) (n <= infer_batch): integer
+ * weights, alias thresholds and aliases, [n][K] each; z is unchanged. */
+int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t *weights, uint32_t *thresholds,
+                    uint32_t *aliases);
+/* Current doc sketches, [D][256] BF16 bit patterns. */
+int culda_mh_sketch(culda_ctx *ctx, uint16_t *out);
+
+/* ------------------------------------------------------------------------
  * Batch: many independent corpora sampled together (no counterpart in the
  * reference, which runs one corpus per process).  Each corpus is its own
  * model (counts, alpha, beta, seed); one iteration of the whole batch is a
