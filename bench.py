@@ -56,8 +56,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4"])
     ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
+    ap.add_argument("--iters", type=int, default=200, help="c4: plain iterations that fix the target")
+    ap.add_argument("--mh-max-iters", type=int, default=100, help="c4: learned-MH chain length (iterations)")
+    ap.add_argument("--eval-every", type=int, default=5, help="c4: held-out evaluation period")
+    ap.add_argument("--fold-in", type=int, default=10, help="c4: fold-in sweeps per evaluation")
+    ap.add_argument("--epochs", type=int, default=2, help="c4: MLP training epochs")
     ap.add_argument("--k", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--slot-tokens", type=int, default=0)
@@ -74,6 +79,9 @@ def main():
 
     if args.config == "c3":
         run_batch(cg, args)
+        return
+    if args.config == "c4":
+        run_mh(cg, args)
         return
     cfg = CONFIGS[args.config]
     K = args.k
@@ -244,6 +252,132 @@ def run_batch(cg, args):
         "cpu_baseline": cpu,
     }
     b.close()
+    print(json.dumps(line), flush=True)
+
+
+def run_mh(cg, args):
+    """Time to a fixed held-out LL/token, plain CGS vs learned MH, on the
+    1M-token corpus with every 10th doc held out (document completion)."""
+    cfg = CONFIGS["c1"]
+    K = args.k
+    alpha, beta = 50.0 / K, 0.01
+    full = cg.Corpus.generate(cfg["V"], cfg["D"], cfg["KT"], 1, len_kind=cfg["len_kind"], len_mean=cfg["len_mean"],
+                              doc_topic_prior=cfg["doc_prior"], topic_word_prior=cfg["word_prior"],
+                              zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=min(16, os.cpu_count() or 1))
+    train, obs, ev = full.heldout_split(10, 1)
+    N = train.info().num_tokens
+    log(f"c4: train N={N}, held-out observed {obs.info().num_tokens}, evaluated {ev.info().num_tokens}")
+
+    def timed(fn, *a):
+        t0 = time.perf_counter()
+        out = fn(*a)
+        return out, time.perf_counter() - t0
+
+    def sampler():
+        s = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=args.seed, slot_tokens=args.slot_tokens)
+        s.upload(train)
+        s.init_z()
+        s.rebuild_all()
+        s.sync()
+        return s
+
+    fold = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=args.seed + 1000)
+    fold.upload(obs)
+    fold.set_eval(ev)
+    evaluate = lambda s: fold.heldout_ll(s, args.fold_in)
+
+    # plain CGS: the target is its held-out LL after args.iters iterations
+    s = sampler()
+    clock, plain = 0.0, [(0, 0.0, evaluate(s))]
+    for it in range(1, args.iters + 1):
+        _, dt = timed(lambda: (s.iterate(it), s.sync()))
+        clock += dt
+        if it % args.eval_every == 0 or it == args.iters:
+            plain.append((it, clock, evaluate(s)))
+    plain_iter_ms = clock / args.iters * 1e3
+    s.close()
+    target = plain[-1][2]
+    thr = target - 0.005 * abs(target)
+    t_plain = next(t for _, t, ll in plain if ll >= thr)
+    log(f"plain: target {target:.5f} (threshold {thr:.5f}) reached at {t_plain:.3f}s; {plain_iter_ms:.3f} ms/iteration")
+
+    # learned MH: plain burn-in 1..30 collecting 10..30, training, then MH
+    # iterations (a plain sweep every 10th) up to args.mh_max_iters
+    m = sampler()
+    m.mh_enable(init_seed=args.seed)
+    clock, mhc = 0.0, [(0, 0.0, evaluate(m))]
+    mh_ms, rates = [], []
+    for it in range(1, 31):
+        _, dt = timed(lambda: (m.iterate(it), m.sync()))
+        clock += dt
+        if it >= 10:
+            _, dt = timed(m.mh_collect)
+            clock += dt
+        if it % args.eval_every == 0:
+            mhc.append((it, clock, evaluate(m)))
+    burn_in_end = clock
+    (loss, _), dt_train = timed(m.mh_train, args.epochs)
+    clock += dt_train
+    log(f"learned MH: trained {args.epochs} epochs in {dt_train:.2f}s, loss {loss:.4f}")
+    for it in range(31, args.mh_max_iters + 1):
+        if it % 10 == 0:
+            _, dt = timed(lambda: (m.iterate(it), m.sync()))
+        else:
+            (rate, _), dt = timed(m.mh_iterate, it, 2)
+            mh_ms.append(dt * 1e3)
+            rates.append(rate)
+        clock += dt
+        if it % args.eval_every == 0:
+            mhc.append((it, clock, evaluate(m)))
+    m.close()
+    fold.close()
+    mh_iter_ms = float(np.median(mh_ms)) if mh_ms else None
+
+    def first_time(curve, thr):
+        return next((t for _, t, ll in curve if ll >= thr), None)
+
+    tolerances = {}
+    for tol in (0.005, 0.001, 0.0005):
+        thr_t = target - tol * abs(target)
+        tp, tm = first_time(plain, thr_t), first_time(mhc, thr_t)
+        tolerances[str(tol)] = {"threshold": thr_t, "t_plain_s": tp, "t_mh_s": tm,
+                                "mh_reached_before_any_mh_iteration": tm is not None and tm <= burn_in_end,
+                                "speedup": (tp / tm) if (tp and tm) else None}
+    t_mh = tolerances["0.005"]["t_mh_s"]
+    log(f"learned MH: {mh_iter_ms} ms per MH iteration, acceptance {np.mean(rates) if rates else 0:.3f}; "
+        f"time-to-LL {tolerances}")
+    flops = 2.0 * (384 * 512 + 512 * K) * N
+    line = {
+        "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
+        "value": (t_plain / t_mh) if t_mh else 0.0,
+        "unit": "x (plain time-to-LL / learned-MH time-to-LL)",
+        "n_gpus": 1,
+        "steps": args.iters,
+        "warmup": 0,
+        "ms_per_step": plain_iter_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (sampler), bf16 MFMA with f32 accumulation (MLP)",
+        "data": "synthetic (seeded LDA generative model)",
+        "config": {"workload": "time to the plain sampler's 200-iteration held-out LL/token (within 0.5%), "
+                               "plain CGS vs learned MH (2 proposals per token, plain sweep every 10th iteration, "
+                               "MLP trained 2 epochs on iterations 10-30); 1M-token corpus (V=10k, D=10k "
+                               "Poisson(100)), every 10th doc held out, document completion with "
+                               f"{args.fold_in} fold-in sweeps",
+                   "k": K, "alpha": alpha, "beta": beta, "num_tokens_train": N,
+                   "target_ll": target, "threshold_ll": thr, "t_plain_s": t_plain, "t_mh_s": t_mh,
+                   "time_to_ll_by_tolerance": tolerances,
+                   "plain_ms_per_iteration": plain_iter_ms, "mh_ms_per_iteration": mh_iter_ms,
+                   "mh_train_s": dt_train, "mh_train_loss": loss, "mh_acceptance": float(np.mean(rates)) if rates else None,
+                   "mlp_tflops_per_mh_iteration": flops / 1e12,
+                   "curve_plain": plain, "curve_mh": mhc},
+        "roofline": {"bound": "mfma", "achieved": (flops / (mh_iter_ms * 1e-3) / 1e12) if mh_iter_ms else None,
+                     "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": (flops / (mh_iter_ms * 1e-3) / 1e12 / 2500.0) if mh_iter_ms else None,
+                     "traffic": None, "kernel": "learned-MH iteration (MLP inference + MH + rebuilds)"},
+        "cpu_baseline": None,
+    }
     print(json.dumps(line), flush=True)
 
 

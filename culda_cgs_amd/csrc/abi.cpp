@@ -708,11 +708,12 @@ extern "C" int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs
         }
         hip_check(hipMemcpy(s.tgt, tg.data(), B * 4, hipMemcpyHostToDevice), "copy");
         hip_check(hipMemcpy(s.wrow, wr.data(), B * 4, hipMemcpyHostToDevice), "copy");
-        float *lsum = (float *)e.dalloc(16);
-        mh_batch_step(e, s, n, false, lsum, e.stream);
+        unsigned long long *lfx = (unsigned long long *)e.dalloc(16);
+        hip_check(launch_fill_u32((uint32_t *)lfx, 2, 0, e.stream), "clear");
+        mh_batch_step(e, s, n, false, lfx, e.stream);
         const MHSizes z = mh_sizes(s);
-        float hl = 0;
-        hip_check(hipMemcpyAsync(&hl, lsum, 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        unsigned long long hl = 0;
+        hip_check(hipMemcpyAsync(&hl, lfx, 8, hipMemcpyDeviceToHost, e.stream), "copy");
         const std::pair<float *, std::pair<const float *, size_t>> parts[] = {
             {dw1, {s.g_w1, z.w1}}, {db1, {s.g_b1, z.b1}}, {dw2, {s.g_w2, z.w2}}, {db2, {s.g_b2, z.b2}}};
         for (auto &p : parts)
@@ -722,10 +723,10 @@ extern "C" int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs
         std::vector<long long> ge(z.emb);
         hip_check(hipMemcpyAsync(ge.data(), s.g_emb, z.emb * 8, hipMemcpyDeviceToHost, e.stream), "copy");
         hip_check(hipStreamSynchronize(e.stream), "sync");
-        e.dfree(lsum);
+        e.dfree(lfx);
         if (demb)
             for (size_t i = 0; i < z.emb; ++i) demb[i] = (float)((double)ge[i] * (1.0 / 4294967296.0));
-        if (loss) *loss = (double)hl / (double)n;
+        if (loss) *loss = (double)(long long)hl / 16777216.0 / (double)n;
         if (train_step) {
             mh_apply_adam(e, s, e.stream);  // consumes (and clears) the embedding accumulator
         } else {

@@ -23,7 +23,7 @@ int32_t sample_blocks_per_cu();
 // mh.hip
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);
 // ta: A stored [k][m]; tb: B stored [n][k]
-hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s);
+hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, int32_t splits = 1);
 hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
                          uint16_t *out, hipStream_t s);
 hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
@@ -32,9 +32,15 @@ hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, con
                                int32_t *tgt, int32_t *wrow, hipStream_t s);
 hipError_t launch_gather_query(const uint16_t *sketch, const uint32_t *docs, const uint32_t *words,
                                const uint16_t *emb16, int64_t n, int64_t rows_pad, uint16_t *X, hipStream_t s);
+// loss_fx (optional): += sum of the row losses in 2^-24 units
 hipError_t launch_softmax_ce(const float *logits, int32_t K, const int32_t *tgt, float inv_rows, uint16_t *G,
-                             float *loss, int32_t rows_pad, hipStream_t s);
-hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out, hipStream_t s);
+                             float *loss, unsigned long long *loss_fx, int32_t rows_pad, hipStream_t s);
+// part: colsum_parts(rows) * cols floats of scratch
+int32_t colsum_parts(int32_t rows);
+hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *part, float *out,
+                              hipStream_t s);
+// part: splits * m * n floats of scratch; c = sum of the split products, added in order
+hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, bool ta, bool tb, int32_t splits, float *part, hipStream_t s);
 hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc, hipStream_t s);
 hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
                        float step, float b1, float b2, float eps, float sqrt_bc2, hipStream_t s);

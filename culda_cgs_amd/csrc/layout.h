@@ -188,6 +188,7 @@ constexpr int kEmb = 128;
 constexpr int kIn = kSketch + kEmb;  // 384
 constexpr int kHidden = 512;
 constexpr int kAliasBits = 20;       // proposal weights are integers summing to K << kAliasBits
+constexpr int kMaxProposals = 8;     // MH proposals per token and sweep
 constexpr int kGemmTile = 64;        // C tile (rows and columns) of the BF16 GEMM
 constexpr int kGemmK = 32;           // reduction step of the BF16 GEMM
 
@@ -209,8 +210,9 @@ struct GemmArgs {
     const float *bias;     // per column, or null
     const float *aux;      // kEpiGeluGrad: pre-activations [m][n] (ldaux)
     int64_t lda, ldb, ldc, ldc16, ldaux;
-    int32_t m, n, k;
+    int32_t m, n, k;       // k: reduction depth of one split
     int32_t epilogue;
+    int64_t split_stride;  // split z (gridDim.z > 1) covers k range [z k, (z+1) k) and writes c + z * split_stride
 };
 
 // Per-token query of the proposal: doc (global id) and word of every token

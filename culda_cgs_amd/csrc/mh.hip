@@ -2,6 +2,8 @@
 // Metropolis-Hastings proposal (no counterpart in the reference).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.h"
 #include "kernels.h"
 #include "philox.h"
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int64_t m0 = (int64_t)blockIdx.y * kGemmTile, n0 = (int64_t)blockIdx.x * kGemmTile;
+    const int64_t kz = (int64_t)blockIdx.z * g.k;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -134,24 +137,24 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
         if (!TA) {
             const int r = t >> 2, c8 = (t & 3) * 8;
             *reinterpret_cast<uint4 *>(&As[r][c8]) =
-                *reinterpret_cast<const uint4 *>(g.a + (m0 + r) * g.lda + k0 + c8);
+                *reinterpret_cast<const uint4 *>(g.a + (m0 + r) * g.lda + kz + k0 + c8);
         } else {
             const int kk = t >> 3, c8 = (t & 7) * 8;
-            const uint4 v = *reinterpret_cast<const uint4 *>(g.a + (int64_t)(k0 + kk) * g.lda + m0 + c8);
+            const uint4 v = *reinterpret_cast<const uint4 *>(g.a + (kz + k0 + kk) * g.lda + m0 + c8);
             const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
 #pragma unroll
             for (int j = 0; j < 8; ++j) As[c8 + j][kk] = h[j];
         }
         if (!TB) {
             const int kk = t >> 3, c8 = (t & 7) * 8;
-            const uint4 v = *reinterpret_cast<const uint4 *>(g.b + (int64_t)(k0 + kk) * g.ldb + n0 + c8);
+            const uint4 v = *reinterpret_cast<const uint4 *>(g.b + (kz + k0 + kk) * g.ldb + n0 + c8);
             const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
 #pragma unroll
             for (int j = 0; j < 8; ++j) Bs[c8 + j][kk] = h[j];
         } else {
             const int r = t >> 2, c8 = (t & 3) * 8;
             *reinterpret_cast<uint4 *>(&Bs[r][c8]) =
-                *reinterpret_cast<const uint4 *>(g.b + (n0 + r) * g.ldb + k0 + c8);
+                *reinterpret_cast<const uint4 *>(g.b + (n0 + r) * g.ldb + kz + k0 + c8);
         }
         __syncthreads();
         bf16x8 af[2], bfr[2];
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
                     g.c16[row * g.ldc16 + col] = f2bf(x * gelu_grad(g.aux[row * g.ldaux + col]));
                 } else {
                     if (g.bias) x += g.bias[col];
-                    if (g.c) g.c[row * g.ldc + col] = x;
+                    if (g.c) g.c[blockIdx.z * g.split_stride + row * g.ldc + col] = x;
                     if (g.c16) g.c16[row * g.ldc16 + col] = f2bf(g.epilogue == kEpiBiasGelu ? gelu(x) : x);
                 }
             }
@@ -266,7 +269,8 @@ __global__ void gather_query_kernel(const uint16_t *sketch, const uint32_t *docs
 // One wave per row: softmax over K logits, loss = log sum exp - logit[z],
 // G = (softmax - onehot(z)) * inv_rows in BF16 (zero for padding rows).
 __global__ __launch_bounds__(64) void softmax_ce_kernel(const float *logits, int32_t K, const int32_t *tgt,
-                                                        float inv_rows, uint16_t *G, float *loss) {
+                                                        float inv_rows, uint16_t *G, float *loss,
+                                                        unsigned long long *loss_fx) {
     const int lane = threadIdx.x;
     const int64_t r = blockIdx.x;
     const float *l = logits + r * K;
@@ -288,15 +292,36 @@ __global__ __launch_bounds__(64) void softmax_ce_kernel(const float *logits, int
         const float p = expf(l[k] - mx) * inv_s;
         gr[k] = f2bf((p - (k == z ? 1.0f : 0.0f)) * inv_rows);
     }
-    if (lane == 0) loss[r] = logf(s) + mx - l[z];
+    if (lane == 0) {
+        const float lr = logf(s) + mx - l[z];
+        loss[r] = lr;
+        // running sum in 2^-24 units: integer, so exact and order-free
+        if (loss_fx) atomicAdd(loss_fx, (unsigned long long)(long long)rint((double)lr * 16777216.0));
+    }
 }
 
-// Column sums of a BF16 matrix, rows in order (bias gradients).
-__global__ void colsum_bf16_kernel(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out) {
+// Column sums of a BF16 matrix (bias gradients) in two fixed-order
+// stages: block (c, p) sums rows [p * kColRows, ...) of its 256 columns into
+// part[p][col]; then each column adds its parts in order.
+constexpr int kColRows = 128;
+
+__global__ void colsum_part_kernel(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *part) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cols) return;
+    const int r0 = blockIdx.y * kColRows, r1 = min(rows, r0 + kColRows);
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += __uint_as_float((uint32_t)x[(int64_t)r * ld + j] << 16);
+#pragma unroll 8
+    for (int r = r0; r < r1; ++r) s += __uint_as_float((uint32_t)x[(int64_t)r * ld + j] << 16);
+    part[(int64_t)blockIdx.y * cols + j] = s;
+}
+
+// out[j] = sum over p < parts of part[p][j], in order (split-K and column-sum reductions).
+__global__ void sum_parts_kernel(const float *part, int32_t parts, int64_t n, float *out) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = 0.f;
+#pragma unroll 8
+    for (int p = 0; p < parts; ++p) s += part[p * n + j];
     out[j] = s;
 }
 
@@ -368,13 +393,28 @@ __global__ __launch_bounds__(64) void sum_f32_kernel(const float *x, int32_t n, 
 // threshold test on 20 more bits), accept iff
 // u (p(k) W_k') < p(k') W_k.  Draws: Philox((g, sweep, 2, r), (seed, 0)).
 __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
-    __shared__ uint32_t sW[kMaxK], sF[kMaxK], sD[kMaxK], sE[kMaxK];
+    // sDE: light prefix deficits at [0, nL), heavy prefix excesses at [nL, K)
+    __shared__ uint32_t sW[kMaxK], sF[kMaxK], sDE[kMaxK];
     __shared__ uint16_t sA[kMaxK], sHI[kMaxK];
     const int lane = threadIdx.x;
     const int r = blockIdx.x;
     if (r >= a.n) return;
     const int K = a.model.k, KL = K / kWave;
     constexpr uint32_t M = 1u << kAliasBits;
+    // the token, and its doc's theta row (<= K entries, 16 per lane; unused
+    // slots hold 0 = topic 0 with count 0), loaded first
+    const int64_t i = a.start + r;
+    const uint32_t d = a.unit.tokens[i];
+    const uint32_t w = a.word[i];
+    const uint32_t z0 = a.unit.z[i];
+    const uint64_t gid = (uint64_t)(i + a.gdelta[w]);
+    const DocMeta dm = a.unit.meta[d];
+    uint32_t tr[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const uint32_t j = lane + kWave * t;
+        tr[t] = j < dm.nnz ? a.unit.theta[dm.start + j] : 0u;
+    }
     const float *lg = a.logits + (int64_t)r * K;
     float e[16];
     float mx = -INFINITY;
@@ -445,12 +485,12 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             sW[k] = W[j];
             if (W[j] < M) {
                 dAcc += M - W[j];
-                sD[aL] = dAcc;
+                sDE[aL] = dAcc;
                 rank[j] = (uint16_t)aL++;
             } else {
                 eAcc += W[j] - M;
                 sHI[bH] = (uint16_t)k;
-                sE[bH] = eAcc;
+                sDE[nL + bH] = eAcc;
                 rank[j] = (uint16_t)bH++;
             }
         }
@@ -461,11 +501,11 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             const uint32_t k = lane * KL + j;
             const uint32_t rk = rank[j];
             if (W[j] < M) {
-                const uint32_t dprev = rk ? sD[rk - 1] : 0u;
+                const uint32_t dprev = rk ? sDE[rk - 1] : 0u;
                 uint32_t lo = 0, hi = nH;  // first b with E_b > dprev
                 while (lo < hi) {
                     const uint32_t mid = (lo + hi) >> 1;
-                    if (sE[mid] > dprev) hi = mid; else lo = mid + 1;
+                    if (sDE[nL + mid] > dprev) hi = mid; else lo = mid + 1;
                 }
                 sF[k] = W[j];
                 sA[k] = sHI[lo < nH ? lo : nH - 1];
@@ -473,17 +513,17 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
                 sF[k] = M;
                 sA[k] = (uint16_t)k;
             } else {
-                const uint32_t eb = sE[rk];
+                const uint32_t eb = sDE[nL + rk];
                 uint32_t astar = 0;
                 if (eb) {
                     uint32_t lo = 0, hi = nL;  // first i with D_i >= eb
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
-                        if (sD[mid] >= eb) hi = mid; else lo = mid + 1;
+                        if (sDE[mid] >= eb) hi = mid; else lo = mid + 1;
                     }
                     astar = lo + 1;
                 }
-                const uint32_t dstar = astar ? sD[astar - 1] : 0u;
+                const uint32_t dstar = astar ? sDE[astar - 1] : 0u;
                 sF[k] = M + eb - dstar;
                 sA[k] = sHI[rk + 1];
             }
@@ -495,40 +535,57 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             a.f_out[(int64_t)r * K + k] = sF[k];
             a.a_out[(int64_t)r * K + k] = sA[k];
         }
-    // MH steps (wave-uniform)
-    const int64_t i = a.start + r;
-    const uint32_t d = a.unit.tokens[i];
-    const uint32_t w = a.word[i];
-    const uint64_t gid = (uint64_t)(i + a.gdelta[w]);
-    const DocMeta dm = a.unit.meta[d];
-    const uint32_t *trow = a.unit.theta + dm.start;
+    // MH steps (wave-uniform): every candidate (current topic, then each
+    // proposal) is drawn first; its n_dk comes from the doc's theta row
+    // held in registers (one compare per entry and a wave max), its n_wk /
+    // n_k from memory, all loads issued together.
     const uint32_t cb = a.model.count_bits, cmask = (1u << cb) - 1u;
     const int32_t *nrow = a.model.nwk + (int64_t)w * K;
     const double alpha = a.model.alpha, beta = a.model.beta;
     const double vbeta = (double)a.model.num_words * beta;
-    auto target = [&](uint32_t k) {
-        uint32_t lo = 0, hi = dm.nnz;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (((trow[mid] >> cb) >> 2) < k) lo = mid + 1; else hi = mid;
+    const int np = a.proposals;
+    uint32_t cand[kMaxProposals + 1];
+    double ud[kMaxProposals];
+    cand[0] = z0;
+#pragma unroll
+    for (int p = 0; p < kMaxProposals; ++p) {
+        if (p < np) {
+            const u64x4 x = philox4x64_10(gid, a.sweep, kPurposeMH, (uint64_t)p, a.model.seed, 0);
+            const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
+            const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
+            ud[p] = (double)(x.v[2] >> 11) * 0x1.0p-53;
+            cand[p + 1] = rr < sF[j] ? j : (uint32_t)sA[j];
+        } else {
+            ud[p] = 0.0;
+            cand[p + 1] = z0;
         }
-        const uint32_t ndk = (lo < dm.nnz && ((trow[lo] >> cb) >> 2) == k) ? (trow[lo] & cmask) : 0u;
-        return (((double)ndk + alpha) * ((double)nrow[k] + beta)) / ((double)a.model.nk[k] + vbeta);
-    };
-    uint32_t cur = a.unit.z[i];
-    double pc = target(cur);
+    }
+    double pt[kMaxProposals + 1];
+#pragma unroll
+    for (int c = 0; c <= kMaxProposals; ++c) {
+        if (c > np) {
+            pt[c] = 0.0;
+            continue;
+        }
+        const uint32_t k = cand[c];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (((tr[t] >> cb) >> 2) == k) cnt = max(cnt, tr[t] & cmask);  // a padding 0 never lowers it
+#pragma unroll
+        for (int dd = 32; dd >= 1; dd >>= 1) cnt = max(cnt, (uint32_t)__shfl_xor((int)cnt, dd, kWave));
+        pt[c] = (((double)cnt + alpha) * ((double)nrow[k] + beta)) / ((double)a.model.nk[k] + vbeta);
+    }
+    uint32_t cur = z0;
+    double pc = pt[0];
     uint32_t acc = 0;
-    for (int p = 0; p < a.proposals; ++p) {
-        const u64x4 x = philox4x64_10(gid, a.sweep, kPurposeMH, (uint64_t)p, a.model.seed, 0);
-        const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
-        const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
-        const double u = (double)(x.v[2] >> 11) * 0x1.0p-53;
-        const uint32_t kp = rr < sF[j] ? j : (uint32_t)sA[j];
-        if (kp != cur) {
-            const double pp = target(kp);
-            if (u * (pc * (double)sW[kp]) < pp * (double)sW[cur]) {
+#pragma unroll
+    for (int p = 0; p < kMaxProposals; ++p) {
+        if (p < np) {
+            const uint32_t kp = cand[p + 1];
+            if (kp != cur && ud[p] * (pc * (double)sW[kp]) < pt[p + 1] * (double)sW[cur]) {
                 cur = kp;
-                pc = pp;
+                pc = pt[p + 1];
                 ++acc;
             }
         }
@@ -540,13 +597,14 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s) {
+hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, int32_t splits) {
     if (g.m <= 0 || g.n <= 0 || g.k <= 0) return hipSuccess;
+    if (splits < 1 || splits > 65535) return hipErrorInvalidValue;
     if (g.m % kGemmTile || g.n % kGemmTile || g.k % kGemmK || g.lda % 8 || g.ldb % 8 ||
         ((uintptr_t)g.a & 15) || ((uintptr_t)g.b & 15) || g.m / kGemmTile > 65535)
         return hipErrorInvalidValue;
     if (g.epilogue == kEpiGeluGrad && (!g.c16 || !g.aux)) return hipErrorInvalidValue;
-    const dim3 grid(g.n / kGemmTile, g.m / kGemmTile);
+    const dim3 grid(g.n / kGemmTile, g.m / kGemmTile, splits);
     if (!ta && !tb)
         hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, g);
     else if (ta && !tb)
@@ -586,15 +644,38 @@ hipError_t launch_gather_query(const uint16_t *sketch, const uint32_t *docs, con
 }
 
 hipError_t launch_softmax_ce(const float *logits, int32_t K, const int32_t *tgt, float inv_rows, uint16_t *G,
-                             float *loss, int32_t rows_pad, hipStream_t s) {
+                             float *loss, unsigned long long *loss_fx, int32_t rows_pad, hipStream_t s) {
     if (rows_pad <= 0) return hipSuccess;
-    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows_pad), dim3(kWave), 0, s, logits, K, tgt, inv_rows, G, loss);
+    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows_pad), dim3(kWave), 0, s, logits, K, tgt, inv_rows, G, loss,
+                       loss_fx);
     return hipGetLastError();
 }
 
-hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *out, hipStream_t s) {
+int32_t colsum_parts(int32_t rows) { return std::max(1, (rows + kColRows - 1) / kColRows); }
+
+hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *part, float *out,
+                              hipStream_t s) {
     if (cols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, x, ld, rows, cols, out);
+    const int32_t parts = colsum_parts(rows);
+    hipLaunchKernelGGL(colsum_part_kernel, dim3((cols + 255) / 256, parts), dim3(256), 0, s, x, ld, rows, cols, part);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((cols + 255) / 256), dim3(256), 0, s, part, parts, (int64_t)cols, out);
+    return hipGetLastError();
+}
+
+// Deterministic split-K: split p computes the product over its share of k
+// into part + p * m * n; the parts are then added in order into g.c.
+hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, bool ta, bool tb, int32_t splits, float *part, hipStream_t s) {
+    if (splits <= 1) return launch_gemm_bf16(g, ta, tb, s);
+    if (g.epilogue != kEpiStore || g.bias || g.c16 || !g.c || g.ldc != g.n || g.k % (splits * kGemmK))
+        return hipErrorInvalidValue;
+    GemmArgs q = g;
+    q.k = g.k / splits;
+    q.c = part;
+    q.split_stride = (int64_t)g.m * g.n;
+    const hipError_t e = launch_gemm_bf16(q, ta, tb, s, splits);
+    if (e != hipSuccess) return e;
+    const int64_t n = (int64_t)g.m * g.n;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, splits, n, g.c);
     return hipGetLastError();
 }
 
@@ -626,7 +707,8 @@ hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s) 
 
 hipError_t launch_mh(const MHArgs &a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    if (a.model.k % kWave || a.model.k > kMaxK) return hipErrorInvalidValue;
+    if (a.model.k % kWave || a.model.k > kMaxK || a.proposals < 0 || a.proposals > kMaxProposals)
+        return hipErrorInvalidValue;
     hipLaunchKernelGGL(mh_kernel, dim3(a.n), dim3(kWave), 0, s, a);
     return hipGetLastError();
 }

@@ -104,6 +104,16 @@ static void fill_uniform(std::vector<float> &v, float bound, uint64_t &s) {
 
 static int64_t round64(int64_t n) { return (n + 63) / 64 * 64; }
 
+// Split-K factor of an m x n x k weight-gradient product: enough
+// workgroups for the whole chip (about 1024), each split a multiple of
+// kGemmK deep.
+int32_t mh_splits(int32_t m, int32_t n, int32_t k) {
+    const int32_t tiles = (m / kGemmTile) * (n / kGemmTile);
+    int32_t s = 1;
+    while (s * 2 * tiles <= 1024 && k % (s * 2 * kGemmK) == 0 && k / (s * 2) >= 4 * kGemmK) s *= 2;
+    return s;
+}
+
 void Engine::mh_enable(const culda_mh_params &p_in) {
     if (!uploaded || models.size() != 1) fail(CULDA_ERR_STATE, "learned MH needs one uploaded corpus");
     const ModelHost &m = models[0];
@@ -218,6 +228,12 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     s.Hi = (uint16_t *)dalloc(IB * kHidden * 2);
     s.Li = falloc(IB * K);
     s.accepted = (unsigned long long *)dalloc(8);
+    {
+        const int64_t w2 = (int64_t)mh_splits(kHidden, (int32_t)K, (int32_t)B) * kHidden * K;
+        const int64_t w1 = (int64_t)mh_splits(kIn, kHidden, (int32_t)B) * kIn * kHidden;
+        const int64_t cs = (int64_t)colsum_parts((int32_t)B) * std::max<int64_t>(K, kHidden);
+        s.part = falloc(std::max({w2, w1, cs}));
+    }
     hip_check(hipDeviceSynchronize(), "mh buffers");
     // upload the initial weights (and their BF16 copies)
     hip_check(hipMemcpy(s.w1, hw1.data(), hw1.size() * 4, hipMemcpyHostToDevice), "copy");
@@ -264,8 +280,8 @@ int32_t Engine::mh_collect() {
 
 // Forward + backward of the rows already gathered into s.X / s.tgt /
 // s.wrow (rows_pad = batch rows), gradients into s.g_*; per-row loss sum
-// into *loss_sum (device).  Then optionally one Adam step.
-void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, float *loss_sum, hipStream_t st) {
+// into *loss_fx (device, 2^-24 units).  Then optionally one Adam step.
+void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, unsigned long long *loss_fx, hipStream_t st) {
     const int32_t B = s.prm.batch, K = s.K;
     auto gemm = [&](const GemmArgs &g, bool ta, bool tb, const char *what) {
         hip_check(launch_gemm_bf16(g, ta, tb, st), what);
@@ -274,16 +290,16 @@ void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, float *loss
     gemm(f1, false, false, "gemm x.w1");
     GemmArgs f2{s.H, s.w2h, s.L, nullptr, s.b2, nullptr, kHidden, K, K, 0, 0, B, K, kHidden, kEpiStore};
     gemm(f2, false, false, "gemm h.w2");
-    hip_check(launch_softmax_ce(s.L, K, s.tgt, 1.0f / (float)std::max(rows, 1), s.G, s.loss, B, st), "softmax");
-    if (loss_sum) hip_check(launch_sum_f32(s.loss, B, loss_sum, st), "loss sum");
+    hip_check(launch_softmax_ce(s.L, K, s.tgt, 1.0f / (float)std::max(rows, 1), s.G, s.loss, loss_fx, B, st),
+              "softmax");
     GemmArgs gw2{s.H, s.G, s.g_w2, nullptr, nullptr, nullptr, kHidden, K, K, 0, 0, kHidden, K, B, kEpiStore};
-    gemm(gw2, true, false, "gemm h^T.g");
-    hip_check(launch_colsum_bf16(s.G, K, B, K, s.g_b2, st), "colsum");
+    hip_check(launch_gemm_bf16_splitk(gw2, true, false, mh_splits(kHidden, K, B), s.part, st), "gemm h^T.g");
+    hip_check(launch_colsum_bf16(s.G, K, B, K, s.part, s.g_b2, st), "colsum");
     GemmArgs dp{s.G, s.w2h, nullptr, s.dP, nullptr, s.P, K, K, 0, kHidden, kHidden, B, kHidden, K, kEpiGeluGrad};
     gemm(dp, false, true, "gemm g.w2^T");
-    hip_check(launch_colsum_bf16(s.dP, kHidden, B, kHidden, s.g_b1, st), "colsum");
+    hip_check(launch_colsum_bf16(s.dP, kHidden, B, kHidden, s.part, s.g_b1, st), "colsum");
     GemmArgs gw1{s.X, s.dP, s.g_w1, nullptr, nullptr, nullptr, kIn, kHidden, kHidden, 0, 0, kIn, kHidden, B, kEpiStore};
-    gemm(gw1, true, false, "gemm x^T.dp");
+    hip_check(launch_gemm_bf16_splitk(gw1, true, false, mh_splits(kIn, kHidden, B), s.part, st), "gemm x^T.dp");
     GemmArgs dx{s.dP, s.w1h + (int64_t)kSketch * kHidden, s.dXe, nullptr, nullptr, nullptr, kHidden, kHidden, kEmb,
                 0, 0, B, kEmb, kHidden, kEpiStore};
     gemm(dx, false, true, "gemm dp.w1e^T");
@@ -325,28 +341,27 @@ double Engine::mh_train(int32_t epochs) {
     const uint64_t T = (uint64_t)s.nsnap * (uint64_t)s.N;
     const int64_t B = s.prm.batch;
     const int64_t steps = (int64_t)((T + B - 1) / B);
-    float *step_loss = (float *)dalloc(std::max<int64_t>(steps, 1) * 4);
+    unsigned long long *loss_fx = (unsigned long long *)dalloc(8);
     uint64_t rs = s.prm.init_seed ^ 0x747261696eULL;
     for (int32_t ep = 0; ep < epochs; ++ep) {
         uint64_t mul = T > 1 ? splitmix64(rs) % (T - 1) + 1 : 1;
         while (gcd64(mul, T) != 1) mul = mul % (T - 1) + 1;
         const uint64_t add = T > 1 ? splitmix64(rs) % T : 0;
+        hip_check(launch_fill_u32((uint32_t *)loss_fx, 2, 0, stream), "clear");  // loss of the last epoch
         for (int64_t st = 0; st < steps; ++st) {
             const int64_t p0 = st * B;
             const int32_t rows = (int32_t)std::min<int64_t>(B, (int64_t)T - p0);
             hip_check(launch_gather_train(s.sks, s.zs, s.tok_doc, s.tok_word, s.embh, mul, add, T, (uint64_t)s.N, s.D, p0,
                                           rows, (int32_t)B, s.X, s.tgt, s.wrow, stream),
                       "gather");
-            mh_batch_step(*this, s, rows, true, step_loss + st, stream);
+            mh_batch_step(*this, s, rows, true, loss_fx, stream);
         }
     }
-    std::vector<float> hl(steps);
-    hip_check(hipMemcpyAsync(hl.data(), step_loss, steps * 4, hipMemcpyDeviceToHost, stream), "loss copy");
+    unsigned long long fx = 0;
+    hip_check(hipMemcpyAsync(&fx, loss_fx, 8, hipMemcpyDeviceToHost, stream), "loss copy");
     hip_check(hipStreamSynchronize(stream), "train");
-    dfree(step_loss);
-    double sum = 0;
-    for (float v : hl) sum += v;
-    return sum / (double)T;
+    dfree(loss_fx);
+    return (double)(long long)fx / 16777216.0 / (double)T;
 }
 
 // Logits [n][K] (rows padded to 64 inside) for explicit (global doc, word)
@@ -367,7 +382,7 @@ void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, in
 double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
     MHState &s = mh_state();
     if (!theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "MH iteration needs rebuilt counts");
-    if (proposals < 0) fail(CULDA_ERR_ARG, "proposals must be >= 0");
+    if (proposals < 0 || proposals > kMaxProposals) fail(CULDA_ERR_ARG, "proposals must be in [0, %d]", kMaxProposals);
     hip_check(hipStreamSynchronize(stream2), "sync");
     const ModelHost &m = models[0];
     hip_check(launch_fill_u32((uint32_t *)s.accepted, 2, 0, stream), "clear");

@@ -37,12 +37,15 @@ struct MHState {
     uint16_t *Xi = nullptr, *Hi = nullptr;
     float *Li = nullptr;
     unsigned long long *accepted = nullptr;
+    float *part = nullptr;  // split-K and column-sum partials
 };
 
 class Engine;
 // Forward + backward of the rows gathered into X / tgt / wrow (gradients
-// into g_*, per-row loss sum into *loss_sum), then one Adam step if update.
-void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, float *loss_sum, hipStream_t st);
+// into g_*, row losses added to *loss_fx in 2^-24 units), then one Adam
+// step if update.
+void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, unsigned long long *loss_fx, hipStream_t st);
 void mh_apply_adam(Engine &e, MHState &s, hipStream_t st);
+int32_t mh_splits(int32_t m, int32_t n, int32_t k);
 
 }  // namespace culda
