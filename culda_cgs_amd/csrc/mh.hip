@@ -269,8 +269,7 @@ __global__ void gather_query_kernel(const uint16_t *sketch, const uint32_t *docs
 // One wave per row: softmax over K logits, loss = log sum exp - logit[z],
 // G = (softmax - onehot(z)) * inv_rows in BF16 (zero for padding rows).
 __global__ __launch_bounds__(64) void softmax_ce_kernel(const float *logits, int32_t K, const int32_t *tgt,
-                                                        float inv_rows, uint16_t *G, float *loss,
-                                                        unsigned long long *loss_fx) {
+                                                        float inv_rows, uint16_t *G, float *loss) {
     const int lane = threadIdx.x;
     const int64_t r = blockIdx.x;
     const float *l = logits + r * K;
@@ -292,12 +291,21 @@ __global__ __launch_bounds__(64) void softmax_ce_kernel(const float *logits, int
         const float p = expf(l[k] - mx) * inv_s;
         gr[k] = f2bf((p - (k == z ? 1.0f : 0.0f)) * inv_rows);
     }
-    if (lane == 0) {
-        const float lr = logf(s) + mx - l[z];
-        loss[r] = lr;
-        // running sum in 2^-24 units: integer, so exact and order-free
-        if (loss_fx) atomicAdd(loss_fx, (unsigned long long)(long long)rint((double)lr * 16777216.0));
+    if (lane == 0) loss[r] = logf(s) + mx - l[z];
+}
+
+// Row losses summed per block of 256 in a fixed tree order, each block's
+// sum added in 2^-24 units with one integer atomic (exact, order-free).
+__global__ __launch_bounds__(256) void loss_sum_kernel(const float *loss, int32_t n, unsigned long long *loss_fx) {
+    __shared__ float sh[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    sh[threadIdx.x] = i < n ? loss[i] : 0.f;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) atomicAdd(loss_fx, (unsigned long long)(long long)rint((double)sh[0] * 16777216.0));
 }
 
 // Column sums of a BF16 matrix (bias gradients) in two fixed-order
@@ -377,25 +385,61 @@ __global__ __launch_bounds__(64) void sum_f32_kernel(const float *x, int32_t n, 
 }
 
 // ------------------------------------------------------------ MH step
+// Alias-table entry of outcome k with weight Wk and rank rk among the
+// lights (Wk < M) or heavies, from the prefix sums in sDE (lights' D at
+// [0, nL), heavies' E at [nL, nL + nH)) and the heavies' ids in sHI:
+// light: F = Wk, alias = first heavy b with E_b > D_{rk-1};
+// last heavy: F = M, alias = itself; other heavy b = rk: emptied after
+// a* = min{a : D_a >= E_b} lights (0 if E_b = 0), F = M + E_b - D_{a*},
+// alias = heavy b + 1.
+__device__ __forceinline__ void alias_entry(uint32_t k, uint32_t Wk, uint32_t rk, const uint32_t *sDE,
+                                            const uint16_t *sHI, uint32_t nL, uint32_t nH, uint32_t &F,
+                                            uint32_t &A) {
+    constexpr uint32_t M = 1u << kAliasBits;
+    if (Wk < M) {
+        const uint32_t dprev = rk ? sDE[rk - 1] : 0u;
+        uint32_t lo = 0, hi = nH;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (sDE[nL + mid] > dprev) hi = mid; else lo = mid + 1;
+        }
+        F = Wk;
+        A = sHI[lo < nH ? lo : nH - 1];
+    } else if (rk + 1 == nH) {
+        F = M;
+        A = k;
+    } else {
+        const uint32_t eb = sDE[nL + rk];
+        uint32_t astar = 0;
+        if (eb) {
+            uint32_t lo = 0, hi = nL;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sDE[mid] >= eb) hi = mid; else lo = mid + 1;
+            }
+            astar = lo + 1;
+        }
+        F = M + eb - (astar ? sDE[astar - 1] : 0u);
+        A = sHI[rk + 1];
+    }
+}
+
 // One wave per token.  Proposal weights: W_k = max(1, floor(softmax_k *
 // K * 2^20)), the remainder added to the largest weight so that
 // sum W = K * 2^20 exactly; the proposal is then exactly W_k / (K 2^20).
-// Alias table (integer sweep, built in parallel): lights (W < M, M = 2^20)
-// and heavies in index order with inclusive prefix deficits D_a =
-// sum (M - W) and excesses E_b = sum (W - M).  Light a keeps W and aliases
-// the first heavy b with E_b > D_{a-1}; heavy b (not the last) is emptied
-// after a* = min{a : D_a >= E_b} lights, keeps M + E_b - D_{a*} and
-// aliases heavy b + 1; the last heavy keeps M.  Every weight's mass is
-// preserved exactly.  Then `proposals` independence-MH steps on the
-// token's collapsed conditional with frozen counts (own assignment
-// included, as in the plain sampler), p(k) = (n_dk + alpha)(n_wk + beta) /
-// (n_k + V beta) in FP64: propose k' from the table (bucket j = K u0,
-// threshold test on 20 more bits), accept iff
-// u (p(k) W_k') < p(k') W_k.  Draws: Philox((g, sweep, 2, r), (seed, 0)).
+// Its Walker alias table (integer sweep: lights and heavies in index order
+// with inclusive prefix deficits D and excesses E, see alias_entry) keeps
+// every weight's mass exactly.  Only the prefix sums are built; the entry
+// of each drawn bucket is evaluated from them.  Then `proposals`
+// independence-MH steps on the token's collapsed conditional with frozen
+// counts (own assignment included, as in the plain sampler),
+// p(k) = (n_dk + alpha)(n_wk + beta) / (n_k + V beta) in FP64: propose k'
+// from the table (bucket j = K u0, threshold test on 20 more bits), accept
+// iff u (p(k) W_k') < p(k') W_k.  Draws: Philox((g, sweep, 2, r), (seed, 0)).
 __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     // sDE: light prefix deficits at [0, nL), heavy prefix excesses at [nL, K)
-    __shared__ uint32_t sW[kMaxK], sF[kMaxK], sDE[kMaxK];
-    __shared__ uint16_t sA[kMaxK], sHI[kMaxK];
+    __shared__ uint32_t sW[kMaxK], sDE[kMaxK];
+    __shared__ uint16_t sRank[kMaxK], sHI[kMaxK];
     const int lane = threadIdx.x;
     const int r = blockIdx.x;
     if (r >= a.n) return;
@@ -451,9 +495,9 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
         }
     }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        lsum += __shfl_xor(lsum, d, kWave);
-        const unsigned long long o = __shfl_xor(best, d, kWave);
+    for (int dd = 32; dd >= 1; dd >>= 1) {
+        lsum += __shfl_xor(lsum, dd, kWave);
+        const unsigned long long o = __shfl_xor(best, dd, kWave);
         best = o > best ? o : best;
     }
     const long long rem = (long long)KM - (long long)lsum;
@@ -477,7 +521,6 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     uint32_t nL, nH, totDef, totExc;
     uint32_t aL = wave_exclusive_u32(cL, &nL), dAcc = wave_exclusive_u32(sDef, &totDef);
     uint32_t bH = wave_exclusive_u32(cH, &nH), eAcc = wave_exclusive_u32(sExc, &totExc);
-    uint16_t rank[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j)
         if (j < KL) {
@@ -486,59 +529,31 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             if (W[j] < M) {
                 dAcc += M - W[j];
                 sDE[aL] = dAcc;
-                rank[j] = (uint16_t)aL++;
+                sRank[k] = (uint16_t)aL++;
             } else {
                 eAcc += W[j] - M;
-                sHI[bH] = (uint16_t)k;
                 sDE[nL + bH] = eAcc;
-                rank[j] = (uint16_t)bH++;
+                sHI[bH] = (uint16_t)k;
+                sRank[k] = (uint16_t)bH++;
             }
         }
     wave_sync();
+    if (a.w_out)  // whole tables, for inspection
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-        if (j < KL) {
-            const uint32_t k = lane * KL + j;
-            const uint32_t rk = rank[j];
-            if (W[j] < M) {
-                const uint32_t dprev = rk ? sDE[rk - 1] : 0u;
-                uint32_t lo = 0, hi = nH;  // first b with E_b > dprev
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (sDE[nL + mid] > dprev) hi = mid; else lo = mid + 1;
-                }
-                sF[k] = W[j];
-                sA[k] = sHI[lo < nH ? lo : nH - 1];
-            } else if (rk + 1 == nH) {
-                sF[k] = M;
-                sA[k] = (uint16_t)k;
-            } else {
-                const uint32_t eb = sDE[nL + rk];
-                uint32_t astar = 0;
-                if (eb) {
-                    uint32_t lo = 0, hi = nL;  // first i with D_i >= eb
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (sDE[mid] >= eb) hi = mid; else lo = mid + 1;
-                    }
-                    astar = lo + 1;
-                }
-                const uint32_t dstar = astar ? sDE[astar - 1] : 0u;
-                sF[k] = M + eb - dstar;
-                sA[k] = sHI[rk + 1];
+        for (int j = 0; j < 16; ++j)
+            if (j < KL) {
+                const uint32_t k = lane * KL + j;
+                uint32_t f, al;
+                alias_entry(k, W[j], sRank[k], sDE, sHI, nL, nH, f, al);
+                a.w_out[(int64_t)r * K + k] = (int32_t)W[j];
+                a.f_out[(int64_t)r * K + k] = f;
+                a.a_out[(int64_t)r * K + k] = al;
             }
-        }
-    wave_sync();
-    if (a.w_out)
-        for (int k = lane; k < K; k += kWave) {
-            a.w_out[(int64_t)r * K + k] = (int32_t)sW[k];
-            a.f_out[(int64_t)r * K + k] = sF[k];
-            a.a_out[(int64_t)r * K + k] = sA[k];
-        }
     // MH steps (wave-uniform): every candidate (current topic, then each
-    // proposal) is drawn first; its n_dk comes from the doc's theta row
-    // held in registers (one compare per entry and a wave max), its n_wk /
-    // n_k from memory, all loads issued together.
+    // proposal) is drawn first -- the alias entry of a drawn bucket is
+    // evaluated from the prefix sums -- its n_dk comes from the doc's theta
+    // row held in registers (one compare per entry and a wave max), its
+    // n_wk / n_k from memory, all loads issued together.
     const uint32_t cb = a.model.count_bits, cmask = (1u << cb) - 1u;
     const int32_t *nrow = a.model.nwk + (int64_t)w * K;
     const double alpha = a.model.alpha, beta = a.model.beta;
@@ -554,7 +569,9 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
             const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
             ud[p] = (double)(x.v[2] >> 11) * 0x1.0p-53;
-            cand[p + 1] = rr < sF[j] ? j : (uint32_t)sA[j];
+            uint32_t f, al;
+            alias_entry(j, sW[j], sRank[j], sDE, sHI, nL, nH, f, al);
+            cand[p + 1] = rr < f ? j : al;
         } else {
             ud[p] = 0.0;
             cand[p + 1] = z0;
@@ -646,8 +663,9 @@ hipError_t launch_gather_query(const uint16_t *sketch, const uint32_t *docs, con
 hipError_t launch_softmax_ce(const float *logits, int32_t K, const int32_t *tgt, float inv_rows, uint16_t *G,
                              float *loss, unsigned long long *loss_fx, int32_t rows_pad, hipStream_t s) {
     if (rows_pad <= 0) return hipSuccess;
-    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows_pad), dim3(kWave), 0, s, logits, K, tgt, inv_rows, G, loss,
-                       loss_fx);
+    hipLaunchKernelGGL(softmax_ce_kernel, dim3(rows_pad), dim3(kWave), 0, s, logits, K, tgt, inv_rows, G, loss);
+    if (loss_fx)
+        hipLaunchKernelGGL(loss_sum_kernel, dim3((rows_pad + 255) / 256), dim3(256), 0, s, loss, rows_pad, loss_fx);
     return hipGetLastError();
 }
 
