@@ -166,3 +166,26 @@ def test_bf16_rounding_known_values():
     assert list(b[:2]) == [0x3F80, 0x3F80]
     assert b[2] == 0x3F81  # 1 + 3*2^-9 rounds up to 1 + 2^-7
     assert mh.bf16_to_f32(b)[3] == -2.5
+
+
+def test_target_matches_the_pinned_phi_head():
+    # target(k) = (n_dk + a)(n_wk + b) / (n_k + V b) must be proportional to
+    # (n_dk + a) * phi_hat_wk, with phi_hat from the C oracle's FP32 head
+    from helpers import chunks_from_docs, oracle_model, random_docs
+    rng = np.random.default_rng(4)
+    V, K = 40, 16
+    docs = random_docs(rng, V, 30, 25)
+    om = oracle_model(V, K, *chunks_from_docs(V, docs, 1))
+    om.init_z(3)
+    om.rebuild()
+    alpha, beta = 0.1, 0.05
+    head = om.phi_head(beta).astype(np.float64)
+    nwk = om.nwk.reshape(V, K)
+    for d in range(5):
+        cnt, top = om.theta_rows(d)
+        ndk = np.zeros(K, np.int64)
+        ndk[top] = cnt
+        for w in range(0, V, 7):
+            t = np.array([mh.target(ndk[k], nwk[w, k], om.nk[k], V, alpha, beta) for k in range(K)])
+            ref = (ndk + np.float32(alpha)) * head[w]
+            assert np.allclose(t / t.sum(), ref / ref.sum(), rtol=2e-6, atol=0)
