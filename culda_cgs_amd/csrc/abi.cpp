@@ -653,9 +653,9 @@ extern "C" int culda_mh_set_weights(culda_ctx *ctx, const float *w1, const float
         hip_check(hipMemcpy(s.w2, w2, z.w2 * 4, hipMemcpyHostToDevice), "copy");
         hip_check(hipMemcpy(s.b2, b2, z.b2 * 4, hipMemcpyHostToDevice), "copy");
         hip_check(hipMemcpy(s.emb, emb, z.emb * 4, hipMemcpyHostToDevice), "copy");
-        hip_check(launch_to_bf16(s.w1, s.w1h, z.w1, e.stream), "bf16");
-        hip_check(launch_to_bf16(s.w2, s.w2h, z.w2, e.stream), "bf16");
-        hip_check(launch_to_bf16(s.emb, s.embh, z.emb, e.stream), "bf16");
+        hip_check(launch_to_bf16(s.w1, s.w1h, s.w1t, kIn, z.w1, e.stream), "bf16");
+        hip_check(launch_to_bf16(s.w2, s.w2h, s.w2t, kHidden, z.w2, e.stream), "bf16");
+        hip_check(launch_to_bf16(s.emb, s.embh, nullptr, 1, z.emb, e.stream), "bf16");
         hip_check(hipStreamSynchronize(e.stream), "sync");
     });
 }

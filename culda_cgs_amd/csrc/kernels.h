@@ -42,9 +42,14 @@ hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32
 // part: splits * m * n floats of scratch; c = sum of the split products, added in order
 hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, bool ta, bool tb, int32_t splits, float *part, hipStream_t s);
 hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc, hipStream_t s);
-hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
-                       float step, float b1, float b2, float eps, float sqrt_bc2, hipStream_t s);
-hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
+// p16t (optional): BF16 copy of the [rows][n / rows] matrix, transposed
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, uint16_t *p16t,
+                       int64_t rows, int64_t n, float step, float b1, float b2, float eps, float sqrt_bc2,
+                       hipStream_t s);
+hipError_t launch_to_bf16(const float *x, uint16_t *y, uint16_t *yt, int64_t rows, int64_t n, hipStream_t s);
+// dst[c][r] = src[r][c]; rows, cols multiples of 64
+hipError_t launch_transpose_bf16(const uint16_t *src, int64_t rows, int64_t cols, int64_t ld_src, uint16_t *dst,
+                                 int64_t ld_dst, hipStream_t s);
 hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s);
 hipError_t launch_mh(const MHArgs &a, hipStream_t s);
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s);

@@ -189,6 +189,98 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
             }
 }
 
+// 128 x 128 C tile per workgroup of four waves (2 x 2, each 64 x 64 =
+// 4 x 4 MFMA 16x16x32 tiles) for A stored [m][k] and B stored [n][k]
+// (both k-contiguous).  32-deep k stages, double-buffered in LDS: the next
+// stage's global loads are issued before the current stage's MFMAs and
+// stored to the other buffer after them.  Same epilogues as the 64-tile
+// kernel.
+__global__ __launch_bounds__(256) void gemm_bf16_nt128_kernel(GemmArgs g) {
+    constexpr int T = 128, KS = kGemmK, LD = KS + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][T][LD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][T][LD];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * T, n0 = (int64_t)blockIdx.x * T;
+    const int64_t kz = (int64_t)blockIdx.z * g.k;
+    const int lr = t >> 2, lc = (t & 3) * 8;  // rows lr and lr + 64, k offset lc
+    const uint16_t *pa = g.a + (m0 + lr) * g.lda + kz + lc;
+    const uint16_t *pb = g.b + (n0 + lr) * g.ldb + kz + lc;
+    const int64_t a64 = 64 * g.lda, b64 = 64 * g.ldb;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    uint4 ra0 = *reinterpret_cast<const uint4 *>(pa), ra1 = *reinterpret_cast<const uint4 *>(pa + a64);
+    uint4 rb0 = *reinterpret_cast<const uint4 *>(pb), rb1 = *reinterpret_cast<const uint4 *>(pb + b64);
+    *reinterpret_cast<uint4 *>(&As[0][lr][lc]) = ra0;
+    *reinterpret_cast<uint4 *>(&As[0][lr + 64][lc]) = ra1;
+    *reinterpret_cast<uint4 *>(&Bs[0][lr][lc]) = rb0;
+    *reinterpret_cast<uint4 *>(&Bs[0][lr + 64][lc]) = rb1;
+    __syncthreads();
+    const int nk = g.k / KS;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int64_t o = (int64_t)(kt + 1) * KS;
+            ra0 = *reinterpret_cast<const uint4 *>(pa + o);
+            ra1 = *reinterpret_cast<const uint4 *>(pa + a64 + o);
+            rb0 = *reinterpret_cast<const uint4 *>(pb + o);
+            rb1 = *reinterpret_cast<const uint4 *>(pb + b64 + o);
+        }
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            af[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][wm * 64 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bfr[j] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn * 64 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        if (more) {
+            const int nx = cur ^ 1;
+            *reinterpret_cast<uint4 *>(&As[nx][lr][lc]) = ra0;
+            *reinterpret_cast<uint4 *>(&As[nx][lr + 64][lc]) = ra1;
+            *reinterpret_cast<uint4 *>(&Bs[nx][lr][lc]) = rb0;
+            *reinterpret_cast<uint4 *>(&Bs[nx][lr + 64][lc]) = rb1;
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+                float x = acc[i][j][r];
+                if (g.epilogue == kEpiGeluGrad) {
+                    g.c16[row * g.ldc16 + col] = f2bf(x * gelu_grad(g.aux[row * g.ldaux + col]));
+                } else {
+                    if (g.bias) x += g.bias[col];
+                    if (g.c) g.c[blockIdx.z * g.split_stride + row * g.ldc + col] = x;
+                    if (g.c16) g.c16[row * g.ldc16 + col] = f2bf(g.epilogue == kEpiBiasGelu ? gelu(x) : x);
+                }
+            }
+}
+
+// BF16 transpose through 64 x 64 LDS tiles: dst[c][r] = src[r][c] for
+// r < rows, c < cols (both multiples of 64).
+__global__ __launch_bounds__(256) void transpose_bf16_kernel(const uint16_t *src, int64_t lds_, uint16_t *dst,
+                                                             int64_t ldd) {
+    __shared__ uint16_t tile[64][64 + 2];
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) tile[r][tx] = src[(r0 + r) * lds_ + c0 + tx];
+    __syncthreads();
+    for (int c = ty; c < 64; c += 4) dst[(c0 + c) * ldd + r0 + tx] = tile[tx][c];
+}
+
 // ------------------------------------------------------------ features
 // One wave per doc of a unit: integer bucket sums in LDS (exact, so the
 // result does not depend on the order), then bucket / n_d rounded to BF16.
@@ -351,8 +443,9 @@ __global__ void emb_grad_kernel(const float *dxe, const int32_t *wrow, int32_t r
 // v = b2 v + (1 - b2) g^2; p += -step * m / (sqrt(v) / sqrt_bc2 + eps).
 // The gradient is FP32 (g) or fixed point (g64, cleared here).  Also
 // refreshes the BF16 copy.
-__global__ void adam_kernel(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
-                            float step, float b1, float b2, float eps, float sqrt_bc2) {
+__global__ void adam_kernel(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16,
+                            uint16_t *p16t, int64_t rows, int64_t n, float step, float b1, float b2, float eps,
+                            float sqrt_bc2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float gi;
@@ -369,11 +462,21 @@ __global__ void adam_kernel(float *p, float *m, float *v, const float *g, long l
     const float pi = p[i] + (-step) * (mi / (sqrtf(vi) / sqrt_bc2 + eps));
     p[i] = pi;
     if (p16) p16[i] = f2bf(pi);
+    if (p16t) {  // [cols][rows] copy of a [rows][cols] matrix
+        const int64_t cols = n / rows, r = i / cols, c = i % cols;
+        p16t[c * rows + r] = f2bf(pi);
+    }
 }
 
-__global__ void to_bf16_kernel(const float *x, uint16_t *y, int64_t n) {
+__global__ void to_bf16_kernel(const float *x, uint16_t *y, uint16_t *yt, int64_t rows, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = f2bf(x[i]);
+    if (i >= n) return;
+    const uint16_t h = f2bf(x[i]);
+    y[i] = h;
+    if (yt) {
+        const int64_t cols = n / rows;
+        yt[(i % cols) * rows + i / cols] = h;
+    }
 }
 
 // Fixed-order sum of n floats (one wave), stored at out[0].
@@ -621,6 +724,10 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, 
         ((uintptr_t)g.a & 15) || ((uintptr_t)g.b & 15) || g.m / kGemmTile > 65535)
         return hipErrorInvalidValue;
     if (g.epilogue == kEpiGeluGrad && (!g.c16 || !g.aux)) return hipErrorInvalidValue;
+    if (!ta && tb && g.m % 128 == 0 && g.n % 128 == 0) {
+        hipLaunchKernelGGL(gemm_bf16_nt128_kernel, dim3(g.n / 128, g.m / 128, splits), dim3(256), 0, s, g);
+        return hipGetLastError();
+    }
     const dim3 grid(g.n / kGemmTile, g.m / kGemmTile, splits);
     if (!ta && !tb)
         hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, g);
@@ -630,6 +737,15 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, 
         hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_bf16(const uint16_t *src, int64_t rows, int64_t cols, int64_t ld_src, uint16_t *dst,
+                                 int64_t ld_dst, hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    if (rows % 64 || cols % 64 || rows / 64 > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3((unsigned)(cols / 64), (unsigned)(rows / 64)), dim3(256), 0, s, src,
+                       ld_src, dst, ld_dst);
     return hipGetLastError();
 }
 
@@ -704,17 +820,20 @@ hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, 
     return hipGetLastError();
 }
 
-hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, int64_t n,
-                       float step, float b1, float b2, float eps, float sqrt_bc2, hipStream_t s) {
+hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, uint16_t *p16t,
+                       int64_t rows, int64_t n, float step, float b1, float b2, float eps, float sqrt_bc2,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, g64, p16, n, step,
-                       b1, b2, eps, sqrt_bc2);
+    if (p16t && (rows <= 0 || n % rows)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, m, v, g, g64, p16, p16t,
+                       rows, n, step, b1, b2, eps, sqrt_bc2);
     return hipGetLastError();
 }
 
-hipError_t launch_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s) {
+hipError_t launch_to_bf16(const float *x, uint16_t *y, uint16_t *yt, int64_t rows, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
+    if (yt && (rows <= 0 || n % rows)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, yt, rows, n);
     return hipGetLastError();
 }
 

@@ -102,15 +102,16 @@ static void fill_uniform(std::vector<float> &v, float bound, uint64_t &s) {
     for (float &x : v) x = (float)(((double)(splitmix64(s) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * bound);
 }
 
-static int64_t round64(int64_t n) { return (n + 63) / 64 * 64; }
+static int64_t round128(int64_t n) { return (n + 127) / 128 * 128; }
 
-// Split-K factor of an m x n x k weight-gradient product: enough
-// workgroups for the whole chip (about 1024), each split a multiple of
-// kGemmK deep.
+// Split-K factor of an m x n x k weight-gradient product: about 512
+// workgroups (128 x 128 tiles when the shape allows, else 64 x 64), each
+// split a multiple of kGemmK and at least 4 stages deep.
 int32_t mh_splits(int32_t m, int32_t n, int32_t k) {
-    const int32_t tiles = (m / kGemmTile) * (n / kGemmTile);
+    const int32_t tile = (m % 128 == 0 && n % 128 == 0) ? 128 : kGemmTile;
+    const int32_t tiles = (m / tile) * (n / tile);
     int32_t s = 1;
-    while (s * 2 * tiles <= 1024 && k % (s * 2 * kGemmK) == 0 && k / (s * 2) >= 4 * kGemmK) s *= 2;
+    while (s * 2 * tiles <= 512 && k % (s * 2 * kGemmK) == 0 && k / (s * 2) >= 4 * kGemmK) s *= 2;
     return s;
 }
 
@@ -126,8 +127,8 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     if (!(p.beta1 > 0)) p.beta1 = 0.9f;
     if (!(p.beta2 > 0)) p.beta2 = 0.999f;
     if (!(p.eps > 0)) p.eps = 1e-8f;
-    if (p.batch < 64 || p.batch % 64 || p.infer_batch < 64 || p.infer_batch % 64 || p.max_snapshots < 1)
-        fail(CULDA_ERR_ARG, "batch and infer_batch must be positive multiples of 64, max_snapshots >= 1");
+    if (p.batch < 128 || p.batch % 128 || p.infer_batch < 128 || p.infer_batch % 128 || p.max_snapshots < 1)
+        fail(CULDA_ERR_ARG, "batch and infer_batch must be positive multiples of 128, max_snapshots >= 1");
     if ((int64_t)p.infer_batch / 64 > 65535) fail(CULDA_ERR_ARG, "infer_batch too large");
     mh.reset(new MHState);
     MHState &s = *mh;
@@ -165,6 +166,8 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     }
     s.w1h = (uint16_t *)dalloc(hw1.size() * 2);
     s.w2h = (uint16_t *)dalloc(hw2.size() * 2);
+    s.w1t = (uint16_t *)dalloc(hw1.size() * 2);
+    s.w2t = (uint16_t *)dalloc(hw2.size() * 2);
     s.embh = (uint16_t *)dalloc(he.size() * 2);
     s.g_w1 = falloc(hw1.size());
     s.g_b1 = falloc(kHidden);
@@ -215,6 +218,10 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     s.H = (uint16_t *)dalloc(B * kHidden * 2);
     s.G = (uint16_t *)dalloc(B * K * 2);
     s.dP = (uint16_t *)dalloc(B * kHidden * 2);
+    s.Xt = (uint16_t *)dalloc(B * kIn * 2);
+    s.Ht = (uint16_t *)dalloc(B * kHidden * 2);
+    s.Gt = (uint16_t *)dalloc(B * K * 2);
+    s.dPt = (uint16_t *)dalloc(B * kHidden * 2);
     s.P = falloc(B * kHidden);
     s.L = falloc(B * K);
     s.dXe = falloc(B * kEmb);
@@ -241,9 +248,9 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     hip_check(hipMemcpy(s.w2, hw2.data(), hw2.size() * 4, hipMemcpyHostToDevice), "copy");
     hip_check(hipMemcpy(s.b2, hb2.data(), hb2.size() * 4, hipMemcpyHostToDevice), "copy");
     hip_check(hipMemcpy(s.emb, he.data(), he.size() * 4, hipMemcpyHostToDevice), "copy");
-    hip_check(launch_to_bf16(s.w1, s.w1h, hw1.size(), stream), "bf16");
-    hip_check(launch_to_bf16(s.w2, s.w2h, hw2.size(), stream), "bf16");
-    hip_check(launch_to_bf16(s.emb, s.embh, he.size(), stream), "bf16");
+    hip_check(launch_to_bf16(s.w1, s.w1h, s.w1t, kIn, hw1.size(), stream), "bf16");
+    hip_check(launch_to_bf16(s.w2, s.w2h, s.w2t, kHidden, hw2.size(), stream), "bf16");
+    hip_check(launch_to_bf16(s.emb, s.embh, nullptr, 1, he.size(), stream), "bf16");
     hip_check(hipStreamSynchronize(stream), "mh init");
 }
 
@@ -286,20 +293,27 @@ void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, unsigned lo
     auto gemm = [&](const GemmArgs &g, bool ta, bool tb, const char *what) {
         hip_check(launch_gemm_bf16(g, ta, tb, st), what);
     };
-    GemmArgs f1{s.X, s.w1h, s.P, s.H, s.b1, nullptr, kIn, kHidden, kHidden, kHidden, 0, B, kHidden, kIn, kEpiBiasGelu};
-    gemm(f1, false, false, "gemm x.w1");
-    GemmArgs f2{s.H, s.w2h, s.L, nullptr, s.b2, nullptr, kHidden, K, K, 0, 0, B, K, kHidden, kEpiStore};
-    gemm(f2, false, false, "gemm h.w2");
+    // every product has both operands k-contiguous (A [m][k], B [n][k]):
+    // weights through their transposed copies, weight gradients through
+    // feature-major copies of the activations
+    GemmArgs f1{s.X, s.w1t, s.P, s.H, s.b1, nullptr, kIn, kIn, kHidden, kHidden, 0, B, kHidden, kIn, kEpiBiasGelu};
+    gemm(f1, false, true, "gemm x.w1");
+    GemmArgs f2{s.H, s.w2t, s.L, nullptr, s.b2, nullptr, kHidden, kHidden, K, 0, 0, B, K, kHidden, kEpiStore};
+    gemm(f2, false, true, "gemm h.w2");
     hip_check(launch_softmax_ce(s.L, K, s.tgt, 1.0f / (float)std::max(rows, 1), s.G, s.loss, loss_fx, B, st),
               "softmax");
-    GemmArgs gw2{s.H, s.G, s.g_w2, nullptr, nullptr, nullptr, kHidden, K, K, 0, 0, kHidden, K, B, kEpiStore};
-    hip_check(launch_gemm_bf16_splitk(gw2, true, false, mh_splits(kHidden, K, B), s.part, st), "gemm h^T.g");
+    hip_check(launch_transpose_bf16(s.H, B, kHidden, kHidden, s.Ht, B, st), "transpose");
+    hip_check(launch_transpose_bf16(s.G, B, K, K, s.Gt, B, st), "transpose");
+    hip_check(launch_transpose_bf16(s.X, B, kIn, kIn, s.Xt, B, st), "transpose");
+    GemmArgs gw2{s.Ht, s.Gt, s.g_w2, nullptr, nullptr, nullptr, B, B, K, 0, 0, kHidden, K, B, kEpiStore};
+    hip_check(launch_gemm_bf16_splitk(gw2, false, true, mh_splits(kHidden, K, B), s.part, st), "gemm h^T.g");
     hip_check(launch_colsum_bf16(s.G, K, B, K, s.part, s.g_b2, st), "colsum");
     GemmArgs dp{s.G, s.w2h, nullptr, s.dP, nullptr, s.P, K, K, 0, kHidden, kHidden, B, kHidden, K, kEpiGeluGrad};
     gemm(dp, false, true, "gemm g.w2^T");
     hip_check(launch_colsum_bf16(s.dP, kHidden, B, kHidden, s.part, s.g_b1, st), "colsum");
-    GemmArgs gw1{s.X, s.dP, s.g_w1, nullptr, nullptr, nullptr, kIn, kHidden, kHidden, 0, 0, kIn, kHidden, B, kEpiStore};
-    hip_check(launch_gemm_bf16_splitk(gw1, true, false, mh_splits(kIn, kHidden, B), s.part, st), "gemm x^T.dp");
+    hip_check(launch_transpose_bf16(s.dP, B, kHidden, kHidden, s.dPt, B, st), "transpose");
+    GemmArgs gw1{s.Xt, s.dPt, s.g_w1, nullptr, nullptr, nullptr, B, B, kHidden, 0, 0, kIn, kHidden, B, kEpiStore};
+    hip_check(launch_gemm_bf16_splitk(gw1, false, true, mh_splits(kIn, kHidden, B), s.part, st), "gemm x^T.dp");
     GemmArgs dx{s.dP, s.w1h + (int64_t)kSketch * kHidden, s.dXe, nullptr, nullptr, nullptr, kHidden, kHidden, kEmb,
                 0, 0, B, kEmb, kHidden, kEpiStore};
     gemm(dx, false, true, "gemm dp.w1e^T");
@@ -316,11 +330,18 @@ void mh_apply_adam(Engine &e, MHState &s, hipStream_t st) {
     const double bc2 = 1.0 - std::pow((double)s.prm.beta2, (double)s.adam_t);
     const float step = (float)(s.prm.lr / bc1), sq = (float)std::sqrt(bc2);
     const float b1 = s.prm.beta1, b2 = s.prm.beta2, eps = s.prm.eps;
-    hip_check(launch_adam(s.w1, s.m_w1, s.v_w1, s.g_w1, nullptr, s.w1h, (int64_t)kIn * kHidden, step, b1, b2, eps, sq, st), "adam");
-    hip_check(launch_adam(s.b1, s.m_b1, s.v_b1, s.g_b1, nullptr, nullptr, kHidden, step, b1, b2, eps, sq, st), "adam");
-    hip_check(launch_adam(s.w2, s.m_w2, s.v_w2, s.g_w2, nullptr, s.w2h, (int64_t)kHidden * K, step, b1, b2, eps, sq, st), "adam");
-    hip_check(launch_adam(s.b2, s.m_b2, s.v_b2, s.g_b2, nullptr, nullptr, K, step, b1, b2, eps, sq, st), "adam");
-    hip_check(launch_adam(s.emb, s.m_emb, s.v_emb, nullptr, s.g_emb, s.embh, (int64_t)s.V * kEmb, step, b1, b2, eps, sq, st),
+    hip_check(launch_adam(s.w1, s.m_w1, s.v_w1, s.g_w1, nullptr, s.w1h, s.w1t, kIn, (int64_t)kIn * kHidden, step, b1,
+                          b2, eps, sq, st),
+              "adam");
+    hip_check(launch_adam(s.b1, s.m_b1, s.v_b1, s.g_b1, nullptr, nullptr, nullptr, 1, kHidden, step, b1, b2, eps, sq, st),
+              "adam");
+    hip_check(launch_adam(s.w2, s.m_w2, s.v_w2, s.g_w2, nullptr, s.w2h, s.w2t, kHidden, (int64_t)kHidden * K, step, b1,
+                          b2, eps, sq, st),
+              "adam");
+    hip_check(launch_adam(s.b2, s.m_b2, s.v_b2, s.g_b2, nullptr, nullptr, nullptr, 1, K, step, b1, b2, eps, sq, st),
+              "adam");
+    hip_check(launch_adam(s.emb, s.m_emb, s.v_emb, nullptr, s.g_emb, s.embh, nullptr, 1, (int64_t)s.V * kEmb, step, b1,
+                          b2, eps, sq, st),
               "adam");
     (void)e;
 }
@@ -369,14 +390,14 @@ double Engine::mh_train(int32_t epochs) {
 void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, float *d_logits,
                              hipStream_t st) {
     MHState &s = mh_state();
-    const int64_t np = round64(n);
+    const int64_t np = round128(n);
     hip_check(launch_gather_query(s.sk_cur, d_docs, d_words, s.embh, n, np, s.Xi, st), "gather");
-    GemmArgs f1{s.Xi, s.w1h, nullptr, s.Hi, s.b1, nullptr, kIn, kHidden, 0, kHidden, 0, (int32_t)np, kHidden, kIn,
+    GemmArgs f1{s.Xi, s.w1t, nullptr, s.Hi, s.b1, nullptr, kIn, kIn, 0, kHidden, 0, (int32_t)np, kHidden, kIn,
                 kEpiBiasGelu};
-    hip_check(launch_gemm_bf16(f1, false, false, st), "gemm x.w1");
-    GemmArgs f2{s.Hi, s.w2h, d_logits, nullptr, s.b2, nullptr, kHidden, s.K, s.K, 0, 0, (int32_t)np, s.K, kHidden,
+    hip_check(launch_gemm_bf16(f1, false, true, st), "gemm x.w1");
+    GemmArgs f2{s.Hi, s.w2t, d_logits, nullptr, s.b2, nullptr, kHidden, kHidden, s.K, 0, 0, (int32_t)np, s.K, kHidden,
                 kEpiStore};
-    hip_check(launch_gemm_bf16(f2, false, false, st), "gemm h.w2");
+    hip_check(launch_gemm_bf16(f2, false, true, st), "gemm h.w2");
 }
 
 double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {

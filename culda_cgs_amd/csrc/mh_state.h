@@ -18,6 +18,7 @@ struct MHState {
     float *w1 = nullptr, *b1 = nullptr, *w2 = nullptr, *b2 = nullptr, *emb = nullptr;
     float *m_w1, *v_w1, *m_b1, *v_b1, *m_w2, *v_w2, *m_b2, *v_b2, *m_emb, *v_emb;
     uint16_t *w1h = nullptr, *w2h = nullptr, *embh = nullptr;
+    uint16_t *w1t = nullptr, *w2t = nullptr;  // BF16 transposes: [512][384], [K][512]
     float *g_w1, *g_b1, *g_w2, *g_b2;
     long long *g_emb = nullptr;  // fixed point, 2^-32
     int64_t adam_t = 0;
@@ -30,6 +31,7 @@ struct MHState {
     uint16_t *sk_cur = nullptr;  // D x 256
     // training buffers (batch rows)
     uint16_t *X = nullptr, *H = nullptr, *G = nullptr, *dP = nullptr;
+    uint16_t *Xt = nullptr, *Ht = nullptr, *Gt = nullptr, *dPt = nullptr;  // feature-major copies
     float *P = nullptr, *L = nullptr, *dXe = nullptr, *loss = nullptr, *step_loss = nullptr;
     int32_t *tgt = nullptr, *wrow = nullptr;
     uint32_t *q_doc = nullptr, *q_word = nullptr;  // explicit query rows

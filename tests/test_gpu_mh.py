@@ -9,6 +9,8 @@
 * a full MH iteration: every token's new z bit-exact against the oracle's
   MH steps given the same weights, counts and Philox stream;
 * training is deterministic and lowers the cross-entropy.
+
+K = 64 runs the 64 x 64-tile GEMM, K = 128 the 128 x 128-tile one.
 """
 import numpy as np
 import pytest
@@ -18,8 +20,8 @@ from oracle import mh
 
 pytestmark = pytest.mark.gpu
 
-K = 64
 ALPHA, BETA, SEED = 0.1, 0.05, 5
+KS = [64, 128]
 M = 1 << mh.ALIAS_BITS
 
 
@@ -28,7 +30,7 @@ def _corpus():
                               zipf_s=1.0, seed=4, num_threads=4)
 
 
-def _sampler(corpus, iters=3, **mhkw):
+def _sampler(corpus, iters=3, K=64, **mhkw):
     s = cg.Sampler(k=K, alpha=ALPHA, beta=BETA, seed=SEED)
     s.upload(corpus)
     s.init_z()
@@ -45,7 +47,7 @@ def _token_arrays(corpus):
     return wt.astype(np.int32), words
 
 
-def _theta_dense(s, D):
+def _theta_dense(s, D, K):
     off, cnt, top = s.get_theta(0)
     th = np.zeros((D, K), np.int64)
     for d in range(D):
@@ -58,8 +60,9 @@ def _bf(x):
 
 
 def test_sketch_matches_oracle():
+    K = 64
     c = _corpus()
-    s = _sampler(c)
+    s = _sampler(c, K=K)
     D = c.info().num_docs
     sk = s.mh_sketch()
     off, cnt, top = s.get_theta(0)
@@ -95,9 +98,10 @@ def _random_weights(s, rng, scale=1.0):
     return {k: (v * scale + rng.normal(0, 0.02, v.shape)).astype(np.float32) for k, v in w.items()}
 
 
-def test_logits_match_torch():
+@pytest.mark.parametrize("K", KS)
+def test_logits_match_torch(K):
     c = _corpus()
-    s = _sampler(c)
+    s = _sampler(c, K=K)
     rng = np.random.default_rng(1)
     w = _random_weights(s, rng, 3.0)
     s.mh_set_weights(w)
@@ -114,10 +118,11 @@ def test_logits_match_torch():
     s.close()
 
 
-def test_gradients_match_torch_and_adam_step():
+@pytest.mark.parametrize("K", KS)
+def test_gradients_match_torch_and_adam_step(K):
     import torch
     c = _corpus()
-    s = _sampler(c, batch=256)
+    s = _sampler(c, K=K, batch=256)
     rng = np.random.default_rng(2)
     w = _random_weights(s, rng, 2.0)
     s.mh_set_weights(w)
@@ -154,7 +159,7 @@ def test_gradients_match_torch_and_adam_step():
     s.close()
 
 
-def _tables_ok(W, F, A, logits):
+def _tables_ok(W, F, A, logits, K):
     assert np.all(W >= 1)
     assert np.all(W.astype(np.int64).sum(axis=1) == K * M)
     ex = np.exp(logits.astype(np.float64) - logits.max(axis=1, keepdims=True))
@@ -167,10 +172,11 @@ def _tables_ok(W, F, A, logits):
         assert np.array_equal(F[i], f) and np.array_equal(A[i], a), f"alias table of row {i}"
 
 
-def test_tables_and_mh_iteration_match_oracle():
+@pytest.mark.parametrize("K", KS)
+def test_tables_and_mh_iteration_match_oracle(K):
     import oracle
     c = _corpus()
-    s = _sampler(c, iters=4)
+    s = _sampler(c, iters=4, K=K)
     rng = np.random.default_rng(3)
     s.mh_set_weights(_random_weights(s, rng, 4.0))
     info = c.info()
@@ -178,9 +184,9 @@ def test_tables_and_mh_iteration_match_oracle():
     docs, words = _token_arrays(c)
     z0 = s.get_z(0)
     nwk, nk = s.get_phi()
-    _, _, _, th = _theta_dense(s, D)
+    _, _, _, th = _theta_dense(s, D, K)
     W, F, A = s.mh_tables(0, N)
-    _tables_ok(W, F, A, s.mh_logits(docs, words))
+    _tables_ok(W, F, A, s.mh_logits(docs, words), K)
     # a sub-range gives the same rows
     W2, F2, A2 = s.mh_tables(100, 50)
     assert np.array_equal(W2, W[100:150]) and np.array_equal(A2, A[100:150])
@@ -205,7 +211,8 @@ def test_tables_and_mh_iteration_match_oracle():
     s.close()
 
 
-def test_training_deterministic_and_learns():
+@pytest.mark.parametrize("K", KS)
+def test_training_deterministic_and_learns(K):
     c = _corpus()
     runs = []
     for _ in range(2):
@@ -229,6 +236,7 @@ def test_training_deterministic_and_learns():
 
 
 def test_mh_errors():
+    K = 64
     c = _corpus()
     s = cg.Sampler(k=K, alpha=ALPHA, beta=BETA, seed=SEED)
     s.upload(c)
