@@ -185,6 +185,7 @@ void Engine::upload() {
             u.tokens = upload_vec(*this, tok);
             u.z = (uint16_t *)dalloc(std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t));
             hip_check(hipMemset(u.z, 0, std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t)), "z clear");
+            u.zp = (uint16_t *)dalloc(std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t));
             u.word_idx = upload_vec(*this, ch.word_idx);
             u.rev_off = upload_vec(*this, rev_off);
             u.rev_idx = upload_vec(*this, rev_idx);
@@ -201,6 +202,18 @@ void Engine::upload() {
         models.push_back(std::move(m));
     }
     const int32_t M = (int32_t)models.size(), U = (int32_t)units.size();
+    {
+        int64_t words = 0, rare = 0;
+        for (const ModelHost &m : models)
+            for (int32_t w = 0; w < m.V; ++w) {
+                int64_t n = 0;
+                for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui)
+                    n += units[ui].word_idx_h[w + 1] - units[ui].word_idx_h[w];
+                ++words;
+                rare += n * 16 < K;
+            }
+        sparse_phi = rare * 4 >= words;
+    }
     d_nk_all = (int32_t *)dalloc((size_t)M * K * sizeof(int32_t));
     for (int32_t mi = 0; mi < M; ++mi) models[mi].nk = d_nk_all + (size_t)mi * K;
     d_stats = (unsigned long long *)dalloc(2 * sizeof(unsigned long long) * U);
@@ -218,7 +231,7 @@ void Engine::upload() {
     std::vector<UnitDev> ud(U);
     for (int32_t ui = 0; ui < U; ++ui) {
         const UnitHost &u = units[ui];
-        ud[ui] = UnitDev{u.tokens, u.z, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+        ud[ui] = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
                          u.doc_end - u.doc_start};
     }
     d_units = upload_vec(*this, ud);
@@ -297,6 +310,7 @@ void Engine::upload() {
     hip_check(hipDeviceSynchronize(), "upload");
     pending.clear();
     theta_ready = phi_ready = sampled = false;
+    zp_valid = false;
 }
 
 // Sampler slots: each unit's word ranges cut into pieces of at most
@@ -407,6 +421,35 @@ void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStr
     PhiArgs a{L.slots, L.packets, d_units, d_models, L.num_packets, accumulate ? 1 : 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
     phi_ready = false;
+    if (unit < 0) {
+        if (sparse_phi) copy_zp(mi, s);
+        zp_valid = sparse_phi && models.size() == 1;
+    } else {
+        zp_valid = false;  // one chunk's counts: complete only after the caller's loop
+    }
+}
+
+void Engine::copy_zp(int32_t model, hipStream_t s) {
+    for (const UnitHost &u : units)
+        if (model < 0 || u.model == model) hip_check(launch_copy_u16(u.z, u.zp, u.num_tokens, s), "zp copy");
+}
+
+void Engine::update_phi(hipStream_t s) {
+    if (zp_valid && sparse_phi) {
+        hip_check(launch_fill_u32((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, 0, s), "fill n_k");
+        hip_check(launch_zero_rows(d_models, phi_all.zero_rows, phi_all.num_zero_rows, s), "zero rows");
+        PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 2};
+        hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
+        phi_ready = false;
+    } else if (prm.fused_phi || units.size() == models.size()) {
+        rebuild_phi_all(s);
+    } else {
+        for (size_t mi = 0; mi < models.size(); ++mi)
+            for (int32_t ui = models[mi].unit_begin; ui < models[mi].unit_end; ++ui)
+                rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, s);
+        if (sparse_phi) copy_zp(-1, s);
+        zp_valid = sparse_phi;
+    }
 }
 
 void Engine::rebuild_phi_all(hipStream_t s) {
@@ -415,6 +458,8 @@ void Engine::rebuild_phi_all(hipStream_t s) {
     PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
     phi_ready = false;
+    if (sparse_phi) copy_zp(-1, s);
+    zp_valid = sparse_phi;
 }
 
 void Engine::phi_head(hipStream_t s) {
@@ -479,6 +524,8 @@ void Engine::rebuild_all() {
         for (size_t mi = 0; mi < models.size(); ++mi)
             for (int32_t ui = models[mi].unit_begin; ui < models[mi].unit_end; ++ui)
                 rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, stream);
+        if (sparse_phi) copy_zp(-1, stream);
+        zp_valid = sparse_phi;
     }
     phi_head(stream);
     rebuild_theta_all(stream);
@@ -517,11 +564,7 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     }
     if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
     ll_reduce(stream);
-    if (prm.fused_phi || U == 1) {
-        rebuild_phi_model(0, -1, false, stream);
-    } else {
-        for (int32_t u = 0; u < U; ++u) rebuild_phi_model(0, u, u > 0, stream);
-    }
+    update_phi(stream);
     phi_head(stream);
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
     hip_check(hipEventRecord(ev_theta_done, stream2), "event");
@@ -565,7 +608,7 @@ void Engine::enqueue_batch_iteration(hipStream_t s, hipEvent_t before, hipEvent_
     sample_all(s);
     if (after) hip_check(hipEventRecord(after, s), "event");
     ll_reduce(s);
-    rebuild_phi_all(s);
+    update_phi(s);
     phi_head(s);
     rebuild_theta_all(s);
     hip_check(launch_add_u32(d_sweep, 1, s), "sweep advance");

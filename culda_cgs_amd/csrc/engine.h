@@ -48,6 +48,7 @@ struct UnitHost {
     int32_t slot_begin = 0, slot_end = 0;
     uint32_t *tokens = nullptr;
     uint16_t *z = nullptr;
+    uint16_t *zp = nullptr;  // z of the last count build (see Engine::zp_valid)
     int64_t *word_idx = nullptr;
     uint32_t *rev_off = nullptr, *rev_idx = nullptr;
     DocMeta *meta = nullptr;
@@ -88,6 +89,13 @@ public:
     std::vector<float> pending_alpha, pending_beta;
     std::vector<uint64_t> pending_seed;
     bool uploaded = false, theta_ready = false, phi_ready = false, sampled = false;
+    // n_wk / n_k equal the counts of every unit's zp: set by full count
+    // rebuilds (which copy z to zp), kept by count deltas, cleared by
+    // anything else that writes the counts
+    bool zp_valid = false;
+    // rebuilds touch only the entries of rare words (at least a quarter of
+    // the words below K / 16 tokens); otherwise every row is rewritten
+    bool sparse_phi = false;
 
     // device tables and work lists
     ModelDev *d_models = nullptr;
@@ -134,6 +142,11 @@ public:
     void init_z();
     void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s);
     void rebuild_phi_all(hipStream_t s);
+    void copy_zp(int32_t model, hipStream_t s);  // zp = z for the model's units (all when model < 0)
+    // counts of every model after a sweep: a rebuild that rewrites only the
+    // touched entries of rare words when zp_valid, else a full rebuild
+    // (fused, or chunks in turn for a single corpus)
+    void update_phi(hipStream_t s);
     void phi_head(hipStream_t s);
     void rebuild_theta_unit(int32_t unit, hipStream_t s);
     void rebuild_theta_all(hipStream_t s);

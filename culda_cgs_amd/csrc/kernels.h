@@ -17,6 +17,7 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
 hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s);
 // grid is used in persistent mode (a.queue != null) only
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
+hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s);
 // workgroups of the persistent sampler resident per CU (0 if unknown)
 int32_t sample_blocks_per_cu();
 

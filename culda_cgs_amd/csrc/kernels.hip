@@ -63,6 +63,11 @@ __global__ void zero_rows_kernel(const ModelDev *models, const int2 *rows, int32
 // slot: LDS histogram of the word's topics, whole rows written with plain
 // 16-byte stores, hot-word pieces added with atomics.  n_k is accumulated
 // per workgroup in LDS and flushed with one atomic per topic.
+// accumulate == 2 (counts known to equal the counts of zp): rare words
+// (fewer than K / 16 tokens) clear only the entries of their previous
+// topics and store the bins of their new ones, with every step a loop over
+// the word's tokens instead of over K; every token's zp becomes its z.
+// The counts are the same either way.
 // Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
 // chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
 __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
@@ -71,38 +76,88 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     const Packet pk = a.packets[blockIdx.x];
     const ModelDev &m = a.models[pk.owner];
     const int K = m.k;
+    const bool keep_zp = a.accumulate == 2;
     for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
     for (int si = pk.first + wave; si < pk.last; si += kWavesPerGroup) {
         const PhiSlot sl = a.slots[si];
-        if (!sl.split && a.accumulate) {
-            // nothing to add when the word is absent from these units
+        int32_t *row = m.nwk + (int64_t)sl.word * K;
+        bool sparse = false;
+        if (!sl.split) {
             int64_t n = 0;
             for (int u = sl.unit_lo; u < sl.unit_hi; ++u)
                 n += a.units[u].word_idx[sl.word + 1] - a.units[u].word_idx[sl.word];
-            if (n == 0) continue;
+            if (n == 0 && a.accumulate != 0) continue;  // nothing to add, row already right
+            sparse = keep_zp && n * 16 < K;
+        }
+        if (sparse) {
+            // Every loop runs over the word's tokens, not over K.  The row
+            // holds the counts of zp: entries of old topics that have no
+            // new token are cleared, entries of new topics get their new
+            // counts -- disjoint stores, so no ordering between them.
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+                const uint16_t *z = a.units[u].z, *zp = a.units[u].zp;
+                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
+                for (int64_t i = b + lane; i < e; i += kWave) {
+                    h[z[i]] = 0;
+                    h[zp[i]] = 0;
+                }
+            }
+            wave_sync();
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+                const uint16_t *z = a.units[u].z;
+                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
+                for (int64_t i = b + lane; i < e; i += kWave) {
+                    const uint16_t t = z[i];
+                    atomicAdd(&h[t], 1u);
+                    atomicAdd(&nk_s[t], 1u);
+                }
+            }
+            wave_sync();
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+                const uint16_t *z = a.units[u].z;
+                uint16_t *zp = a.units[u].zp;
+                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
+                for (int64_t i = b + lane; i < e; i += kWave) {
+                    const uint16_t t = z[i], p = zp[i];
+                    if (h[p] == 0) row[p] = 0;
+                    row[t] = (int)h[t];
+                    zp[i] = t;
+                }
+            }
+            wave_sync();
+            continue;
         }
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         if (!sl.split) {
             for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
                 const uint16_t *z = a.units[u].z;
+                uint16_t *zp = a.units[u].zp;
                 const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
-                for (int64_t i = b + lane; i < e; i += kWave) atomicAdd(&h[z[i]], 1u);
+                for (int64_t i = b + lane; i < e; i += kWave) {
+                    const uint16_t t = z[i];
+                    atomicAdd(&h[t], 1u);
+                    if (keep_zp) zp[i] = t;
+                }
             }
         } else {
             const uint16_t *z = a.units[sl.unit_lo].z;
-            for (int64_t i = sl.begin + lane; i < sl.end; i += kWave) atomicAdd(&h[z[i]], 1u);
+            uint16_t *zp = a.units[sl.unit_lo].zp;
+            for (int64_t i = sl.begin + lane; i < sl.end; i += kWave) {
+                const uint16_t t = z[i];
+                atomicAdd(&h[t], 1u);
+                if (keep_zp) zp[i] = t;
+            }
         }
         wave_sync();
-        int32_t *row = m.nwk + (int64_t)sl.word * K;
         if (!sl.split) {
             for (int k = lane * 4; k < K; k += 4 * kWave) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(h + k);
                 int4 o = make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
-                if (a.accumulate) {
+                if (a.accumulate == 1) {
                     const int4 old = *reinterpret_cast<const int4 *>(row + k);
                     o.x += old.x;
                     o.y += old.y;
@@ -447,6 +502,11 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     }
 }
 
+__global__ void copy_u16_kernel(const uint16_t *src, uint16_t *dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Fixed-order FP64 sums of the per-slot LL partials, per model, in two
 // launches: stage 1 sums each packet (a contiguous slot range of one
 // model), stage 2 sums a model's packets in order and stores the total at
@@ -590,6 +650,12 @@ int32_t sample_blocks_per_cu() {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave, 0) != hipSuccess) n = 0;
     return n;
+}
+
+hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_u16_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s) {

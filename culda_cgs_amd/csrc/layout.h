@@ -60,6 +60,7 @@ struct ModelDev {
 struct UnitDev {
     const uint32_t *tokens;
     uint16_t *z;
+    uint16_t *zp;               // z the current counts were built from (count deltas)
     const int64_t *word_idx;
     const uint32_t *rev_off;
     const uint32_t *rev_idx;
@@ -129,7 +130,7 @@ struct PhiArgs {
     const UnitDev *units;
     const ModelDev *models;
     int32_t num_packets;
-    int32_t accumulate;
+    int32_t accumulate;  // 0: write rows; 1: add to rows; 2: write rows, counts of zp known, zp = z
 };
 
 struct ThetaArgs {

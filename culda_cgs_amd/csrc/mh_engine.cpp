@@ -57,6 +57,7 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     hip_check(hipStreamSynchronize(stream2), "sync");
     hip_check(hipMemcpyAsync(m.nwk, tm.nwk, (size_t)m.V * m.K * 4, hipMemcpyDeviceToDevice, stream), "phi copy");
     hip_check(hipMemcpyAsync(m.nk, tm.nk, (size_t)m.K * 4, hipMemcpyDeviceToDevice, stream), "phi copy");
+    zp_valid = false;  // these counts are the training context's
     phi_head(stream);
     init_z();
     rebuild_theta_all(stream);
@@ -419,7 +420,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             a.logits = s.Li;
             a.word = s.tok_word + base;
             a.gdelta = s.gdelta[ui - m.unit_begin];
-            a.unit = UnitDev{u.tokens, u.z, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+            a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
                              u.doc_end - u.doc_start};
             a.model = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
                                m.count_bits, 0};
@@ -431,8 +432,8 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             hip_check(launch_mh(a, stream), "mh launch");
         }
     }
-    // the count rebuilds of a plain iteration
-    rebuild_phi_all(stream);
+    // the count updates of a plain iteration
+    update_phi(stream);
     phi_head(stream);
     rebuild_theta_all(stream);
     unsigned long long acc = 0;

@@ -137,3 +137,30 @@ def test_single_sweep_from_common_state_and_boundaries():
     ref = out.ll_sum / om.num_tokens
     assert abs(ll - ref) <= LL_RTOL * abs(ref)
     s.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("K", [64, 256])
+def test_counts_after_each_iteration_equal_a_full_rebuild(fused, K):
+    """After every iteration n_wk / n_k equal the oracle's rebuild from the
+    device's own z, also after z is replaced (set_z + rebuild_all) mid-run.
+    At K = 256 most words have fewer than K / 16 tokens, so the rebuild
+    touches only their previous and new topics' entries; at K = 64 it
+    rewrites every row."""
+    V, _, arrays = _setup(6, V=350, D=140, mean_len=60, chunks=3, K=K)
+    s, om = _pair(V, K, arrays, 0.1, 0.02, 9, fused_phi=fused)
+    s.init_z()
+    s.rebuild_all()
+    for it in range(1, 7):
+        s.iterate(it)
+        om.set_z([s.get_z(c) for c in range(3)])
+        om.rebuild()
+        nwk, nk = s.get_phi()
+        assert np.array_equal(nwk.ravel(), om.nwk), f"iteration {it}: n_wk differs from a rebuild"
+        assert np.array_equal(nk, om.nk), f"iteration {it}: n_k differs from a rebuild"
+        if it == 3:
+            rng = np.random.default_rng(1)
+            for c in range(3):
+                s.set_z(c, rng.integers(0, K, s.get_z(c).size).astype(np.uint16))
+            s.rebuild_all()
+    s.close()
