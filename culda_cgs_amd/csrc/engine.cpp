@@ -199,6 +199,7 @@ void Engine::upload() {
         m.unit_end = (int32_t)units.size();
         m.nwk = (int32_t *)dalloc((size_t)m.V * K * sizeof(int32_t));
         m.den = (float *)dalloc(K * sizeof(float));
+        m.zero_phi = (float *)dalloc(K * sizeof(float));
         models.push_back(std::move(m));
     }
     const int32_t M = (int32_t)models.size(), U = (int32_t)units.size();
@@ -224,8 +225,8 @@ void Engine::upload() {
     std::vector<ModelDev> md(M);
     for (int32_t mi = 0; mi < M; ++mi) {
         const ModelHost &m = models[mi];
-        md[mi] = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K, m.V,
-                          m.count_bits, 0};
+        md[mi] = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K,
+                          m.V, m.count_bits, 0};
     }
     d_models = upload_vec(*this, md);
     std::vector<UnitDev> ud(U);
@@ -350,7 +351,9 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
                 if (n > 0) {
                     const int64_t pieces = (n + slot_tokens - 1) / slot_tokens;
                     const int64_t step = (n + pieces - 1) / pieces;
-                    for (int64_t x = b; x < e; x += step) sl.push_back(SlotDesc{w, ui, x, std::min(e, x + step), run[w] - b});
+                    const int32_t rare = m.unit_end - m.unit_begin == 1 && pieces == 1 && n * 16 < m.K;
+                    for (int64_t x = b; x < e; x += step)
+                        sl.push_back(SlotDesc{w, ui, x, std::min(e, x + step), run[w] - b, rare, 0});
                 }
                 run[w] += n;
             }
@@ -366,6 +369,12 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
         mp[2 * mi + 1] = (int32_t)llp.size();
     }
     if (all.size() >= (size_t)INT32_MAX) fail(CULDA_ERR_LIMIT, "too many sampler slots");
+    rare_slots = rare_tokens = 0;
+    for (const SlotDesc &sd : all)
+        if (sd.rare) {
+            ++rare_slots;
+            rare_tokens += sd.end - sd.begin;
+        }
     num_slots = (int32_t)all.size();
     std::vector<int32_t> order(all.size());
     std::iota(order.begin(), order.end(), 0);
@@ -483,12 +492,13 @@ void Engine::rebuild_theta_all(hipStream_t s) {
 void Engine::sample_unit(int32_t ui, hipStream_t s) {
     UnitHost &u = units[ui];
     SampleArgs a{d_slots + u.slot_begin, nullptr, d_units, d_models, d_ll_partial + u.slot_begin, d_sweep, nullptr,
-                 u.slot_end - u.slot_begin, 0};
+                 u.slot_end - u.slot_begin, (zp_valid && sparse_phi) ? 1 : 0};
     hip_check(launch_sample(a, 0, s), "sampler launch");
 }
 
 void Engine::sample_all(hipStream_t s) {
-    SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_sweep, nullptr, num_slots, 0};
+    SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_sweep, nullptr, num_slots,
+                 (zp_valid && sparse_phi) ? 1 : 0};
     int32_t grid = 0;
     if (persistent_groups > 0) {
         hip_check(launch_fill_u32(d_queue, 1, 0, s), "queue reset");
@@ -675,11 +685,17 @@ void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
+// Per token: doc id, doc meta, z write, the doc's theta nonzeros; per slot
+// the word's n_wk row, or for a rare slot staged from zp (when the counts
+// allow) zp, the count and the denominator of each token's topic.
 double Engine::sweep_bytes() const {
     int64_t n = 0, nnz = 0;
     for (auto &m : models) n += m.num_tokens;
     for (int64_t v : unit_token_nnz) nnz += v;
-    return (double)n * (4.0 + 16.0 + 2.0) + 4.0 * (double)nnz + 4.0 * (double)prm.k * (double)num_slots;
+    const bool zp_stage = zp_valid && sparse_phi;
+    const double rows = (double)(num_slots - (zp_stage ? rare_slots : 0)) * 4.0 * (double)prm.k;
+    const double rare = zp_stage ? 10.0 * (double)rare_tokens : 0.0;
+    return (double)n * (4.0 + 16.0 + 2.0) + 4.0 * (double)nnz + rows + rare;
 }
 
 // Times one sweep per candidate slot size from the same state (z is saved

@@ -38,6 +38,7 @@ struct ModelHost {
     std::vector<int64_t> doc_len;
     int32_t *nwk = nullptr, *nk = nullptr;
     float *den = nullptr;
+    float *zero_phi = nullptr;
     PhiList phi_fused;  // all units of the model in one pass
 };
 
@@ -104,6 +105,7 @@ public:
     int32_t *d_order = nullptr;  // slot indices, longest first over all units
     int32_t *d_order_claims = nullptr;  // the same, interleaved so each kQueueClaim run spans all lengths
     int32_t num_slots = 0;
+    int64_t rare_slots = 0, rare_tokens = 0;  // slots staged from zp when the counts allow
     double *d_ll_partial = nullptr;
     Packet *d_ll_packets = nullptr;
     int32_t *d_model_packets = nullptr;

@@ -191,7 +191,11 @@ __global__ void den_kernel(const ModelDev *models, int32_t num_models) {
     for (int mi = blockIdx.x; mi < num_models; mi += gridDim.x) {
         const ModelDev &m = models[mi];
         const float bv = m.beta * (float)m.num_words;
-        for (int i = threadIdx.x; i < m.k; i += blockDim.x) m.den[i] = (float)m.nk[i] + bv;
+        for (int i = threadIdx.x; i < m.k; i += blockDim.x) {
+            const float d = (float)m.nk[i] + bv;
+            m.den[i] = d;
+            m.zero_phi[i] = m.beta / d;  // = fl(fl(0 + beta) / d)
+        }
     }
 }
 
@@ -310,13 +314,26 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
 
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
     const int32_t *row = md.nwk + (int64_t)s.word * K;
-    for (int k = lane * 4; k < K; k += 4 * kWave) {
-        const int4 c = *reinterpret_cast<const int4 *>(row + k);
-        const float4 dn = *reinterpret_cast<const float4 *>(md.den + k);
-        ph[k + 0] = ((float)c.x + beta) / dn.x;
-        ph[k + 1] = ((float)c.y + beta) / dn.y;
-        ph[k + 2] = ((float)c.z + beta) / dn.z;
-        ph[k + 3] = ((float)c.w + beta) / dn.w;
+    if (a.use_zp && s.rare) {
+        // A rare word's counts are nonzero only at its tokens' zp topics
+        // (every token of the word is in this slot): the zero-count row
+        // (fl(beta / den_k), the same value) plus those entries.
+        for (int k = lane * 4; k < K; k += 4 * kWave)
+            *reinterpret_cast<float4 *>(ph + k) = *reinterpret_cast<const float4 *>(md.zero_phi + k);
+        wave_sync();
+        for (int64_t i = s.begin + lane; i < s.end; i += kWave) {
+            const uint32_t t = un.zp[i];
+            ph[t] = ((float)row[t] + beta) / md.den[t];
+        }
+    } else {
+        for (int k = lane * 4; k < K; k += 4 * kWave) {
+            const int4 c = *reinterpret_cast<const int4 *>(row + k);
+            const float4 dn = *reinterpret_cast<const float4 *>(md.den + k);
+            ph[k + 0] = ((float)c.x + beta) / dn.x;
+            ph[k + 1] = ((float)c.y + beta) / dn.y;
+            ph[k + 2] = ((float)c.z + beta) / dn.z;
+            ph[k + 3] = ((float)c.w + beta) / dn.w;
+        }
     }
     wave_sync();
     // Q-bucket block totals (balanced tree of 16 = last element of the

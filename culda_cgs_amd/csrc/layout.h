@@ -47,6 +47,7 @@ struct ModelDev {
     int32_t *nwk;
     int32_t *nk;
     float *den;
+    float *zero_phi;  // phi_k of a zero count, fl(beta / den_k)
     uint64_t seed;
     double k_alpha;  // (double)K * (double)alpha
     float alpha;
@@ -80,6 +81,8 @@ struct SlotDesc {
     int64_t begin;
     int64_t end;
     int64_t gdelta;
+    int32_t rare;  // the word's only slot, in a one-unit model, with fewer than K/16 tokens
+    int32_t pad;
 };
 
 // Phi work item.  split == 0: every token of `word` in units
@@ -113,7 +116,7 @@ struct SampleArgs {
     const uint32_t *sweep;  // sweep index t >= 1, read on the device
     uint32_t *queue;        // persistent mode: next list position; else null
     int32_t num_slots;
-    int32_t pad;
+    int32_t use_zp;         // counts equal counts(zp): rare slots stage phi from zp
 };
 
 struct InitArgs {

@@ -422,7 +422,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             a.gdelta = s.gdelta[ui - m.unit_begin];
             a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
                              u.doc_end - u.doc_start};
-            a.model = ModelDev{m.nwk, m.nk, m.den, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
+            a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
                                m.count_bits, 0};
             a.accepted = s.accepted;
             a.start = start;
