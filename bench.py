@@ -56,7 +56,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4", "c5"])
     ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
     ap.add_argument("--iters", type=int, default=200, help="c4: plain iterations that fix the target")
     ap.add_argument("--mh-max-iters", type=int, default=100, help="c4: learned-MH chain length (iterations)")
@@ -82,6 +82,9 @@ def main():
         return
     if args.config == "c4":
         run_mh(cg, args)
+        return
+    if args.config == "c5":
+        run_large(cg, args)
         return
     cfg = CONFIGS[args.config]
     K = args.k
@@ -252,6 +255,88 @@ def run_batch(cg, args):
         "cpu_baseline": cpu,
     }
     b.close()
+    print(json.dumps(line), flush=True)
+
+
+def run_large(cg, args):
+    """The large corpus (V=140k Zipf(1.05), D=8.2M Poisson(90) docs, ~738M
+    tokens, K_true=1000, Dir(0.05) / Dir(0.01), generator seed 5): every
+    10th doc held out, the rest in 16 data_chunks resident in HBM, sampled
+    in turn with on-device count accumulation; held-out LL/token at the end."""
+    K = args.k
+    alpha, beta = 50.0 / K, 0.01
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.time()
+    full = cg.Corpus.generate(140_000, 8_200_000, 1000, 1, len_kind=0, len_mean=90.0, doc_topic_prior=0.05,
+                              topic_word_prior=0.01, zipf_s=1.05, seed=5, num_threads=threads)
+    log(f"c5: {full.info().num_tokens} tokens generated ({time.time() - t0:.0f}s)")
+    t0 = time.time()
+    train, obs, ev = full.heldout_split(10, 16)
+    del full
+    info = train.info()
+    N = info.num_tokens
+    log(f"c5: train N={N} in {info.num_chunks} chunks, held-out {obs.info().num_tokens} observed / "
+        f"{ev.info().num_tokens} evaluated ({time.time() - t0:.0f}s)")
+    s = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=args.seed, profile=True, slot_tokens=args.slot_tokens)
+    t0 = time.time()
+    s.upload(train)
+    s.init_z()
+    s.rebuild_all()
+    s.sync()
+    log(f"upload + init {time.time() - t0:.1f}s")
+    fold = cg.Sampler(k=K, alpha=alpha, beta=beta, seed=args.seed + 1000)
+    fold.upload(obs)
+    fold.set_eval(ev)
+    ll0 = fold.heldout_ll(s, args.fold_in)
+    it = 1
+    for _ in range(args.warmup):
+        s.iterate(it)
+        it += 1
+    s.sync()
+    samp_ms, samp_bytes, lls = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ll, _ = s.iterate(it)
+        st = s.stats()
+        samp_ms.append(st.sampler_ms)
+        samp_bytes.append(s.sampler_bytes())
+        lls.append(ll)
+        it += 1
+    s.sync()
+    elapsed = time.perf_counter() - t_start
+    launches = s.stats().sampler_launches
+    avg_launch_ms = float(np.sum(samp_ms)) / (len(samp_ms) * launches)
+    bytes_per_launch = float(np.sum(samp_bytes)) / (len(samp_bytes) * launches)
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    hll = fold.heldout_ll(s, args.fold_in)
+    log(f"{N * args.steps / elapsed / 1e6:.1f} M tokens/s, {elapsed / args.steps * 1e3:.1f} ms/iteration, "
+        f"held-out LL/token {ll0:.4f} -> {hll:.4f} after {it - 1} iterations")
+    line = {
+        "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
+        "value": N * args.steps / elapsed,
+        "unit": "tokens/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded LDA generative model)",
+        "config": {"workload": "plain CGS K=1024 on the large corpus: V=140k Zipf(1.05), D=8.2M Poisson(90) docs, "
+                               "K_true=1000, every 10th doc held out, 16 data_chunks resident in HBM",
+                   "k": K, "alpha": alpha, "beta": beta, "num_tokens": N, "chunks": info.num_chunks,
+                   "iterations_timed": [it - args.steps, it - 1], "ll_per_token_last": lls[-1],
+                   "heldout_ll_per_token_initial": ll0, "heldout_ll_per_token_last": hll,
+                   "fold_in_sweeps": args.fold_in},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "sample_kernel",
+                     "avg_launch_ms": avg_launch_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": None,
+    }
+    fold.close()
+    s.close()
     print(json.dumps(line), flush=True)
 
 
