@@ -23,8 +23,8 @@ int32_t sample_blocks_per_cu();
 
 // mh.hip
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);
-// ta: A stored [k][m]; tb: B stored [n][k]
-hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, int32_t splits = 1);
+// A stored [m][k], B stored [n][k]; 128 x 128 tiles when m and n allow
+hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits = 1);
 hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
                          uint16_t *out, hipStream_t s);
 hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
@@ -41,7 +41,7 @@ int32_t colsum_parts(int32_t rows);
 hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32_t cols, float *part, float *out,
                               hipStream_t s);
 // part: splits * m * n floats of scratch; c = sum of the split products, added in order
-hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, bool ta, bool tb, int32_t splits, float *part, hipStream_t s);
+hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, int32_t splits, float *part, hipStream_t s);
 hipError_t launch_emb_grad(const float *dxe, const int32_t *wrow, int32_t rows, long long *acc, hipStream_t s);
 // p16t (optional): BF16 copy of the [rows][n / rows] matrix, transposed
 hipError_t launch_adam(float *p, float *m, float *v, const float *g, long long *g64, uint16_t *p16, uint16_t *p16t,
@@ -51,7 +51,6 @@ hipError_t launch_to_bf16(const float *x, uint16_t *y, uint16_t *yt, int64_t row
 // dst[c][r] = src[r][c]; rows, cols multiples of 64
 hipError_t launch_transpose_bf16(const uint16_t *src, int64_t rows, int64_t cols, int64_t ld_src, uint16_t *dst,
                                  int64_t ld_dst, hipStream_t s);
-hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s);
 hipError_t launch_mh(const MHArgs &a, hipStream_t s);
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s);
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,

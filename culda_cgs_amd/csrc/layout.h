@@ -196,10 +196,9 @@ constexpr int kMaxProposals = 8;     // MH proposals per token and sweep
 constexpr int kGemmTile = 64;        // C tile (rows and columns) of the BF16 GEMM
 constexpr int kGemmK = 32;           // reduction step of the BF16 GEMM
 
-// C[m x n] = sum_k A(i, k) B(k, j), A and B in BF16 (raw bits).  A is
-// stored [m][k] (lda), or [k][m] when transposed; B is stored [k][n]
-// (ldb), or [n][k] when transposed.  m, n multiples of kGemmTile, k a
-// multiple of kGemmK, every row start 16-byte aligned.
+// C[m x n] = sum_k A(i, k) B(k, j), A and B in BF16 (raw bits), A stored
+// [m][k] (lda) and B stored [n][k] (ldb).  m, n multiples of kGemmTile, k
+// a multiple of kGemmK, every row start 16-byte aligned.
 enum GemmEpilogue : int32_t {
     kEpiStore = 0,     // c = acc (+ bias[j])
     kEpiBiasGelu = 1,  // c = acc + bias[j] (optional), c16 = bf16(gelu(acc + bias[j]))

@@ -114,13 +114,13 @@ __device__ __host__ __forceinline__ bool sketch_negative(uint32_t k) { return ((
 }  // namespace
 
 // ------------------------------------------------------------ BF16 GEMM
+// Both kernels take A stored [m][k] and B stored [n][k] (k-contiguous).
 // 64 x 64 C tile per workgroup of four waves (2 x 2, each 32 x 32 = 2 x 2
-// MFMA 16x16x32 tiles); A and B staged through LDS 32 deep as [row][k]
-// (B as [col][k]) so that every lane reads its 8 consecutive k as one
-// 16-byte LDS load.  MFMA lane layout (checked on gfx950): lane l feeds
-// A[l % 16][8 (l / 16) .. +7] and B[8 (l / 16) .. +7][l % 16] and receives
-// C[4 (l / 16) + r][l % 16], r = 0..3.
-template <bool TA, bool TB>
+// MFMA 16x16x32 tiles), for shapes that are not multiples of 128; A and B
+// staged through LDS 32 deep as [row][k] so that every lane reads its 8
+// consecutive k as one 16-byte LDS load.  MFMA lane layout (checked on
+// gfx950): lane l feeds A[l % 16][8 (l / 16) .. +7] and
+// B[8 (l / 16) .. +7][l % 16] and receives C[4 (l / 16) + r][l % 16].
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
     __shared__ __attribute__((aligned(16))) uint16_t As[kGemmTile][kGemmK + 8];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[kGemmTile][kGemmK + 8];
@@ -134,25 +134,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < g.k; k0 += kGemmK) {
-        if (!TA) {
+        {
             const int r = t >> 2, c8 = (t & 3) * 8;
             *reinterpret_cast<uint4 *>(&As[r][c8]) =
                 *reinterpret_cast<const uint4 *>(g.a + (m0 + r) * g.lda + kz + k0 + c8);
-        } else {
-            const int kk = t >> 3, c8 = (t & 7) * 8;
-            const uint4 v = *reinterpret_cast<const uint4 *>(g.a + (kz + k0 + kk) * g.lda + m0 + c8);
-            const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) As[c8 + j][kk] = h[j];
-        }
-        if (!TB) {
-            const int kk = t >> 3, c8 = (t & 7) * 8;
-            const uint4 v = *reinterpret_cast<const uint4 *>(g.b + (kz + k0 + kk) * g.ldb + n0 + c8);
-            const uint16_t *h = reinterpret_cast<const uint16_t *>(&v);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) Bs[c8 + j][kk] = h[j];
-        } else {
-            const int r = t >> 2, c8 = (t & 3) * 8;
             *reinterpret_cast<uint4 *>(&Bs[r][c8]) =
                 *reinterpret_cast<const uint4 *>(g.b + (n0 + r) * g.ldb + kz + k0 + c8);
         }
@@ -479,14 +464,6 @@ __global__ void to_bf16_kernel(const float *x, uint16_t *y, uint16_t *yt, int64_
     }
 }
 
-// Fixed-order sum of n floats (one wave), stored at out[0].
-__global__ __launch_bounds__(64) void sum_f32_kernel(const float *x, int32_t n, float *out) {
-    float s = 0.f;
-    for (int i = threadIdx.x; i < n; i += kWave) s += x[i];
-    s = wave_sum_f(s);
-    if (threadIdx.x == 0) out[0] = s;
-}
-
 // ------------------------------------------------------------ MH step
 // Alias-table entry of outcome k with weight Wk and rank rk among the
 // lights (Wk < M) or heavies, from the prefix sums in sDE (lights' D at
@@ -717,26 +694,17 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
 }
 
 // ------------------------------------------------------------ launchers
-hipError_t launch_gemm_bf16(const GemmArgs &g, bool ta, bool tb, hipStream_t s, int32_t splits) {
+hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits) {
     if (g.m <= 0 || g.n <= 0 || g.k <= 0) return hipSuccess;
     if (splits < 1 || splits > 65535) return hipErrorInvalidValue;
     if (g.m % kGemmTile || g.n % kGemmTile || g.k % kGemmK || g.lda % 8 || g.ldb % 8 ||
         ((uintptr_t)g.a & 15) || ((uintptr_t)g.b & 15) || g.m / kGemmTile > 65535)
         return hipErrorInvalidValue;
     if (g.epilogue == kEpiGeluGrad && (!g.c16 || !g.aux)) return hipErrorInvalidValue;
-    if (!ta && tb && g.m % 128 == 0 && g.n % 128 == 0) {
+    if (g.m % 128 == 0 && g.n % 128 == 0)
         hipLaunchKernelGGL(gemm_bf16_nt128_kernel, dim3(g.n / 128, g.m / 128, splits), dim3(256), 0, s, g);
-        return hipGetLastError();
-    }
-    const dim3 grid(g.n / kGemmTile, g.m / kGemmTile, splits);
-    if (!ta && !tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, g);
-    else if (ta && !tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, g);
-    else if (!ta && tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, g);
     else
-        hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, g);
+        hipLaunchKernelGGL(gemm_bf16_kernel, dim3(g.n / kGemmTile, g.m / kGemmTile, splits), dim3(256), 0, s, g);
     return hipGetLastError();
 }
 
@@ -798,15 +766,15 @@ hipError_t launch_colsum_bf16(const uint16_t *x, int64_t ld, int32_t rows, int32
 
 // Deterministic split-K: split p computes the product over its share of k
 // into part + p * m * n; the parts are then added in order into g.c.
-hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, bool ta, bool tb, int32_t splits, float *part, hipStream_t s) {
-    if (splits <= 1) return launch_gemm_bf16(g, ta, tb, s);
+hipError_t launch_gemm_bf16_splitk(const GemmArgs &g, int32_t splits, float *part, hipStream_t s) {
+    if (splits <= 1) return launch_gemm_bf16(g, s);
     if (g.epilogue != kEpiStore || g.bias || g.c16 || !g.c || g.ldc != g.n || g.k % (splits * kGemmK))
         return hipErrorInvalidValue;
     GemmArgs q = g;
     q.k = g.k / splits;
     q.c = part;
     q.split_stride = (int64_t)g.m * g.n;
-    const hipError_t e = launch_gemm_bf16(q, ta, tb, s, splits);
+    const hipError_t e = launch_gemm_bf16(q, s, splits);
     if (e != hipSuccess) return e;
     const int64_t n = (int64_t)g.m * g.n;
     hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, splits, n, g.c);
@@ -834,11 +802,6 @@ hipError_t launch_to_bf16(const float *x, uint16_t *y, uint16_t *yt, int64_t row
     if (n <= 0) return hipSuccess;
     if (yt && (rows <= 0 || n % rows)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, yt, rows, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_sum_f32(const float *x, int32_t n, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(sum_f32_kernel, dim3(1), dim3(kWave), 0, s, x, n, out);
     return hipGetLastError();
 }
 

@@ -291,33 +291,31 @@ int32_t Engine::mh_collect() {
 // into *loss_fx (device, 2^-24 units).  Then optionally one Adam step.
 void mh_batch_step(Engine &e, MHState &s, int32_t rows, bool update, unsigned long long *loss_fx, hipStream_t st) {
     const int32_t B = s.prm.batch, K = s.K;
-    auto gemm = [&](const GemmArgs &g, bool ta, bool tb, const char *what) {
-        hip_check(launch_gemm_bf16(g, ta, tb, st), what);
-    };
+    auto gemm = [&](const GemmArgs &g, const char *what) { hip_check(launch_gemm_bf16(g, st), what); };
     // every product has both operands k-contiguous (A [m][k], B [n][k]):
     // weights through their transposed copies, weight gradients through
     // feature-major copies of the activations
     GemmArgs f1{s.X, s.w1t, s.P, s.H, s.b1, nullptr, kIn, kIn, kHidden, kHidden, 0, B, kHidden, kIn, kEpiBiasGelu};
-    gemm(f1, false, true, "gemm x.w1");
+    gemm(f1, "gemm x.w1");
     GemmArgs f2{s.H, s.w2t, s.L, nullptr, s.b2, nullptr, kHidden, kHidden, K, 0, 0, B, K, kHidden, kEpiStore};
-    gemm(f2, false, true, "gemm h.w2");
+    gemm(f2, "gemm h.w2");
     hip_check(launch_softmax_ce(s.L, K, s.tgt, 1.0f / (float)std::max(rows, 1), s.G, s.loss, loss_fx, B, st),
               "softmax");
     hip_check(launch_transpose_bf16(s.H, B, kHidden, kHidden, s.Ht, B, st), "transpose");
     hip_check(launch_transpose_bf16(s.G, B, K, K, s.Gt, B, st), "transpose");
     hip_check(launch_transpose_bf16(s.X, B, kIn, kIn, s.Xt, B, st), "transpose");
     GemmArgs gw2{s.Ht, s.Gt, s.g_w2, nullptr, nullptr, nullptr, B, B, K, 0, 0, kHidden, K, B, kEpiStore};
-    hip_check(launch_gemm_bf16_splitk(gw2, false, true, mh_splits(kHidden, K, B), s.part, st), "gemm h^T.g");
+    hip_check(launch_gemm_bf16_splitk(gw2, mh_splits(kHidden, K, B), s.part, st), "gemm h^T.g");
     hip_check(launch_colsum_bf16(s.G, K, B, K, s.part, s.g_b2, st), "colsum");
     GemmArgs dp{s.G, s.w2h, nullptr, s.dP, nullptr, s.P, K, K, 0, kHidden, kHidden, B, kHidden, K, kEpiGeluGrad};
-    gemm(dp, false, true, "gemm g.w2^T");
+    gemm(dp, "gemm g.w2^T");
     hip_check(launch_colsum_bf16(s.dP, kHidden, B, kHidden, s.part, s.g_b1, st), "colsum");
     hip_check(launch_transpose_bf16(s.dP, B, kHidden, kHidden, s.dPt, B, st), "transpose");
     GemmArgs gw1{s.Xt, s.dPt, s.g_w1, nullptr, nullptr, nullptr, B, B, kHidden, 0, 0, kIn, kHidden, B, kEpiStore};
-    hip_check(launch_gemm_bf16_splitk(gw1, false, true, mh_splits(kIn, kHidden, B), s.part, st), "gemm x^T.dp");
+    hip_check(launch_gemm_bf16_splitk(gw1, mh_splits(kIn, kHidden, B), s.part, st), "gemm x^T.dp");
     GemmArgs dx{s.dP, s.w1h + (int64_t)kSketch * kHidden, s.dXe, nullptr, nullptr, nullptr, kHidden, kHidden, kEmb,
                 0, 0, B, kEmb, kHidden, kEpiStore};
-    gemm(dx, false, true, "gemm dp.w1e^T");
+    gemm(dx, "gemm dp.w1e^T");
     hip_check(launch_emb_grad(s.dXe, s.wrow, B, s.g_emb, st), "emb grad");
     if (update) mh_apply_adam(e, s, st);
 }
@@ -395,10 +393,10 @@ void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, in
     hip_check(launch_gather_query(s.sk_cur, d_docs, d_words, s.embh, n, np, s.Xi, st), "gather");
     GemmArgs f1{s.Xi, s.w1t, nullptr, s.Hi, s.b1, nullptr, kIn, kIn, 0, kHidden, 0, (int32_t)np, kHidden, kIn,
                 kEpiBiasGelu};
-    hip_check(launch_gemm_bf16(f1, false, true, st), "gemm x.w1");
+    hip_check(launch_gemm_bf16(f1, st), "gemm x.w1");
     GemmArgs f2{s.Hi, s.w2t, d_logits, nullptr, s.b2, nullptr, kHidden, kHidden, s.K, 0, 0, (int32_t)np, s.K, kHidden,
                 kEpiStore};
-    hip_check(launch_gemm_bf16(f2, false, true, st), "gemm h.w2");
+    hip_check(launch_gemm_bf16(f2, st), "gemm h.w2");
 }
 
 double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {

@@ -32,7 +32,7 @@ struct MHState {
     // training buffers (batch rows)
     uint16_t *X = nullptr, *H = nullptr, *G = nullptr, *dP = nullptr;
     uint16_t *Xt = nullptr, *Ht = nullptr, *Gt = nullptr, *dPt = nullptr;  // feature-major copies
-    float *P = nullptr, *L = nullptr, *dXe = nullptr, *loss = nullptr, *step_loss = nullptr;
+    float *P = nullptr, *L = nullptr, *dXe = nullptr, *loss = nullptr;
     int32_t *tgt = nullptr, *wrow = nullptr;
     uint32_t *q_doc = nullptr, *q_word = nullptr;  // explicit query rows
     // inference buffers
