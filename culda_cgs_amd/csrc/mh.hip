@@ -519,7 +519,7 @@ __device__ __forceinline__ void alias_entry(uint32_t k, uint32_t Wk, uint32_t rk
 __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     // sDE: light prefix deficits at [0, nL), heavy prefix excesses at [nL, K)
     __shared__ uint32_t sW[kMaxK], sDE[kMaxK];
-    __shared__ uint16_t sRank[kMaxK], sHI[kMaxK];
+    __shared__ uint16_t sHI[kMaxK];
     const int lane = threadIdx.x;
     const int r = blockIdx.x;
     if (r >= a.n) return;
@@ -551,7 +551,7 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     float s = 0.f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        e[j] = j < KL ? expf(e[j] - mx) : 0.f;
+        e[j] = j < KL ? __expf(e[j] - mx) : 0.f;
         s += e[j];
     }
     s = wave_sum_f(s);
@@ -601,6 +601,19 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     uint32_t nL, nH, totDef, totExc;
     uint32_t aL = wave_exclusive_u32(cL, &nL), dAcc = wave_exclusive_u32(sDef, &totDef);
     uint32_t bH = wave_exclusive_u32(cH, &nH), eAcc = wave_exclusive_u32(sExc, &totExc);
+    const uint32_t aL0 = aL, bH0 = bH;  // ranks of this lane's first light / heavy
+    // rank of this lane's element jj among the lights or the heavies
+    auto rank_of = [&](uint32_t jj) {
+        uint32_t wj = 0, nl = 0, nh = 0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            if ((uint32_t)q == jj) wj = W[q];
+            if ((uint32_t)q < jj && q < KL) {
+                if (W[q] < M) ++nl; else ++nh;
+            }
+        }
+        return wj < M ? aL0 + nl : bH0 + nh;
+    };
 #pragma unroll
     for (int j = 0; j < 16; ++j)
         if (j < KL) {
@@ -608,13 +621,11 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             sW[k] = W[j];
             if (W[j] < M) {
                 dAcc += M - W[j];
-                sDE[aL] = dAcc;
-                sRank[k] = (uint16_t)aL++;
+                sDE[aL++] = dAcc;
             } else {
                 eAcc += W[j] - M;
                 sDE[nL + bH] = eAcc;
-                sHI[bH] = (uint16_t)k;
-                sRank[k] = (uint16_t)bH++;
+                sHI[bH++] = (uint16_t)k;
             }
         }
     wave_sync();
@@ -624,7 +635,7 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             if (j < KL) {
                 const uint32_t k = lane * KL + j;
                 uint32_t f, al;
-                alias_entry(k, W[j], sRank[k], sDE, sHI, nL, nH, f, al);
+                alias_entry(k, W[j], rank_of((uint32_t)j), sDE, sHI, nL, nH, f, al);
                 a.w_out[(int64_t)r * K + k] = (int32_t)W[j];
                 a.f_out[(int64_t)r * K + k] = f;
                 a.a_out[(int64_t)r * K + k] = al;
@@ -649,8 +660,10 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
             const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
             ud[p] = (double)(x.v[2] >> 11) * 0x1.0p-53;
+            // the rank of bucket j, from the lane that holds it
+            const uint32_t rk = (uint32_t)__shfl((int)rank_of(j % (uint32_t)KL), (int)(j / (uint32_t)KL), kWave);
             uint32_t f, al;
-            alias_entry(j, sW[j], sRank[j], sDE, sHI, nL, nH, f, al);
+            alias_entry(j, sW[j], rk, sDE, sHI, nL, nH, f, al);
             cand[p + 1] = rr < f ? j : al;
         } else {
             ud[p] = 0.0;
