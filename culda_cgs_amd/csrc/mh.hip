@@ -94,16 +94,55 @@ __device__ __forceinline__ float wave_max_f(float x) {
     return x;
 }
 
-// Exclusive prefix over the 64 lanes (integer: any order gives the same).
-__device__ __forceinline__ uint32_t wave_exclusive_u32(uint32_t x, uint32_t *total) {
-    uint32_t inc = x;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint32_t y = __shfl_up(inc, d, kWave);
-        if ((int)(threadIdx.x & 63) >= d) inc += y;
-    }
-    *total = __shfl(inc, kWave - 1, kWave);
-    return inc - x;
+// 64-lane inclusive scans with DPP (row shifts inside each 16-lane row,
+// then row_bcast:15 / row_bcast:31 across rows): a few cycles per step
+// instead of an LDS round trip per shuffle.  `id` fills lanes without a
+// source.  wave_*_all returns the total to every lane.
+template <int CTRL, int RM, bool BC>
+__device__ __forceinline__ uint32_t dppu(uint32_t old, uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, RM, 0xf, BC);
+}
+
+template <class Op>
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t id, Op op) {
+    x = op(x, dppu<0x111, 0xf, false>(id, x));
+    x = op(x, dppu<0x112, 0xf, false>(id, x));
+    x = op(x, dppu<0x114, 0xf, false>(id, x));
+    x = op(x, dppu<0x118, 0xf, false>(id, x));
+    x = op(x, dppu<0x142, 0xa, false>(id, x));
+    x = op(x, dppu<0x143, 0xc, false>(id, x));
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_add_scan_u32(uint32_t x) {
+    return wave_scan_u32(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+__device__ __forceinline__ uint32_t wave_max_all_u32(uint32_t x) {
+    x = wave_scan_u32(x, 0u, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
+}
+
+__device__ __forceinline__ uint32_t wave_min_all_u32(uint32_t x) {
+    x = wave_scan_u32(x, 0xffffffffu, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, kWave - 1);
+}
+
+__device__ __forceinline__ float wave_max_all_f(float x) {
+    const uint32_t ninf = 0xff800000u;
+    uint32_t v = __float_as_uint(x);
+    v = wave_scan_u32(v, ninf, [](uint32_t a, uint32_t b) {
+        return __float_as_uint(fmaxf(__uint_as_float(a), __uint_as_float(b)));
+    });
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1));
+}
+
+__device__ __forceinline__ float wave_sum_all_f(float x) {
+    uint32_t v = __float_as_uint(x);
+    v = wave_scan_u32(v, 0u, [](uint32_t a, uint32_t b) {
+        return __float_as_uint(__uint_as_float(a) + __uint_as_float(b));
+    });
+    return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)v, kWave - 1));
 }
 
 // Signed count sketch of a doc's topic counts: topic k adds +-n_dk to
@@ -547,18 +586,18 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
         e[j] = j < KL ? lg[lane * KL + j] : -INFINITY;
         mx = fmaxf(mx, e[j]);
     }
-    mx = wave_max_f(mx);
+    mx = wave_max_all_f(mx);
     float s = 0.f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         e[j] = j < KL ? __expf(e[j] - mx) : 0.f;
         s += e[j];
     }
-    s = wave_sum_f(s);
+    s = wave_sum_all_f(s);
     const uint32_t KM = (uint32_t)K << kAliasBits;
     const float scale = (float)KM / s;
     uint32_t W[16];
-    unsigned long long lsum = 0, best = 0;
+    uint32_t lsum = 0, lmax = 0;  // lsum wraps mod 2^32 (see rem below)
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         if (j < KL) {
@@ -567,24 +606,25 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             if (wv > KM) wv = KM;
             W[j] = wv;
             lsum += wv;
-            const uint32_t k = lane * KL + j;
-            const unsigned long long key = ((unsigned long long)wv << 32) | (0xffffffffu - k);
-            best = key > best ? key : best;
+            lmax = wv > lmax ? wv : lmax;
         } else {
             W[j] = 0;
         }
     }
+    // the sum of all weights is within a few K of K 2^20 < 2^31, so the
+    // remainder is exact from sums modulo 2^32
+    const uint32_t wsum = (uint32_t)__builtin_amdgcn_readlane((int)wave_add_scan_u32(lsum), kWave - 1);
+    const int32_t rem = (int32_t)(KM - wsum);
+    // the largest weight, lowest topic among equals, takes the remainder
+    const uint32_t wmax = wave_max_all_u32(lmax);
+    uint32_t lk = 0xffffffffu;
 #pragma unroll
-    for (int dd = 32; dd >= 1; dd >>= 1) {
-        lsum += __shfl_xor(lsum, dd, kWave);
-        const unsigned long long o = __shfl_xor(best, dd, kWave);
-        best = o > best ? o : best;
-    }
-    const long long rem = (long long)KM - (long long)lsum;
-    const uint32_t kmax = 0xffffffffu - (uint32_t)(best & 0xffffffffu);
+    for (int j = 15; j >= 0; --j)
+        if (j < KL && W[j] == wmax) lk = lane * KL + j;
+    const uint32_t kmax = wave_min_all_u32(lk);
 #pragma unroll
     for (int j = 0; j < 16; ++j)
-        if (j < KL && (uint32_t)(lane * KL + j) == kmax) W[j] = (uint32_t)((long long)W[j] + rem);
+        if (j < KL && (uint32_t)(lane * KL + j) == kmax) W[j] = (uint32_t)((int32_t)W[j] + rem);
     // lane-local light / heavy counts and sums, then wave prefixes
     uint32_t cL = 0, sDef = 0, cH = 0, sExc = 0;
 #pragma unroll
@@ -598,9 +638,12 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
                 sExc += W[j] - M;
             }
         }
-    uint32_t nL, nH, totDef, totExc;
-    uint32_t aL = wave_exclusive_u32(cL, &nL), dAcc = wave_exclusive_u32(sDef, &totDef);
-    uint32_t bH = wave_exclusive_u32(cH, &nH), eAcc = wave_exclusive_u32(sExc, &totExc);
+    // exclusive prefixes (inclusive DPP scan minus the lane's own value)
+    const uint32_t iL = wave_add_scan_u32(cL), iD = wave_add_scan_u32(sDef);
+    const uint32_t iH = wave_add_scan_u32(cH), iE = wave_add_scan_u32(sExc);
+    const uint32_t nL = (uint32_t)__builtin_amdgcn_readlane((int)iL, kWave - 1);
+    const uint32_t nH = (uint32_t)__builtin_amdgcn_readlane((int)iH, kWave - 1);
+    uint32_t aL = iL - cL, dAcc = iD - sDef, bH = iH - cH, eAcc = iE - sExc;
     const uint32_t aL0 = aL, bH0 = bH;  // ranks of this lane's first light / heavy
     // rank of this lane's element jj among the lights or the heavies
     auto rank_of = [&](uint32_t jj) {
@@ -682,8 +725,7 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
 #pragma unroll
         for (int t = 0; t < 16; ++t)
             if (((tr[t] >> cb) >> 2) == k) cnt = max(cnt, tr[t] & cmask);  // a padding 0 never lowers it
-#pragma unroll
-        for (int dd = 32; dd >= 1; dd >>= 1) cnt = max(cnt, (uint32_t)__shfl_xor((int)cnt, dd, kWave));
+        cnt = wave_max_all_u32(cnt);
         pt[c] = (((double)cnt + alpha) * ((double)nrow[k] + beta)) / ((double)a.model.nk[k] + vbeta);
     }
     uint32_t cur = z0;
