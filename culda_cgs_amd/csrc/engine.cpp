@@ -94,11 +94,12 @@ static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last
 }
 
 // Phi list of model m over units [u0, u1): whole words, hot words as pieces
-// of at most kPhiSplit tokens per unit (rows zeroed first, atomics).
+// of at most kPhiSplit tokens per unit (rows zeroed first, atomics); items
+// in descending token count, tokens[i] being item i's.
 static void build_phi_list(const std::vector<UnitHost> &units, int32_t m, int32_t u0, int32_t u1, int32_t V,
-                           std::vector<PhiSlot> &slots, std::vector<int2> &zero) {
-    std::vector<std::pair<int64_t, int32_t>> whole;
-    whole.reserve(V);
+                           std::vector<PhiSlot> &slots, std::vector<int64_t> &tokens, std::vector<int2> &zero) {
+    std::vector<std::pair<int64_t, PhiSlot>> items;
+    items.reserve(V);
     for (int32_t w = 0; w < V; ++w) {
         int64_t tot = 0;
         for (int32_t u = u0; u < u1; ++u) tot += units[u].word_idx_h[w + 1] - units[u].word_idx_h[w];
@@ -107,14 +108,71 @@ static void build_phi_list(const std::vector<UnitHost> &units, int32_t m, int32_
             for (int32_t u = u0; u < u1; ++u) {
                 const int64_t b = units[u].word_idx_h[w], e = units[u].word_idx_h[w + 1];
                 for (int64_t x = b; x < e; x += kPhiSplit)
-                    slots.push_back(PhiSlot{w, u, u + 1, 1, x, std::min(e, x + kPhiSplit)});
+                    items.emplace_back(std::min(e, x + kPhiSplit) - x, PhiSlot{w, u, u + 1, 1, x, std::min(e, x + kPhiSplit)});
             }
         } else {
-            whole.emplace_back(tot, w);
+            items.emplace_back(tot, PhiSlot{w, u0, u1, 0, 0, 0});
         }
     }
-    std::stable_sort(whole.begin(), whole.end(), [](auto &a, auto &b) { return a.first > b.first; });
-    for (auto &pr : whole) slots.push_back(PhiSlot{pr.second, u0, u1, 0, 0, 0});
+    std::stable_sort(items.begin(), items.end(), [](auto &a, auto &b) { return a.first > b.first; });
+    slots.clear();
+    tokens.clear();
+    for (auto &pr : items) {
+        tokens.push_back(pr.first);
+        slots.push_back(pr.second);
+    }
+}
+
+// Deals a list in descending token order round-robin into packets of about
+// `per` items (packet p takes items p, p + P, p + 2P, ...), so every packet
+// gets a like share of long and short items; reorders `slots` to match and
+// appends the packets, their ranges offset by `base`.
+static void deal_packets(std::vector<PhiSlot> &slots, int32_t owner, int32_t per, int32_t base,
+                         std::vector<Packet> &packets) {
+    const int32_t n = (int32_t)slots.size();
+    if (n == 0) return;
+    const int32_t P = (n + per - 1) / per;
+    std::vector<PhiSlot> out;
+    out.reserve(n);
+    for (int32_t p = 0; p < P; ++p) {
+        const int32_t first = base + (int32_t)out.size();
+        for (int32_t i = p; i < n; i += P) out.push_back(slots[i]);
+        packets.push_back(Packet{first, base + (int32_t)out.size(), owner, 0});
+    }
+    slots.swap(out);
+}
+
+// Rare tiles of a single-unit model: words with 0 < n and n * 16 < K (so
+// n < 64, kMaxK being 1024), packed greedily in word order while the tile
+// keeps at most 64 tokens and spans at most 64 word ids; a word with tokens
+// that is not rare ends the tile (its tokens would break the contiguous
+// range).  covered[w] marks the words the tiles take.
+static void build_rare_tiles(const UnitHost &u, int32_t ui, int32_t V, int32_t K, std::vector<RareTile> &tiles,
+                             std::vector<char> &covered) {
+    RareTile cur{};
+    bool open = false;
+    for (int32_t w = 0; w < V; ++w) {
+        const int64_t b = u.word_idx_h[w], n = u.word_idx_h[w + 1] - b;
+        if (n == 0) continue;
+        if (n * 16 >= K || n > kWave) {
+            if (open) tiles.push_back(cur);
+            open = false;
+            continue;
+        }
+        if (open && (cur.count + n > kWave || w - cur.w0 >= kWave)) {
+            tiles.push_back(cur);
+            open = false;
+        }
+        if (!open) {
+            cur = RareTile{0, 0, b, w, ui, 0, 0};
+            open = true;
+        }
+        cur.start_mask |= 1ull << cur.count;
+        cur.word_mask |= 1ull << (w - cur.w0);
+        cur.count += (int32_t)n;
+        covered[w] = 1;
+    }
+    if (open) tiles.push_back(cur);
 }
 
 int32_t Engine::add_corpus(const Corpus &c, float alpha, float beta, uint64_t seed) {
@@ -238,17 +296,29 @@ void Engine::upload() {
     d_units = upload_vec(*this, ud);
     // phi work lists: fused per model (and all models concatenated), and per unit
     {
-        std::vector<PhiSlot> all_slots;
+        std::vector<PhiSlot> all_slots, sparse_slots;
         std::vector<Packet> all_packets;
         std::vector<int2> all_zero;
+        std::vector<RareTile> tiles;
+        std::vector<int64_t> model_tiles(M + 1, 0);
+        std::vector<std::vector<PhiSlot>> model_sparse(M);
         for (int32_t mi = 0; mi < M; ++mi) {
             ModelHost &m = models[mi];
             std::vector<PhiSlot> sl;
+            std::vector<int64_t> tok;
             std::vector<int2> zr;
-            build_phi_list(units, mi, m.unit_begin, m.unit_end, m.V, sl, zr);
+            build_phi_list(units, mi, m.unit_begin, m.unit_end, m.V, sl, tok, zr);
+            // the zp-based update skips words without tokens and the words
+            // of the rare tiles
+            std::vector<char> covered(m.V, 0);
+            if (m.unit_end - m.unit_begin == 1)
+                build_rare_tiles(units[m.unit_begin], m.unit_begin, m.V, K, tiles, covered);
+            model_tiles[mi + 1] = (int64_t)tiles.size();
+            for (size_t i = 0; i < sl.size(); ++i)
+                if (sl[i].split || (tok[i] > 0 && !covered[sl[i].word])) model_sparse[mi].push_back(sl[i]);
             const int32_t per = std::max<int32_t>(4, (int32_t)((sl.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
             std::vector<Packet> pk;
-            append_packets(pk, 0, (int32_t)sl.size(), mi, per);
+            deal_packets(sl, mi, per, 0, pk);
             m.phi_fused.slots = upload_vec(*this, sl);
             m.phi_fused.packets = upload_vec(*this, pk);
             m.phi_fused.zero_rows = upload_vec(*this, zr);
@@ -261,12 +331,13 @@ void Engine::upload() {
             all_zero.insert(all_zero.end(), zr.begin(), zr.end());
             for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
                 std::vector<PhiSlot> s1;
+                std::vector<int64_t> t1;
                 std::vector<int2> z1;
-                build_phi_list(units, mi, ui, ui + 1, m.V, s1, z1);
+                build_phi_list(units, mi, ui, ui + 1, m.V, s1, t1, z1);
                 const int32_t per1 =
                     std::max<int32_t>(4, (int32_t)((s1.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
                 std::vector<Packet> p1;
-                append_packets(p1, 0, (int32_t)s1.size(), mi, per1);
+                deal_packets(s1, mi, per1, 0, p1);
                 UnitHost &u = units[ui];
                 u.phi_single.slots = upload_vec(*this, s1);
                 u.phi_single.packets = upload_vec(*this, p1);
@@ -282,6 +353,30 @@ void Engine::upload() {
         phi_all.num_slots = (int32_t)all_slots.size();
         phi_all.num_packets = (int32_t)all_packets.size();
         phi_all.num_zero_rows = (int32_t)all_zero.size();
+        // packet sizes from the totals over every model, so that a batch of
+        // many small corpora still gets a few hundred items per workgroup
+        size_t sparse_total = 0;
+        for (const auto &v : model_sparse) sparse_total += v.size();
+        const size_t groups = (size_t)cu_count * 8;
+        const int32_t per_s = std::max<int32_t>(4, (int32_t)((sparse_total + groups - 1) / groups));
+        const int32_t per_t = std::max<int32_t>(4, (int32_t)((tiles.size() + groups - 1) / groups));
+        std::vector<Packet> sparse_packets, tile_packets;
+        for (int32_t mi = 0; mi < M; ++mi) {
+            deal_packets(model_sparse[mi], mi, per_s, (int32_t)sparse_slots.size(), sparse_packets);
+            sparse_slots.insert(sparse_slots.end(), model_sparse[mi].begin(), model_sparse[mi].end());
+            append_packets(tile_packets, (int32_t)model_tiles[mi], (int32_t)model_tiles[mi + 1], mi, per_t);
+        }
+        if (tiles.size() >= (size_t)INT32_MAX) fail(CULDA_ERR_LIMIT, "too many rare-word tiles");
+        phi_sparse.slots = upload_vec(*this, sparse_slots);
+        phi_sparse.packets = upload_vec(*this, sparse_packets);
+        phi_sparse.zero_rows = phi_all.zero_rows;
+        phi_sparse.num_slots = (int32_t)sparse_slots.size();
+        phi_sparse.num_packets = (int32_t)sparse_packets.size();
+        phi_sparse.num_zero_rows = phi_all.num_zero_rows;
+        d_rare_tiles = upload_vec(*this, tiles);
+        d_rare_packets = upload_vec(*this, tile_packets);
+        num_rare_packets = (int32_t)tile_packets.size();
+        num_rare_tiles = (int64_t)tiles.size();
     }
     // theta packets per unit (and all units concatenated)
     {
@@ -447,8 +542,10 @@ void Engine::update_phi(hipStream_t s) {
     if (zp_valid && sparse_phi) {
         hip_check(launch_fill_u32((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, 0, s), "fill n_k");
         hip_check(launch_zero_rows(d_models, phi_all.zero_rows, phi_all.num_zero_rows, s), "zero rows");
-        PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 2};
+        PhiArgs a{phi_sparse.slots, phi_sparse.packets, d_units, d_models, phi_sparse.num_packets, 2};
         hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
+        RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
+        hip_check(launch_phi_rare(r, s), "rare phi launch");
         phi_ready = false;
     } else if (prm.fused_phi || units.size() == models.size()) {
         rebuild_phi_all(s);

@@ -113,6 +113,13 @@ public:
     double *d_ll_hist = nullptr;
     int32_t num_ll_packets = 0;
     PhiList phi_all;  // every model, fused over its units
+    // the zp-based update (update_phi when zp_valid): words with tokens that
+    // no rare tile covers, and the rare tiles of single-unit models
+    PhiList phi_sparse;
+    RareTile *d_rare_tiles = nullptr;
+    Packet *d_rare_packets = nullptr;
+    int32_t num_rare_packets = 0;
+    int64_t num_rare_tiles = 0;
     Packet *d_theta_packets_all = nullptr;
     int32_t num_theta_packets_all = 0;
     int32_t *d_nk_all = nullptr;  // M*K

@@ -11,6 +11,7 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s);
 hipError_t launch_zero_rows(const ModelDev *models, const int2 *rows, int32_t n, hipStream_t s);
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s);
+hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
 hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s);
 hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, int32_t num_words, int32_t k,
                            float beta, hipStream_t s);

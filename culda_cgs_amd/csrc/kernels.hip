@@ -70,6 +70,29 @@ __global__ void zero_rows_kernel(const ModelDev *models, const int2 *rows, int32
 // The counts are the same either way.
 // Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
 // chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
+// Histogram of z[b, e) into the wave's LDS bins h; zp = z when keep_zp.
+// Each lane issues eight loads before their LDS atomics, so a long word
+// waits for one memory round trip per 512 tokens instead of per 64.
+__device__ __forceinline__ void count_range(const uint16_t *z, uint16_t *zp, int64_t b, int64_t e, int lane,
+                                            uint32_t *h, bool keep_zp) {
+    for (int64_t i = b + lane; i < e; i += 8 * kWave) {
+        uint16_t t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t x = i + j * kWave;
+            t[j] = x < e ? z[x] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t x = i + j * kWave;
+            if (x < e) {
+                atomicAdd(&h[t[j]], 1u);
+                if (keep_zp) zp[x] = t[j];
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
     __shared__ uint32_t nk_s[kMaxK];
@@ -133,24 +156,11 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         if (!sl.split) {
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
-                const uint16_t *z = a.units[u].z;
-                uint16_t *zp = a.units[u].zp;
-                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
-                for (int64_t i = b + lane; i < e; i += kWave) {
-                    const uint16_t t = z[i];
-                    atomicAdd(&h[t], 1u);
-                    if (keep_zp) zp[i] = t;
-                }
-            }
+            for (int u = sl.unit_lo; u < sl.unit_hi; ++u)
+                count_range(a.units[u].z, a.units[u].zp, a.units[u].word_idx[sl.word], a.units[u].word_idx[sl.word + 1],
+                            lane, h, keep_zp);
         } else {
-            const uint16_t *z = a.units[sl.unit_lo].z;
-            uint16_t *zp = a.units[sl.unit_lo].zp;
-            for (int64_t i = sl.begin + lane; i < sl.end; i += kWave) {
-                const uint16_t t = z[i];
-                atomicAdd(&h[t], 1u);
-                if (keep_zp) zp[i] = t;
-            }
+            count_range(a.units[sl.unit_lo].z, a.units[sl.unit_lo].zp, sl.begin, sl.end, lane, h, keep_zp);
         }
         wave_sync();
         if (!sl.split) {
@@ -177,6 +187,63 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
             if (v) atomicAdd(&nk_s[k], v);
         }
         wave_sync();
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const uint32_t v = nk_s[k];
+        if (v) atomicAdd(m.nk + k, (int)v);
+    }
+}
+
+// ------------------------------------------------------------ rare tiles
+// Count update of the rare words of single-unit models when the rows hold
+// the counts of zp: one wave per tile of up to 64 tokens (one per lane) of
+// several consecutive words, so a word costs a share of two memory round
+// trips instead of a pass of its own.  A token's word is its start_mask
+// rank; its new bin is the number of lanes with the same (word, z), and an
+// old topic p whose (word, p) no lane holds any more is cleared -- the
+// cleared and the stored entries are disjoint, so no ordering between the
+// stores.  zp becomes z; n_k gets one per token (LDS, flushed per packet).
+__global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
+    __shared__ uint32_t nk_s[kMaxK];
+    const Packet pk = a.packets[blockIdx.x];
+    const ModelDev &m = a.models[pk.owner];
+    const int K = m.k;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int ti = pk.first + wave; ti < pk.last; ti += kWavesPerGroup) {
+        const RareTile t = a.tiles[ti];
+        const UnitDev &u = a.units[t.unit];
+        const bool live = lane < t.count;
+        uint32_t zt = 0, zo = 0;
+        if (live) {
+            zt = u.z[t.begin + lane];
+            zo = u.zp[t.begin + lane];
+        }
+        // word of present-word rank q: lane r (word w0 + r present) pushes
+        // w0 + r to lane rank(r); absent words push to lane 63, which no
+        // rank reads unless all 64 words are present
+        const bool here = (t.word_mask >> lane) & 1ull;
+        const int dst = here ? __popcll(t.word_mask & ((1ull << lane) - 1ull)) : 63;
+        const int by_rank = __builtin_amdgcn_ds_permute(dst << 2, t.w0 + lane);
+        const uint32_t rank = (uint32_t)__popcll(t.start_mask & ((2ull << lane) - 1ull)) - 1u;
+        const int word = __builtin_amdgcn_ds_bpermute((int)(rank << 2), by_rank);
+        const uint32_t key = (rank << 16) | zt, okey = (rank << 16) | zo;
+        uint32_t c = 0;
+        bool kept = false;
+        for (int j = 0; j < t.count; ++j) {
+            const uint32_t kj = __builtin_amdgcn_readlane(key, j);
+            c += kj == key;
+            kept |= kj == okey;
+        }
+        if (live) {
+            int32_t *row = m.nwk + (int64_t)word * K;
+            if (!kept) row[zo] = 0;
+            row[zt] = (int32_t)c;
+            u.zp[t.begin + lane] = (uint16_t)zt;
+            atomicAdd(&nk_s[zt], 1u);
+        }
     }
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
@@ -631,6 +698,12 @@ hipError_t launch_zero_rows(const ModelDev *models, const int2 *rows, int32_t n,
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
     hipLaunchKernelGGL(phi_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s) {
+    if (a.num_packets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(phi_rare_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

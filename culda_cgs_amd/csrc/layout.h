@@ -136,6 +136,30 @@ struct PhiArgs {
     int32_t accumulate;  // 0: write rows; 1: add to rows; 2: write rows, counts of zp known, zp = z
 };
 
+// Rare-word tile: consecutive words of one unit whose tokens (at most 64,
+// every word whole) sit in z[begin, begin + count).  Words without tokens
+// may lie between them; bit r of word_mask marks word w0 + r as present,
+// bit i of start_mask marks token i as the first of its word.
+struct RareTile {
+    uint64_t start_mask;
+    uint64_t word_mask;
+    int64_t begin;
+    int32_t w0;
+    int32_t unit;
+    int32_t count;
+    int32_t pad;
+};
+
+// Packets of rare tiles (owner = model) for the zp-based count update.
+struct RareArgs {
+    const RareTile *tiles;
+    const Packet *packets;
+    const UnitDev *units;
+    const ModelDev *models;
+    int32_t num_packets;
+    int32_t pad;
+};
+
 struct ThetaArgs {
     const Packet *packets;  // owner = unit, [first, last) = its docs
     const UnitDev *units;

@@ -164,3 +164,31 @@ def test_counts_after_each_iteration_equal_a_full_rebuild(fused, K):
                 s.set_z(c, rng.integers(0, K, s.get_z(c).size).astype(np.uint16))
             s.rebuild_all()
     s.close()
+
+
+@pytest.mark.parametrize("V,D,K", [(350, 140, 256), (350, 140, 1024), (3000, 100, 1024), (3000, 100, 256)])
+def test_rare_tiles_equal_a_full_rebuild(V, D, K):
+    """One chunk: the rare words' counts are updated in tiles of up to 64
+    tokens of consecutive words.  Word ids are shuffled, so frequent words
+    end tiles between rare ones; V = 350 fills tiles up to 64 tokens;
+    V = 3000 leaves many words without tokens, so tiles also end at 64 word
+    ids.  After every iteration (and after set_z + rebuild_all) n_wk / n_k
+    equal the oracle's rebuild."""
+    rng = np.random.default_rng(8)
+    perm = rng.permutation(V)
+    docs = [perm[d] for d in random_docs(rng, V, D, 60)]
+    arrays = chunks_from_docs(V, docs, 1)
+    s, om = _pair(V, K, arrays, 0.1, 0.02, 4)
+    s.init_z()
+    s.rebuild_all()
+    for it in range(1, 6):
+        s.iterate(it)
+        om.set_z([s.get_z(0)])
+        om.rebuild()
+        nwk, nk = s.get_phi()
+        assert np.array_equal(nwk.ravel(), om.nwk), f"iteration {it}: n_wk differs from a rebuild"
+        assert np.array_equal(nk, om.nk), f"iteration {it}: n_k differs from a rebuild"
+        if it == 2:
+            s.set_z(0, np.random.default_rng(2).integers(0, K, s.get_z(0).size).astype(np.uint16))
+            s.rebuild_all()
+    s.close()
