@@ -15,6 +15,22 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Loads and stores through a pointer in the global address space: a plain
+// pointer read from a descriptor is generic, and a flat access also counts
+// against the LDS counter, so a wait for an LDS read would wait for it too.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CULDA_AS1 __attribute__((address_space(1)))
+#else
+#define CULDA_AS1
+#endif
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t load_global(const uint32_t *p) { return *(const CULDA_AS1 uint32_t *)p; }
+__device__ __forceinline__ u32x2 load_global2(const uint32_t *p) { return *(const CULDA_AS1 u32x2 *)p; }
+__device__ __forceinline__ u32x4 load_global4(const void *p) { return *(const CULDA_AS1 u32x4 *)p; }
+__device__ __forceinline__ void store_global_u16(uint16_t *p, uint16_t v) { *(CULDA_AS1 uint16_t *)p = v; }
+
 // DPP move with zero for lanes whose source is outside the 16-lane row.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float x) {

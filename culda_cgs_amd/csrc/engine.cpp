@@ -236,7 +236,7 @@ void Engine::upload() {
                 const int64_t nd = m.doc_len[ch.doc_start + d];
                 meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
                 cap += (std::min<int64_t>(nd, K) + 15) / 16 * 16;
-                if (cap + 64 >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^32 entries");
+                if (cap + kThetaPad >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^32 entries");
             }
             u.theta_cap = cap;
             u.word_idx_h = ch.word_idx;
@@ -248,10 +248,11 @@ void Engine::upload() {
             u.rev_off = upload_vec(*this, rev_off);
             u.rev_idx = upload_vec(*this, rev_idx);
             u.meta = upload_vec(*this, meta);
-            // 64 entries of tail padding: the sampler loads whole 64-entry
-            // groups and may read past the last slab (those values are unused)
-            u.theta = (uint32_t *)dalloc((cap + 64) * sizeof(uint32_t));
-            hip_check(hipMemset(u.theta, 0, (cap + 64) * sizeof(uint32_t)), "theta clear");
+            // kThetaPad entries of tail padding: the sampler loads whole
+            // 64-entry groups and may read past the last slab (those values
+            // are unused)
+            u.theta = (uint32_t *)dalloc((cap + kThetaPad) * sizeof(uint32_t));
+            hip_check(hipMemset(u.theta, 0, (cap + kThetaPad) * sizeof(uint32_t)), "theta clear");
             units.push_back(std::move(u));
         }
         m.unit_end = (int32_t)units.size();

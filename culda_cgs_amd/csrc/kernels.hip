@@ -349,7 +349,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     // the phi lookup is a shift of the packed entry plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
     __shared__ float rq[kWave];
-    __shared__ float rs_s[4][kWave + 4];
+    __shared__ __attribute__((aligned(16))) float rs_s[4][kWave + 4];
     __shared__ uint32_t zres[kWave];
     __shared__ float sqres[kWave];
     const int lane = threadIdx.x;
@@ -427,7 +427,9 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     wave_sync();
 
     const int rrow = lane >> 4, lr = lane & 15;
-    float *rs = rs_s[rrow];
+    // rs[b] = S prefix before block b (rs[0] unused); offset so that
+    // rs[4g + 1 .. 4g + 4] is one aligned 16-byte store
+    float *rs = rs_s[rrow] + 3;
     double ll = 0.0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
@@ -436,8 +438,8 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
         float u = 0.0f;
         if (valid) {
             const int64_t i = base + lane;
-            const uint32_t doc = un.tokens[i];
-            const uint4 m = *reinterpret_cast<const uint4 *>(un.meta + doc);
+            const uint32_t doc = load_global(un.tokens + i);
+            const u32x4 m = load_global4(un.meta + doc);
             mstart = m.x;
             mnnz = m.y;
             mnd = m.z;
@@ -469,57 +471,74 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
             if (p < ntok) {
                 // S = sum over the doc's nonzeros, 16 per block; every block
                 // lies inside the doc's 16-aligned slab, entries past nnz
-                // are masked
-                // Blocks are loaded four at a time (a group of 64 entries),
-                // one group ahead.  Loads past the last block stay inside the
-                // theta array (64 entries of tail padding) and those blocks
-                // are skipped; inside the last block the slab is zero past
-                // nnz, so its extra products are exact zeros (R + 0 == R).
+                // are zero (the slab tail), so their products are exact
+                // zeros.  A block total is the balanced tree of its 16
+                // products (lane 15 of the in-block Kogge-Stone scan): lane
+                // lr holds entries 2(lr & 7) and 2(lr & 7) + 1 of block
+                // lr >> 3 of a pair of blocks and adds them (the tree's
+                // first level), three DPP steps over 8 lanes finish it at
+                // lanes 7 and 15.  Totals are carried sequentially,
+                // R_{b+1} = fl(R_b + T_b).
+                // Groups of four blocks (64 entries) are loaded one group
+                // ahead.  The last group is loaded whole: loads past the
+                // last block stay inside the theta array (kThetaPad entries
+                // of tail padding), and its second pair of blocks is summed
+                // only when it holds a block of the doc.  rs entries past nb
+                // may hold sums of such blocks; nothing reads them.
                 const uint32_t nb = (nnz + 15u) >> 4;
                 const uint32_t ng = (nb + 3u) >> 2;
-                const uint32_t *tp = theta + start + lr;
-                uint32_t cur[4], nxt[4];
+                const uint32_t *tp = theta + start + ((lr >> 3) << 4) + ((lr & 7) << 1);
+                u32x2 cur[2];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) cur[q] = tp[q << 4];
+                for (int h = 0; h < 2; ++h) cur[h] = load_global2(tp + (h << 5));
                 float R = 0.0f;
                 uint32_t g = 0;
                 for (; g + 1 < ng; ++g) {
                     const uint32_t *gp = tp + ((g + 1) << 6);
+                    u32x2 nxt[2];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) nxt[q] = gp[q << 4];
-                    float x[4];
+                    for (int h = 0; h < 2; ++h) nxt[h] = load_global2(gp + (h << 5));
+                    float y[2];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = COUNT_OF(cur[q]) * PHI_OF(cur[q]);
+                    for (int h = 0; h < 2; ++h)
+                        y[h] = COUNT_OF(cur[h].x) * PHI_OF(cur[h].x) + COUNT_OF(cur[h].y) * PHI_OF(cur[h].y);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x111>(x[q]);
+                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x111>(y[h]);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x112>(x[q]);
+                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x112>(y[h]);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x114>(x[q]);
+                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x114>(y[h]);
+                    const float r0 = R + dpp_f<0x157>(y[0]);
+                    const float r1 = r0 + row_last(y[0]);
+                    const float r2 = r1 + dpp_f<0x157>(y[1]);
+                    const float r3 = r2 + row_last(y[1]);
+                    if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
+                    R = r3;
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = x[q] + dpp_f<0x118>(x[q]);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        R = R + row_last(x[q]);
-                        if (lr == 0) rs[(g << 2) + q + 1] = R;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
+                    for (int h = 0; h < 2; ++h) cur[h] = nxt[h];
                 }
-                // last group: blocks 4g .. nb-1
-                const uint32_t b0 = g << 2;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (b0 + q < nb) {
-                        const float x = scan16(COUNT_OF(cur[q]) * PHI_OF(cur[q]));
-                        R = R + row_last(x);
-                        if (lr == 0) rs[b0 + q + 1] = R;
-                    }
+                // last group: nl = 1..4 blocks
+                const uint32_t nl = nb - (g << 2);
+                float y0 = COUNT_OF(cur[0].x) * PHI_OF(cur[0].x) + COUNT_OF(cur[0].y) * PHI_OF(cur[0].y);
+                y0 = y0 + dpp_f<0x111>(y0);
+                y0 = y0 + dpp_f<0x112>(y0);
+                y0 = y0 + dpp_f<0x114>(y0);
+                const float r0 = R + dpp_f<0x157>(y0);
+                const float r1 = r0 + row_last(y0);
+                float r2 = r1, r3 = r1;
+                if (nl > 2) {
+                    float y1 = COUNT_OF(cur[1].x) * PHI_OF(cur[1].x) + COUNT_OF(cur[1].y) * PHI_OF(cur[1].y);
+                    y1 = y1 + dpp_f<0x111>(y1);
+                    y1 = y1 + dpp_f<0x112>(y1);
+                    y1 = y1 + dpp_f<0x114>(y1);
+                    r2 = r1 + dpp_f<0x157>(y1);
+                    r3 = r2 + row_last(y1);
                 }
+                if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
+                const float S = nl == 1 ? r0 : nl == 2 ? r1 : nl == 3 ? r2 : r3;
                 wave_sync();
                 // t = fl(u * (S + Q)); the S bucket takes t < S with target
                 // t, the Q bucket target fl(t - S)
-                const float S = R;
                 const float SQ = S + Q;
                 const float tu = uu * SQ;
                 const bool sb = tu < S;
@@ -537,14 +556,14 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 }
                 uint32_t topic;
                 if (bsel < 0) {
-                    topic = sb ? (theta[start + nnz - 1] >> (cb + 2)) : (uint32_t)(K - 1);
+                    topic = sb ? (load_global(theta + start + nnz - 1) >> (cb + 2)) : (uint32_t)(K - 1);
                 } else {
                     const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
                     uint32_t e = 0;
                     float v;
                     bool ok = true;
                     if (sb) {
-                        e = theta[start + j];
+                        e = load_global(theta + start + j);
                         ok = j < nnz;
                         v = ok ? COUNT_OF(e) * PHI_OF(e) : 0.0f;
                     } else {
@@ -571,7 +590,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
         }
         wave_sync();
         if (lane < ntok) {
-            un.z[base + tok] = (uint16_t)zres[lane];
+            store_global_u16(un.z + base + tok, (uint16_t)zres[lane]);
             ll += log((double)sqres[lane] / ((double)pnd + md.k_alpha));
         }
         wave_sync();

@@ -17,12 +17,12 @@
 //   rev_idx[N_u]  u32  word-major position of each token, doc-major
 //                      (docRevIndices / docRevIdx, doc_chunk.cpp:225-255)
 //   meta[D_u]     DocMeta: theta slab start, nnz, n_d
-//   theta[cap_u + 64] u32 packed ((4 * topic) << count_bits) | count,
+//   theta[cap_u + kThetaPad] u32 packed ((4 * topic) << count_bits) | count,
 //                      count_bits = 30 - ceil(log2 K) (20 at K=1024), so
 //                      e >> count_bits is the byte offset of phi_topic in
 //                      LDS; ascending topic within a doc; slab capacity of
 //                      doc d is roundup16(min(n_d, K)) (model_theta.cpp:49-56
-//                      uses roundup32 for 32-lane warps); 64 padding entries
+//                      uses roundup32 for 32-lane warps); kThetaPad padding entries
 // Per model: n_wk[V*K] i32 (row per word), n_k[K] i32, den[K] f32.
 #pragma once
 #include <stdint.h>
@@ -34,6 +34,7 @@ constexpr int kBlock = 16;         // FP32 prefix block = one DPP row
 constexpr int kMaxK = 1024;        // Q block totals scanned by one wave
 constexpr int kWavesPerGroup = 4;  // rebuild kernels
 constexpr int kLLHistory = 1024;   // LL kept per model for this many sweeps
+constexpr int kThetaPad = 64;      // theta entries past the last slab (sampler group loads)
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 
 struct DocMeta {
