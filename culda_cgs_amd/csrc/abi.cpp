@@ -69,28 +69,35 @@ HostPhi download_phi(Engine &e, int32_t m) {
 }
 
 // Compact CSR of one unit's theta rows.
+// Theta rows as (topic, n_dk) pairs: the device's entries of one topic
+// (several when n_dk > kThetaMaxCount) are merged.
 void download_theta(Engine &e, int32_t unit, std::vector<int64_t> &off, std::vector<int32_t> &cnt,
                     std::vector<int16_t> &top) {
     check_unit(e, unit);
     UnitHost &u = e.units[unit];
-    const uint32_t cb = e.models[u.model].count_bits;
     const int32_t Dc = u.doc_end - u.doc_start;
     std::vector<DocMeta> meta(Dc);
-    std::vector<uint32_t> th(u.theta_cap);
+    std::vector<uint16_t> th(u.theta_cap);
     e.sync();
     if (Dc) hip_check(hipMemcpy(meta.data(), u.meta, Dc * sizeof(DocMeta), hipMemcpyDeviceToHost), "meta copy");
-    if (u.theta_cap) hip_check(hipMemcpy(th.data(), u.theta, u.theta_cap * 4, hipMemcpyDeviceToHost), "theta copy");
+    if (u.theta_cap) hip_check(hipMemcpy(th.data(), u.theta, u.theta_cap * 2, hipMemcpyDeviceToHost), "theta copy");
     off.assign(Dc + 1, 0);
-    for (int32_t d = 0; d < Dc; ++d) off[d + 1] = off[d] + meta[d].nnz;
-    cnt.resize(off[Dc]);
-    top.resize(off[Dc]);
-    const uint32_t cmask = (1u << cb) - 1u;
-    for (int32_t d = 0; d < Dc; ++d)
+    cnt.clear();
+    top.clear();
+    for (int32_t d = 0; d < Dc; ++d) {
         for (uint32_t j = 0; j < meta[d].nnz; ++j) {
             const uint32_t v = th[meta[d].start + j];
-            cnt[off[d] + j] = (int32_t)(v & cmask);
-            top[off[d] + j] = (int16_t)(v >> (cb + 2));
+            const int16_t k = (int16_t)(v & kThetaTopicMask);
+            const int32_t c = (int32_t)(v >> kThetaCountShift);
+            if (j > 0 && top.back() == k)
+                cnt.back() += c;
+            else {
+                top.push_back(k);
+                cnt.push_back(c);
+            }
         }
+        off[d + 1] = (int64_t)top.size();
+    }
 }
 
 FILE *open_or_fail(const std::string &path, const char *mode) {
@@ -271,7 +278,7 @@ extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
         hip_check(launch_joint_ll_words(m.nwk, V, K, std::lgamma(beta), m.beta, part, G, e.stream), "joint");
         for (size_t k = 0; k < U; ++k) {
             UnitHost &u = e.units[k];
-            hip_check(launch_joint_ll_docs(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits, m.alpha,
+            hip_check(launch_joint_ll_docs(u.meta, u.theta, u.doc_end - u.doc_start, m.alpha,
                                            part + G * (1 + k), G, e.stream),
                       "joint");
         }
@@ -759,8 +766,7 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
         a.gdelta = s.gdelta[0];
         a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, e.d_stats, u.model,
                          u.doc_end - u.doc_start};
-        a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
-                           m.count_bits, 0};
+        a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V};
         a.accepted = s.accepted;
         a.w_out = dw;
         a.f_out = df;

@@ -29,6 +29,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint32_t load_global(const uint32_t *p) { return *(const CULDA_AS1 uint32_t *)p; }
 __device__ __forceinline__ u32x2 load_global2(const uint32_t *p) { return *(const CULDA_AS1 u32x2 *)p; }
 __device__ __forceinline__ u32x4 load_global4(const void *p) { return *(const CULDA_AS1 u32x4 *)p; }
+__device__ __forceinline__ uint32_t load_global_u16(const uint16_t *p) { return *(const CULDA_AS1 uint16_t *)p; }
 __device__ __forceinline__ void store_global_u16(uint16_t *p, uint16_t v) { *(CULDA_AS1 uint16_t *)p = v; }
 
 // DPP move with zero for lanes whose source is outside the 16-lane row.

@@ -80,10 +80,10 @@ static T *upload_vec(Engine &e, const std::vector<T> &v) {
     return d;
 }
 
-static uint32_t ceil_log2(uint32_t k) {
-    uint32_t b = 0;
-    while ((1u << b) < k) ++b;
-    return b ? b : 1;
+// Largest number of theta entries a doc of nd tokens can need: at most one
+// per token, and at most one per topic plus one per kThetaMaxCount tokens.
+int64_t theta_entries_max(int64_t nd, int32_t K) {
+    return std::min<int64_t>(nd, (int64_t)K + (nd + kThetaMaxCount - 1) / kThetaMaxCount);
 }
 
 static constexpr int64_t kPhiSplit = 8192;  // hot words are rebuilt in pieces
@@ -191,8 +191,6 @@ void Engine::upload() {
     if (uploaded) fail(CULDA_ERR_STATE, "already uploaded");
     if (pending.empty()) fail(CULDA_ERR_STATE, "no corpus to upload");
     const int32_t K = prm.k;
-    const uint32_t cbits = 30 - ceil_log2((uint32_t)K);
-    const int64_t max_count = ((int64_t)1 << cbits) - 1;
     int64_t total_docs = 0;
     for (size_t mi = 0; mi < pending.size(); ++mi) {
         const Corpus &c = *pending[mi];
@@ -203,13 +201,9 @@ void Engine::upload() {
         m.alpha = pending_alpha[mi];
         m.beta = pending_beta[mi];
         m.seed = pending_seed[mi];
-        m.count_bits = cbits;
         m.num_tokens = c.num_tokens();
         if (m.num_tokens >= (int64_t)INT32_MAX)
             fail(CULDA_ERR_LIMIT, "corpus %zu has %lld tokens; counts are int32", mi, (long long)m.num_tokens);
-        if (c.max_doc_len() > max_count)
-            fail(CULDA_ERR_LIMIT, "doc length %d exceeds %lld, the largest count a packed theta entry holds at K=%d",
-                 c.max_doc_len(), (long long)max_count, K);
         m.doc_len.resize(m.num_docs);
         for (int32_t d = 0; d < m.num_docs; ++d) m.doc_len[d] = c.doc_len(d);
         total_docs += m.num_docs;
@@ -235,7 +229,7 @@ void Engine::upload() {
             for (int32_t d = 0; d < Dc; ++d) {
                 const int64_t nd = m.doc_len[ch.doc_start + d];
                 meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
-                cap += (std::min<int64_t>(nd, K) + 15) / 16 * 16;
+                cap += (theta_entries_max(nd, K) + 15) / 16 * 16;
                 if (cap + kThetaPad >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^32 entries");
             }
             u.theta_cap = cap;
@@ -251,8 +245,8 @@ void Engine::upload() {
             // kThetaPad entries of tail padding: the sampler loads whole
             // 64-entry groups and may read past the last slab (those values
             // are unused)
-            u.theta = (uint32_t *)dalloc((cap + kThetaPad) * sizeof(uint32_t));
-            hip_check(hipMemset(u.theta, 0, (cap + kThetaPad) * sizeof(uint32_t)), "theta clear");
+            u.theta = (uint16_t *)dalloc((cap + kThetaPad) * sizeof(uint16_t));
+            hip_check(hipMemset(u.theta, 0, (cap + kThetaPad) * sizeof(uint16_t)), "theta clear");
             units.push_back(std::move(u));
         }
         m.unit_end = (int32_t)units.size();
@@ -285,7 +279,7 @@ void Engine::upload() {
     for (int32_t mi = 0; mi < M; ++mi) {
         const ModelHost &m = models[mi];
         md[mi] = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K,
-                          m.V, m.count_bits, 0};
+                          m.V};
     }
     d_models = upload_vec(*this, md);
     std::vector<UnitDev> ud(U);
@@ -793,7 +787,7 @@ double Engine::sweep_bytes() const {
     const bool zp_stage = zp_valid && sparse_phi;
     const double rows = (double)(num_slots - (zp_stage ? rare_slots : 0)) * 4.0 * (double)prm.k;
     const double rare = zp_stage ? 10.0 * (double)rare_tokens : 0.0;
-    return (double)n * (4.0 + 16.0 + 2.0) + 4.0 * (double)nnz + rows + rare;
+    return (double)n * (4.0 + 16.0 + 2.0) + 2.0 * (double)nnz + rows + rare;
 }
 
 // Times one sweep per candidate slot size from the same state (z is saved

@@ -18,6 +18,7 @@
 namespace culda {
 
 void hip_check(hipError_t e, const char *what);
+int64_t theta_entries_max(int64_t nd, int32_t K);
 
 // A phi work list with its workgroup packets and the hot-word rows to zero.
 struct PhiList {
@@ -31,7 +32,6 @@ struct ModelHost {
     int32_t V = 0, K = 0, num_docs = 0;
     float alpha = 0, beta = 0;
     uint64_t seed = 0;
-    uint32_t count_bits = 0;
     int32_t unit_begin = 0, unit_end = 0;
     int32_t slot_begin = 0, slot_end = 0;  // contiguous in the slot array
     int64_t num_tokens = 0;
@@ -53,7 +53,7 @@ struct UnitHost {
     int64_t *word_idx = nullptr;
     uint32_t *rev_off = nullptr, *rev_idx = nullptr;
     DocMeta *meta = nullptr;
-    uint32_t *theta = nullptr;
+    uint16_t *theta = nullptr;
     PhiList phi_single;  // this unit alone (chunks-in-turn accumulation)
     Packet *theta_packets = nullptr;
     int32_t num_theta_packets = 0;

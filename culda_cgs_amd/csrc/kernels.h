@@ -26,8 +26,8 @@ int32_t sample_blocks_per_cu();
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);
 // A stored [m][k], B stored [n][k]; 128 x 128 tiles when m and n allow
 hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits = 1);
-hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
-                         uint16_t *out, hipStream_t s);
+hipError_t launch_sketch(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, uint16_t *out,
+                         hipStream_t s);
 hipError_t launch_gather_train(const uint16_t *sketches, const uint16_t *zs, const uint32_t *tok_doc,
                                const uint32_t *tok_word, const uint16_t *emb16, uint64_t mul, uint64_t add, uint64_t T,
                                uint64_t N, int64_t D, int64_t p0, int32_t rows, int32_t rows_pad, uint16_t *X,
@@ -56,8 +56,8 @@ hipError_t launch_mh(const MHArgs &a, hipStream_t s);
 hipError_t launch_ll_reduce(const LLArgs &a, hipStream_t s);
 hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t k, double lg_beta, float beta,
                                  double *partial, int32_t grid, hipStream_t s);
-hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
-                                uint32_t count_bits, float alpha, double *partial, int32_t grid, hipStream_t s);
+hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, float alpha,
+                                double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_copy(const float4 *src, float4 *dst, int64_t n, hipStream_t s);
 
 }  // namespace culda

@@ -276,7 +276,10 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 // ------------------------------------------------------------ theta rebuild
 // One workgroup per packet (a run of docs of one unit), one wave per doc:
 // LDS histogram over the doc's tokens (gathered through rev_idx), then an
-// ascending ballot compaction of the nonzero topics.
+// ascending compaction of the nonzero topics into 16-bit entries; a count
+// above kThetaMaxCount takes several entries (kThetaMaxCount each, the
+// remainder last), placed by a wave prefix sum (a ballot when no topic of
+// the 64 needs more than one).
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
@@ -284,7 +287,6 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     const UnitDev &un = a.units[pk.owner];
     const ModelDev &m = a.models[un.model];
     const int K = m.k;
-    const uint32_t cb = m.count_bits;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
     unsigned long long s_nnz = 0, s_tok = 0;
@@ -310,10 +312,26 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
         for (int k0 = 0; k0 < K; k0 += kWave) {
             const int k = k0 + lane;
             const uint32_t v = k < K ? h[k] : 0u;
-            const unsigned long long msk = __ballot(v != 0);
-            const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-            if (v) un.theta[start + pos] = ((uint32_t)k << (cb + 2)) | v;
-            base += (uint32_t)__popcll(msk);
+            if (__ballot(v > kThetaMaxCount) == 0ull) {
+                const unsigned long long msk = __ballot(v != 0);
+                const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                if (v) un.theta[start + pos] = (uint16_t)((v << kThetaCountShift) | (uint32_t)k);
+                base += (uint32_t)__popcll(msk);
+            } else {
+                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
+                uint32_t incl = ne;
+#pragma unroll
+                for (int d2 = 1; d2 < kWave; d2 <<= 1) {
+                    const uint32_t y = __shfl_up(incl, d2, kWave);
+                    if (lane >= d2) incl += y;
+                }
+                const uint32_t pos = base + incl - ne;
+                for (uint32_t q = 0; q < ne; ++q) {
+                    const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
+                    un.theta[start + pos + q] = (uint16_t)((c << kThetaCountShift) | (uint32_t)k);
+                }
+                base += __shfl(incl, kWave - 1, kWave);
+            }
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
         if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
@@ -373,11 +391,15 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     const ModelDev md = uniform_copy(a.models + un.model);
     const int K = md.k;
     const float alpha = md.alpha, beta = md.beta;
-    const uint32_t cb = md.count_bits, cmask = (1u << cb) - 1u;
-    const uint32_t *theta = un.theta;
-    // phi of a packed theta entry e (byte offset e >> cb) and its count
-#define PHI_OF(e) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + ((e) >> cb)))
-#define COUNT_OF(e) ((float)((e)&cmask))
+    const uint16_t *theta = un.theta;
+    // phi and count of a 16-bit theta entry e, and of the low and high
+    // entries of a dword d holding two of them
+#define PHI_OF(e) (ph[(e)&kThetaTopicMask])
+#define COUNT_OF(e) ((float)((e) >> kThetaCountShift))
+#define PHI_LO(d) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + (((d) << 2) & 0xffcu)))
+#define PHI_HI(d) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + (((d) >> 14) & 0xffcu)))
+#define COUNT_LO(d) ((float)(((d) >> kThetaCountShift) & kThetaMaxCount))
+#define COUNT_HI(d) ((float)((d) >> (16 + kThetaCountShift)))
 
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
     const int32_t *row = md.nwk + (int64_t)s.word * K;
@@ -474,7 +496,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 // are zero (the slab tail), so their products are exact
                 // zeros.  A block total is the balanced tree of its 16
                 // products (lane 15 of the in-block Kogge-Stone scan): lane
-                // lr holds entries 2(lr & 7) and 2(lr & 7) + 1 of block
+                // lr holds entries 2(lr & 7) and 2(lr & 7) + 1 (one dword) of block
                 // lr >> 3 of a pair of blocks and adds them (the tree's
                 // first level), three DPP steps over 8 lanes finish it at
                 // lanes 7 and 15.  Totals are carried sequentially,
@@ -487,21 +509,22 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 // may hold sums of such blocks; nothing reads them.
                 const uint32_t nb = (nnz + 15u) >> 4;
                 const uint32_t ng = (nb + 3u) >> 2;
-                const uint32_t *tp = theta + start + ((lr >> 3) << 4) + ((lr & 7) << 1);
-                u32x2 cur[2];
+                const uint32_t *tp =
+                    reinterpret_cast<const uint32_t *>(theta + start) + ((lr >> 3) << 3) + (lr & 7);
+                uint32_t cur[2];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) cur[h] = load_global2(tp + (h << 5));
+                for (int h = 0; h < 2; ++h) cur[h] = load_global(tp + (h << 4));
                 float R = 0.0f;
                 uint32_t g = 0;
                 for (; g + 1 < ng; ++g) {
-                    const uint32_t *gp = tp + ((g + 1) << 6);
-                    u32x2 nxt[2];
+                    const uint32_t *gp = tp + ((g + 1) << 5);
+                    uint32_t nxt[2];
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) nxt[h] = load_global2(gp + (h << 5));
+                    for (int h = 0; h < 2; ++h) nxt[h] = load_global(gp + (h << 4));
                     float y[2];
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
-                        y[h] = COUNT_OF(cur[h].x) * PHI_OF(cur[h].x) + COUNT_OF(cur[h].y) * PHI_OF(cur[h].y);
+                        y[h] = COUNT_LO(cur[h]) * PHI_LO(cur[h]) + COUNT_HI(cur[h]) * PHI_HI(cur[h]);
 #pragma unroll
                     for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x111>(y[h]);
 #pragma unroll
@@ -519,7 +542,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 }
                 // last group: nl = 1..4 blocks
                 const uint32_t nl = nb - (g << 2);
-                float y0 = COUNT_OF(cur[0].x) * PHI_OF(cur[0].x) + COUNT_OF(cur[0].y) * PHI_OF(cur[0].y);
+                float y0 = COUNT_LO(cur[0]) * PHI_LO(cur[0]) + COUNT_HI(cur[0]) * PHI_HI(cur[0]);
                 y0 = y0 + dpp_f<0x111>(y0);
                 y0 = y0 + dpp_f<0x112>(y0);
                 y0 = y0 + dpp_f<0x114>(y0);
@@ -527,7 +550,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 const float r1 = r0 + row_last(y0);
                 float r2 = r1, r3 = r1;
                 if (nl > 2) {
-                    float y1 = COUNT_OF(cur[1].x) * PHI_OF(cur[1].x) + COUNT_OF(cur[1].y) * PHI_OF(cur[1].y);
+                    float y1 = COUNT_LO(cur[1]) * PHI_LO(cur[1]) + COUNT_HI(cur[1]) * PHI_HI(cur[1]);
                     y1 = y1 + dpp_f<0x111>(y1);
                     y1 = y1 + dpp_f<0x112>(y1);
                     y1 = y1 + dpp_f<0x114>(y1);
@@ -556,14 +579,14 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                 }
                 uint32_t topic;
                 if (bsel < 0) {
-                    topic = sb ? (load_global(theta + start + nnz - 1) >> (cb + 2)) : (uint32_t)(K - 1);
+                    topic = sb ? (load_global_u16(theta + start + nnz - 1) & kThetaTopicMask) : (uint32_t)(K - 1);
                 } else {
                     const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
                     uint32_t e = 0;
                     float v;
                     bool ok = true;
                     if (sb) {
-                        e = load_global(theta + start + j);
+                        e = load_global_u16(theta + start + j);
                         ok = j < nnz;
                         v = ok ? COUNT_OF(e) * PHI_OF(e) : 0.0f;
                     } else {
@@ -579,7 +602,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                         const int nvalid = (int)nnz - (bsel << 4);
                         msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
                     }
-                    const uint32_t et = __shfl(e >> (cb + 2), msel, kBlock);
+                    const uint32_t et = __shfl(e & kThetaTopicMask, msel, kBlock);
                     topic = sb ? et : (uint32_t)((bsel << 4) + msel);
                 }
                 if (lr == 0) {
@@ -600,6 +623,10 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
     if (lane == 0) a.ll_partial[slot] = ll;
 #undef PHI_OF
 #undef COUNT_OF
+#undef PHI_LO
+#undef PHI_HI
+#undef COUNT_LO
+#undef COUNT_HI
     if (!kPersistent) break;
     wave_sync();  // LDS is reused by the next slot
     }
@@ -658,16 +685,23 @@ __global__ __launch_bounds__(256) void joint_words_kernel(const int32_t *nwk, in
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-// sum over docs and nonzero n_dk of lgamma(n_dk + alpha) - lgamma(alpha)
-__global__ __launch_bounds__(256) void joint_docs_kernel(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
-                                                         uint32_t cb, float alpha, double *partial) {
+// sum over docs and nonzero n_dk of lgamma(n_dk + alpha) - lgamma(alpha);
+// n_dk is summed over the topic's consecutive entries, by the lane of its
+// last entry (every entry before it holds kThetaMaxCount)
+__global__ __launch_bounds__(256) void joint_docs_kernel(const DocMeta *meta, const uint16_t *theta, int32_t num_docs,
+                                                         float alpha, double *partial) {
     __shared__ double s[256];
     const double lg_alpha = lgamma((double)alpha);
     double acc = 0.0;
     for (int d = blockIdx.x * 4 + (threadIdx.x >> 6); d < num_docs; d += gridDim.x * 4) {
         const DocMeta m = meta[d];
-        for (uint32_t j = threadIdx.x & 63; j < m.nnz; j += 64)
-            acc += lgamma((double)(theta[m.start + j] & ((1u << cb) - 1u)) + (double)alpha) - lg_alpha;
+        for (uint32_t j = threadIdx.x & 63; j < m.nnz; j += 64) {
+            const uint32_t e = theta[m.start + j], k = e & kThetaTopicMask;
+            if (j + 1 < m.nnz && (theta[m.start + j + 1] & kThetaTopicMask) == k) continue;
+            uint32_t n = e >> kThetaCountShift;
+            for (uint32_t i = j; i > 0 && (theta[m.start + i - 1] & kThetaTopicMask) == k; --i) n += kThetaMaxCount;
+            acc += lgamma((double)n + (double)alpha) - lg_alpha;
+        }
     }
     s[threadIdx.x] = acc;
     __syncthreads();
@@ -781,10 +815,9 @@ hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t 
     return hipGetLastError();
 }
 
-hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
-                                float alpha, double *partial, int32_t grid, hipStream_t s) {
-    hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, count_bits, alpha,
-                       partial);
+hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, float alpha,
+                                double *partial, int32_t grid, hipStream_t s) {
+    hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, alpha, partial);
     return hipGetLastError();
 }
 

@@ -17,12 +17,15 @@
 //   rev_idx[N_u]  u32  word-major position of each token, doc-major
 //                      (docRevIndices / docRevIdx, doc_chunk.cpp:225-255)
 //   meta[D_u]     DocMeta: theta slab start, nnz, n_d
-//   theta[cap_u + kThetaPad] u32 packed ((4 * topic) << count_bits) | count,
-//                      count_bits = 30 - ceil(log2 K) (20 at K=1024), so
-//                      e >> count_bits is the byte offset of phi_topic in
-//                      LDS; ascending topic within a doc; slab capacity of
-//                      doc d is roundup16(min(n_d, K)) (model_theta.cpp:49-56
-//                      uses roundup32 for 32-lane warps); kThetaPad padding entries
+//   theta[cap_u + kThetaPad] u16 entries (count << 10) | topic with
+//                      1 <= count <= 63, ascending topic within a doc; a
+//                      topic with n_dk > 63 takes ceil(n_dk / 63) consecutive
+//                      entries (63, ..., 63, remainder); 0 = padding (count
+//                      0).  Slab capacity of doc d is roundup16(min(n_d,
+//                      K + ceil(n_d / 63))) entries (model_theta.cpp:49-56
+//                      sizes rows by min(n_d, K), roundup32 for 32-lane
+//                      warps, with (topic, count) as two int arrays);
+//                      kThetaPad padding entries at the end
 // Per model: n_wk[V*K] i32 (row per word), n_k[K] i32, den[K] f32.
 #pragma once
 #include <stdint.h>
@@ -35,11 +38,14 @@ constexpr int kMaxK = 1024;        // Q block totals scanned by one wave
 constexpr int kWavesPerGroup = 4;  // rebuild kernels
 constexpr int kLLHistory = 1024;   // LL kept per model for this many sweeps
 constexpr int kThetaPad = 64;      // theta entries past the last slab (sampler group loads)
+constexpr int kThetaCountShift = 10;    // theta entry: count in bits 10..15
+constexpr uint32_t kThetaTopicMask = 1023u;  // topic in bits 0..9 (K <= kMaxK)
+constexpr uint32_t kThetaMaxCount = 63u;     // largest count of one entry
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 
 struct DocMeta {
     uint32_t start;  // theta slab start (entries)
-    uint32_t nnz;    // nonzero topics after the last rebuild
+    uint32_t nnz;    // theta entries after the last rebuild
     uint32_t nd;     // doc length
     uint32_t pad;
 };
@@ -55,8 +61,6 @@ struct ModelDev {
     float beta;
     int32_t k;
     int32_t num_words;
-    uint32_t count_bits;
-    int32_t pad;
 };
 
 struct UnitDev {
@@ -67,8 +71,8 @@ struct UnitDev {
     const uint32_t *rev_off;
     const uint32_t *rev_idx;
     DocMeta *meta;
-    uint32_t *theta;
-    unsigned long long *stats;  // [0] += sum nnz, [1] += sum n_d * nnz (last rebuild)
+    uint16_t *theta;
+    unsigned long long *stats;  // [0] += sum entries, [1] += sum n_d * entries (last rebuild)
     int32_t model;
     int32_t num_docs;
 };
@@ -195,7 +199,7 @@ struct HeldoutArgs {
     const EvalSlot *slots;
     const uint32_t *tokens;  // held-out doc id of each evaluated token, word-major
     const DocMeta *meta;     // fold-in theta of the held-out docs
-    const uint32_t *theta;
+    const uint16_t *theta;
     const int32_t *nwk;
     const int32_t *nk;
     double *partial;         // one per slot
@@ -204,7 +208,7 @@ struct HeldoutArgs {
     int32_t num_slots;
     int32_t k;
     int32_t num_words;
-    uint32_t count_bits;
+    int32_t pad;
 };
 
 // ------------------------------------------------------------ learned MH
@@ -218,6 +222,7 @@ constexpr int kIn = kSketch + kEmb;  // 384
 constexpr int kHidden = 512;
 constexpr int kAliasBits = 20;       // proposal weights are integers summing to K << kAliasBits
 constexpr int kMaxProposals = 8;     // MH proposals per token and sweep
+constexpr int kMHThetaEntries = 2048;  // longest theta row the MH step holds in registers
 constexpr int kGemmTile = 64;        // C tile (rows and columns) of the BF16 GEMM
 constexpr int kGemmK = 32;           // reduction step of the BF16 GEMM
 

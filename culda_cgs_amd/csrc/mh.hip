@@ -14,7 +14,7 @@ namespace culda {
 // One wave per slot (evaluated tokens of one word).  The word's phi_k =
 // (n_wk + beta) / (n_k + V beta) is staged in LDS in FP64 and
 // Q = alpha * sum_k phi_k is reduced in a fixed order; each lane then takes
-// one token: S = sum over the doc's fold-in nonzeros of n_dk * phi_k, and
+// one token: S = sum over the doc's fold-in theta entries of count * phi_k, and
 // adds log((S + Q) / (n_d + K alpha)).  Slot partials are summed on the
 // host in slot order.
 __global__ __launch_bounds__(64) void heldout_kernel(HeldoutArgs a) {
@@ -32,7 +32,6 @@ __global__ __launch_bounds__(64) void heldout_kernel(HeldoutArgs a) {
     }
     q = wave_sum(q) * a.alpha;
     wave_sync();
-    const uint32_t cb = a.count_bits, cmask = (1u << cb) - 1u;
     const double kalpha = (double)K * a.alpha;
     double ll = 0.0;
     for (int64_t i = s.begin + lane; i < s.end; i += kWave) {
@@ -40,7 +39,7 @@ __global__ __launch_bounds__(64) void heldout_kernel(HeldoutArgs a) {
         double sum = 0.0;
         for (uint32_t j = 0; j < m.nnz; ++j) {
             const uint32_t e = a.theta[m.start + j];
-            sum += (double)(e & cmask) * ph[(e >> cb) >> 2];
+            sum += (double)(e >> kThetaCountShift) * ph[e & kThetaTopicMask];
         }
         ll += log((sum + q) / ((double)m.nd + kalpha));
     }
@@ -116,6 +115,10 @@ __device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t id, Op op
 
 __device__ __forceinline__ uint32_t wave_add_scan_u32(uint32_t x) {
     return wave_scan_u32(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
+
+__device__ __forceinline__ uint32_t wave_add_all_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)wave_add_scan_u32(x), kWave - 1);
 }
 
 __device__ __forceinline__ uint32_t wave_max_all_u32(uint32_t x) {
@@ -308,8 +311,8 @@ __global__ __launch_bounds__(256) void transpose_bf16_kernel(const uint16_t *src
 // ------------------------------------------------------------ features
 // One wave per doc of a unit: integer bucket sums in LDS (exact, so the
 // result does not depend on the order), then bucket / n_d rounded to BF16.
-__global__ __launch_bounds__(64) void sketch_kernel(const DocMeta *meta, const uint32_t *theta, int32_t num_docs,
-                                                    uint32_t count_bits, uint16_t *out) {
+__global__ __launch_bounds__(64) void sketch_kernel(const DocMeta *meta, const uint16_t *theta, int32_t num_docs,
+                                                    uint16_t *out) {
     __shared__ int32_t acc[kSketch];
     const int lane = threadIdx.x;
     const int d = blockIdx.x;
@@ -317,11 +320,10 @@ __global__ __launch_bounds__(64) void sketch_kernel(const DocMeta *meta, const u
     for (int b = lane; b < kSketch; b += kWave) acc[b] = 0;
     wave_sync();
     const DocMeta m = meta[d];
-    const uint32_t cmask = (1u << count_bits) - 1u;
     for (uint32_t j = lane; j < m.nnz; j += kWave) {
         const uint32_t e = theta[m.start + j];
-        const uint32_t k = (e >> count_bits) >> 2;
-        const int32_t c = (int32_t)(e & cmask);
+        const uint32_t k = e & kThetaTopicMask;
+        const int32_t c = (int32_t)(e >> kThetaCountShift);
         atomicAdd(&acc[sketch_bucket(k)], sketch_negative(k) ? -c : c);
     }
     wave_sync();
@@ -564,8 +566,9 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     if (r >= a.n) return;
     const int K = a.model.k, KL = K / kWave;
     constexpr uint32_t M = 1u << kAliasBits;
-    // the token, and its doc's theta row (<= K entries, 16 per lane; unused
-    // slots hold 0 = topic 0 with count 0), loaded first
+    // the token, and its doc's theta row (<= kMHThetaEntries entries, two
+    // per dword, 16 dwords per lane; unused entries hold 0 = topic 0 with
+    // count 0), loaded first
     const int64_t i = a.start + r;
     const uint32_t d = a.unit.tokens[i];
     const uint32_t w = a.word[i];
@@ -573,10 +576,11 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     const uint64_t gid = (uint64_t)(i + a.gdelta[w]);
     const DocMeta dm = a.unit.meta[d];
     uint32_t tr[16];
+    const uint32_t *trow = reinterpret_cast<const uint32_t *>(a.unit.theta + dm.start);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        const uint32_t j = lane + kWave * t;
-        tr[t] = j < dm.nnz ? a.unit.theta[dm.start + j] : 0u;
+        const uint32_t j = lane + kWave * t;  // entries 2j, 2j + 1 (2j + 1 past nnz is a zero of the slab tail)
+        tr[t] = 2 * j < dm.nnz ? trow[j] : 0u;
     }
     const float *lg = a.logits + (int64_t)r * K;
     float e[16];
@@ -688,7 +692,6 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     // evaluated from the prefix sums -- its n_dk comes from the doc's theta
     // row held in registers (one compare per entry and a wave max), its
     // n_wk / n_k from memory, all loads issued together.
-    const uint32_t cb = a.model.count_bits, cmask = (1u << cb) - 1u;
     const int32_t *nrow = a.model.nwk + (int64_t)w * K;
     const double alpha = a.model.alpha, beta = a.model.beta;
     const double vbeta = (double)a.model.num_words * beta;
@@ -721,11 +724,14 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
             continue;
         }
         const uint32_t k = cand[c];
+        // n_dk = sum of the counts of the topic's entries (a padding 0 adds 0)
         uint32_t cnt = 0;
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (((tr[t] >> cb) >> 2) == k) cnt = max(cnt, tr[t] & cmask);  // a padding 0 never lowers it
-        cnt = wave_max_all_u32(cnt);
+        for (int t = 0; t < 16; ++t) {
+            if ((tr[t] & kThetaTopicMask) == k) cnt += (tr[t] >> kThetaCountShift) & kThetaMaxCount;
+            if (((tr[t] >> 16) & kThetaTopicMask) == k) cnt += tr[t] >> (16 + kThetaCountShift);
+        }
+        cnt = wave_add_all_u32(cnt);
         pt[c] = (((double)cnt + alpha) * ((double)nrow[k] + beta)) / ((double)a.model.nk[k] + vbeta);
     }
     uint32_t cur = z0;
@@ -772,10 +778,10 @@ hipError_t launch_transpose_bf16(const uint16_t *src, int64_t rows, int64_t cols
     return hipGetLastError();
 }
 
-hipError_t launch_sketch(const DocMeta *meta, const uint32_t *theta, int32_t num_docs, uint32_t count_bits,
-                         uint16_t *out, hipStream_t s) {
+hipError_t launch_sketch(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, uint16_t *out,
+                         hipStream_t s) {
     if (num_docs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sketch_kernel, dim3(num_docs), dim3(kWave), 0, s, meta, theta, num_docs, count_bits, out);
+    hipLaunchKernelGGL(sketch_kernel, dim3(num_docs), dim3(kWave), 0, s, meta, theta, num_docs, out);
     return hipGetLastError();
 }
 

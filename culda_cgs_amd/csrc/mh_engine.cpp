@@ -68,7 +68,7 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     }
     const UnitHost &u = units[0];
     HeldoutArgs a{eval.slots, eval.tokens, u.meta, u.theta, m.nwk, m.nk, eval.partial, (double)m.alpha,
-                  (double)m.beta, eval.num_slots, m.K, m.V, m.count_bits};
+                  (double)m.beta, eval.num_slots, m.K, m.V, 0};
     hip_check(launch_heldout(a, stream), "held-out launch");
     std::vector<double> part(eval.num_slots);
     hip_check(hipMemcpyAsync(part.data(), eval.partial, part.size() * 8, hipMemcpyDeviceToHost, stream), "copy");
@@ -120,6 +120,10 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     if (!uploaded || models.size() != 1) fail(CULDA_ERR_STATE, "learned MH needs one uploaded corpus");
     const ModelHost &m = models[0];
     if (m.K % 64) fail(CULDA_ERR_ARG, "learned MH needs k %% 64 == 0 (k = %d)", m.K);
+    for (int64_t nd : m.doc_len)
+        if (theta_entries_max(nd, m.K) > kMHThetaEntries)
+            fail(CULDA_ERR_LIMIT, "learned MH holds theta rows of at most %d entries; a doc of %lld tokens may need more",
+                 kMHThetaEntries, (long long)nd);
     culda_mh_params p = p_in;
     if (!p.batch) p.batch = 8192;
     if (!p.infer_batch) p.infer_batch = 65536;
@@ -260,8 +264,8 @@ void Engine::mh_sketch_current(hipStream_t st) {
     const ModelHost &m = models[0];
     for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
         const UnitHost &u = units[ui];
-        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits,
-                                s.sk_cur + (int64_t)u.doc_start * kSketch, st),
+        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start, s.sk_cur + (int64_t)u.doc_start * kSketch,
+                                st),
                   "sketch");
     }
 }
@@ -278,7 +282,7 @@ int32_t Engine::mh_collect() {
             hip_check(hipMemcpyAsync(s.zs + (int64_t)s.nsnap * s.N + s.unit_base[ui - m.unit_begin], u.z,
                                      u.num_tokens * 2, hipMemcpyDeviceToDevice, stream),
                       "z snapshot");
-        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start, m.count_bits,
+        hip_check(launch_sketch(u.meta, u.theta, u.doc_end - u.doc_start,
                                 s.sks + ((int64_t)s.nsnap * s.D + u.doc_start) * kSketch, stream),
                   "sketch");
     }
@@ -420,8 +424,8 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             a.gdelta = s.gdelta[ui - m.unit_begin];
             a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
                              u.doc_end - u.doc_start};
-            a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V,
-                               m.count_bits, 0};
+            a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K,
+                               m.V};
             a.accepted = s.accepted;
             a.start = start;
             a.n = (int32_t)n;

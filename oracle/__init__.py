@@ -32,6 +32,9 @@ class SweepOut(C.Structure):
                 ("mismatch", C.c_int64), ("s_bucket", C.c_int64)]
 
 
+THETA_MAX_COUNT = 63  # largest count of one theta entry
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -57,7 +60,7 @@ def lib():
                                     P(C.c_int32), P(C.c_int64), P(C.c_int32), P(C.c_int16), P(C.c_double)]
         L.oc_sweep_fast.restype = None
         L.oc_loglik_joint.argtypes = [P(_OcCorpus), C.c_float, C.c_float, P(C.c_int32), P(C.c_int32),
-                                      P(C.c_int64), P(C.c_int32)]
+                                      P(C.c_int64), P(C.c_int32), P(C.c_int16)]
         L.oc_loglik_joint.restype = C.c_double
         L.oc_max_threads.restype = C.c_int
         L.oc_set_threads.argtypes = [C.c_int]
@@ -112,7 +115,8 @@ class OracleModel:
                             _p(self.chunk_doc_end, C.c_int32), self._wi, self._tk, self._z)
         lens = np.diff(np.concatenate([[0], self.doc_cum]))
         self.doc_len = lens
-        self.theta_cap = int(np.minimum(lens, self.K).sum())
+        # theta entries: a topic with count c takes ceil(c / THETA_MAX_COUNT)
+        self.theta_cap = int(np.minimum(lens, self.K + (lens + THETA_MAX_COUNT - 1) // THETA_MAX_COUNT).sum())
         self.nwk = np.zeros(self.V * self.K, np.int32)
         self.nk = np.zeros(self.K, np.int32)
         self.th_off = np.zeros(self.D + 1, np.int64)
@@ -148,9 +152,20 @@ class OracleModel:
                           _p(h, C.c_float))
         return h.reshape(self.V, self.K)
 
-    def theta_rows(self, d):
+    def theta_entries(self, d):
+        """Doc d's theta entries (counts, topics), a topic with n_dk > 63
+        split into ceil(n_dk / 63) entries -- the order S is summed in."""
         o0, o1 = self.th_off[d], self.th_off[d + 1]
         return self.th_cnt[o0:o1], self.th_top[o0:o1]
+
+    def theta_rows(self, d):
+        """Doc d's nonzero (n_dk, topic) pairs, ascending topic."""
+        cnt, top = self.theta_entries(d)
+        if top.size == 0:
+            return cnt, top
+        first = np.concatenate([[True], top[1:] != top[:-1]])
+        starts = np.flatnonzero(first)
+        return np.add.reduceat(cnt, starts).astype(np.int32), top[starts]
 
     def sweep(self, alpha, beta, seed, sweep, want_exact=False):
         """Faithful FP32 sweep (writes self.z); returns (SweepOut, exact_z, flags)."""
@@ -177,4 +192,4 @@ class OracleModel:
     def loglik_joint(self, alpha, beta):
         return lib().oc_loglik_joint(C.byref(self._c), alpha, beta, _p(self.nwk, C.c_int32),
                                      _p(self.nk, C.c_int32), _p(self.th_off, C.c_int64),
-                                     _p(self.th_cnt, C.c_int32))
+                                     _p(self.th_cnt, C.c_int32), _p(self.th_top, C.c_int16))

@@ -10,7 +10,10 @@
  *   - frozen counts for a whole sweep, own assignment included
  *     (kernel/lda_train_kernel.cu:104-259, train/SingleChunkSingleGPU.h:75-116);
  *   - two buckets S (theta nonzeros) and Q (alpha * phi head)
- *     (kernel/lda_train_kernel.cu:68-137, :145-253);
+ *     (kernel/lda_train_kernel.cu:68-137, :145-253); S runs over the
+ *     doc's theta *entries*: ascending topics, a topic with n_dk > 63
+ *     split into ceil(n_dk / 63) entries (63, ..., 63, remainder), the
+ *     layout of the device's 16-bit theta entries;
  *   - counts rebuilt from z after the sweep
  *     (kernel/lda_phi_kernel.cu:76-117, kernel/lda_theta_kernel.cu:5-87),
  *     summed over chunks (kernel/lda_phi_kernel.cu:162-183);
@@ -134,9 +137,16 @@ void oc_rebuild_phi(const oc_corpus *c, int32_t *nwk, int32_t *nk) {
             }
 }
 
-/* Theta rows, global doc order, ascending topics (kernel/lda_theta_kernel.cu:44-84).
- * th_off has D+1 entries; th_cnt / th_top have room for
- * sum_d min(n_d, K) entries.  Returns sum_d n_d * nnz(d). */
+#define OC_THETA_MAX_COUNT 63
+
+/* Entries of one topic with count c: ceil(c / 63). */
+static inline int64_t theta_entries(int32_t c) { return (c + OC_THETA_MAX_COUNT - 1) / OC_THETA_MAX_COUNT; }
+
+/* Theta rows, global doc order, ascending topics (kernel/lda_theta_kernel.cu:44-84),
+ * as entries: a topic with count c > 63 takes ceil(c / 63) consecutive
+ * entries, 63 each and the remainder last.  th_off has D+1 entries; th_cnt /
+ * th_top have room for sum_d min(n_d, K + ceil(n_d / 63)) entries.  Returns
+ * sum_d n_d * entries(d). */
 int64_t oc_rebuild_theta(const oc_corpus *c, int64_t *th_off, int32_t *th_cnt, int16_t *th_top) {
     const int32_t K = c->K, D = c->D;
     /* doc-major list of token topics */
@@ -161,7 +171,7 @@ int64_t oc_rebuild_theta(const oc_corpus *c, int64_t *th_off, int32_t *th_cnt, i
         for (int32_t d = 0; d < D; ++d) {
             int64_t m = 0;
             for (int64_t t = pos[d]; t < pos[d + 1]; ++t) h[dz[t]]++;
-            for (int32_t k = 0; k < K; ++k) m += h[k] != 0;
+            for (int32_t k = 0; k < K; ++k) m += theta_entries(h[k]);
             nnz[d] = m;
             for (int64_t t = pos[d]; t < pos[d + 1]; ++t) h[dz[t]] = 0;
         }
@@ -176,13 +186,14 @@ int64_t oc_rebuild_theta(const oc_corpus *c, int64_t *th_off, int32_t *th_cnt, i
         for (int32_t d = 0; d < D; ++d) {
             for (int64_t t = pos[d]; t < pos[d + 1]; ++t) h[dz[t]]++;
             int64_t o = th_off[d];
-            for (int32_t k = 0; k < K; ++k)
-                if (h[k]) {
-                    th_cnt[o] = h[k];
+            for (int32_t k = 0; k < K; ++k) {
+                for (int32_t c = h[k]; c > 0; c -= OC_THETA_MAX_COUNT) {
+                    th_cnt[o] = c > OC_THETA_MAX_COUNT ? OC_THETA_MAX_COUNT : c;
                     th_top[o] = (int16_t)k;
                     ++o;
-                    h[k] = 0;
                 }
+                h[k] = 0;
+            }
             total += (pos[d + 1] - pos[d]) * nnz[d];
         }
         free(h);
@@ -556,7 +567,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
  * form); the reference declares LDALikelihood but never defines it
  * (train/lda_train.h:16). */
 double oc_loglik_joint(const oc_corpus *c, float alpha, float beta, const int32_t *nwk, const int32_t *nk,
-                       const int64_t *th_off, const int32_t *th_cnt) {
+                       const int64_t *th_off, const int32_t *th_cnt, const int16_t *th_top) {
     const int32_t K = c->K, V = c->V, D = c->D;
     const double a = alpha, b = beta, vb = (double)V * b, ka = (double)K * a;
     double s = 0;
@@ -565,7 +576,13 @@ double oc_loglik_joint(const oc_corpus *c, float alpha, float beta, const int32_
         if (nwk[i]) s += lgamma((double)nwk[i] + b) - lgamma(b);
     for (int32_t d = 0; d < D; ++d) {
         s += lgamma(ka) - lgamma((double)doc_len(c, d) + ka);
-        for (int64_t j = th_off[d]; j < th_off[d + 1]; ++j) s += lgamma((double)th_cnt[j] + a) - lgamma(a);
+        /* n_dk = the sum over the topic's consecutive entries */
+        for (int64_t j = th_off[d]; j < th_off[d + 1];) {
+            int64_t n = 0, i = j;
+            for (; i < th_off[d + 1] && th_top[i] == th_top[j]; ++i) n += th_cnt[i];
+            s += lgamma((double)n + a) - lgamma(a);
+            j = i;
+        }
     }
     return s;
 }

@@ -79,7 +79,7 @@ def test_rebuild_matches_numpy_histograms(chunks):
         assert np.array_equal(top, nz)                   # ascending topics
         assert np.array_equal(cnt, ndk[d][nz])
         assert cnt.sum() == lens[d]
-        tok_nnz += lens[d] * nz.size
+        tok_nnz += lens[d] * int(((ndk[d][nz] + 62) // 63).sum())
     assert om.token_nnz == tok_nnz
     # validPhi invariants (model/model_phi.cpp:194-280)
     assert np.array_equal(nwk.sum(1), np.concatenate([np.diff(w) for w in wi]).reshape(chunks, V).sum(0))
@@ -217,3 +217,53 @@ def test_loglik_closed_forms():
            + len(docs) * (g(K * alpha) - K * g(alpha)) + g(ndk + alpha).sum() - g(lens + K * alpha).sum())
     got = before.loglik_joint(alpha, beta)
     assert abs(got - ref) <= 1e-9 * abs(ref)
+
+
+def test_theta_entries_split_counts_above_63():
+    """A topic with n_dk > 63 takes ceil(n_dk / 63) entries (63, ..., 63,
+    remainder): the entry list S is summed over.  Merged rows, the joint
+    log-likelihood (gammaln of the merged n_dk) and a sweep's exact FP64
+    log-likelihood are those of the merged counts."""
+    K, V = 32, 5
+    counts = {3: 300, 4: 63, 9: 64, 20: 126, 31: 1}
+    z = np.concatenate([np.full(c, k, np.uint16) for k, c in counts.items()])
+    docs = [np.arange(z.size) % V, np.array([0, 1, 2])]
+    arrays = chunks_from_docs(V, docs, 1)
+    om = oracle.OracleModel(V, K, *arrays)
+    # z is word-major: give doc 0's tokens the planted topics in that order
+    words = np.repeat(np.arange(V), np.diff(arrays[2][0]))
+    zw = np.zeros(arrays[3][0].size, np.uint16)
+    d0 = np.flatnonzero(arrays[3][0] == 0)
+    zw[d0] = np.sort(z)  # any assignment of the planted multiset to doc 0's tokens
+    zw[arrays[3][0] == 1] = [5, 5, 6]
+    om.set_z([zw])
+    om.rebuild()
+    cnt, top = om.theta_entries(0)
+    assert list(top) == [3] * 5 + [4] + [9] * 2 + [20] * 2 + [31]
+    assert list(cnt) == [63, 63, 63, 63, 48, 63, 63, 1, 63, 63, 1]
+    mc, mt = om.theta_rows(0)
+    assert list(mt) == sorted(counts) and list(mc) == [counts[k] for k in sorted(counts)]
+    cnt1, top1 = om.theta_entries(1)
+    assert list(top1) == [5, 6] and list(cnt1) == [2, 1]
+    assert om.token_nnz == z.size * 11 + 3 * 2
+    # joint log p(w, z): gammaln of the merged counts
+    alpha, beta = 0.5, float(np.float32(0.1))
+    nwk = om.nwk.reshape(V, K)
+    ndk = np.zeros((2, K))
+    ndk[0, list(counts)] = list(counts.values())
+    ndk[1, [5, 6]] = [2, 1]
+    lens = np.array([z.size, 3])
+    g = special.gammaln
+    ref = (K * (g(V * beta) - V * g(beta)) + (g(nwk + beta)).sum() - g(om.nk + V * beta).sum()
+           + 2 * (g(K * alpha) - K * g(alpha)) + g(ndk + alpha).sum() - g(lens + K * alpha).sum())
+    assert abs(om.loglik_joint(alpha, beta) - ref) <= 1e-9 * abs(ref)
+    # the sweep's exact LL uses the merged conditional
+    before = oracle.OracleModel(V, K, *arrays, z=om.z)
+    before.rebuild()
+    out, _, _ = om.sweep(alpha, beta, 1, 1)
+    ll = 0.0
+    for i in range(zw.size):
+        mass, _ = _conditional(before, int(arrays[3][0][i]), int(words[i]), alpha, beta)
+        ll += np.log(mass.sum() / (lens[arrays[3][0][i]] + K * alpha))
+    assert abs(out.ll_sum_exact - ll) <= 1e-10 * abs(ll)
+    assert abs(out.ll_sum - ll) <= 1e-5 * abs(ll)
