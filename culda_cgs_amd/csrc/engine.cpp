@@ -34,6 +34,8 @@ Engine::~Engine() {
     for (hipEvent_t &e : ev_sampled)
         if (e) (void)hipEventDestroy(e);
     if (ev_theta_done) (void)hipEventDestroy(ev_theta_done);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
     if (stream2) (void)hipStreamDestroy(stream2);
 }
@@ -705,14 +707,26 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
 // sweep counter advances -- everything reads the sweep index on the device,
 // so the sequence can be captured once and replayed.  No host memset and
 // no host synchronisation inside.
+// One batched iteration on stream s.  The theta rebuild and the count
+// update both only read the new z, so theta runs on stream2 beside the LL,
+// phi and head-denominator launches; s waits for it before the sweep
+// counter advances.  Captured into a graph, this is a fork and a join.
 void Engine::enqueue_batch_iteration(hipStream_t s, hipEvent_t before, hipEvent_t after) {
+    if (!ev_fork) {
+        hip_check(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "hipEventCreate");
+    }
     if (before) hip_check(hipEventRecord(before, s), "event");
     sample_all(s);
     if (after) hip_check(hipEventRecord(after, s), "event");
+    hip_check(hipEventRecord(ev_fork, s), "event");
+    hip_check(hipStreamWaitEvent(stream2, ev_fork, 0), "wait");
+    rebuild_theta_all(stream2);
+    hip_check(hipEventRecord(ev_join, stream2), "event");
     ll_reduce(s);
     update_phi(s);
     phi_head(s);
-    rebuild_theta_all(s);
+    hip_check(hipStreamWaitEvent(s, ev_join, 0), "wait");
     hip_check(launch_add_u32(d_sweep, 1, s), "sweep advance");
 }
 

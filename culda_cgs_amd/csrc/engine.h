@@ -133,6 +133,7 @@ public:
     // profiling / stats
     std::vector<hipEvent_t> ev_sampled;
     hipEvent_t ev_theta_done = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // batched iteration: theta on stream2
     hipEvent_t ev[4] = {};
     double sampler_ms = 0, phi_ms = 0, theta_ms = 0, iter_ms = 0;
     int32_t sampler_launches = 0;
