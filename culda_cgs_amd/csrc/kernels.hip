@@ -361,14 +361,16 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // is exhausted (the host interleaves the order so that every claim holds
 // slots of all lengths); otherwise workgroup b takes position b.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
+// Seven waves per SIMD (at most 72 VGPRs): the per-token chains of
+// dependent loads need the occupancy more than the registers.
 template <bool kPersistent>
-__global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sample_kernel(SampleArgs a) {
     // one wave per workgroup: every LDS array sits at a fixed address, so
     // the phi lookup is a shift of the packed entry plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
     __shared__ float rq[kWave];
     __shared__ __attribute__((aligned(16))) float rs_s[4][kWave + 4];
-    __shared__ uint32_t zres[kWave];
+    __shared__ uint16_t zres[kWave];
     __shared__ float sqres[kWave];
     const int lane = threadIdx.x;
     const uint32_t sweep = *a.sweep;
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs a) {
                     topic = sb ? et : (uint32_t)((bsel << 4) + msel);
                 }
                 if (lr == 0) {
-                    zres[p] = topic;
+                    zres[p] = (uint16_t)topic;
                     sqres[p] = SQ;
                 }
             }
