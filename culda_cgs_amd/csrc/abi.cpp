@@ -830,7 +830,7 @@ extern "C" int culda_batch_autotune(culda_batch *batch, culda_tune *out) {
         best.best_ms = 1e30;
         int32_t best_groups = 0;
         std::vector<std::pair<int32_t, double>> all;
-        const int32_t per_cu = std::max(1, sample_blocks_per_cu());
+        const int32_t per_cu = std::max(1, sample_blocks_per_cu(e.rs_stride));
         const int32_t resident = per_cu * e.cu_count;
         for (int32_t groups : {0, resident, std::max(1, resident / 2)}) {
             e.persistent_groups = groups;

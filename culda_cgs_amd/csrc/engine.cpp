@@ -259,6 +259,12 @@ void Engine::upload() {
     }
     const int32_t M = (int32_t)models.size(), U = (int32_t)units.size();
     {
+        int64_t longest = 0;
+        for (const ModelHost &m : models)
+            for (int64_t nd : m.doc_len) longest = std::max(longest, theta_entries_max(nd, K));
+        rs_stride = sample_rs_stride(longest);
+    }
+    {
         int64_t words = 0, rare = 0;
         for (const ModelHost &m : models)
             for (int32_t w = 0; w < m.V; ++w) {
@@ -586,13 +592,13 @@ void Engine::rebuild_theta_all(hipStream_t s) {
 void Engine::sample_unit(int32_t ui, hipStream_t s) {
     UnitHost &u = units[ui];
     SampleArgs a{d_slots + u.slot_begin, nullptr, d_units, d_models, d_ll_partial + u.slot_begin, d_sweep, nullptr,
-                 u.slot_end - u.slot_begin, (zp_valid && sparse_phi) ? 1 : 0};
+                 u.slot_end - u.slot_begin, (zp_valid && sparse_phi) ? 1 : 0, rs_stride, 0};
     hip_check(launch_sample(a, 0, s), "sampler launch");
 }
 
 void Engine::sample_all(hipStream_t s) {
     SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_sweep, nullptr, num_slots,
-                 (zp_valid && sparse_phi) ? 1 : 0};
+                 (zp_valid && sparse_phi) ? 1 : 0, rs_stride, 0};
     int32_t grid = 0;
     if (persistent_groups > 0) {
         hip_check(launch_fill_u32(d_queue, 1, 0, s), "queue reset");

@@ -127,6 +127,7 @@ public:
     uint32_t *d_sweep = nullptr;
     uint32_t *d_queue = nullptr;
     int32_t persistent_groups = 0;  // 0 = per-slot grid
+    int32_t rs_stride = 0;          // sampler S-prefix row stride (longest theta row of any doc)
     hipGraphExec_t graph = nullptr;
     std::vector<void *> allocations;
 

@@ -16,6 +16,8 @@
 //   the last element.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.h"
 #include "kernels.h"
 #include "philox.h"
@@ -369,7 +371,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     // the phi lookup is a shift of the packed entry plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
     __shared__ float rq[kWave];
-    __shared__ __attribute__((aligned(16))) float rs_s[4][kWave + 4];
+    // four rows of a.rs_stride floats: S block prefixes of each row's doc
+    extern __shared__ __attribute__((aligned(16))) float rs_s[];
     __shared__ uint16_t zres[kWave];
     __shared__ float sqres[kWave];
     const int lane = threadIdx.x;
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     const int rrow = lane >> 4, lr = lane & 15;
     // rs[b] = S prefix before block b (rs[0] unused); offset so that
     // rs[4g + 1 .. 4g + 4] is one aligned 16-byte store
-    float *rs = rs_s[rrow] + 3;
+    float *rs = rs_s + rrow * a.rs_stride + 3;
     double ll = 0.0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
@@ -782,18 +785,31 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Row stride of the sampler's S block prefixes for rows of at most
+// max_entries theta entries: rs = row + 3 is written at [1, 4 * ng] with
+// ng = ceil(nb / 4) groups of four 16-entry blocks.  68 floats while no
+// row has more than 1024 entries.
+int32_t sample_rs_stride(int64_t max_entries) {
+    const int64_t nb = (std::max<int64_t>(max_entries, 1) + 15) / 16;
+    return (int32_t)(4 * ((nb + 3) / 4) + 4);
+}
+
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
+    if (a.rs_stride < 8 || a.rs_stride % 4) return hipErrorInvalidValue;
+    const size_t lds = (size_t)4 * a.rs_stride * sizeof(float);
     if (a.queue)
-        hipLaunchKernelGGL(sample_kernel<true>, dim3(grid), dim3(kWave), 0, s, a);
+        hipLaunchKernelGGL(sample_kernel<true>, dim3(grid), dim3(kWave), lds, s, a);
     else
-        hipLaunchKernelGGL(sample_kernel<false>, dim3(a.num_slots), dim3(kWave), 0, s, a);
+        hipLaunchKernelGGL(sample_kernel<false>, dim3(a.num_slots), dim3(kWave), lds, s, a);
     return hipGetLastError();
 }
 
-int32_t sample_blocks_per_cu() {
+int32_t sample_blocks_per_cu(int32_t rs_stride) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave, 0) != hipSuccess) n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave,
+                                                     (size_t)4 * rs_stride * sizeof(float)) != hipSuccess)
+        n = 0;
     return n;
 }
 

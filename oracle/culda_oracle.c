@@ -139,6 +139,16 @@ void oc_rebuild_phi(const oc_corpus *c, int32_t *nwk, int32_t *nk) {
 
 #define OC_THETA_MAX_COUNT 63
 
+/* Most 16-entry blocks any doc's theta row has (from th_off). */
+static int64_t max_blocks(const oc_corpus *c, const int64_t *th_off) {
+    int64_t m = 0;
+    for (int32_t d = 0; d < c->D; ++d) {
+        const int64_t nb = (th_off[d + 1] - th_off[d] + 15) / 16;
+        if (nb > m) m = nb;
+    }
+    return m;
+}
+
 /* Entries of one topic with count c: ceil(c / 63). */
 static inline int64_t theta_entries(int32_t c) { return (c + OC_THETA_MAX_COUNT - 1) / OC_THETA_MAX_COUNT; }
 
@@ -269,12 +279,13 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
     }
     double ll = 0, lle = 0;
     int64_t nbound = 0, nmis = 0, ns = 0;
+    const int64_t nbmax = max_blocks(c, th_off);
 #pragma omp parallel reduction(+ : ll, lle, nbound, nmis, ns)
     {
         float *ph = (float *)malloc(sizeof(float) * K);
         double *phd = (double *)malloc(sizeof(double) * K);
         float rq[64];
-        float rs[65];
+        float *rs = (float *)malloc(sizeof(float) * (size_t)(nbmax + 1)); /* S prefix after each block */
 #pragma omp for schedule(dynamic, 16)
         for (int32_t w = 0; w < V; ++w) {
             int64_t ntok = 0;
@@ -420,6 +431,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
             }
         }
         free(ph);
+        free(rs);
         free(phd);
     }
     free(cbase);
@@ -454,10 +466,12 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
         }
     }
     double ll = 0;
+    const int64_t nbmax = max_blocks(c, th_off);
 #pragma omp parallel reduction(+ : ll)
     {
         float *ph = (float *)malloc(sizeof(float) * K);
-        float rq[64], rs[65];
+        float rq[64];
+        float *rs = (float *)malloc(sizeof(float) * (size_t)(nbmax + 1)); /* S prefix after each block */
 #pragma omp for schedule(dynamic, 16)
         for (int32_t w = 0; w < V; ++w) {
             int64_t ntok = 0;
@@ -556,6 +570,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
             }
         }
         free(ph);
+        free(rs);
     }
     free(cbase);
     free(pre);

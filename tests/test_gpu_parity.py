@@ -192,3 +192,34 @@ def test_rare_tiles_equal_a_full_rebuild(V, D, K):
             s.set_z(0, np.random.default_rng(2).integers(0, K, s.get_z(0).size).astype(np.uint16))
             s.rebuild_all()
     s.close()
+
+
+def test_long_doc_with_more_than_1024_theta_entries():
+    """A 100,000-token doc at K = 1024: every topic holds ~98 of its tokens,
+    so each takes two entries and the row has ~2048 entries (128 blocks of
+    16, more than the 64 of any row with one entry per topic).  z, n_wk,
+    n_k and theta equal the oracle's over three sweeps; the LL within
+    1e-5."""
+    V, K = 300, 1024
+    rng = np.random.default_rng(12)
+    docs = random_docs(rng, V, 8, 40) + [rng.integers(0, V, 100_000)]
+    arrays = chunks_from_docs(V, docs, 1)
+    alpha, beta, seed = 50.0 / K, 0.01, 3
+    s, om = _pair(V, K, arrays, alpha, beta, seed)
+    s.init_z()
+    s.rebuild_all()
+    om.init_z(seed)
+    om.rebuild()
+    assert int(np.diff(om.th_off).max()) > 1024 + 512
+    for it in range(1, 4):
+        ll, _ = s.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+        bad = np.flatnonzero(s.get_z(0) != om.z[0])
+        assert bad.size == 0, f"sweep {it}: {bad.size} tokens differ (first {bad[:5]})"
+        ref = out.ll_sum / om.num_tokens
+        assert abs(ll - ref) <= LL_RTOL * abs(ref)
+    nwk, nk = s.get_phi()
+    assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
+    _theta_equal(s, om, arrays)
+    s.close()
