@@ -241,8 +241,10 @@ int culda_get_phi(culda_ctx *ctx, int32_t *n_wk, int32_t *n_k);
 int culda_get_phi_head(culda_ctx *ctx, float *head);
 
 /* Theta rows of one chunk in compact CSR over the chunk's docs:
- * offsets[chunk_docs+1], counts[nnz_total], topics[nnz_total].  Call with
- * counts == topics == NULL to obtain offsets (and so the total) first.
+ * offsets[chunk_docs+1], counts[nnz_total], topics[nnz_total], one
+ * (topic, n_dk) pair per nonzero topic, ascending topic (the device's split
+ * entries of a topic are merged).  Call with counts == topics == NULL to
+ * obtain offsets (and so the total) first.
  * Replaces ModelThetaChunk::toCPU (model/model_theta_chunk.cpp:288-323). */
 int culda_get_theta(culda_ctx *ctx, int32_t chunk, int64_t *offsets, int32_t *counts,
                     int16_t *topics);
@@ -266,8 +268,10 @@ int culda_load_checkpoint(culda_ctx *ctx, const char *path, int32_t *next_iterat
 typedef struct culda_stats {
     int64_t num_tokens;
     int64_t num_slots;          /* sampler wave slots over all chunks          */
-    int64_t sum_doc_nnz;        /* sum_d nnz(d) after the last theta rebuild    */
-    int64_t sum_token_nnz;      /* sum_d n_d * nnz(d): theta entries gathered   */
+    int64_t sum_doc_nnz;        /* sum_d entries(d) after the last theta rebuild
+                                   (16-bit entries; a topic with n_dk > 63 takes
+                                   ceil(n_dk / 63) of them)                      */
+    int64_t sum_token_nnz;      /* sum_d n_d * entries(d): theta entries gathered */
     int64_t theta_capacity;     /* theta slab entries over all chunks           */
     double  sampler_ms;         /* last iteration: the chunks' sweeps (profile=1) */
     double  phi_ms;             /* LL reduce + phi rebuild + head denominators    */
