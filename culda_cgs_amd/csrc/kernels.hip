@@ -496,72 +496,50 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
             const uint32_t nnz = __shfl(pnnz, p, kWave);
             const float uu = __shfl(pu, p, kWave);
             if (p < ntok) {
-                // S = sum over the doc's nonzeros, 16 per block; every block
-                // lies inside the doc's 16-aligned slab, entries past nnz
-                // are zero (the slab tail), so their products are exact
+                // S = sum over the doc's theta entries, 16 per block; every
+                // block lies inside the doc's 16-aligned slab, entries past
+                // nnz are zero (the slab tail), so their products are exact
                 // zeros.  A block total is the balanced tree of its 16
                 // products (lane 15 of the in-block Kogge-Stone scan): lane
-                // lr holds entries 2(lr & 7) and 2(lr & 7) + 1 (one dword) of block
-                // lr >> 3 of a pair of blocks and adds them (the tree's
-                // first level), three DPP steps over 8 lanes finish it at
-                // lanes 7 and 15.  Totals are carried sequentially,
-                // R_{b+1} = fl(R_b + T_b).
-                // Groups of four blocks (64 entries) are loaded one group
-                // ahead.  The last group is loaded whole: loads past the
-                // last block stay inside the theta array (kThetaPad entries
-                // of tail padding), and its second pair of blocks is summed
-                // only when it holds a block of the doc.  rs entries past nb
-                // may hold sums of such blocks; nothing reads them.
+                // lr holds entries 4 lr .. 4 lr + 3 of a group of four
+                // blocks (one 8-byte load; block lr >> 2) and forms the
+                // tree's first two levels in registers; DPP row_shr 1 and 2
+                // finish the four totals at lanes 3, 7, 11 and 15, and
+                // row_newbcast carries R_{b+1} = fl(R_b + T_b) in block
+                // order.  Groups (64 entries) are loaded one group ahead;
+                // the last group is loaded whole: loads past the last block
+                // stay inside the theta array (kThetaPad entries of tail
+                // padding), and rs entries past nb may hold sums of such
+                // blocks, which nothing reads.
                 const uint32_t nb = (nnz + 15u) >> 4;
                 const uint32_t ng = (nb + 3u) >> 2;
-                const uint32_t *tp =
-                    reinterpret_cast<const uint32_t *>(theta + start) + ((lr >> 3) << 3) + (lr & 7);
-                uint32_t cur[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) cur[h] = load_global(tp + (h << 4));
+                const uint32_t *tp = reinterpret_cast<const uint32_t *>(theta + start) + 2 * lr;
+                auto group = [&](u32x2 d, float R0, float &r0, float &r1, float &r2, float &r3) {
+                    const float f0 = PHI_LO(d.x), f1 = PHI_HI(d.x), f2 = PHI_LO(d.y), f3 = PHI_HI(d.y);
+                    const float c0 = COUNT_LO(d.x), c1 = COUNT_HI(d.x), c2 = COUNT_LO(d.y), c3 = COUNT_HI(d.y);
+                    float y = (c0 * f0 + c1 * f1) + (c2 * f2 + c3 * f3);
+                    y = y + dpp_f<0x111>(y);
+                    y = y + dpp_f<0x112>(y);
+                    r0 = R0 + dpp_f<0x153>(y);
+                    r1 = r0 + dpp_f<0x157>(y);
+                    r2 = r1 + dpp_f<0x15B>(y);
+                    r3 = r2 + row_last(y);
+                };
+                u32x2 cur = load_global2(tp);
                 float R = 0.0f;
                 uint32_t g = 0;
                 for (; g + 1 < ng; ++g) {
-                    const uint32_t *gp = tp + ((g + 1) << 5);
-                    uint32_t nxt[2];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) nxt[h] = load_global(gp + (h << 4));
-                    float y[2];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-                        y[h] = COUNT_LO(cur[h]) * PHI_LO(cur[h]) + COUNT_HI(cur[h]) * PHI_HI(cur[h]);
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x111>(y[h]);
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x112>(y[h]);
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) y[h] = y[h] + dpp_f<0x114>(y[h]);
-                    const float r0 = R + dpp_f<0x157>(y[0]);
-                    const float r1 = r0 + row_last(y[0]);
-                    const float r2 = r1 + dpp_f<0x157>(y[1]);
-                    const float r3 = r2 + row_last(y[1]);
+                    const u32x2 nxt = load_global2(tp + ((g + 1) << 5));
+                    float r0, r1, r2, r3;
+                    group(cur, R, r0, r1, r2, r3);
                     if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
                     R = r3;
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) cur[h] = nxt[h];
+                    cur = nxt;
                 }
                 // last group: nl = 1..4 blocks
                 const uint32_t nl = nb - (g << 2);
-                float y0 = COUNT_LO(cur[0]) * PHI_LO(cur[0]) + COUNT_HI(cur[0]) * PHI_HI(cur[0]);
-                y0 = y0 + dpp_f<0x111>(y0);
-                y0 = y0 + dpp_f<0x112>(y0);
-                y0 = y0 + dpp_f<0x114>(y0);
-                const float r0 = R + dpp_f<0x157>(y0);
-                const float r1 = r0 + row_last(y0);
-                float r2 = r1, r3 = r1;
-                if (nl > 2) {
-                    float y1 = COUNT_LO(cur[1]) * PHI_LO(cur[1]) + COUNT_HI(cur[1]) * PHI_HI(cur[1]);
-                    y1 = y1 + dpp_f<0x111>(y1);
-                    y1 = y1 + dpp_f<0x112>(y1);
-                    y1 = y1 + dpp_f<0x114>(y1);
-                    r2 = r1 + dpp_f<0x157>(y1);
-                    r3 = r2 + row_last(y1);
-                }
+                float r0, r1, r2, r3;
+                group(cur, R, r0, r1, r2, r3);
                 if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
                 const float S = nl == 1 ? r0 : nl == 2 ? r1 : nl == 3 ? r2 : r3;
                 wave_sync();
