@@ -348,6 +348,24 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     }
 }
 
+// Bitonic sort of the keys of lanes [0, M) of each M-lane group into
+// descending order (the last stage's direction is the same for every lane
+// below M).
+template <int M>
+__device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
+#pragma unroll
+    for (int kk = 2; kk <= M; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            const uint32_t other = __shfl_xor(key, j, kWave);
+            const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
+            const uint32_t mx = key > other ? key : other, mn = key > other ? other : key;
+            key = (lower == up) ? mx : mn;
+        }
+    }
+    return key;
+}
+
 // ------------------------------------------------------------ sampler
 // One wave per slot (one word's token range of one unit).  The wave stages
 // the word's head row phi_k in LDS (computed from n_wk), builds the
@@ -473,18 +491,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
             u = bits_to_uniform(token_bits(md.seed, (uint64_t)(i + s.gdelta), sweep, kPurposeSweep));
         }
         // bitonic sort of (valid, nnz, lane), descending: positions
-        // 0..ntok-1 hold the valid tokens, longest theta rows first
+        // 0..ntok-1 hold the valid tokens, longest theta rows first.  The
+        // valid tokens are lanes 0..ntok-1, so sorting the first 16 or 32
+        // lanes suffices for short batches, and a batch of at most 4 tokens
+        // is one step whatever its order.
         uint32_t key = (valid ? (0x80000000u | ((mnnz < 0xffffu ? mnnz : 0xffffu) << 6)) : 0u) | (uint32_t)lane;
-#pragma unroll
-        for (int kk = 2; kk <= kWave; kk <<= 1) {
-#pragma unroll
-            for (int j = kk >> 1; j > 0; j >>= 1) {
-                const uint32_t other = __shfl_xor(key, j, kWave);
-                const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
-                const uint32_t mx = key > other ? key : other, mn = key > other ? other : key;
-                key = (lower == up) ? mx : mn;
-            }
-        }
+        if (ntok > 32)
+            key = bitonic_desc<64>(key, lane);
+        else if (ntok > 16)
+            key = bitonic_desc<32>(key, lane);
+        else if (ntok > 4)
+            key = bitonic_desc<16>(key, lane);
         const int tok = (int)(key & 63u);
         const uint32_t pstart = __shfl(mstart, tok, kWave);
         const uint32_t pnnz = __shfl(mnnz, tok, kWave);
