@@ -62,6 +62,7 @@ HostPhi download_phi(Engine &e, int32_t m) {
     HostPhi h;
     h.nwk.resize((size_t)mh.V * mh.K);
     h.nk.resize(mh.K);
+    e.fresh_rows(e.stream);
     e.sync();
     if (!h.nwk.empty()) hip_check(hipMemcpy(h.nwk.data(), mh.nwk, h.nwk.size() * 4, hipMemcpyDeviceToHost), "phi copy");
     hip_check(hipMemcpy(h.nk.data(), mh.nk, h.nk.size() * 4, hipMemcpyDeviceToHost), "n_k copy");
@@ -268,6 +269,7 @@ extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
     CTX_CALL({
         if (!out) fail(CULDA_ERR_ARG, "null output");
         if (!e.theta_ready) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
+        e.fresh_rows(e.stream);
         e.sync();
         const ModelHost &m = e.models[0];
         const int32_t K = m.K, V = m.V;
@@ -313,6 +315,7 @@ extern "C" int culda_get_phi_head(culda_ctx *ctx, float *head) {
         const ModelHost &m = e.models[0];
         const size_t n = (size_t)m.V * m.K;
         float *d = (float *)e.dalloc(n * 4);
+        e.fresh_rows(e.stream);
         hip_check(launch_phi_head(m.nwk, m.den, d, m.V, m.K, m.beta, e.stream), "head");
         hip_check(hipStreamSynchronize(e.stream), "sync");
         hip_check(hipMemcpy(head, d, n * 4, hipMemcpyDeviceToHost), "head copy");
@@ -757,6 +760,7 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
         int32_t *dw = (int32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
         uint32_t *df = (uint32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
         uint32_t *da = (uint32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
+        e.fresh_rows(e.stream);
         e.mh_sketch_current(e.stream);
         e.mh_query_logits(s.tok_doc + first, s.tok_word + first, n, s.Li, e.stream);
         const ModelHost &m = e.models[0];

@@ -290,11 +290,18 @@ void Engine::upload() {
                           m.V};
     }
     d_models = upload_vec(*this, md);
+    // zm for the units whose rare words are updated in tiles (one-unit models)
+    if (sparse_phi)
+        for (ModelHost &m : models)
+            if (m.unit_end - m.unit_begin == 1) {
+                UnitHost &u = units[m.unit_begin];
+                u.zm = (uint16_t *)dalloc(std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t));
+            }
     std::vector<UnitDev> ud(U);
     for (int32_t ui = 0; ui < U; ++ui) {
         const UnitHost &u = units[ui];
         ud[ui] = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
-                         u.doc_end - u.doc_start};
+                         u.doc_end - u.doc_start, u.zm};
     }
     d_units = upload_vec(*this, ud);
     // phi work lists: fused per model (and all models concatenated), and per unit
@@ -528,17 +535,30 @@ void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStr
     PhiArgs a{L.slots, L.packets, d_units, d_models, L.num_packets, accumulate ? 1 : 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
     phi_ready = false;
-    if (unit < 0) {
-        if (sparse_phi) copy_zp(mi, s);
+    // rows written from z: zp and zm follow (a one-unit model's unit list
+    // is the whole model), so a later pass over the rare tiles keeps them
+    const bool whole = unit < 0 || m.unit_end - m.unit_begin == 1;
+    if (whole && sparse_phi) copy_zp(mi, s);
+    if (whole && models.size() == 1) rare_stale = false;
+    if (unit < 0)
         zp_valid = sparse_phi && models.size() == 1;
-    } else {
+    else
         zp_valid = false;  // one chunk's counts: complete only after the caller's loop
-    }
 }
 
 void Engine::copy_zp(int32_t model, hipStream_t s) {
     for (const UnitHost &u : units)
-        if (model < 0 || u.model == model) hip_check(launch_copy_u16(u.z, u.zp, u.num_tokens, s), "zp copy");
+        if (model < 0 || u.model == model) {
+            hip_check(launch_copy_u16(u.z, u.zp, u.num_tokens, s), "zp copy");
+            if (u.zm) hip_check(launch_copy_u16(u.z, u.zm, u.num_tokens, s), "zm copy");
+        }
+}
+
+void Engine::fresh_rows(hipStream_t s) {
+    if (!rare_stale) return;
+    RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 1};
+    hip_check(launch_phi_rare(r, s), "rare rows launch");
+    rare_stale = false;
 }
 
 void Engine::update_phi(hipStream_t s) {
@@ -549,6 +569,7 @@ void Engine::update_phi(hipStream_t s) {
         hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
         RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
         hip_check(launch_phi_rare(r, s), "rare phi launch");
+        rare_stale = num_rare_packets > 0;
         phi_ready = false;
     } else if (prm.fused_phi || units.size() == models.size()) {
         rebuild_phi_all(s);
@@ -558,6 +579,7 @@ void Engine::update_phi(hipStream_t s) {
                 rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, s);
         if (sparse_phi) copy_zp(-1, s);
         zp_valid = sparse_phi;
+        rare_stale = false;
     }
 }
 
@@ -569,6 +591,7 @@ void Engine::rebuild_phi_all(hipStream_t s) {
     phi_ready = false;
     if (sparse_phi) copy_zp(-1, s);
     zp_valid = sparse_phi;
+    rare_stale = false;
 }
 
 void Engine::phi_head(hipStream_t s) {
@@ -636,6 +659,7 @@ void Engine::rebuild_all() {
                 rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, stream);
         if (sparse_phi) copy_zp(-1, stream);
         zp_valid = sparse_phi;
+        rare_stale = false;
     }
     phi_head(stream);
     rebuild_theta_all(stream);
@@ -758,6 +782,9 @@ void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double
         hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         hip_check(e, "graph instantiate");
+        // the captured count update leaves the rare-tile rows stale after
+        // every replay, which no host code sees: remember it here
+        graph_rare_stale = rare_stale;
     }
     set_sweep(first, stream);
     auto t0 = std::chrono::steady_clock::now();
@@ -773,6 +800,7 @@ void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double
         else
             enqueue_batch_iteration(stream);
     }
+    if (use_graph && graph_rare_stale) rare_stale = true;
     if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
     hip_check(hipStreamSynchronize(stream), "iteration sync");
     auto t1 = std::chrono::steady_clock::now();
@@ -806,7 +834,7 @@ double Engine::sweep_bytes() const {
     for (int64_t v : unit_token_nnz) nnz += v;
     const bool zp_stage = zp_valid && sparse_phi;
     const double rows = (double)(num_slots - (zp_stage ? rare_slots : 0)) * 4.0 * (double)prm.k;
-    const double rare = zp_stage ? 10.0 * (double)rare_tokens : 0.0;
+    const double rare = zp_stage ? 6.0 * (double)rare_tokens : 0.0;  // zp topic + den_k
     return (double)n * (4.0 + 16.0 + 2.0) + 2.0 * (double)nnz + rows + rare;
 }
 

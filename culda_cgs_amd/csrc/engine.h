@@ -50,6 +50,7 @@ struct UnitHost {
     uint32_t *tokens = nullptr;
     uint16_t *z = nullptr;
     uint16_t *zp = nullptr;  // z of the last count build (see Engine::zp_valid)
+    uint16_t *zm = nullptr;  // z the rare-word rows hold the counts of (Engine::rare_stale)
     int64_t *word_idx = nullptr;
     uint32_t *rev_off = nullptr, *rev_idx = nullptr;
     DocMeta *meta = nullptr;
@@ -94,6 +95,10 @@ public:
     // rebuilds (which copy z to zp), kept by count deltas, cleared by
     // anything else that writes the counts
     bool zp_valid = false;
+    // the n_wk rows of the rare-tile words hold the counts of zm, not of zp;
+    // fresh_rows() brings them up to date before anything reads n_wk
+    bool rare_stale = false;
+    bool graph_rare_stale = false;  // a replay of the captured batched iteration leaves them stale
     // rebuilds touch only the entries of rare words (at least a quarter of
     // the words below K / 16 tokens); otherwise every row is rewritten
     bool sparse_phi = false;
@@ -153,7 +158,8 @@ public:
     void init_z();
     void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s);
     void rebuild_phi_all(hipStream_t s);
-    void copy_zp(int32_t model, hipStream_t s);  // zp = z for the model's units (all when model < 0)
+    void copy_zp(int32_t model, hipStream_t s);  // zp (and zm) = z for the model's units (all when model < 0)
+    void fresh_rows(hipStream_t s);  // n_wk rows current (rare-tile words moved from zm to zp)
     // counts of every model after a sweep: a rebuild that rewrites only the
     // touched entries of rare words when zp_valid, else a full rebuild
     // (fused, or chunks in turn for a single corpus)

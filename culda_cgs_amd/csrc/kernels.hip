@@ -198,30 +198,43 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
 }
 
 // ------------------------------------------------------------ rare tiles
-// Count update of the rare words of single-unit models when the rows hold
-// the counts of zp: one wave per tile of up to 64 tokens (one per lane) of
-// several consecutive words, so a word costs a share of two memory round
-// trips instead of a pass of its own.  A token's word is its start_mask
-// rank; its new bin is the number of lanes with the same (word, z), and an
-// old topic p whose (word, p) no lane holds any more is cleared -- the
-// cleared and the stored entries are disjoint, so no ordering between the
-// stores.  zp becomes z; n_k gets one per token (LDS, flushed per packet).
+// Rare words of one-unit models: their n_wk rows are brought up to date
+// only when something reads them.  After a sweep (kMaterialize false) one
+// wave per tile of up to 64 tokens adds each token's topic to n_k (LDS,
+// flushed per packet) and sets zp = z -- the sampler stages such a word's
+// phi from the zp topics of its tokens, all in its one slot, without the
+// row.  Before a read of n_wk (kMaterialize true) the rows move from the
+// counts of zm to the counts of zp: a token's word is its start_mask rank,
+// its new bin the number of lanes with the same (word, zp), and a zm topic
+// that no lane keeps is cleared -- the cleared and the stored entries are
+// disjoint, so no ordering between the stores; zm = zp.
+template <bool kMaterialize>
 __global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
     __shared__ uint32_t nk_s[kMaxK];
     const Packet pk = a.packets[blockIdx.x];
     const ModelDev &m = a.models[pk.owner];
     const int K = m.k;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
-    __syncthreads();
+    if (!kMaterialize) {
+        for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
+        __syncthreads();
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int ti = pk.first + wave; ti < pk.last; ti += kWavesPerGroup) {
         const RareTile t = a.tiles[ti];
         const UnitDev &u = a.units[t.unit];
         const bool live = lane < t.count;
+        if (!kMaterialize) {
+            if (live) {
+                const uint16_t zt = u.z[t.begin + lane];
+                u.zp[t.begin + lane] = zt;
+                atomicAdd(&nk_s[zt], 1u);
+            }
+            continue;
+        }
         uint32_t zt = 0, zo = 0;
         if (live) {
-            zt = u.z[t.begin + lane];
-            zo = u.zp[t.begin + lane];
+            zt = u.zp[t.begin + lane];
+            zo = u.zm[t.begin + lane];
         }
         // word of present-word rank q: lane r (word w0 + r present) pushes
         // w0 + r to lane rank(r); absent words push to lane 63, which no
@@ -243,14 +256,15 @@ __global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
             int32_t *row = m.nwk + (int64_t)word * K;
             if (!kept) row[zo] = 0;
             row[zt] = (int32_t)c;
-            u.zp[t.begin + lane] = (uint16_t)zt;
-            atomicAdd(&nk_s[zt], 1u);
+            u.zm[t.begin + lane] = (uint16_t)zt;
         }
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
-        const uint32_t v = nk_s[k];
-        if (v) atomicAdd(m.nk + k, (int)v);
+    if (!kMaterialize) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < K; k += blockDim.x) {
+            const uint32_t v = nk_s[k];
+            if (v) atomicAdd(m.nk + k, (int)v);
+        }
     }
 }
 
@@ -427,16 +441,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     // stage phi_k = fl(fl(n_wk) + beta) / den_k
     const int32_t *row = md.nwk + (int64_t)s.word * K;
     if (a.use_zp && s.rare) {
-        // A rare word's counts are nonzero only at its tokens' zp topics
-        // (every token of the word is in this slot): the zero-count row
-        // (fl(beta / den_k), the same value) plus those entries.
+        // A rare word's counts are those of its tokens' zp topics, and
+        // every token of the word is in this slot (fewer than K / 16 <= 64
+        // of them, one per lane): the zero-count row (fl(beta / den_k), the
+        // same value) plus, at each zp topic, its count among the lanes --
+        // the row itself need not be current.
         for (int k = lane * 4; k < K; k += 4 * kWave)
             *reinterpret_cast<float4 *>(ph + k) = *reinterpret_cast<const float4 *>(md.zero_phi + k);
+        const int n = (int)(s.end - s.begin);
+        const uint32_t t = lane < n ? un.zp[s.begin + lane] : 0xffffffffu;
+        uint32_t c = 0;
+        for (int j = 0; j < n; ++j) c += (uint32_t)__builtin_amdgcn_readlane((int)t, j) == t;
         wave_sync();
-        for (int64_t i = s.begin + lane; i < s.end; i += kWave) {
-            const uint32_t t = un.zp[i];
-            ph[t] = ((float)row[t] + beta) / md.den[t];
-        }
+        if (lane < n) ph[t] = ((float)c + beta) / md.den[t];
     } else {
         for (int k = lane * 4; k < K; k += 4 * kWave) {
             const int4 c = *reinterpret_cast<const int4 *>(row + k);
@@ -756,7 +773,10 @@ hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s) {
 
 hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(phi_rare_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    if (a.mode)
+        hipLaunchKernelGGL(phi_rare_kernel<true>, dim3(a.num_packets), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(phi_rare_kernel<false>, dim3(a.num_packets), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -75,6 +75,7 @@ struct UnitDev {
     unsigned long long *stats;  // [0] += sum entries, [1] += sum n_d * entries (last rebuild)
     int32_t model;
     int32_t num_docs;
+    uint16_t *zm;               // z the rare-word rows of n_wk hold the counts of (one-unit models)
 };
 
 // Sampler / z-init work item: tokens [begin, end) of one word of one unit.
@@ -157,14 +158,17 @@ struct RareTile {
     int32_t pad;
 };
 
-// Packets of rare tiles (owner = model) for the zp-based count update.
+// Packets of rare tiles (owner = model).  mode 0 (after a sweep): n_k
+// gets the tiles' tokens, zp = z; the rows are left alone.  mode 1 (before
+// anything reads n_wk): the rows move from the counts of zm to the counts
+// of zp, zm = zp.
 struct RareArgs {
     const RareTile *tiles;
     const Packet *packets;
     const UnitDev *units;
     const ModelDev *models;
     int32_t num_packets;
-    int32_t pad;
+    int32_t mode;
 };
 
 struct ThetaArgs {

@@ -53,11 +53,13 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     ModelHost &m = models[0];
     if (tm.V != m.V || tm.K != m.K) fail(CULDA_ERR_ARG, "held-out and training contexts differ in V or K");
     if (!train.phi_ready) fail(CULDA_ERR_STATE, "training counts not built");
+    train.fresh_rows(train.stream);
     train.sync();
     hip_check(hipStreamSynchronize(stream2), "sync");
     hip_check(hipMemcpyAsync(m.nwk, tm.nwk, (size_t)m.V * m.K * 4, hipMemcpyDeviceToDevice, stream), "phi copy");
     hip_check(hipMemcpyAsync(m.nk, tm.nk, (size_t)m.K * 4, hipMemcpyDeviceToDevice, stream), "phi copy");
     zp_valid = false;  // these counts are the training context's
+    rare_stale = false;
     phi_head(stream);
     init_z();
     rebuild_theta_all(stream);
@@ -410,6 +412,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
     hip_check(hipStreamSynchronize(stream2), "sync");
     const ModelHost &m = models[0];
     hip_check(launch_fill_u32((uint32_t *)s.accepted, 2, 0, stream), "clear");
+    fresh_rows(stream);  // the MH step reads n_wk
     mh_sketch_current(stream);
     const int64_t IB = s.prm.infer_batch;
     for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
