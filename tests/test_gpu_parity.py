@@ -223,3 +223,31 @@ def test_long_doc_with_more_than_1024_theta_entries():
     assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
     _theta_equal(s, om, arrays)
     s.close()
+
+
+def test_row_readers_after_lazy_rare_updates():
+    """One chunk, K = 1024, mostly rare words: after each sweep only n_k and
+    zp follow the new z, the rare words' rows are brought up to date when
+    read.  Every reader sees the counts of the current z: the joint
+    log-likelihood (1e-9), the FP32 head and the counts (bit-exact), also
+    after a second round of sweeps."""
+    V, K = 3000, 1024
+    rng = np.random.default_rng(21)
+    docs = random_docs(rng, V, 100, 60)
+    arrays = chunks_from_docs(V, docs, 1)
+    alpha, beta = 0.05, 0.01
+    s, om = _pair(V, K, arrays, alpha, beta, 6)
+    s.init_z()
+    s.rebuild_all()
+    for rnd in range(2):
+        for it in range(1 + 3 * rnd, 4 + 3 * rnd):
+            s.iterate(it)
+        om.set_z([s.get_z(0)])
+        om.rebuild()
+        ref = om.loglik_joint(alpha, beta)
+        got = s.loglik_joint()
+        assert abs(got - ref) <= 1e-9 * abs(ref), (rnd, got, ref)
+        assert np.array_equal(s.get_phi_head(), om.phi_head(beta)), f"round {rnd}: FP32 head differs"
+        nwk, nk = s.get_phi()
+        assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
+    s.close()
