@@ -258,3 +258,20 @@ def test_mh_errors():
     with pytest.raises(cg.CuldaError):
         t.mh_enable()  # K not a multiple of 64
     t.close()
+
+
+def test_mh_refuses_rows_longer_than_it_holds():
+    # the MH step holds a doc's theta row in registers (2048 16-bit
+    # entries); a 200,000-token doc at K = 1024 may need 1024 + 3175
+    from helpers import chunks_from_docs, product_corpus
+    rng = np.random.default_rng(3)
+    V = 50
+    docs = [rng.integers(0, V, 200_000), rng.integers(0, V, 30)]
+    c = product_corpus(V, *chunks_from_docs(V, docs, 1))
+    s = cg.Sampler(k=1024, alpha=ALPHA, beta=BETA, seed=SEED)
+    s.upload(c)
+    s.init_z()
+    s.rebuild_all()
+    with pytest.raises(cg.CuldaError):
+        s.mh_enable()
+    s.close()
