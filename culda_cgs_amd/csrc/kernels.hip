@@ -631,7 +631,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
         wave_sync();
         if (lane < ntok) {
             store_global_u16(un.z + base + tok, (uint16_t)zres[lane]);
-            ll += log((double)sqres[lane] / ((double)pnd + md.k_alpha));
+            // FP32 term log(fl(SQ / fl(fl(n_d) + fl(K alpha)))), summed in FP64
+            ll += (double)logf(sqres[lane] / ((float)pnd + (float)md.k_alpha));
         }
         wave_sync();
     }
