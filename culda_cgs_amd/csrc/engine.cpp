@@ -193,7 +193,6 @@ void Engine::upload() {
     if (uploaded) fail(CULDA_ERR_STATE, "already uploaded");
     if (pending.empty()) fail(CULDA_ERR_STATE, "no corpus to upload");
     const int32_t K = prm.k;
-    int64_t total_docs = 0;
     for (size_t mi = 0; mi < pending.size(); ++mi) {
         const Corpus &c = *pending[mi];
         ModelHost m;
@@ -208,7 +207,6 @@ void Engine::upload() {
             fail(CULDA_ERR_LIMIT, "corpus %zu has %lld tokens; counts are int32", mi, (long long)m.num_tokens);
         m.doc_len.resize(m.num_docs);
         for (int32_t d = 0; d < m.num_docs; ++d) m.doc_len[d] = c.doc_len(d);
-        total_docs += m.num_docs;
         m.unit_begin = (int32_t)units.size();
         for (size_t k = 0; k < c.chunks.size(); ++k) {
             const Chunk &ch = c.chunks[k];
@@ -388,17 +386,26 @@ void Engine::upload() {
         num_rare_packets = (int32_t)tile_packets.size();
         num_rare_tiles = (int64_t)tiles.size();
     }
-    // theta packets per unit (and all units concatenated)
+    // theta packets.  Per unit (a chunk's rebuild, which runs beside the
+    // next chunk's sweep): about two workgroups per CU, so that it keeps
+    // pace with the sweeps without taking the CUs from them.  All units at
+    // once (batches, full rebuilds): eight workgroups per CU over the total.
     {
-        const int64_t per_docs = std::max<int64_t>(16, (total_docs + (int64_t)cu_count * 8 - 1) / ((int64_t)cu_count * 8));
+        int64_t total_docs = 0;
+        for (const UnitHost &u : units) total_docs += u.doc_end - u.doc_start;
+        const int64_t per_all = std::max<int64_t>(16, (total_docs + (int64_t)cu_count * 8 - 1) / ((int64_t)cu_count * 8));
         std::vector<Packet> all;
         for (int32_t ui = 0; ui < U; ++ui) {
             UnitHost &u = units[ui];
+            const int64_t Du = u.doc_end - u.doc_start;
+            const int64_t per_u = std::max<int64_t>(16, (Du + (int64_t)cu_count * 2 - 1) / ((int64_t)cu_count * 2));
             std::vector<Packet> pk;
-            append_packets(pk, 0, u.doc_end - u.doc_start, ui, (int32_t)per_docs);
+            append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
             u.theta_packets = upload_vec(*this, pk);
             u.num_theta_packets = (int32_t)pk.size();
-            all.insert(all.end(), pk.begin(), pk.end());
+            std::vector<Packet> pa;
+            append_packets(pa, 0, (int32_t)Du, ui, (int32_t)per_all);
+            all.insert(all.end(), pa.begin(), pa.end());
         }
         d_theta_packets_all = upload_vec(*this, all);
         num_theta_packets_all = (int32_t)all.size();
