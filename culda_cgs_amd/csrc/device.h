@@ -67,6 +67,14 @@ __device__ __forceinline__ T uniform_copy(const T *p) {
     return v;
 }
 
+// Lane j's 64-bit value (j wave-uniform), through two scalar reads.
+__device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), j);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Row-local ballot: bit m set when lane m of this row voted true.
 __device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
     return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;

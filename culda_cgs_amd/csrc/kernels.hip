@@ -109,11 +109,26 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     for (int si = pk.first + wave; si < pk.last; si += kWavesPerGroup) {
         const PhiSlot sl = a.slots[si];
         int32_t *row = m.nwk + (int64_t)sl.word * K;
+        // fn(unit, b, e) over the word's token range [b, e) in every unit of
+        // the slot, in unit order: the ranges of up to 64 units are loaded
+        // at once, one unit per lane, and read back from registers, so a
+        // word costs one round trip for its ranges instead of one per unit
+        auto each_unit = [&](auto &&fn) {
+            for (int g0 = sl.unit_lo; g0 < sl.unit_hi; g0 += kWave) {
+                const int cnt = sl.unit_hi - g0 < kWave ? sl.unit_hi - g0 : kWave;
+                int64_t wb = 0, we = 0;
+                if (lane < cnt) {
+                    const int64_t *wi = a.units[g0 + lane].word_idx;
+                    wb = wi[sl.word];
+                    we = wi[sl.word + 1];
+                }
+                for (int j = 0; j < cnt; ++j) fn(g0 + j, readlane64(wb, j), readlane64(we, j));
+            }
+        };
         bool sparse = false;
         if (!sl.split) {
             int64_t n = 0;
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u)
-                n += a.units[u].word_idx[sl.word + 1] - a.units[u].word_idx[sl.word];
+            each_unit([&](int, int64_t b, int64_t e) { n += e - b; });
             if (n == 0 && a.accumulate != 0) continue;  // nothing to add, row already right
             sparse = keep_zp && n * 16 < K;
         }
@@ -122,45 +137,40 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
             // holds the counts of zp: entries of old topics that have no
             // new token are cleared, entries of new topics get their new
             // counts -- disjoint stores, so no ordering between them.
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+            each_unit([&](int u, int64_t b, int64_t e) {
                 const uint16_t *z = a.units[u].z, *zp = a.units[u].zp;
-                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
                 for (int64_t i = b + lane; i < e; i += kWave) {
                     h[z[i]] = 0;
                     h[zp[i]] = 0;
                 }
-            }
+            });
             wave_sync();
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+            each_unit([&](int u, int64_t b, int64_t e) {
                 const uint16_t *z = a.units[u].z;
-                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
                 for (int64_t i = b + lane; i < e; i += kWave) {
                     const uint16_t t = z[i];
                     atomicAdd(&h[t], 1u);
                     atomicAdd(&nk_s[t], 1u);
                 }
-            }
+            });
             wave_sync();
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u) {
+            each_unit([&](int u, int64_t b, int64_t e) {
                 const uint16_t *z = a.units[u].z;
                 uint16_t *zp = a.units[u].zp;
-                const int64_t b = a.units[u].word_idx[sl.word], e = a.units[u].word_idx[sl.word + 1];
                 for (int64_t i = b + lane; i < e; i += kWave) {
                     const uint16_t t = z[i], p = zp[i];
                     if (h[p] == 0) row[p] = 0;
                     row[t] = (int)h[t];
                     zp[i] = t;
                 }
-            }
+            });
             wave_sync();
             continue;
         }
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         if (!sl.split) {
-            for (int u = sl.unit_lo; u < sl.unit_hi; ++u)
-                count_range(a.units[u].z, a.units[u].zp, a.units[u].word_idx[sl.word], a.units[u].word_idx[sl.word + 1],
-                            lane, h, keep_zp);
+            each_unit([&](int u, int64_t b, int64_t e) { count_range(a.units[u].z, a.units[u].zp, b, e, lane, h, keep_zp); });
         } else {
             count_range(a.units[sl.unit_lo].z, a.units[sl.unit_lo].zp, sl.begin, sl.end, lane, h, keep_zp);
         }
