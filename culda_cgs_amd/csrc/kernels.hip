@@ -410,7 +410,7 @@ __device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
 template <bool kPersistent>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sample_kernel(SampleArgs a) {
     // one wave per workgroup: every LDS array sits at a fixed address, so
-    // the phi lookup is a shift of the packed entry plus a constant offset
+    // the phi lookup is the entry's topic times 4 plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
     __shared__ float rq[kWave];
     // four rows of a.rs_stride floats: S block prefixes of each row's doc

@@ -690,7 +690,7 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     // MH steps (wave-uniform): every candidate (current topic, then each
     // proposal) is drawn first -- the alias entry of a drawn bucket is
     // evaluated from the prefix sums -- its n_dk comes from the doc's theta
-    // row held in registers (one compare per entry and a wave max), its
+    // row held in registers (the counts of its entries, a wave sum), its
     // n_wk / n_k from memory, all loads issued together.
     const int32_t *nrow = a.model.nwk + (int64_t)w * K;
     const double alpha = a.model.alpha, beta = a.model.beta;
