@@ -72,6 +72,8 @@ __global__ void zero_rows_kernel(const ModelDev *models, const int2 *rows, int32
 // The counts are the same either way.
 // Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
 // chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
+constexpr int kSmallWordUnits = 16;  // units whose loads the small-word path keeps in flight together
+
 // Histogram of z[b, e) into the wave's LDS bins h; zp = z when keep_zp.
 // Each lane issues eight loads before their LDS atomics, so a long word
 // waits for one memory round trip per 512 tokens instead of per 64.
@@ -170,7 +172,38 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         if (!sl.split) {
-            each_unit([&](int u, int64_t b, int64_t e) { count_range(a.units[u].z, a.units[u].zp, b, e, lane, h, keep_zp); });
+            const int nu = sl.unit_hi - sl.unit_lo;
+            int64_t wb = 0, we = 0;
+            if (lane < nu && nu <= kSmallWordUnits) {
+                const int64_t *wi = a.units[sl.unit_lo + lane].word_idx;
+                wb = wi[sl.word];
+                we = wi[sl.word + 1];
+            }
+            if (nu <= kSmallWordUnits && __ballot(we - wb > kWave) == 0ull) {
+                // at most 64 of the word's tokens in every unit: one load
+                // per unit per lane, all in flight at once
+                uint16_t t[kSmallWordUnits];
+                uint32_t have = 0;  // bit j: this lane holds a token of unit j
+#pragma unroll
+                for (int j = 0; j < kSmallWordUnits; ++j) {
+                    t[j] = 0;
+                    if (j < nu) {
+                        const int64_t i = readlane64(wb, j) + lane;
+                        if (i < readlane64(we, j)) {
+                            t[j] = a.units[sl.unit_lo + j].z[i];
+                            have |= 1u << j;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kSmallWordUnits; ++j)
+                    if ((have >> j) & 1u) {
+                        atomicAdd(&h[t[j]], 1u);
+                        if (keep_zp) a.units[sl.unit_lo + j].zp[readlane64(wb, j) + lane] = t[j];
+                    }
+            } else {
+                each_unit([&](int u, int64_t b, int64_t e) { count_range(a.units[u].z, a.units[u].zp, b, e, lane, h, keep_zp); });
+            }
         } else {
             count_range(a.units[sl.unit_lo].z, a.units[sl.unit_lo].zp, sl.begin, sl.end, lane, h, keep_zp);
         }
