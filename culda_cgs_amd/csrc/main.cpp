@@ -8,7 +8,7 @@
 // Same flags and per-iteration line as the reference.  Differences: every
 // flag has a default (the reference leaves omitted fields uninitialised,
 // culda_argument.h:50-67), -a/-b accept a fraction such as 50/1024 (atof
-// read "50/1024" as 50), -g is clamped to the one device this build drives,
+// read "50/1024" as 50), -g is clamped to the one device the library drives,
 // and -o writes the phi and theta save files.
 #include <cctype>
 #include <chrono>

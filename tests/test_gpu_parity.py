@@ -1,7 +1,7 @@
 """HIP path vs the CPU oracle, called through the C ABI (libculda.so).
 
 Bit-exact for z, n_wk, n_k and theta; LL/token within 1e-5 relative of the
-oracle's FP64 sum (north-star tolerance).
+oracle's FP64 sum.
 """
 import numpy as np
 import pytest
