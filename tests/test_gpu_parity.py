@@ -251,3 +251,46 @@ def test_row_readers_after_lazy_rare_updates():
         nwk, nk = s.get_phi()
         assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
     s.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_configurations_bit_exact(seed):
+    """Seeded mixes of K (multiples of 16 up to 1024), chunk counts, word
+    shuffles, empty and long docs (counts above 63 split theta entries,
+    rare words take the tiled path): z, n_wk, n_k and theta equal the
+    oracle's after each of three sweeps, counts read only at the end and in
+    between."""
+    rng = np.random.default_rng(1000 + seed)
+    K = int(rng.choice([16, 48, 96, 256, 640, 1024]))
+    V = int(rng.integers(40, 2500))
+    D = int(rng.integers(5, 120))
+    chunks = int(rng.integers(1, 4))
+    perm = rng.permutation(V)
+    docs = [perm[d] for d in random_docs(rng, V, D, float(rng.uniform(5, 80)), empty_docs=int(rng.integers(0, 3)),
+                                         zipf=float(rng.uniform(0.6, 1.3)))]
+    if seed % 2:
+        docs.append(perm[rng.integers(0, min(V, 30), int(rng.integers(2000, 9000)))])  # one long doc
+    arrays = chunks_from_docs(V, docs, min(chunks, len(docs)))
+    alpha, beta, s_seed = float(rng.uniform(0.01, 0.5)), float(rng.uniform(0.005, 0.1)), int(rng.integers(1, 1 << 30))
+    s, om = _pair(V, K, arrays, alpha, beta, s_seed)
+    nch = len(arrays[1])
+    s.init_z()
+    s.rebuild_all()
+    om.init_z(s_seed)
+    om.rebuild()
+    for it in range(1, 4):
+        ll, _ = s.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, s_seed, it)
+        om.rebuild()
+        for c in range(nch):
+            bad = np.flatnonzero(s.get_z(c) != om.z[c])
+            assert bad.size == 0, f"K={K} V={V} chunks={nch} sweep {it}: {bad.size} tokens differ in chunk {c}"
+        ref = out.ll_sum / om.num_tokens
+        assert abs(ll - ref) <= LL_RTOL * abs(ref)
+        if it == 2:
+            nwk, nk = s.get_phi()
+            assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
+    nwk, nk = s.get_phi()
+    assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
+    _theta_equal(s, om, arrays)
+    s.close()
