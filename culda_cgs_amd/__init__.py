@@ -70,8 +70,8 @@ class MHParams(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("slot_tokens", C.c_int32), ("persistent_groups", C.c_int32), ("num_tried", C.c_int32),
-                ("pad", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 16),
-                ("tried_ms", C.c_double * 16)]
+                ("pad", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 32),
+                ("tried_ms", C.c_double * 32)]
 
     def tried(self):
         return [(self.tried_slot_tokens[i], self.tried_ms[i]) for i in range(self.num_tried)]

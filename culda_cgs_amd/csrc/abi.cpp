@@ -130,14 +130,14 @@ void fill_tune(const TuneResult &r, culda_tune *out) {
     std::memset(out, 0, sizeof *out);
     out->slot_tokens = r.slot_tokens;
     out->best_ms = r.best_ms;
-    out->num_tried = (int32_t)std::min<size_t>(r.tried.size(), 16);
+    out->num_tried = (int32_t)std::min<size_t>(r.tried.size(), 32);
     for (int32_t i = 0; i < out->num_tried; ++i) {
         out->tried_slot_tokens[i] = r.tried[i].first;
         out->tried_ms[i] = r.tried[i].second;
     }
 }
 
-const std::vector<int32_t> kSlotCandidates = {256, 512, 1024, 2048, 4096};
+const std::vector<int32_t> kSlotCandidates = {128, 256, 512, 1024, 2048, 4096};
 
 }  // namespace
 

@@ -426,6 +426,23 @@ void Engine::upload() {
     zp_valid = false;
 }
 
+// Default tokens per sampler slot: about 32 slots per CU for the tokens of
+// one launch (a unit; every unit of a batch), as a power of two in
+// [128, 1024] -- a small corpus needs short slots to fill the device, a
+// large one gains nothing below 1024.
+int32_t Engine::default_slot_tokens() const {
+    int64_t per_launch = 0, total = 0;
+    for (const UnitHost &u : units) {
+        per_launch = std::max(per_launch, u.num_tokens);
+        total += u.num_tokens;
+    }
+    if (models.size() > 1) per_launch = total;
+    const int64_t want = per_launch / ((int64_t)cu_count * 32);
+    int32_t st = 128;
+    while (st < 1024 && (int64_t)st * 2 <= want) st *= 2;
+    return st;
+}
+
 // Sampler slots: each unit's word ranges cut into pieces of at most
 // slot_tokens, longest first within the unit; units in order, so a model's
 // slots are contiguous (its LL partials too).  d_order lists every slot
@@ -442,7 +459,7 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     dfree(d_ll_packets);
     dfree(d_model_packets);
     dfree(d_ll_scratch);
-    const int64_t slot_tokens = slot_tokens_in > 0 ? slot_tokens_in : 1024;
+    const int64_t slot_tokens = slot_tokens_in > 0 ? slot_tokens_in : default_slot_tokens();
     std::vector<SlotDesc> all;
     std::vector<Packet> llp;
     std::vector<int32_t> mp(2 * models.size());

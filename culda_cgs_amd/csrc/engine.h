@@ -153,6 +153,7 @@ public:
     int32_t add_corpus(const Corpus &c, float alpha, float beta, uint64_t seed);
     void upload();
     void rebuild_slots(int32_t slot_tokens);
+    int32_t default_slot_tokens() const;
 
     void set_sweep(uint32_t t, hipStream_t s);
     void init_z();

@@ -164,7 +164,8 @@ typedef struct culda_params {
     uint64_t seed;           /* Philox key; z init and every sweep derive
                                 from (seed, global token id, iteration)    */
     int32_t  device;         /* HIP device ordinal                          */
-    int32_t  slot_tokens;    /* sampler tokens per wave slot (0 = default)  */
+    int32_t  slot_tokens;    /* sampler tokens per wave slot (0 = default:
+                                about 32 slots per CU per launch, 128..1024) */
     int32_t  profile;        /* 1 = time each kernel with HIP events        */
     int32_t  fused_phi;      /* 1 = rebuild phi over all chunks in one pass */
 } culda_params;
@@ -299,11 +300,11 @@ typedef struct culda_tune {
     int32_t num_tried;
     int32_t pad;
     double  best_ms;              /* sampler time of the chosen candidate    */
-    int32_t tried_slot_tokens[16];
-    double  tried_ms[16];
+    int32_t tried_slot_tokens[32];
+    double  tried_ms[32];
 } culda_tune;
 
-/* Time slot sizes 256..4096 for this corpus and keep the fastest.  Needs
+/* Time slot sizes 128..4096 for this corpus and keep the fastest.  Needs
  * the initial rebuild (culda_rebuild_all). */
 int culda_autotune(culda_ctx *ctx, culda_tune *out);
 
