@@ -80,6 +80,54 @@ __device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
     return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
 }
 
+// ---- 8-lane rows: each 16-lane DPP row holds two of them (lanes 0-7 and
+// 8-15 of the DPP row); m = lane & 7 is the lane's place in its row.
+
+// Ballot within 8-lane rows: bit m set when lane m of row `row` voted true.
+__device__ __forceinline__ uint32_t row8_ballot(bool pred, int row) {
+    return (uint32_t)(__ballot(pred) >> (row * 8)) & 0xffu;
+}
+
+// Lane 7 of each 8-lane row broadcast to the row: row_newbcast:7 written to
+// banks 0-1 (lanes 0-7) and row_newbcast:15 to banks 2-3 (lanes 8-15).
+__device__ __forceinline__ float row8_last(float x) {
+    const int v = __builtin_bit_cast(int, x);
+    const int r = __builtin_amdgcn_update_dpp(v, v, 0x157, 0xf, 0x3, false);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(r, v, 0x15F, 0xf, 0xC, false));
+}
+
+// Value of the lane S places below in the same 8-lane row, 0 for m < S:
+// row_shr:S over the 16-lane DPP row, after zeroing the lanes m >= 8 - S,
+// which are read only by the first S lanes of the upper row.  S = 4 masks
+// banks 0 and 2 instead (lanes m < 4 keep the 0 passed as old value).
+template <int S>
+__device__ __forceinline__ float row8_shr(float x, int m) {
+    if (S == 4)
+        return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x114, 0xf, 0xA, false));
+    return dpp_f<0x110 + S>(m >= 8 - S ? 0.0f : x);
+}
+
+// scan16 of 16 values held two per lane by an 8-lane row (lane m: values
+// 2m and 2m + 1): each of the four Kogge-Stone steps x_i <- x_i + x_{i-d}
+// adds the same operands as scan16 does over 16 lanes, so every prefix is
+// bit-identical to it.
+__device__ __forceinline__ void scan16x2(float &lo, float &hi, int m) {
+    const float hb = row8_shr<1>(hi, m);  // d = 1: x_{2m} += x_{2m-1}
+    hi = hi + lo;                         //        x_{2m+1} += x_{2m}
+    lo = lo + hb;
+    float a = row8_shr<1>(lo, m), b = row8_shr<1>(hi, m);  // d = 2
+    lo = lo + a;
+    hi = hi + b;
+    a = row8_shr<2>(lo, m);  // d = 4
+    b = row8_shr<2>(hi, m);
+    lo = lo + a;
+    hi = hi + b;
+    a = row8_shr<4>(lo, m);  // d = 8
+    b = row8_shr<4>(hi, m);
+    lo = lo + a;
+    hi = hi + b;
+}
+
 }  // namespace
 
 namespace {

@@ -428,8 +428,8 @@ __device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
 // the word's head row phi_k in LDS (computed from n_wk), builds the
 // Q-bucket block prefix, then takes its tokens 64 at a time: lane l loads
 // the doc id, doc meta and Philox uniform of token l; the batch is sorted
-// by nnz so that the four 16-lane rows, which process four tokens side by
-// side, run S loops of similar length.  Theta blocks are prefetched a
+// by nnz so that the eight 8-lane rows, which process eight tokens side by
+// side, run S loops of similar length.  Theta groups are prefetched a
 // group ahead; scans are DPP row operations.  z and the FP64 LL term are
 // written per token.  The processing order never changes a result: every
 // token depends only on the frozen counts and its own Philox draw.
@@ -446,7 +446,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     // the phi lookup is the entry's topic times 4 plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
     __shared__ float rq[kWave];
-    // four rows of a.rs_stride floats: S block prefixes of each row's doc
+    // eight rows of a.rs_stride floats: S block prefixes of each row's doc
     extern __shared__ __attribute__((aligned(16))) float rs_s[];
     __shared__ uint16_t zres[kWave];
     __shared__ float sqres[kWave];
@@ -472,10 +472,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     const int K = md.k;
     const float alpha = md.alpha, beta = md.beta;
     const uint16_t *theta = un.theta;
-    // phi and count of a 16-bit theta entry e, and of the low and high
-    // entries of a dword d holding two of them
-#define PHI_OF(e) (ph[(e)&kThetaTopicMask])
-#define COUNT_OF(e) ((float)((e) >> kThetaCountShift))
+    // phi and count of the low and high 16-bit theta entries of a dword d
 #define PHI_LO(d) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + (((d) << 2) & 0xffcu)))
 #define PHI_HI(d) (*reinterpret_cast<const float *>(reinterpret_cast<const char *>(ph) + (((d) >> 14) & 0xffcu)))
 #define COUNT_LO(d) ((float)(((d) >> kThetaCountShift) & kThetaMaxCount))
@@ -531,10 +528,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     const float Q = __shfl(T, nbq - 1, kWave);
     wave_sync();
 
-    const int rrow = lane >> 4, lr = lane & 15;
-    // rs[b] = S prefix before block b (rs[0] unused); offset so that
-    // rs[4g + 1 .. 4g + 4] is one aligned 16-byte store
-    float *rs = rs_s + rrow * a.rs_stride + 3;
+    const int rrow = lane >> 3, lr = lane & 7;
+    // rr[b] = inclusive S prefix after block b of this row's doc (8-byte
+    // aligned: a group's two prefixes are one store)
+    float *rr = rs_s + rrow * a.rs_stride + 2;
     double ll = 0.0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
@@ -553,21 +550,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
         // bitonic sort of (valid, nnz, lane), descending: positions
         // 0..ntok-1 hold the valid tokens, longest theta rows first.  The
         // valid tokens are lanes 0..ntok-1, so sorting the first 16 or 32
-        // lanes suffices for short batches, and a batch of at most 4 tokens
+        // lanes suffices for short batches, and a batch of at most 8 tokens
         // is one step whatever its order.
         uint32_t key = (valid ? (0x80000000u | ((mnnz < 0xffffu ? mnnz : 0xffffu) << 6)) : 0u) | (uint32_t)lane;
         if (ntok > 32)
             key = bitonic_desc<64>(key, lane);
         else if (ntok > 16)
             key = bitonic_desc<32>(key, lane);
-        else if (ntok > 4)
+        else if (ntok > 8)
             key = bitonic_desc<16>(key, lane);
         const int tok = (int)(key & 63u);
         const uint32_t pstart = __shfl(mstart, tok, kWave);
         const uint32_t pnnz = __shfl(mnnz, tok, kWave);
         const uint32_t pnd = __shfl(mnd, tok, kWave);
         const float pu = __shfl(u, tok, kWave);
-        for (int r = 0; r < ntok; r += 4) {
+        for (int r = 0; r < ntok; r += 8) {
             const int p = r + rrow;
             const uint32_t start = __shfl(pstart, p, kWave);
             const uint32_t nnz = __shfl(pnnz, p, kWave);
@@ -577,48 +574,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 // block lies inside the doc's 16-aligned slab, entries past
                 // nnz are zero (the slab tail), so their products are exact
                 // zeros.  A block total is the balanced tree of its 16
-                // products (lane 15 of the in-block Kogge-Stone scan): lane
-                // lr holds entries 4 lr .. 4 lr + 3 of a group of four
-                // blocks (one 8-byte load; block lr >> 2) and forms the
-                // tree's first two levels in registers; DPP row_shr 1 and 2
-                // finish the four totals at lanes 3, 7, 11 and 15, and
-                // row_newbcast carries R_{b+1} = fl(R_b + T_b) in block
-                // order.  Groups (64 entries) are loaded one group ahead;
-                // the last group is loaded whole: loads past the last block
-                // stay inside the theta array (kThetaPad entries of tail
-                // padding), and rs entries past nb may hold sums of such
-                // blocks, which nothing reads.
+                // products (lane 15 of scan16): lane m holds entries
+                // 4m .. 4m + 3 of a group of two blocks (one 8-byte load;
+                // block m >> 2), forms the tree's first two levels in
+                // registers, and DPP row_shr 1 and 2 finish the two totals
+                // at lanes 3 and 7; lane 7 carries R_{b+1} = fl(R_b + T_b)
+                // in block order (row_shr 4 brings lane 3's total).  Groups
+                // (32 entries) are loaded one group ahead; the last group is
+                // loaded whole: loads past the last block stay inside the
+                // theta array (kThetaPad entries of tail padding), and rr
+                // entries past nb may hold sums of such blocks, which
+                // nothing reads.
                 const uint32_t nb = (nnz + 15u) >> 4;
-                const uint32_t ng = (nb + 3u) >> 2;
+                const uint32_t ng = (nb + 1u) >> 1;
                 const uint32_t *tp = reinterpret_cast<const uint32_t *>(theta + start) + 2 * lr;
-                auto group = [&](u32x2 d, float R0, float &r0, float &r1, float &r2, float &r3) {
+                auto group = [&](u32x2 d, float R0, float &r0, float &r1) {
                     const float f0 = PHI_LO(d.x), f1 = PHI_HI(d.x), f2 = PHI_LO(d.y), f3 = PHI_HI(d.y);
                     const float c0 = COUNT_LO(d.x), c1 = COUNT_HI(d.x), c2 = COUNT_LO(d.y), c3 = COUNT_HI(d.y);
                     float y = (c0 * f0 + c1 * f1) + (c2 * f2 + c3 * f3);
                     y = y + dpp_f<0x111>(y);
                     y = y + dpp_f<0x112>(y);
-                    r0 = R0 + dpp_f<0x153>(y);
-                    r1 = r0 + dpp_f<0x157>(y);
-                    r2 = r1 + dpp_f<0x15B>(y);
-                    r3 = r2 + row_last(y);
+                    r0 = R0 + dpp_f<0x114>(y);
+                    r1 = r0 + y;
                 };
                 u32x2 cur = load_global2(tp);
                 float R = 0.0f;
                 uint32_t g = 0;
                 for (; g + 1 < ng; ++g) {
-                    const u32x2 nxt = load_global2(tp + ((g + 1) << 5));
-                    float r0, r1, r2, r3;
-                    group(cur, R, r0, r1, r2, r3);
-                    if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
-                    R = r3;
+                    const u32x2 nxt = load_global2(tp + ((g + 1) << 4));
+                    float r0, r1;
+                    group(cur, R, r0, r1);
+                    if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
+                    R = r1;
                     cur = nxt;
                 }
-                // last group: nl = 1..4 blocks
-                const uint32_t nl = nb - (g << 2);
-                float r0, r1, r2, r3;
-                group(cur, R, r0, r1, r2, r3);
-                if (lr == 0) *reinterpret_cast<float4 *>(rs + (g << 2) + 1) = make_float4(r0, r1, r2, r3);
-                const float S = nl == 1 ? r0 : nl == 2 ? r1 : nl == 3 ? r2 : r3;
+                // last group: one or two blocks
+                float r0, r1;
+                group(cur, R, r0, r1);
+                if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
+                const float S = row8_last(nb - (g << 1) == 1u ? r0 : r1);
                 wave_sync();
                 // t = fl(u * (S + Q)); the S bucket takes t < S with target
                 // t, the Q bucket target fl(t - S)
@@ -626,14 +620,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 const float tu = uu * SQ;
                 const bool sb = tu < S;
                 const float t = sb ? tu : tu - S;
-                const float *rr = sb ? rs + 1 : rq;  // inclusive prefix after each block
+                const float *rsel = sb ? rr : rq;  // inclusive prefix after each block
                 const int nbb = sb ? (int)nb : nbq;
+                // first block b with t < rsel[b]: lane m tests blocks
+                // q0 + 2m and q0 + 2m + 1
                 int bsel = -1;
                 for (int q0 = 0; q0 < nbb; q0 += 16) {
-                    const int b = q0 + lr;
-                    const uint32_t msk = row_ballot((b < nbb) && (t < rr[b]), rrow);
-                    if (msk) {
-                        bsel = q0 + __builtin_ctz(msk);
+                    const int b = q0 + 2 * lr;
+                    bool clo = false, chi = false;
+                    if (b < nbb) {
+                        const float2 v = *reinterpret_cast<const float2 *>(rsel + b);
+                        clo = t < v.x;
+                        chi = b + 1 < nbb && t < v.y;
+                    }
+                    const uint32_t mlo = row8_ballot(clo, rrow), mhi = row8_ballot(chi, rrow);
+                    if (mlo | mhi) {
+                        const int m = __builtin_ctz(mlo | mhi);
+                        bsel = q0 + 2 * m + (int)(((mlo >> m) & 1u) ^ 1u);
                         break;
                     }
                 }
@@ -641,29 +644,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 if (bsel < 0) {
                     topic = sb ? (load_global_u16(theta + start + nnz - 1) & kThetaTopicMask) : (uint32_t)(K - 1);
                 } else {
-                    const uint32_t j = ((uint32_t)bsel << 4) + (uint32_t)lr;
-                    uint32_t e = 0;
-                    float v;
-                    bool ok = true;
+                    // the block's 16 products, two per lane (entries j, j + 1)
+                    const uint32_t j = ((uint32_t)bsel << 4) + 2u * (uint32_t)lr;
+                    uint32_t e2 = 0;
+                    float vlo, vhi;
+                    bool oklo = true, okhi = true;
                     if (sb) {
-                        e = load_global_u16(theta + start + j);
-                        ok = j < nnz;
-                        v = ok ? COUNT_OF(e) * PHI_OF(e) : 0.0f;
+                        e2 = load_global(reinterpret_cast<const uint32_t *>(theta + start + j));
+                        oklo = j < nnz;
+                        okhi = j + 1u < nnz;
+                        vlo = oklo ? COUNT_LO(e2) * PHI_LO(e2) : 0.0f;
+                        vhi = okhi ? COUNT_HI(e2) * PHI_HI(e2) : 0.0f;
                     } else {
-                        v = alpha * ph[j];
+                        const float2 f = *reinterpret_cast<const float2 *>(ph + j);
+                        vlo = alpha * f.x;
+                        vhi = alpha * f.y;
                     }
-                    v = scan16(v);
-                    const float C = bsel == 0 ? v : rr[bsel - 1] + v;
-                    const uint32_t msk = row_ballot(ok && (t < C), rrow);
+                    scan16x2(vlo, vhi, lr);
+                    float Clo = vlo, Chi = vhi;
+                    if (bsel != 0) {
+                        const float rb = rsel[bsel - 1];
+                        Clo = rb + vlo;
+                        Chi = rb + vhi;
+                    }
+                    const uint32_t mlo = row8_ballot(oklo && t < Clo, rrow), mhi = row8_ballot(okhi && t < Chi, rrow);
                     int msel;
-                    if (msk) {
-                        msel = __builtin_ctz(msk);
+                    if (mlo | mhi) {
+                        const int m = __builtin_ctz(mlo | mhi);
+                        msel = 2 * m + (int)(((mlo >> m) & 1u) ^ 1u);
                     } else {
                         const int nvalid = (int)nnz - (bsel << 4);
                         msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
                     }
-                    const uint32_t et = __shfl(e & kThetaTopicMask, msel, kBlock);
-                    topic = sb ? et : (uint32_t)((bsel << 4) + msel);
+                    const uint32_t ew = __shfl(e2, (rrow << 3) + (msel >> 1), kWave);
+                    topic = sb ? (((msel & 1) ? (ew >> 16) : ew) & kThetaTopicMask) : (uint32_t)((bsel << 4) + msel);
                 }
                 if (lr == 0) {
                     zres[p] = (uint16_t)topic;
@@ -682,8 +696,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
     if (lane == 0) a.ll_partial[slot] = ll;
-#undef PHI_OF
-#undef COUNT_OF
 #undef PHI_LO
 #undef PHI_HI
 #undef COUNT_LO
@@ -856,7 +868,7 @@ int32_t sample_rs_stride(int64_t max_entries) {
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
     if (a.rs_stride < 8 || a.rs_stride % 4) return hipErrorInvalidValue;
-    const size_t lds = (size_t)4 * a.rs_stride * sizeof(float);
+    const size_t lds = (size_t)8 * a.rs_stride * sizeof(float);
     if (a.queue)
         hipLaunchKernelGGL(sample_kernel<true>, dim3(grid), dim3(kWave), lds, s, a);
     else
@@ -867,7 +879,7 @@ hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
 int32_t sample_blocks_per_cu(int32_t rs_stride) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave,
-                                                     (size_t)4 * rs_stride * sizeof(float)) != hipSuccess)
+                                                     (size_t)8 * rs_stride * sizeof(float)) != hipSuccess)
         n = 0;
     return n;
 }

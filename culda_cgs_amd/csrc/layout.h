@@ -123,7 +123,7 @@ struct SampleArgs {
     uint32_t *queue;        // persistent mode: next list position; else null
     int32_t num_slots;
     int32_t use_zp;         // counts equal counts(zp): rare slots stage phi from zp
-    int32_t rs_stride;      // floats per 16-lane row of S block prefixes (dynamic LDS), sample_rs_stride()
+    int32_t rs_stride;      // floats per 8-lane row of S block prefixes (dynamic LDS), sample_rs_stride()
     int32_t pad;
 };
 
