@@ -88,6 +88,15 @@ __device__ __forceinline__ uint32_t row8_ballot(bool pred, int row) {
     return (uint32_t)(__ballot(pred) >> (row * 8)) & 0xffu;
 }
 
+// Minimum over each 8-lane row, in every lane of the row: quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], then row_half_mirror (lane m meets lane
+// 7 - m, joining the row's two quads).
+__device__ __forceinline__ uint32_t row8_min(uint32_t x) {
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xf, 0xf, false));
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xf, 0xf, false));
+    return min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xf, 0xf, false));
+}
+
 // Lane 7 of each 8-lane row broadcast to the row: row_newbcast:7 written to
 // banks 0-1 (lanes 0-7) and row_newbcast:15 to banks 2-3 (lanes 8-15).
 __device__ __forceinline__ float row8_last(float x) {

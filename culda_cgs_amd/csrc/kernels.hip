@@ -623,20 +623,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 const float *rsel = sb ? rr : rq;  // inclusive prefix after each block
                 const int nbb = sb ? (int)nb : nbq;
                 // first block b with t < rsel[b]: lane m tests blocks
-                // q0 + 2m and q0 + 2m + 1
+                // q0 + 2m and q0 + 2m + 1, the row takes the least hit
                 int bsel = -1;
                 for (int q0 = 0; q0 < nbb; q0 += 16) {
                     const int b = q0 + 2 * lr;
-                    bool clo = false, chi = false;
+                    uint32_t hit = 0xffu;
                     if (b < nbb) {
                         const float2 v = *reinterpret_cast<const float2 *>(rsel + b);
-                        clo = t < v.x;
-                        chi = b + 1 < nbb && t < v.y;
+                        hit = t < v.x ? 2u * lr : (b + 1 < nbb && t < v.y) ? 2u * lr + 1u : 0xffu;
                     }
-                    const uint32_t mlo = row8_ballot(clo, rrow), mhi = row8_ballot(chi, rrow);
-                    if (mlo | mhi) {
-                        const int m = __builtin_ctz(mlo | mhi);
-                        bsel = q0 + 2 * m + (int)(((mlo >> m) & 1u) ^ 1u);
+                    hit = row8_min(hit);
+                    if (hit != 0xffu) {
+                        bsel = q0 + (int)hit;
                         break;
                     }
                 }
@@ -644,17 +642,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 if (bsel < 0) {
                     topic = sb ? (load_global_u16(theta + start + nnz - 1) & kThetaTopicMask) : (uint32_t)(K - 1);
                 } else {
-                    // the block's 16 products, two per lane (entries j, j + 1)
+                    // the block's 16 products, two per lane (entries j, j + 1);
+                    // entries past nnz are zero-count padding (exact zero
+                    // products) and never chosen: their prefixes, summed in
+                    // another order, may round above the last entry's
                     const uint32_t j = ((uint32_t)bsel << 4) + 2u * (uint32_t)lr;
                     uint32_t e2 = 0;
                     float vlo, vhi;
-                    bool oklo = true, okhi = true;
                     if (sb) {
                         e2 = load_global(reinterpret_cast<const uint32_t *>(theta + start + j));
-                        oklo = j < nnz;
-                        okhi = j + 1u < nnz;
-                        vlo = oklo ? COUNT_LO(e2) * PHI_LO(e2) : 0.0f;
-                        vhi = okhi ? COUNT_HI(e2) * PHI_HI(e2) : 0.0f;
+                        vlo = COUNT_LO(e2) * PHI_LO(e2);
+                        vhi = COUNT_HI(e2) * PHI_HI(e2);
                     } else {
                         const float2 f = *reinterpret_cast<const float2 *>(ph + j);
                         vlo = alpha * f.x;
@@ -667,11 +665,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                         Clo = rb + vlo;
                         Chi = rb + vhi;
                     }
-                    const uint32_t mlo = row8_ballot(oklo && t < Clo, rrow), mhi = row8_ballot(okhi && t < Chi, rrow);
+                    const uint32_t lim = sb ? nnz : 0xffffffffu;
+                    const uint32_t hit = row8_min((j < lim && t < Clo)        ? 2u * lr
+                                                  : (j + 1u < lim && t < Chi) ? 2u * lr + 1u
+                                                                              : 0xffu);
                     int msel;
-                    if (mlo | mhi) {
-                        const int m = __builtin_ctz(mlo | mhi);
-                        msel = 2 * m + (int)(((mlo >> m) & 1u) ^ 1u);
+                    if (hit != 0xffu) {
+                        msel = (int)hit;
                     } else {
                         const int nvalid = (int)nnz - (bsel << 4);
                         msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
