@@ -597,20 +597,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                     r0 = R0 + dpp_f<0x114>(y);
                     r1 = r0 + y;
                 };
-                u32x2 cur = load_global2(tp);
-                float R = 0.0f;
+                // two groups per pass, each loaded while the one before is
+                // summed (no register copy between passes)
+                u32x2 ca = load_global2(tp);
+                float R = 0.0f, r0, r1;
                 uint32_t g = 0;
-                for (; g + 1 < ng; ++g) {
-                    const u32x2 nxt = load_global2(tp + ((g + 1) << 4));
-                    float r0, r1;
-                    group(cur, R, r0, r1);
+                for (; g + 2 < ng; g += 2) {
+                    const u32x2 cb = load_global2(tp + ((g + 1) << 4));
+                    group(ca, R, r0, r1);
+                    if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
+                    ca = load_global2(tp + ((g + 2) << 4));
+                    group(cb, r1, r0, R);
+                    if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1) + 2) = make_float2(r0, R);
+                }
+                if (g + 1 < ng) {
+                    const u32x2 cb = load_global2(tp + ((g + 1) << 4));
+                    group(ca, R, r0, r1);
                     if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
                     R = r1;
-                    cur = nxt;
+                    ca = cb;
+                    ++g;
                 }
                 // last group: one or two blocks
-                float r0, r1;
-                group(cur, R, r0, r1);
+                group(ca, R, r0, r1);
                 if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
                 const float S = row8_last(nb - (g << 1) == 1u ? r0 : r1);
                 wave_sync();
