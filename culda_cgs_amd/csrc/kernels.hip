@@ -529,9 +529,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
     wave_sync();
 
     const int rrow = lane >> 3, lr = lane & 7;
-    // rr[b] = inclusive S prefix after block b of this row's doc (8-byte
-    // aligned: a group's two prefixes are one store)
-    float *rr = rs_s + rrow * a.rs_stride + 2;
+    // rr[b] = inclusive S prefix after block b of this row's doc (16-byte
+    // aligned: two groups' four prefixes are one store)
+    float *rr = rs_s + rrow * a.rs_stride + 4;
     double ll = 0.0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
@@ -604,11 +604,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 uint32_t g = 0;
                 for (; g + 2 < ng; g += 2) {
                     const u32x2 cb = load_global2(tp + ((g + 1) << 4));
+                    float r2, r3;
                     group(ca, R, r0, r1);
-                    if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
                     ca = load_global2(tp + ((g + 2) << 4));
-                    group(cb, r1, r0, R);
-                    if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1) + 2) = make_float2(r0, R);
+                    group(cb, r1, r2, r3);
+                    if (lr == 7) *reinterpret_cast<float4 *>(rr + (g << 1)) = make_float4(r0, r1, r2, r3);
+                    R = r3;
                 }
                 if (g + 1 < ng) {
                     const u32x2 cb = load_global2(tp + ((g + 1) << 4));
@@ -866,12 +867,12 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s) {
 }
 
 // Row stride of the sampler's S block prefixes for rows of at most
-// max_entries theta entries: rs = row + 3 is written at [1, 4 * ng] with
-// ng = ceil(nb / 4) groups of four 16-entry blocks.  68 floats while no
-// row has more than 1024 entries.
+// max_entries theta entries (nb = ceil(max_entries / 16) blocks): rr = row
+// + 4 is written and read at [0, nb].  72 floats while no row has more than
+// 1024 entries.
 int32_t sample_rs_stride(int64_t max_entries) {
     const int64_t nb = (std::max<int64_t>(max_entries, 1) + 15) / 16;
-    return (int32_t)(4 * ((nb + 3) / 4) + 4);
+    return (int32_t)(4 * ((nb + 3) / 4) + 8);
 }
 
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
