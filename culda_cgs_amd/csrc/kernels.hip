@@ -633,14 +633,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                 const float *rsel = sb ? rr : rq;  // inclusive prefix after each block
                 const int nbb = sb ? (int)nb : nbq;
                 // first block b with t < rsel[b]: lane m tests blocks
-                // q0 + 2m and q0 + 2m + 1, the row takes the least hit
+                // q0 + 4m .. q0 + 4m + 3, the row takes the least hit
                 int bsel = -1;
-                for (int q0 = 0; q0 < nbb; q0 += 16) {
-                    const int b = q0 + 2 * lr;
+                for (int q0 = 0; q0 < nbb; q0 += 32) {
+                    const int b = q0 + 4 * lr;
                     uint32_t hit = 0xffu;
                     if (b < nbb) {
-                        const float2 v = *reinterpret_cast<const float2 *>(rsel + b);
-                        hit = t < v.x ? 2u * lr : (b + 1 < nbb && t < v.y) ? 2u * lr + 1u : 0xffu;
+                        const float4 v = *reinterpret_cast<const float4 *>(rsel + b);
+                        hit = t < v.x                       ? 4u * lr
+                              : (b + 1 < nbb && t < v.y) ? 4u * lr + 1u
+                              : (b + 2 < nbb && t < v.z) ? 4u * lr + 2u
+                              : (b + 3 < nbb && t < v.w) ? 4u * lr + 3u
+                                                         : 0xffu;
                     }
                     hit = row8_min(hit);
                     if (hit != 0xffu) {
@@ -868,8 +872,8 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s) {
 
 // Row stride of the sampler's S block prefixes for rows of at most
 // max_entries theta entries (nb = ceil(max_entries / 16) blocks): rr = row
-// + 4 is written and read at [0, nb].  72 floats while no row has more than
-// 1024 entries.
+// + 4 is written at [0, nb] and read at [0, nb + 2].  72 floats while no row
+// has more than 1024 entries.
 int32_t sample_rs_stride(int64_t max_entries) {
     const int64_t nb = (std::max<int64_t>(max_entries, 1) + 15) / 16;
     return (int32_t)(4 * ((nb + 3) / 4) + 8);
