@@ -639,12 +639,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                     const int b = q0 + 4 * lr;
                     uint32_t hit = 0xffu;
                     if (b < nbb) {
+                        // one 16-byte read, conditions without short-circuit
                         const float4 v = *reinterpret_cast<const float4 *>(rsel + b);
-                        hit = t < v.x                       ? 4u * lr
-                              : (b + 1 < nbb && t < v.y) ? 4u * lr + 1u
-                              : (b + 2 < nbb && t < v.z) ? 4u * lr + 2u
-                              : (b + 3 < nbb && t < v.w) ? 4u * lr + 3u
-                                                         : 0xffu;
+                        const int nv = nbb - b;
+                        hit = ((nv > 3) & (t < v.w)) ? 4u * lr + 3u : hit;
+                        hit = ((nv > 2) & (t < v.z)) ? 4u * lr + 2u : hit;
+                        hit = ((nv > 1) & (t < v.y)) ? 4u * lr + 1u : hit;
+                        hit = (t < v.x) ? 4u * lr : hit;
                     }
                     hit = row8_min(hit);
                     if (hit != 0xffu) {
@@ -680,9 +681,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sa
                         Chi = rb + vhi;
                     }
                     const uint32_t lim = sb ? nnz : 0xffffffffu;
-                    const uint32_t hit = row8_min((j < lim && t < Clo)        ? 2u * lr
-                                                  : (j + 1u < lim && t < Chi) ? 2u * lr + 1u
-                                                                              : 0xffu);
+                    uint32_t hit = ((j + 1u < lim) & (t < Chi)) ? 2u * lr + 1u : 0xffu;
+                    hit = row8_min(((j < lim) & (t < Clo)) ? 2u * lr : hit);
                     int msel;
                     if (hit != 0xffu) {
                         msel = (int)hit;
