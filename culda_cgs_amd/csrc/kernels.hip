@@ -438,10 +438,11 @@ __device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
 // is exhausted (the host interleaves the order so that every claim holds
 // slots of all lengths); otherwise workgroup b takes position b.
 // Reference: LDAKernelTrain (kernel/lda_train_kernel.cu:20-271).
-// Seven waves per SIMD (at most 72 VGPRs): the per-token chains of
-// dependent loads need the occupancy more than the registers.
-template <bool kPersistent>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7))) void sample_kernel(SampleArgs a) {
+// kWaves waves per SIMD bound the registers (7: at most 72 VGPRs, 6: 80):
+// the per-token chains of dependent loads need the occupancy more than the
+// registers, unless the S-prefix rows in LDS already hold it below seven.
+template <bool kPersistent, int kWaves>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) void sample_kernel(SampleArgs a) {
     // one wave per workgroup: every LDS array sits at a fixed address, so
     // the phi lookup is the entry's topic times 4 plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
@@ -879,23 +880,49 @@ int32_t sample_rs_stride(int64_t max_entries) {
     return (int32_t)(4 * ((nb + 3) / 4) + 8);
 }
 
+// Whether the sampler's LDS (dynamic part: eight S-prefix rows) keeps it
+// below seven waves per SIMD, in which case the six-wave build, with more
+// registers, runs instead.
+static bool sample_lds_bound(int32_t rs_stride) {
+    static thread_local int32_t last_stride = -1;
+    static thread_local bool last = false;
+    if (rs_stride != last_stride) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 7>, kWave,
+                                                         (size_t)8 * rs_stride * sizeof(float)) != hipSuccess)
+            n = 0;
+        last = n > 0 && n < 7 * 4;
+        last_stride = rs_stride;
+    }
+    return last;
+}
+
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
     if (a.num_slots <= 0) return hipSuccess;
     if (a.rs_stride < 8 || a.rs_stride % 4) return hipErrorInvalidValue;
     const size_t lds = (size_t)8 * a.rs_stride * sizeof(float);
-    if (a.queue)
-        hipLaunchKernelGGL(sample_kernel<true>, dim3(grid), dim3(kWave), lds, s, a);
-    else
-        hipLaunchKernelGGL(sample_kernel<false>, dim3(a.num_slots), dim3(kWave), lds, s, a);
+    const bool six = sample_lds_bound(a.rs_stride);
+    if (a.queue) {
+        if (six)
+            hipLaunchKernelGGL((sample_kernel<true, 6>), dim3(grid), dim3(kWave), lds, s, a);
+        else
+            hipLaunchKernelGGL((sample_kernel<true, 7>), dim3(grid), dim3(kWave), lds, s, a);
+    } else {
+        if (six)
+            hipLaunchKernelGGL((sample_kernel<false, 6>), dim3(a.num_slots), dim3(kWave), lds, s, a);
+        else
+            hipLaunchKernelGGL((sample_kernel<false, 7>), dim3(a.num_slots), dim3(kWave), lds, s, a);
+    }
     return hipGetLastError();
 }
 
 int32_t sample_blocks_per_cu(int32_t rs_stride) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true>, kWave,
-                                                     (size_t)8 * rs_stride * sizeof(float)) != hipSuccess)
-        n = 0;
-    return n;
+    const size_t lds = (size_t)8 * rs_stride * sizeof(float);
+    const hipError_t e = sample_lds_bound(rs_stride)
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 6>, kWave, lds)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 7>, kWave, lds);
+    return e == hipSuccess ? n : 0;
 }
 
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s) {
