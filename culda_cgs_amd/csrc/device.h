@@ -49,9 +49,6 @@ __device__ __forceinline__ float scan16(float x) {
     return x;
 }
 
-// Lane 15 of each row broadcast to the whole row (DPP row_newbcast:15).
-__device__ __forceinline__ float row_last(float x) { return dpp_f<0x15F>(x); }
-
 // Copy of a wave-uniform struct held in scalar registers (each dword
 // through readfirstlane), so descriptors read inside a loop that also
 // stores do not occupy vector registers.
@@ -75,18 +72,8 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int j) {
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-// Row-local ballot: bit m set when lane m of this row voted true.
-__device__ __forceinline__ uint32_t row_ballot(bool pred, int row) {
-    return (uint32_t)(__ballot(pred) >> (row * kBlock)) & 0xffffu;
-}
-
 // ---- 8-lane rows: each 16-lane DPP row holds two of them (lanes 0-7 and
 // 8-15 of the DPP row); m = lane & 7 is the lane's place in its row.
-
-// Ballot within 8-lane rows: bit m set when lane m of row `row` voted true.
-__device__ __forceinline__ uint32_t row8_ballot(bool pred, int row) {
-    return (uint32_t)(__ballot(pred) >> (row * 8)) & 0xffu;
-}
 
 // Minimum over each 8-lane row, in every lane of the row: quad_perm
 // [1,0,3,2], quad_perm [2,3,0,1], then row_half_mirror (lane m meets lane
@@ -95,14 +82,6 @@ __device__ __forceinline__ uint32_t row8_min(uint32_t x) {
     x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xf, 0xf, false));
     x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xf, 0xf, false));
     return min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xf, 0xf, false));
-}
-
-// Lane 7 of each 8-lane row broadcast to the row: row_newbcast:7 written to
-// banks 0-1 (lanes 0-7) and row_newbcast:15 to banks 2-3 (lanes 8-15).
-__device__ __forceinline__ float row8_last(float x) {
-    const int v = __builtin_bit_cast(int, x);
-    const int r = __builtin_amdgcn_update_dpp(v, v, 0x157, 0xf, 0x3, false);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(r, v, 0x15F, 0xf, 0xC, false));
 }
 
 // Value of the lane S places below in the same 8-lane row, 0 for m < S:

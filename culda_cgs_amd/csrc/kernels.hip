@@ -623,8 +623,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // last group: one or two blocks
                 group(ca, R, r0, r1);
                 if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
-                const float S = row8_last(nb - (g << 1) == 1u ? r0 : r1);
                 wave_sync();
+                const float S = rr[nb - 1];  // nb >= 1: a token's doc has an entry
                 // t = fl(u * (S + Q)); the S bucket takes t < S with target
                 // t, the Q bucket target fl(t - S)
                 const float SQ = S + Q;
