@@ -38,17 +38,6 @@ __device__ __forceinline__ float dpp_f(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xf, 0xf, false));
 }
 
-// Inclusive Kogge-Stone scan inside each 16-lane row (DPP row_shr:1,2,4,8):
-// x_m <- x_m + x_{m-d} for m >= d; lanes with m < d add 0, which leaves
-// the non-negative values unchanged.
-__device__ __forceinline__ float scan16(float x) {
-    x = x + dpp_f<0x111>(x);
-    x = x + dpp_f<0x112>(x);
-    x = x + dpp_f<0x114>(x);
-    x = x + dpp_f<0x118>(x);
-    return x;
-}
-
 // Copy of a wave-uniform struct held in scalar registers (each dword
 // through readfirstlane), so descriptors read inside a loop that also
 // stores do not occupy vector registers.
@@ -95,25 +84,18 @@ __device__ __forceinline__ float row8_shr(float x, int m) {
     return dpp_f<0x110 + S>(m >= 8 - S ? 0.0f : x);
 }
 
-// scan16 of 16 values held two per lane by an 8-lane row (lane m: values
-// 2m and 2m + 1): each of the four Kogge-Stone steps x_i <- x_i + x_{i-d}
-// adds the same operands as scan16 does over 16 lanes, so every prefix is
-// bit-identical to it.
-__device__ __forceinline__ void scan16x2(float &lo, float &hi, int m) {
-    const float hb = row8_shr<1>(hi, m);  // d = 1: x_{2m} += x_{2m-1}
-    hi = hi + lo;                         //        x_{2m+1} += x_{2m}
-    lo = lo + hb;
-    float a = row8_shr<1>(lo, m), b = row8_shr<1>(hi, m);  // d = 2
-    lo = lo + a;
-    hi = hi + b;
-    a = row8_shr<2>(lo, m);  // d = 4
-    b = row8_shr<2>(hi, m);
-    lo = lo + a;
-    hi = hi + b;
-    a = row8_shr<4>(lo, m);  // d = 8
-    b = row8_shr<4>(hi, m);
-    lo = lo + a;
-    hi = hi + b;
+// In-block prefixes of 16 values held two per lane by an 8-lane row (lane
+// m: values 2m and 2m + 1): pair sums q_m = fl(x_2m + x_2m+1), their
+// inclusive Kogge-Stone scan P_m (offsets 1, 2, 4), then C_2m+1 = P_m and
+// C_2m = fl(P_m-1 + x_2m) (P_-1 = 0, and 0 + x = x for x >= 0).  C_15 = P_7
+// is the balanced tree of the 16 values, the block total.
+__device__ __forceinline__ void prefix16x2(float &lo, float &hi, int m) {
+    float P = lo + hi;
+    P = P + row8_shr<1>(P, m);
+    P = P + row8_shr<2>(P, m);
+    P = P + row8_shr<4>(P, m);
+    lo = row8_shr<1>(P, m) + lo;
+    hi = P;
 }
 
 }  // namespace

@@ -6,10 +6,11 @@
 //   phi_k   = fl(fl(fl(n_wk) + beta) / den_k),  den_k = fl(fl(n_k) + fl(beta * fl(V)))
 //   a_k     = fl(alpha * phi_k)                              (Q bucket)
 //   p_j     = fl(fl(count_j) * phi_{topic_j})                (S bucket)
-//   blocks of 16 values, inclusive Kogge-Stone scan inside a block
-//   (offsets 1, 2, 4, 8); S: block totals carried sequentially
-//   R_{b+1} = fl(R_b + T_b); Q: 64-lane Kogge-Stone scan of the block
-//   totals.  t = fl(u * fl(S + Q)); S bucket iff t < S (target t), else
+//   blocks of 16 values; a block total T_b is the balanced tree of its
+//   values; S: block totals carried sequentially R_{b+1} = fl(R_b + T_b);
+//   Q: 64-lane Kogge-Stone scan of the block totals.  In-block prefixes:
+//   pair sums q_m = fl(v_2m + v_2m+1), their Kogge-Stone scan P_m (offsets
+//   1, 2, 4), prefix_2m+1 = P_m, prefix_2m = fl(P_m-1 + v_2m).  t = fl(u * fl(S + Q)); S bucket iff t < S (target t), else
 //   Q bucket (target fl(t - S)).  Search: first block whose inclusive
 //   prefix exceeds the target, then first element of that block whose
 //   fl(prefix_before_block + in-block prefix) exceeds it; fallbacks take
@@ -506,8 +507,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         }
     }
     wave_sync();
-    // Q-bucket block totals (balanced tree of 16 = last element of the
-    // in-block scan), then a 64-lane Kogge-Stone scan of the totals
+    // Q-bucket block totals (balanced tree of 16), then a 64-lane
+    // Kogge-Stone scan of the totals
     const int nbq = K >> 4;
     float T = 0.0f;
     if (lane < nbq) {
@@ -575,7 +576,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // block lies inside the doc's 16-aligned slab, entries past
                 // nnz are zero (the slab tail), so their products are exact
                 // zeros.  A block total is the balanced tree of its 16
-                // products (lane 15 of scan16): lane m holds entries
+                // products: lane m holds entries
                 // 4m .. 4m + 3 of a group of two blocks (one 8-byte load;
                 // block m >> 2), forms the tree's first two levels in
                 // registers, and DPP row_shr 1 and 2 finish the two totals
@@ -674,7 +675,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                         vlo = alpha * f.x;
                         vhi = alpha * f.y;
                     }
-                    scan16x2(vlo, vhi, lr);
+                    prefix16x2(vlo, vhi, lr);
                     float Clo = vlo, Chi = vhi;
                     if (bsel != 0) {
                         const float rb = rsel[bsel - 1];

@@ -20,8 +20,10 @@
  *   - Philox4x64-10 keyed by (seed, global token id, sweep, purpose)
  *     instead of the reference's clock-seeded curand state;
  *   - FP32 head (n_wk + beta) / (n_k + beta V) (kernel/lda_phi_kernel.cu:26);
- *   - FP32 bucket arithmetic in a fixed order: blocks of 16, Kogge-Stone
- *     inclusive scan inside a block, S block totals carried sequentially,
+ *   - FP32 bucket arithmetic in a fixed order: blocks of 16 whose totals
+ *     are balanced trees (last element of a Kogge-Stone scan, ks16), in-block
+ *     prefixes from pair sums scanned Kogge-Stone (pfx16), S block totals
+ *     carried sequentially,
  *     Q block totals scanned Kogge-Stone over 64 lanes; t = u * (S + Q)
  *     picks the S bucket when t < S (target t) and otherwise the Q bucket
  *     (target t - S) -- the reference's u < S/(S+Q) test and
@@ -234,6 +236,24 @@ static void ks16(float *x) {
     }
 }
 
+/* In-block prefixes of 16 products, in place: pair sums q_m = p_2m + p_2m+1,
+ * their inclusive Kogge-Stone scan P_m (offsets 1, 2, 4), then
+ * C_2m+1 = P_m and C_2m = P_m-1 + p_2m (P_-1 = 0).  C_15 = P_7 is the balanced
+ * tree of the 16 products, the same value as ks16's last element. */
+static void pfx16(float *x) {
+    float P[8];
+    for (int m = 0; m < 8; ++m) P[m] = x[2 * m] + x[2 * m + 1];
+    for (int d = 1; d < 8; d <<= 1) {
+        float y[8];
+        for (int m = 0; m < 8; ++m) y[m] = m >= d ? P[m] + P[m - d] : P[m];
+        memcpy(P, y, sizeof y);
+    }
+    for (int m = 0; m < 8; ++m) {
+        x[2 * m] = m ? P[m - 1] + x[2 * m] : x[2 * m];
+        x[2 * m + 1] = P[m];
+    }
+}
+
 /* Inclusive Kogge-Stone scan of 64 values, offsets 1 .. 32. */
 static void ks64(float *x) {
     for (int d = 1; d < 64; d <<= 1) {
@@ -350,7 +370,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                                 const int64_t j = bsel * 16 + m;
                                 p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
                             }
-                            ks16(p);
+                            pfx16(p);
                             const int64_t nvalid = nnz - bsel * 16 < 16 ? nnz - bsel * 16 : 16;
                             int64_t msel = nvalid - 1;
                             for (int64_t m = 0; m < nvalid; ++m) {
@@ -375,7 +395,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                         } else {
                             float v[16];
                             for (int m = 0; m < 16; ++m) v[m] = alpha * ph[bsel * 16 + m];
-                            ks16(v);
+                            pfx16(v);
                             int msel = 15;
                             for (int m = 0; m < 16; ++m) {
                                 const float C = bsel == 0 ? v[m] : rq[bsel - 1] + v[m];
@@ -527,7 +547,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
                                 const int64_t j = bsel * 16 + m;
                                 p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
                             }
-                            ks16(p);
+                            pfx16(p);
                             const int64_t nvalid = nnz - bsel * 16 < 16 ? nnz - bsel * 16 : 16;
                             int64_t msel = nvalid - 1;
                             for (int64_t m = 0; m < nvalid; ++m) {
@@ -552,7 +572,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
                         } else {
                             float v[16];
                             for (int m = 0; m < 16; ++m) v[m] = alpha * ph[bsel * 16 + m];
-                            ks16(v);
+                            pfx16(v);
                             int msel = 15;
                             for (int m = 0; m < 16; ++m) {
                                 const float C = bsel == 0 ? v[m] : rq[bsel - 1] + v[m];
