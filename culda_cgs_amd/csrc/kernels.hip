@@ -535,11 +535,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
     // aligned: two groups' four prefixes are one store)
     float *rr = rs_s + rrow * a.rs_stride + 4;
     double ll = 0.0;
+    // Philox blocks [pc0, pc0 + 64), block pc0 + l's four words (top 24
+    // bits) in lane l: token g draws from word g & 3 of block g >> 2, so one
+    // evaluation per lane serves up to four batches of the slot
+    int64_t pc0 = -1;
+    uint32_t pw0 = 0, pw1 = 0, pw2 = 0, pw3 = 0;
     for (int64_t base = s.begin; base < s.end; base += kWave) {
         const int ntok = (int)((s.end - base) < kWave ? (s.end - base) : kWave);
         const bool valid = lane < ntok;
+        const int64_t g0 = base + s.gdelta;
+        if (pc0 < 0 || ((g0 + ntok - 1) >> 2) >= pc0 + kWave) {
+            pc0 = g0 >> 2;
+            const u64x4 x = philox4x64_10((uint64_t)(pc0 + lane), sweep, kPurposeSweep, 0, md.seed, 0);
+            pw0 = uniform_bits(x.v[0]);
+            pw1 = uniform_bits(x.v[1]);
+            pw2 = uniform_bits(x.v[2]);
+            pw3 = uniform_bits(x.v[3]);
+        }
+        float u;
+        {
+            const int64_t g = g0 + lane;
+            const int src = (int)(((g >> 2) - pc0) & (kWave - 1));  // < 64 for valid lanes
+            const uint32_t w0 = __shfl(pw0, src, kWave), w1 = __shfl(pw1, src, kWave);
+            const uint32_t w2 = __shfl(pw2, src, kWave), w3 = __shfl(pw3, src, kWave);
+            const uint32_t k = (uint32_t)g & 3u;
+            u = bits24_to_uniform(k == 0 ? w0 : k == 1 ? w1 : k == 2 ? w2 : w3);
+        }
         uint32_t mstart = 0, mnnz = 0, mnd = 0;
-        float u = 0.0f;
         if (valid) {
             const int64_t i = base + lane;
             const uint32_t doc = load_global(un.tokens + i);
@@ -547,7 +569,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             mstart = m.x;
             mnnz = m.y;
             mnd = m.z;
-            u = bits_to_uniform(token_bits(md.seed, (uint64_t)(i + s.gdelta), sweep, kPurposeSweep));
         }
         // bitonic sort of (valid, nnz, lane), descending: positions
         // 0..ntok-1 hold the valid tokens, longest theta rows first.  The

@@ -71,10 +71,11 @@ CULDA_HD uint64_t token_bits(uint64_t seed, uint64_t gid, uint64_t sweep, uint64
     return philox4x64_10(gid, sweep, purpose, 0, seed, 0).v[0];
 }
 
-// Uniform on (0, 1]: (top 24 bits + 1) * 2^-24, exact in FP32.
-CULDA_HD float bits_to_uniform(uint64_t x) {
-    return (float)((uint32_t)(x >> 40) + 1u) * 5.9604644775390625e-8f;
-}
+// Sweep draws: token g takes word g & 3 of Philox((g >> 2, sweep,
+// kPurposeSweep, 0), (seed, 0)), so one block serves four tokens; the
+// uniform on (0, 1] is (top 24 bits + 1) * 2^-24, exact in FP32.
+CULDA_HD uint32_t uniform_bits(uint64_t x) { return (uint32_t)(x >> 40); }
+CULDA_HD float bits24_to_uniform(uint32_t b) { return (float)(b + 1u) * 5.9604644775390625e-8f; }
 
 // Initial topic: floor(x * k / 2^64).
 CULDA_HD uint32_t bits_to_topic(uint64_t x, uint32_t k) { return (uint32_t)mulhi64(x, k); }

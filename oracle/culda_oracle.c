@@ -18,7 +18,8 @@
  *     (kernel/lda_phi_kernel.cu:76-117, kernel/lda_theta_kernel.cu:5-87),
  *     summed over chunks (kernel/lda_phi_kernel.cu:162-183);
  *   - Philox4x64-10 keyed by (seed, global token id, sweep, purpose)
- *     instead of the reference's clock-seeded curand state;
+ *     instead of the reference's clock-seeded curand state (a sweep draw
+ *     takes word g & 3 of the block of counter g >> 2);
  *   - FP32 head (n_wk + beta) / (n_k + beta V) (kernel/lda_phi_kernel.cu:26);
  *   - FP32 bucket arithmetic in a fixed order: blocks of 16 whose totals
  *     are balanced trees (last element of a Kogge-Stone scan, ks16), in-block
@@ -81,6 +82,14 @@ static inline uint64_t token_word(uint64_t seed, uint64_t gid, uint64_t sweep, u
     uint64_t c[4] = {gid, sweep, purpose, 0}, k[2] = {seed, 0}, o[4];
     oc_philox4x64_10(c, k, o);
     return o[0];
+}
+
+/* Sweep draw of token g: word g & 3 of Philox((g >> 2, sweep, 1, 0), (seed, 0)),
+ * one block per four consecutive global token ids. */
+static inline uint64_t sweep_word(uint64_t seed, uint64_t gid, uint64_t sweep) {
+    uint64_t c[4] = {gid >> 2, sweep, 1, 0}, k[2] = {seed, 0}, o[4];
+    oc_philox4x64_10(c, k, o);
+    return o[gid & 3];
 }
 
 /* ------------------------------------------------------------------ corpus */
@@ -337,7 +346,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                     const int64_t o = th_off[d];
                     const int64_t nnz = th_off[d + 1] - o;
                     const uint64_t g = (uint64_t)(cbase[(size_t)ck * V + w] + (i - b0));
-                    const float u = (float)((uint32_t)(token_word(seed, g, sweep, 1) >> 40) + 1u) * 5.9604644775390625e-8f;
+                    const float u = (float)((uint32_t)(sweep_word(seed, g, sweep) >> 40) + 1u) * 5.9604644775390625e-8f;
                     /* S: blocks of 16 */
                     const int64_t nb = (nnz + 15) / 16;
                     float R = 0.0f;
@@ -516,7 +525,7 @@ void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, u
                     const int32_t d = c->tokens[ck][i];
                     const int64_t o = th_off[d], nnz = th_off[d + 1] - o;
                     const uint64_t g = (uint64_t)(cbase[(size_t)ck * V + w] + (i - b0));
-                    const float u = (float)((uint32_t)(token_word(seed, g, sweep, 1) >> 40) + 1u) * 5.9604644775390625e-8f;
+                    const float u = (float)((uint32_t)(sweep_word(seed, g, sweep) >> 40) + 1u) * 5.9604644775390625e-8f;
                     const int64_t nb = (nnz + 15) / 16;
                     float R = 0.0f;
                     for (int64_t b = 0; b < nb; ++b) {

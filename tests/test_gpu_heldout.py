@@ -44,8 +44,10 @@ def _train(train, iters):
 
 
 def _brute_force(V, nwk, nk, theta_dense, nd, ev):
-    phi = (nwk.astype(np.float64) + BETA) / (nk.astype(np.float64) + V * BETA)
-    th = (theta_dense + ALPHA) / (nd[:, None] + K * ALPHA)
+    # alpha and beta as the library holds them (FP32), widened exactly
+    alpha, beta = float(np.float32(ALPHA)), float(np.float32(BETA))
+    phi = (nwk.astype(np.float64) + beta) / (nk.astype(np.float64) + V * beta)
+    th = (theta_dense + alpha) / (nd[:, None] + K * alpha)
     _, _, wi, wt = ev.chunk(0)
     words = np.repeat(np.arange(V), np.diff(wi))
     p = np.einsum("ik,ik->i", th[wt], phi[words])

@@ -128,7 +128,7 @@ def test_sweep_matches_exact_inverse_cdf():
     checked = 0
     for i in range(wt[0].size):
         d, w = int(wt[0][i]), int(gid_words[i])
-        x = int(oracle.philox4x64_10([i, sweep, 1, 0], [seed, 0])[0])
+        x = int(oracle.philox4x64_10([i >> 2, sweep, 1, 0], [seed, 0])[i & 3])
         u = np.float32(((x >> 40) + 1) * 2.0**-24)
         mass, topics = _conditional(om_before, d, w, alpha, beta)
         cum = np.cumsum(mass)
