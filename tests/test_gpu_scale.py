@@ -32,13 +32,17 @@ def _c2(chunks):
                               doc_topic_prior=0.1, topic_word_prior=0.01, zipf_s=1.0, seed=2, num_threads=16)
 
 
-def test_c1_full_size_bit_exact_vs_oracle():
+@pytest.mark.parametrize("slot_tokens", [0, 2048])
+def test_c1_full_size_bit_exact_vs_oracle(slot_tokens):
+    """Default slots (128 tokens at this size) and 2048-token slots, whose
+    batches run past the 256 tokens one set of Philox blocks serves."""
     import oracle
     corpus = _c1()
     info = corpus.info()
     assert 950_000 < info.num_tokens < 1_050_000
     _, end, wi, wt = corpus.chunk(0)
-    with cg.Sampler(k=1024, alpha=ALPHA, beta=BETA, seed=1) as s:
+    assert np.diff(wi).max() > 2048
+    with cg.Sampler(k=1024, alpha=ALPHA, beta=BETA, seed=1, slot_tokens=slot_tokens) as s:
         s.upload(corpus)
         s.init_z()
         s.rebuild_all()
