@@ -119,6 +119,7 @@ _SIGS = {
     "culda_load_checkpoint": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_int32)]),
     "culda_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "culda_last_sweeps": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "culda_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
     "culda_mh_enable": (C.c_int, [_P, C.POINTER(MHParams)]),
@@ -394,6 +395,13 @@ class Sampler:
         v = C.c_double()
         _check(lib().culda_sampler_bytes(self._h, C.byref(v)))
         return v.value
+
+    def last_sweeps(self):
+        """(sampler ms, algorithmic bytes, launches) of the last iterate(),
+        without waiting for the theta rebuilds it left running."""
+        ms, b, n = C.c_double(), C.c_double(), C.c_int32()
+        _check(lib().culda_last_sweeps(self._h, C.byref(ms), C.byref(b), C.byref(n)))
+        return ms.value, b.value, n.value
 
     def stream_copy(self, n_bytes, repeats=10):
         v = C.c_double()

@@ -141,7 +141,17 @@ const std::vector<int32_t> kSlotCandidates = {128, 256, 512, 1024, 2048, 4096};
 
 }  // namespace
 
+// Every call first waits for the theta rebuilds a culda_iterate may have
+// left running on the second stream (CTX_CALL_LIVE: calls that read only
+// host-side values, or order their launches after those rebuilds).
 #define CTX_CALL(...)         \
+    return guarded([&] {      \
+        Engine &e = eng(ctx); \
+        e.settle();           \
+        __VA_ARGS__;          \
+        return CULDA_OK;      \
+    })
+#define CTX_CALL_LIVE(...)    \
     return guarded([&] {      \
         Engine &e = eng(ctx); \
         (void)e;              \
@@ -247,7 +257,7 @@ extern "C" int culda_loglik(culda_ctx *ctx, double *ll_per_token) {
 }
 
 extern "C" int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds) {
-    CTX_CALL({
+    CTX_CALL_LIVE({
         if (iteration < 1) fail(CULDA_ERR_ARG, "iteration must be >= 1");
         e.iterate_units((uint32_t)iteration, ll, seconds);
         ctx->last_sweep = (uint32_t)iteration;
@@ -496,9 +506,18 @@ extern "C" int culda_get_stats(culda_ctx *ctx, culda_stats *out) {
 }
 
 extern "C" int culda_sampler_bytes(culda_ctx *ctx, double *bytes) {
-    CTX_CALL({
+    CTX_CALL_LIVE({
         if (!bytes) fail(CULDA_ERR_ARG, "null output");
         *bytes = e.last_sampler_bytes;
+    });
+}
+
+extern "C" int culda_last_sweeps(culda_ctx *ctx, double *sampler_ms, double *sampler_bytes, int32_t *launches) {
+    CTX_CALL_LIVE({
+        if (!sampler_ms || !sampler_bytes || !launches) fail(CULDA_ERR_ARG, "null output");
+        *sampler_ms = e.sampler_ms;
+        *sampler_bytes = e.last_sampler_bytes;
+        *launches = e.sampler_launches;
     });
 }
 
@@ -807,6 +826,7 @@ extern "C" int culda_mh_sketch(culda_ctx *ctx, uint16_t *out) {
 extern "C" int culda_heldout_set_eval(culda_ctx *fold, const culda_corpus *eval) {
     return guarded([&] {
         if (!eval) fail(CULDA_ERR_ARG, "null corpus");
+        eng(fold).settle();
         eng(fold).set_eval(corpus_ref(eval));
         return CULDA_OK;
     });
@@ -819,6 +839,8 @@ extern "C" int culda_heldout_ll(culda_ctx *train, culda_ctx *fold, int32_t fold_
         Engine &t = eng(train);
         Engine &f = eng(fold);
         if (t.prm.device != f.prm.device) fail(CULDA_ERR_ARG, "contexts on different devices");
+        t.settle();
+        f.settle();
         *ll_per_token = f.heldout_ll(t, fold_in_sweeps);
         return CULDA_OK;
     });

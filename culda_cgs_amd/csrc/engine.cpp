@@ -33,7 +33,10 @@ Engine::~Engine() {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ev_sampled)
         if (e) (void)hipEventDestroy(e);
-    if (ev_theta_done) (void)hipEventDestroy(ev_theta_done);
+    for (hipEvent_t &e : ev_theta_unit)
+        if (e) (void)hipEventDestroy(e);
+    for (unsigned long long *p : h_stats)
+        if (p) (void)hipHostFree(p);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
@@ -417,7 +420,10 @@ void Engine::upload() {
     hip_check(hipMemset(d_ll_hist, 0, sizeof(double) * kLLHistory * M), "ll clear");
     ev_sampled.assign(U, nullptr);
     for (hipEvent_t &e : ev_sampled) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-    hip_check(hipEventCreateWithFlags(&ev_theta_done, hipEventDisableTiming), "event");
+    ev_theta_unit.assign(U, nullptr);
+    for (hipEvent_t &e : ev_theta_unit) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    for (unsigned long long *&p : h_stats)
+        hip_check(hipHostMalloc((void **)&p, 2 * sizeof(unsigned long long) * U, hipHostMallocDefault), "stats buffer");
     uploaded = true;
     rebuild_slots(prm.slot_tokens);
     hip_check(hipDeviceSynchronize(), "upload");
@@ -666,9 +672,24 @@ void Engine::read_stats() {
     std::vector<unsigned long long> h(2 * units.size());
     hip_check(hipMemcpyAsync(h.data(), d_stats, h.size() * 8, hipMemcpyDeviceToHost, stream), "stats copy");
     hip_check(hipStreamSynchronize(stream), "stats sync");
+    take_stats(h.data());
+}
+
+void Engine::take_stats(const unsigned long long *h) {
     for (size_t u = 0; u < units.size(); ++u) {
         unit_doc_nnz[u] = (int64_t)h[2 * u];
         unit_token_nnz[u] = (int64_t)h[2 * u + 1];
+    }
+}
+
+void Engine::settle() {
+    if (stream2_busy) {
+        hip_check(hipStreamSynchronize(stream2), "sync");
+        stream2_busy = false;
+    }
+    if (stats_pending >= 0) {
+        take_stats(h_stats[stats_pending]);
+        stats_pending = -1;
     }
 }
 
@@ -705,49 +726,57 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     const bool prof = prm.profile != 0;
     if (prof && !ev[0])
         for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-    const double bytes = sweep_bytes();  // theta entries read by this sweep: the last rebuild's
     auto t0 = std::chrono::steady_clock::now();
     set_sweep(sweep, stream);
     // stream: sweeps of all chunks, LL, phi + head denominators.
     // stream2: theta of chunk c as soon as chunk c's sweep is done, so the
     // (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta
-    // of chunk c is read only by chunk c's sweep, which has finished.
+    // of chunk c is read only by chunk c's sweep: the last chunks' rebuilds
+    // also overlap the next iteration, whose sweep of chunk c waits for
+    // ev_theta_unit[c] only.  A theta rebuild reads z of its own chunk, which
+    // no other launch of the iteration writes.
+    const bool after_iteration = stream2_busy;
     if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
-    hip_check(hipStreamWaitEvent(stream2, ev_theta_done, 0), "wait");
     for (int32_t u = 0; u < U; ++u) {
+        if (after_iteration) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[u], 0), "wait");
         sample_unit(u, stream);
         hip_check(hipEventRecord(ev_sampled[u], stream), "event");
         hip_check(hipStreamWaitEvent(stream2, ev_sampled[u], 0), "wait");
         rebuild_theta_unit(u, stream2);
+        if (u + 1 < U) hip_check(hipEventRecord(ev_theta_unit[u], stream2), "event");
     }
+    // the rebuilds' statistics, copied before the last unit's event: the
+    // next iteration's last sweep waits for it, so they are complete when
+    // that iteration's stream is synchronised
+    const int32_t slot = stats_pending == 0 ? 1 : 0;
+    hip_check(hipMemcpyAsync(h_stats[slot], d_stats, 2 * sizeof(unsigned long long) * U, hipMemcpyDeviceToHost, stream2),
+              "stats copy");
+    hip_check(hipEventRecord(ev_theta_unit[U - 1], stream2), "event");
     if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
     ll_reduce(stream);
     update_phi(stream);
     phi_head(stream);
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
-    hip_check(hipEventRecord(ev_theta_done, stream2), "event");
-    hip_check(hipStreamWaitEvent(stream, ev_theta_done, 0), "wait");
-    if (prof) hip_check(hipEventRecord(ev[3], stream), "event");
     double h_ll = 0;
-    std::vector<unsigned long long> h_st(2 * U);
     hip_check(hipMemcpyAsync(&h_ll, d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
                              stream),
               "ll copy");
-    hip_check(hipMemcpyAsync(h_st.data(), d_stats, h_st.size() * 8, hipMemcpyDeviceToHost, stream), "stats copy");
     hip_check(hipStreamSynchronize(stream), "iteration sync");
     auto t1 = std::chrono::steady_clock::now();
-    for (int32_t u = 0; u < U; ++u) {
-        unit_doc_nnz[u] = (int64_t)h_st[2 * u];
-        unit_token_nnz[u] = (int64_t)h_st[2 * u + 1];
-    }
+    // statistics of the rebuild this sweep read: the previous iteration's
+    // (complete: this iteration's last sweep waited for their copy), or
+    // current already
+    if (stats_pending >= 0) take_stats(h_stats[stats_pending]);
+    const double bytes = sweep_bytes();
+    stats_pending = slot;
+    stream2_busy = true;
     if (prof) {
-        float a = 0, b = 0, c = 0;
+        float a = 0, b = 0;
         hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
         hip_check(hipEventElapsedTime(&b, ev[1], ev[2]), "elapsed");
-        hip_check(hipEventElapsedTime(&c, ev[2], ev[3]), "elapsed");
         sampler_ms = a;
         phi_ms = b;
-        theta_ms = c;
+        theta_ms = 0;  // the rebuilds overlap the sweeps and the next iteration
     }
     sampler_launches = U;
     last_sampler_bytes = bytes;

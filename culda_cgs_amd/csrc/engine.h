@@ -138,7 +138,14 @@ public:
 
     // profiling / stats
     std::vector<hipEvent_t> ev_sampled;
-    hipEvent_t ev_theta_done = nullptr;
+    std::vector<hipEvent_t> ev_theta_unit;  // per-chunk iteration: theta of unit u rebuilt (stream2)
+    // The per-chunk iteration returns with the last chunks' theta rebuilds
+    // still running on stream2; their statistics land in h_stats[slot]
+    // (pinned, written by a copy on stream2) and are read one iteration
+    // later or by settle().
+    bool stream2_busy = false;
+    int32_t stats_pending = -1;  // slot of h_stats holding stats not yet read, or -1
+    unsigned long long *h_stats[2] = {nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // batched iteration: theta on stream2
     hipEvent_t ev[4] = {};
     double sampler_ms = 0, phi_ms = 0, theta_ms = 0, iter_ms = 0;
@@ -172,6 +179,10 @@ public:
     void sample_all(hipStream_t s);
     void ll_reduce(hipStream_t s);
     void read_stats();
+    void take_stats(const unsigned long long *h);
+    // wait for the theta rebuilds a per-chunk iteration left on stream2 and
+    // read their statistics (before anything else touches device state)
+    void settle();
     void rebuild_all();
 
     // culda_ctx iteration: per-chunk sweeps, theta of chunk c overlapped

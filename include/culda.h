@@ -220,8 +220,11 @@ int culda_sample(culda_ctx *ctx, int32_t chunk, int32_t iteration);
 int culda_loglik(culda_ctx *ctx, double *ll_per_token);
 
 /* One full iteration: sample every chunk, LL, phi rebuild + head, theta
- * rebuild, synchronise.  ll = predictive LL/token of the pre-sweep model,
- * seconds = wall time of the whole iteration.  Replaces one pass of the
+ * rebuild.  ll = predictive LL/token of the pre-sweep model, seconds = wall
+ * time until the LL is known.  The last chunks' theta rebuilds may still be
+ * running when it returns: the next culda_iterate orders each chunk's sweep
+ * after that chunk's rebuild, and every other call (culda_sampler_bytes and
+ * culda_last_sweeps aside) waits for them first.  Replaces one pass of the
  * loop body train/SingleChunkSingleGPU.h:70-127 (and, for C > 1, of
  * train/MultiChunkMultiGPUequal.h:88-169 on one device). */
 int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds);
@@ -276,8 +279,9 @@ typedef struct culda_stats {
     int64_t theta_capacity;     /* theta slab entries over all chunks           */
     double  sampler_ms;         /* last iteration: the chunks' sweeps (profile=1) */
     double  phi_ms;             /* LL reduce + phi rebuild + head denominators    */
-    double  theta_ms;           /* wait for the theta rebuilds, which run on a
-                                   second stream overlapped with the sweeps     */
+    double  theta_ms;           /* 0: the theta rebuilds run on a second stream,
+                                   overlapped with the sweeps and the next
+                                   iteration                                    */
     double  other_ms;
     int32_t sampler_launches;   /* launches in the last iteration               */
     int32_t pad;
@@ -289,6 +293,12 @@ int culda_get_stats(culda_ctx *ctx, culda_stats *out);
  * iteration (doc id + meta + z write + theta entries gathered + count rows
  * staged), summed over chunks. */
 int culda_sampler_bytes(culda_ctx *ctx, double *bytes);
+
+/* The last culda_iterate's sampler time (HIP events; params->profile = 1,
+ * else 0), algorithmic bytes (as culda_sampler_bytes) and launch count,
+ * without waiting for the theta rebuilds it left running (no counterpart
+ * in the reference; for per-iteration measurement loops). */
+int culda_last_sweeps(culda_ctx *ctx, double *sampler_ms, double *sampler_bytes, int32_t *launches);
 
 /* Autotuner result: the sampler's tokens per wave slot (and, for batches,
  * the persistent grid) chosen by timing one sweep per candidate from the

@@ -294,3 +294,36 @@ def test_random_configurations_bit_exact(seed):
     assert np.array_equal(nwk.ravel(), om.nwk) and np.array_equal(nk, om.nk)
     _theta_equal(s, om, arrays)
     s.close()
+
+
+@pytest.mark.parametrize("chunks", [1, 4])
+def test_back_to_back_iterations_equal_synchronised_ones(chunks):
+    """culda_iterate returns with the last chunks' theta rebuilds still on the
+    second stream; the next iteration orders each chunk's sweep after that
+    chunk's rebuild.  Back-to-back iterations (nothing in between that waits)
+    must give the chain of iterations each followed by a full sync, and the
+    per-iteration sampler bytes must be the ones of the rebuild each sweep
+    read."""
+    V, K, arrays = _setup(31, V=500, D=400, mean_len=90, chunks=chunks, K=128)
+    alpha, beta, seed = 0.2, 0.01, 17
+    live, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
+    synced, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
+    for s in (live, synced):
+        s.init_z()
+        s.rebuild_all()
+    ll_live, ll_synced, b_live, b_synced = [], [], [], []
+    for it in range(1, 7):
+        ll_live.append(live.iterate(it)[0])
+        ms, b, n = live.last_sweeps()
+        assert n == chunks and ms > 0 and b == live.sampler_bytes()
+        b_live.append(b)
+        ll_synced.append(synced.iterate(it)[0])
+        synced.sync()
+        b_synced.append(synced.sampler_bytes())
+    assert ll_live == ll_synced and b_live == b_synced
+    for c in range(chunks):
+        assert np.array_equal(live.get_z(c), synced.get_z(c))
+        assert all(np.array_equal(x, y) for x, y in zip(live.get_theta(c), synced.get_theta(c)))
+    assert live.stats().sum_token_nnz == synced.stats().sum_token_nnz
+    live.close()
+    synced.close()
