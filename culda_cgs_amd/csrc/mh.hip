@@ -93,30 +93,7 @@ __device__ __forceinline__ float wave_max_f(float x) {
     return x;
 }
 
-// 64-lane inclusive scans with DPP (row shifts inside each 16-lane row,
-// then row_bcast:15 / row_bcast:31 across rows): a few cycles per step
-// instead of an LDS round trip per shuffle.  `id` fills lanes without a
-// source.  wave_*_all returns the total to every lane.
-template <int CTRL, int RM, bool BC>
-__device__ __forceinline__ uint32_t dppu(uint32_t old, uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, RM, 0xf, BC);
-}
-
-template <class Op>
-__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x, uint32_t id, Op op) {
-    x = op(x, dppu<0x111, 0xf, false>(id, x));
-    x = op(x, dppu<0x112, 0xf, false>(id, x));
-    x = op(x, dppu<0x114, 0xf, false>(id, x));
-    x = op(x, dppu<0x118, 0xf, false>(id, x));
-    x = op(x, dppu<0x142, 0xa, false>(id, x));
-    x = op(x, dppu<0x143, 0xc, false>(id, x));
-    return x;
-}
-
-__device__ __forceinline__ uint32_t wave_add_scan_u32(uint32_t x) {
-    return wave_scan_u32(x, 0u, [](uint32_t a, uint32_t b) { return a + b; });
-}
-
+// wave_*_all: the total of a 64-lane DPP scan (device.h) in every lane.
 __device__ __forceinline__ uint32_t wave_add_all_u32(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readlane((int)wave_add_scan_u32(x), kWave - 1);
 }
