@@ -340,7 +340,7 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 // above kThetaMaxCount takes several entries (kThetaMaxCount each, the
 // remainder last), placed by a wave prefix sum (a ballot when no topic of
 // the 64 needs more than one).  The histogram is cleared once; afterwards
-// the compaction clears each counter it reads.
+// the compaction reads each lane's 16 counters at once and clears them.
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
@@ -369,11 +369,21 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
                 if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
         }
         wave_sync();
+        // the lane's 16 counters (topics r * 64 + lane < K; the rest of
+        // the array is never cleared) read at once, then cleared: one LDS
+        // round trip instead of one per 64 topics
+        uint32_t hv[kMaxK / kWave];
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r) hv[r] = r * kWave + lane < K ? h[r * kWave + lane] : 0u;
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r)
+            if (hv[r]) h[r * kWave + lane] = 0u;
         uint32_t base = 0;
-        for (int k0 = 0; k0 < K; k0 += kWave) {
-            const int k = k0 + lane;
-            const uint32_t v = k < K ? h[k] : 0u;
-            if (v) h[k] = 0u;
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r) {
+            if (r * kWave >= K) break;
+            const int k = r * kWave + lane;
+            const uint32_t v = hv[r];
             if (__ballot(v > kThetaMaxCount) == 0ull) {
                 const unsigned long long msk = __ballot(v != 0);
                 const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
