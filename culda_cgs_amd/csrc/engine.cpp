@@ -731,10 +731,10 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     // stream: sweeps of all chunks, LL, phi + head denominators.
     // stream2: theta of chunk c as soon as chunk c's sweep is done, so the
     // (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta
-    // of chunk c is read only by chunk c's sweep: the last chunks' rebuilds
-    // also overlap the next iteration, whose sweep of chunk c waits for
-    // ev_theta_unit[c] only.  A theta rebuild reads z of its own chunk, which
-    // no other launch of the iteration writes.
+    // of chunk c is read only by chunk c's sweep: the last chunk's rebuild
+    // also overlaps the next iteration, whose sweep of that chunk waits for
+    // ev_theta_unit[c].  A theta rebuild reads z of its own chunk, which no
+    // other launch of the iteration writes.
     const bool after_iteration = stream2_busy;
     if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
     for (int32_t u = 0; u < U; ++u) {
@@ -756,6 +756,11 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     ll_reduce(stream);
     update_phi(stream);
     phi_head(stream);
+    // all rebuilds but the last one before the next iteration's sweeps:
+    // they run beside the sweeps and may fall behind (many chunks, short
+    // sweeps); here they catch up while nothing else competes, so their
+    // lag cannot carry from one iteration to the next
+    if (U >= 2) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[U - 2], 0), "wait");
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
     double h_ll = 0;
     hip_check(hipMemcpyAsync(&h_ll, d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,

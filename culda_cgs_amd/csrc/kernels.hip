@@ -339,8 +339,7 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 // ascending compaction of the nonzero topics into 16-bit entries; a count
 // above kThetaMaxCount takes several entries (kThetaMaxCount each, the
 // remainder last), placed by a wave prefix sum (a ballot when no topic of
-// the 64 needs more than one).  The histogram is cleared once; afterwards
-// the compaction reads each lane's 16 counters at once and clears them.
+// the 64 needs more than one).
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
@@ -350,12 +349,11 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     const int K = m.k;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
-    for (int k = 4 * lane; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0, 0, 0, 0);
     unsigned long long s_nnz = 0, s_tok = 0;
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
+        for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
         const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
-        const uint32_t start = un.meta[d].start;  // issued before the histogram's waits
         // four index loads, then four z gathers, in flight per lane
         for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
             uint32_t t[4];
@@ -369,21 +367,11 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
                 if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
         }
         wave_sync();
-        // the lane's 16 counters (topics r * 64 + lane < K; the rest of
-        // the array is never cleared) read at once, then cleared: one LDS
-        // round trip instead of one per 64 topics
-        uint32_t hv[kMaxK / kWave];
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r) hv[r] = r * kWave + lane < K ? h[r * kWave + lane] : 0u;
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r)
-            if (hv[r]) h[r * kWave + lane] = 0u;
+        const uint32_t start = un.meta[d].start;
         uint32_t base = 0;
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r) {
-            if (r * kWave >= K) break;
-            const int k = r * kWave + lane;
-            const uint32_t v = hv[r];
+        for (int k0 = 0; k0 < K; k0 += kWave) {
+            const int k = k0 + lane;
+            const uint32_t v = k < K ? h[k] : 0u;
             if (__ballot(v > kThetaMaxCount) == 0ull) {
                 const unsigned long long msk = __ballot(v != 0);
                 const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
@@ -391,13 +379,18 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
                 base += (uint32_t)__popcll(msk);
             } else {
                 const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
-                const uint32_t incl = wave_add_scan_u32(ne);
+                uint32_t incl = ne;
+#pragma unroll
+                for (int d2 = 1; d2 < kWave; d2 <<= 1) {
+                    const uint32_t y = __shfl_up(incl, d2, kWave);
+                    if (lane >= d2) incl += y;
+                }
                 const uint32_t pos = base + incl - ne;
                 for (uint32_t q = 0; q < ne; ++q) {
                     const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
                     un.theta[start + pos + q] = (uint16_t)((c << kThetaCountShift) | (uint32_t)k);
                 }
-                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+                base += __shfl(incl, kWave - 1, kWave);
             }
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
