@@ -246,8 +246,8 @@ void Engine::upload() {
             u.rev_idx = upload_vec(*this, rev_idx);
             u.meta = upload_vec(*this, meta);
             // kThetaPad entries of tail padding: the sampler loads whole
-            // 64-entry groups and may read past the last slab (those values
-            // are unused)
+            // 32-entry groups and may read up to 16 entries past the last
+            // slab (those values are unused)
             u.theta = (uint16_t *)dalloc((cap + kThetaPad) * sizeof(uint16_t));
             hip_check(hipMemset(u.theta, 0, (cap + kThetaPad) * sizeof(uint16_t)), "theta clear");
             units.push_back(std::move(u));
