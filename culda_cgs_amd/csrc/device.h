@@ -31,6 +31,17 @@ __device__ __forceinline__ u32x2 load_global2(const uint32_t *p) { return *(cons
 __device__ __forceinline__ u32x4 load_global4(const void *p) { return *(const CULDA_AS1 u32x4 *)p; }
 __device__ __forceinline__ uint32_t load_global_u16(const uint16_t *p) { return *(const CULDA_AS1 uint16_t *)p; }
 __device__ __forceinline__ void store_global_u16(uint16_t *p, uint16_t v) { *(CULDA_AS1 uint16_t *)p = v; }
+// The same at base + a 32-bit byte offset (base wave-uniform): the address
+// is the scalar base plus a 32-bit vector offset, no 64-bit vector math.
+__device__ __forceinline__ u32x2 load_global2_at(const void *base, uint32_t off) {
+    return *(const CULDA_AS1 u32x2 *)((const char *)base + off);
+}
+__device__ __forceinline__ uint32_t load_global_at(const void *base, uint32_t off) {
+    return *(const CULDA_AS1 uint32_t *)((const char *)base + off);
+}
+__device__ __forceinline__ uint32_t load_global_u16_at(const void *base, uint32_t off) {
+    return *(const CULDA_AS1 uint16_t *)((const char *)base + off);
+}
 
 // DPP move with zero for lanes whose source is outside the 16-lane row.
 template <int CTRL>

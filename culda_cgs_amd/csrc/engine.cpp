@@ -233,7 +233,9 @@ void Engine::upload() {
                 const int64_t nd = m.doc_len[ch.doc_start + d];
                 meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
                 cap += (theta_entries_max(nd, K) + 15) / 16 * 16;
-                if (cap + kThetaPad >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^32 entries");
+                // the sampler addresses a chunk's theta with 32-bit byte offsets
+                if (cap + kThetaPad >= ((int64_t)1 << 31))
+                    fail(CULDA_ERR_LIMIT, "a chunk's theta capacity exceeds 2^31 entries: use more data_chunks");
             }
             u.theta_cap = cap;
             u.word_idx_h = ch.word_idx;

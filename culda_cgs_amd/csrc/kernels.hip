@@ -610,7 +610,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // nothing reads.
                 const uint32_t nb = (nnz + 15u) >> 4;
                 const uint32_t ng = (nb + 1u) >> 1;
-                const uint32_t *tp = reinterpret_cast<const uint32_t *>(theta + start) + 2 * lr;
+                // byte offset of this lane's entries in the first group (a
+                // chunk's theta holds fewer than 2^31 entries)
+                const uint32_t tp = (start + 4u * (uint32_t)lr) * 2u;
                 auto group = [&](u32x2 d, float R0, float &r0, float &r1) {
                     const float f0 = PHI_LO(d.x), f1 = PHI_HI(d.x), f2 = PHI_LO(d.y), f3 = PHI_HI(d.y);
                     const float c0 = COUNT_LO(d.x), c1 = COUNT_HI(d.x), c2 = COUNT_LO(d.y), c3 = COUNT_HI(d.y);
@@ -622,20 +624,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 };
                 // two groups per pass, each loaded while the one before is
                 // summed (no register copy between passes)
-                u32x2 ca = load_global2(tp);
+                u32x2 ca = load_global2_at(theta, tp);
                 float R = 0.0f, r0, r1;
                 uint32_t g = 0;
                 for (; g + 2 < ng; g += 2) {
-                    const u32x2 cb = load_global2(tp + ((g + 1) << 4));
+                    const u32x2 cb = load_global2_at(theta, tp + ((g + 1) << 6));
                     float r2, r3;
                     group(ca, R, r0, r1);
-                    ca = load_global2(tp + ((g + 2) << 4));
+                    ca = load_global2_at(theta, tp + ((g + 2) << 6));
                     group(cb, r1, r2, r3);
                     if (lr == 7) *reinterpret_cast<float4 *>(rr + (g << 1)) = make_float4(r0, r1, r2, r3);
                     R = r3;
                 }
                 if (g + 1 < ng) {
-                    const u32x2 cb = load_global2(tp + ((g + 1) << 4));
+                    const u32x2 cb = load_global2_at(theta, tp + ((g + 1) << 6));
                     group(ca, R, r0, r1);
                     if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
                     R = r1;
@@ -678,7 +680,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 }
                 uint32_t topic;
                 if (bsel < 0) {
-                    topic = sb ? (load_global_u16(theta + start + nnz - 1) & kThetaTopicMask) : (uint32_t)(K - 1);
+                    topic = sb ? (load_global_u16_at(theta, (start + nnz - 1u) * 2u) & kThetaTopicMask) : (uint32_t)(K - 1);
                 } else {
                     // the block's 16 products, two per lane (entries j, j + 1);
                     // entries past nnz are zero-count padding (exact zero
@@ -688,7 +690,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                     uint32_t e2 = 0;
                     float vlo, vhi;
                     if (sb) {
-                        e2 = load_global(reinterpret_cast<const uint32_t *>(theta + start + j));
+                        e2 = load_global_at(theta, (start + j) * 2u);
                         vlo = COUNT_LO(e2) * PHI_LO(e2);
                         vhi = COUNT_HI(e2) * PHI_HI(e2);
                     } else {
