@@ -634,13 +634,14 @@ void Engine::phi_head(hipStream_t s) {
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
     hip_check(launch_fill_u32((uint32_t *)(d_stats + 2 * ui), 4, 0, s), "stats clear");
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
-    hip_check(launch_theta_rebuild(a, s), "theta rebuild launch");
+    // a single chunk's rebuild has nothing beside it: the next sweep waits
+    hip_check(launch_theta_rebuild(a, units.size() == 1, s), "theta rebuild launch");
 }
 
 void Engine::rebuild_theta_all(hipStream_t s) {
     hip_check(launch_fill_u32((uint32_t *)d_stats, 4 * (int64_t)units.size(), 0, s), "stats clear");
     ThetaArgs a{d_theta_packets_all, d_units, d_models, num_theta_packets_all, 0};
-    hip_check(launch_theta_rebuild(a, s), "theta rebuild launch");
+    hip_check(launch_theta_rebuild(a, false, s), "theta rebuild launch");
     theta_ready = true;
 }
 

@@ -15,7 +15,8 @@ hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
 hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s);
 hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, int32_t num_words, int32_t k,
                            float beta, hipStream_t s);
-hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s);
+// alone: nothing else runs beside it (a single-chunk model), see theta_rebuild_alone_kernel
+hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, hipStream_t s);
 // grid is used in persistent mode (a.queue != null) only
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s);

@@ -406,6 +406,82 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     }
 }
 
+// The same rebuild for a launch that runs alone (a single-chunk model,
+// whose next sweep waits for it): the histogram is cleared once, and each
+// doc's compaction reads all of a lane's counters in one LDS round trip and
+// clears them -- shorter per doc, but at 78 VGPRs it does not fit beside
+// the sampler's waves, as the per-chunk rebuilds of larger models must.
+__global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    const Packet pk = a.packets[blockIdx.x];
+    const UnitDev &un = a.units[pk.owner];
+    const ModelDev &m = a.models[un.model];
+    const int K = m.k;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *h = hist_s[wave];
+    for (int k = 4 * lane; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0, 0, 0, 0);
+    unsigned long long s_nnz = 0, s_tok = 0;
+    for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
+        wave_sync();
+        const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
+        const uint32_t start = un.meta[d].start;  // issued before the histogram's waits
+        // four index loads, then four z gathers, in flight per lane
+        for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
+            uint32_t t[4];
+            uint16_t zz[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? un.rev_idx[i0 + q * kWave] : 0xffffffffu;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? un.z[t[q]] : (uint16_t)0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
+        }
+        wave_sync();
+        // the lane's 16 counters (topics r * 64 + lane < K; the rest of
+        // the array is never cleared) read at once, then cleared: one LDS
+        // round trip instead of one per 64 topics
+        uint32_t hv[kMaxK / kWave];
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r) hv[r] = r * kWave + lane < K ? h[r * kWave + lane] : 0u;
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r)
+            if (hv[r]) h[r * kWave + lane] = 0u;
+        uint32_t base = 0;
+#pragma unroll
+        for (int r = 0; r < kMaxK / kWave; ++r) {
+            if (r * kWave >= K) break;
+            const int k = r * kWave + lane;
+            const uint32_t v = hv[r];
+            if (__ballot(v > kThetaMaxCount) == 0ull) {
+                const unsigned long long msk = __ballot(v != 0);
+                const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                if (v) un.theta[start + pos] = (uint16_t)((v << kThetaCountShift) | (uint32_t)k);
+                base += (uint32_t)__popcll(msk);
+            } else {
+                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
+                const uint32_t incl = wave_add_scan_u32(ne);
+                const uint32_t pos = base + incl - ne;
+                for (uint32_t q = 0; q < ne; ++q) {
+                    const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
+                    un.theta[start + pos + q] = (uint16_t)((c << kThetaCountShift) | (uint32_t)k);
+                }
+                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+            }
+        }
+        // zero the slab tail up to the next multiple of 16 (read by the sampler)
+        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
+        if (lane == 0) un.meta[d].nnz = base;
+        s_nnz += base;
+        s_tok += (unsigned long long)base * (e - b);
+        wave_sync();
+    }
+    if (lane == 0 && s_nnz) {
+        atomicAdd(&un.stats[0], s_nnz);
+        atomicAdd(&un.stats[1], s_tok);
+    }
+}
+
 // Bitonic sort of the keys of lanes [0, M) of each M-lane group into
 // descending order (the last stage's direction is the same for every lane
 // below M).
@@ -889,9 +965,12 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
     return hipGetLastError();
 }
 
-hipError_t launch_theta_rebuild(const ThetaArgs &a, hipStream_t s) {
+hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(theta_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    if (alone)
+        hipLaunchKernelGGL(theta_rebuild_alone_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(theta_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
