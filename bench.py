@@ -111,27 +111,22 @@ def main():
     log(f"stream copy {copy_gbps:.0f} GB/s")
 
     it = 1
-    for _ in range(args.warmup):
-        s.iterate(it)
-        it += 1
+    if args.warmup:
+        s.iterate_n(it, args.warmup)
+        it += args.warmup
     s.sync()
-    samp_ms, samp_bytes, iter_s, lls = [], [], [], []
+    # the timed steps: one call enqueues them back to back (no host round
+    # trip between iterations); the LL of every step comes back at the end
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ll, sec = s.iterate(it)
-        ms, b, _ = s.last_sweeps()  # waits for nothing: the theta rebuilds overlap the next iteration
-        samp_ms.append(ms)
-        samp_bytes.append(b)
-        iter_s.append(sec)
-        lls.append(ll)
-        it += 1
+    lls, _ = s.iterate_n(it, args.steps)
+    samp_ms_total, samp_bytes_total, launches = s.last_sweeps()  # sums over the timed steps
     s.sync()
     elapsed = time.perf_counter() - t_start
+    it += args.steps
     st = s.stats()
     tokens_per_s = N * args.steps / elapsed
-    launches = st.sampler_launches
-    avg_launch_ms = float(np.sum(samp_ms)) / (len(samp_ms) * launches)
-    bytes_per_launch = float(np.sum(samp_bytes)) / (len(samp_bytes) * launches)
+    avg_launch_ms = samp_ms_total / launches
+    bytes_per_launch = samp_bytes_total / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     log(f"{tokens_per_s / 1e6:.1f} M tokens/s, LL/token {lls[-1]:.6f}, sampler {avg_launch_ms:.3f} ms/launch, "
         f"{achieved:.0f} GB/s algorithmic")
@@ -289,24 +284,18 @@ def run_large(cg, args):
     fold.set_eval(ev)
     ll0 = fold.heldout_ll(s, args.fold_in)
     it = 1
-    for _ in range(args.warmup):
-        s.iterate(it)
-        it += 1
+    if args.warmup:
+        s.iterate_n(it, args.warmup)
+        it += args.warmup
     s.sync()
-    samp_ms, samp_bytes, lls = [], [], []
     t_start = time.perf_counter()
-    for _ in range(args.steps):
-        ll, _ = s.iterate(it)
-        ms, b, _ = s.last_sweeps()
-        samp_ms.append(ms)
-        samp_bytes.append(b)
-        lls.append(ll)
-        it += 1
+    lls, _ = s.iterate_n(it, args.steps)
+    samp_ms_total, samp_bytes_total, launches = s.last_sweeps()
     s.sync()
     elapsed = time.perf_counter() - t_start
-    launches = s.stats().sampler_launches
-    avg_launch_ms = float(np.sum(samp_ms)) / (len(samp_ms) * launches)
-    bytes_per_launch = float(np.sum(samp_bytes)) / (len(samp_bytes) * launches)
+    it += args.steps
+    avg_launch_ms = samp_ms_total / launches
+    bytes_per_launch = samp_bytes_total / launches
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     hll = fold.heldout_ll(s, args.fold_in)
     log(f"{N * args.steps / elapsed / 1e6:.1f} M tokens/s, {elapsed / args.steps * 1e3:.1f} ms/iteration, "

@@ -120,6 +120,7 @@ _SIGS = {
     "culda_get_stats": (C.c_int, [_P, C.POINTER(Stats)]),
     "culda_sampler_bytes": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "culda_last_sweeps": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    "culda_iterate_n": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "culda_stream_copy": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_double)]),
     "culda_autotune": (C.c_int, [_P, C.POINTER(Tune)]),
     "culda_mh_enable": (C.c_int, [_P, C.POINTER(MHParams)]),
@@ -328,6 +329,13 @@ class Sampler:
         ll, sec = C.c_double(), C.c_double()
         _check(lib().culda_iterate(self._h, iteration, C.byref(ll), C.byref(sec)))
         return ll.value, sec.value
+
+    def iterate_n(self, first, count):
+        """count iterations from sweep `first` with no host round trip between
+        them: (LL/token per iteration, seconds)."""
+        ll, sec = (C.c_double * count)(), C.c_double()
+        _check(lib().culda_iterate_n(self._h, first, count, ll, C.byref(sec)))
+        return list(ll), sec.value
 
     def sync(self):
         _check(lib().culda_sync(self._h))

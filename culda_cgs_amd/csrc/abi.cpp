@@ -264,6 +264,14 @@ extern "C" int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, doub
     });
 }
 
+extern "C" int culda_iterate_n(culda_ctx *ctx, int32_t first_iteration, int32_t count, double *ll, double *seconds) {
+    CTX_CALL_LIVE({
+        if (first_iteration < 1) fail(CULDA_ERR_ARG, "iteration must be >= 1");
+        e.iterate_units_n((uint32_t)first_iteration, count, ll, seconds);
+        ctx->last_sweep = (uint32_t)(first_iteration + count - 1);
+    });
+}
+
 extern "C" int culda_rebuild_all(culda_ctx *ctx) { CTX_CALL(e.rebuild_all()); }
 
 extern "C" int culda_sync(culda_ctx *ctx) { CTX_CALL(e.sync()); }

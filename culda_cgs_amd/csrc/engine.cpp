@@ -35,8 +35,9 @@ Engine::~Engine() {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ev_theta_unit)
         if (e) (void)hipEventDestroy(e);
-    for (unsigned long long *p : h_stats)
-        if (p) (void)hipHostFree(p);
+    if (h_stats_ring) (void)hipHostFree(h_stats_ring);
+    for (hipEvent_t &e : ev_iter)
+        if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
@@ -424,8 +425,9 @@ void Engine::upload() {
     for (hipEvent_t &e : ev_sampled) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     ev_theta_unit.assign(U, nullptr);
     for (hipEvent_t &e : ev_theta_unit) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-    for (unsigned long long *&p : h_stats)
-        hip_check(hipHostMalloc((void **)&p, 2 * sizeof(unsigned long long) * U, hipHostMallocDefault), "stats buffer");
+    hip_check(hipHostMalloc((void **)&h_stats_ring, (size_t)kStatsSlots * 2 * sizeof(unsigned long long) * U,
+                            hipHostMallocDefault),
+              "stats buffer");
     uploaded = true;
     rebuild_slots(prm.slot_tokens);
     hip_check(hipDeviceSynchronize(), "upload");
@@ -691,7 +693,7 @@ void Engine::settle() {
         stream2_busy = false;
     }
     if (stats_pending >= 0) {
-        take_stats(h_stats[stats_pending]);
+        take_stats(stats_slot(stats_pending));
         stats_pending = -1;
     }
 }
@@ -722,40 +724,36 @@ double Engine::model_ll(int32_t mi, uint32_t sweep) {
     return h / (double)models[mi].num_tokens;
 }
 
-void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
-    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
-    if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
+// One per-chunk iteration enqueued on the two streams.
+// stream: sweeps of all chunks, LL, phi + head denominators.
+// stream2: theta of chunk c as soon as chunk c's sweep is done, so the
+// (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta of
+// chunk c is read only by chunk c's sweep: the last chunk's rebuild also
+// overlaps the next iteration, whose sweep of that chunk waits for
+// ev_theta_unit[c].  A theta rebuild reads z of its own chunk, which no
+// other launch of the iteration writes.  The rebuilds' statistics go to
+// the pinned stats_dst; ev_a / ev_b (profiling) bracket the sweeps.
+void Engine::enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_dst, hipEvent_t ev_a, hipEvent_t ev_b) {
     const int32_t U = (int32_t)units.size();
-    const bool prof = prm.profile != 0;
-    if (prof && !ev[0])
-        for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-    auto t0 = std::chrono::steady_clock::now();
     set_sweep(sweep, stream);
-    // stream: sweeps of all chunks, LL, phi + head denominators.
-    // stream2: theta of chunk c as soon as chunk c's sweep is done, so the
-    // (gather-bound) theta rebuilds overlap the next chunks' sweeps.  Theta
-    // of chunk c is read only by chunk c's sweep: the last chunk's rebuild
-    // also overlaps the next iteration, whose sweep of that chunk waits for
-    // ev_theta_unit[c].  A theta rebuild reads z of its own chunk, which no
-    // other launch of the iteration writes.
     const bool after_iteration = stream2_busy;
-    if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
     for (int32_t u = 0; u < U; ++u) {
         if (after_iteration) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[u], 0), "wait");
+        // the sweeps' time starts once the first one may run
+        if (u == 0 && ev_a) hip_check(hipEventRecord(ev_a, stream), "event");
         sample_unit(u, stream);
         hip_check(hipEventRecord(ev_sampled[u], stream), "event");
         hip_check(hipStreamWaitEvent(stream2, ev_sampled[u], 0), "wait");
         rebuild_theta_unit(u, stream2);
         if (u + 1 < U) hip_check(hipEventRecord(ev_theta_unit[u], stream2), "event");
     }
-    // the rebuilds' statistics, copied before the last unit's event: the
-    // next iteration's last sweep waits for it, so they are complete when
-    // that iteration's stream is synchronised
-    const int32_t slot = stats_pending == 0 ? 1 : 0;
-    hip_check(hipMemcpyAsync(h_stats[slot], d_stats, 2 * sizeof(unsigned long long) * U, hipMemcpyDeviceToHost, stream2),
+    // the statistics are copied before the last unit's event: the next
+    // iteration's last sweep waits for it, so they are complete when that
+    // iteration's stream is synchronised
+    hip_check(hipMemcpyAsync(stats_dst, d_stats, 2 * sizeof(unsigned long long) * U, hipMemcpyDeviceToHost, stream2),
               "stats copy");
     hip_check(hipEventRecord(ev_theta_unit[U - 1], stream2), "event");
-    if (prof) hip_check(hipEventRecord(ev[1], stream), "event");
+    if (ev_b) hip_check(hipEventRecord(ev_b, stream), "event");
     ll_reduce(stream);
     update_phi(stream);
     phi_head(stream);
@@ -764,6 +762,20 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     // sweeps); here they catch up while nothing else competes, so their
     // lag cannot carry from one iteration to the next
     if (U >= 2) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[U - 2], 0), "wait");
+    stream2_busy = true;
+}
+
+void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
+    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
+    const int32_t U = (int32_t)units.size();
+    const bool prof = prm.profile != 0;
+    if (prof && !ev[0])
+        for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    auto t0 = std::chrono::steady_clock::now();
+    const int32_t slot = stats_next;
+    stats_next = (stats_next + 1) % kStatsSlots;
+    enqueue_units_iteration(sweep, stats_slot(slot), prof ? ev[0] : nullptr, prof ? ev[1] : nullptr);
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
     double h_ll = 0;
     hip_check(hipMemcpyAsync(&h_ll, d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
@@ -774,10 +786,9 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     // statistics of the rebuild this sweep read: the previous iteration's
     // (complete: this iteration's last sweep waited for their copy), or
     // current already
-    if (stats_pending >= 0) take_stats(h_stats[stats_pending]);
+    if (stats_pending >= 0) take_stats(stats_slot(stats_pending));
     const double bytes = sweep_bytes();
     stats_pending = slot;
-    stream2_busy = true;
     if (prof) {
         float a = 0, b = 0;
         hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
@@ -790,6 +801,61 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     last_sampler_bytes = bytes;
     sampled = true;
     if (ll) *ll = h_ll / (double)models[0].num_tokens;
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+void Engine::iterate_units_n(uint32_t first, int32_t count, double *ll, double *seconds) {
+    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
+    if (count < 1 || count > kLLHistory) fail(CULDA_ERR_LIMIT, "count must be in [1, %d]", kLLHistory);
+    const int32_t U = (int32_t)units.size();
+    const bool prof = prm.profile != 0;
+    if (prof)
+        while ((int32_t)ev_iter.size() < 2 * count) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreate(&e), "hipEventCreate");
+            ev_iter.push_back(e);
+        }
+    std::vector<int32_t> slots(count);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int32_t i = 0; i < count; ++i) {
+        slots[i] = stats_next;
+        stats_next = (stats_next + 1) % kStatsSlots;
+        enqueue_units_iteration(first + (uint32_t)i, stats_slot(slots[i]), prof ? ev_iter[2 * i] : nullptr,
+                                prof ? ev_iter[2 * i + 1] : nullptr);
+    }
+    std::vector<double> hist(kLLHistory);
+    hip_check(hipMemcpyAsync(hist.data(), d_ll_hist, kLLHistory * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "ll copy");
+    hip_check(hipStreamSynchronize(stream), "iteration sync");
+    auto t1 = std::chrono::steady_clock::now();
+    // sweep i read the rebuild of iteration i - 1 (complete: iteration i's
+    // last sweep waited for its copy), sweep 0 the state before the call
+    double bytes = 0, ms = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        if (i == 0) {
+            if (stats_pending >= 0) take_stats(stats_slot(stats_pending));
+        } else {
+            take_stats(stats_slot(slots[i - 1]));
+        }
+        bytes += sweep_bytes();
+        if (prof) {
+            float a = 0;
+            hip_check(hipEventElapsedTime(&a, ev_iter[2 * i], ev_iter[2 * i + 1]), "elapsed");
+            ms += a;
+        }
+    }
+    stats_pending = slots[count - 1];
+    if (prof) {
+        sampler_ms = ms;
+        phi_ms = 0;
+        theta_ms = 0;
+    }
+    sampler_launches = U * count;
+    last_sampler_bytes = bytes;
+    sampled = true;
+    for (int32_t i = 0; i < count; ++i)
+        if (ll) ll[i] = hist[(first + (uint32_t)i) % kLLHistory] / (double)models[0].num_tokens;
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 

@@ -139,13 +139,16 @@ public:
     // profiling / stats
     std::vector<hipEvent_t> ev_sampled;
     std::vector<hipEvent_t> ev_theta_unit;  // per-chunk iteration: theta of unit u rebuilt (stream2)
-    // The per-chunk iteration returns with the last chunks' theta rebuilds
-    // still running on stream2; their statistics land in h_stats[slot]
-    // (pinned, written by a copy on stream2) and are read one iteration
-    // later or by settle().
+    // The per-chunk iteration returns with the last chunk's theta rebuild
+    // still running on stream2; each iteration's rebuild statistics land in
+    // the next slot of a pinned ring (written by a copy on stream2) and are
+    // read one iteration later or by settle().
+    static constexpr int32_t kStatsSlots = kLLHistory + 2;  // > iterations of one call + the pending slot
     bool stream2_busy = false;
-    int32_t stats_pending = -1;  // slot of h_stats holding stats not yet read, or -1
-    unsigned long long *h_stats[2] = {nullptr, nullptr};
+    int32_t stats_pending = -1;  // ring slot holding stats not yet read, or -1
+    int32_t stats_next = 0;      // ring slot of the next iteration's copy
+    unsigned long long *h_stats_ring = nullptr;  // kStatsSlots x 2U
+    std::vector<hipEvent_t> ev_iter;             // profiling: two per iteration of iterate_units_n
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // batched iteration: theta on stream2
     hipEvent_t ev[4] = {};
     double sampler_ms = 0, phi_ms = 0, theta_ms = 0, iter_ms = 0;
@@ -188,6 +191,11 @@ public:
     // culda_ctx iteration: per-chunk sweeps, theta of chunk c overlapped
     // with the sweep of chunk c + 1
     void iterate_units(uint32_t sweep, double *ll, double *seconds);
+    // `count` such iterations from sweep `first` with no host round trip
+    // between them; ll[i] = LL/token of sweep first + i
+    void iterate_units_n(uint32_t first, int32_t count, double *ll, double *seconds);
+    void enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_dst, hipEvent_t ev_a, hipEvent_t ev_b);
+    unsigned long long *stats_slot(int32_t i) { return h_stats_ring + (size_t)i * 2 * units.size(); }
     // culda_batch iteration: one launch per step for every model
     void enqueue_batch_iteration(hipStream_t s, hipEvent_t before = nullptr, hipEvent_t after = nullptr);
     // algorithmic bytes of one sweep over every unit with the current theta

@@ -229,6 +229,15 @@ int culda_loglik(culda_ctx *ctx, double *ll_per_token);
  * train/MultiChunkMultiGPUequal.h:88-169 on one device). */
 int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds);
 
+/* count culda_iterate steps (sweeps first_iteration .. first_iteration +
+ * count - 1, count <= 1024) enqueued back to back with no host round trip
+ * between them; ll[i] = LL/token of sweep first_iteration + i (ll may be
+ * NULL), seconds = wall time until the last LL is known.  culda_last_sweeps
+ * then reports the sums over the count iterations.  Same chain as count
+ * culda_iterate calls (no counterpart in the reference: its loop reads the
+ * LL back every iteration). */
+int culda_iterate_n(culda_ctx *ctx, int32_t first_iteration, int32_t count, double *ll, double *seconds);
+
 /* Rebuild phi, head and theta from the current z (initial model state,
  * train/SingleChunkSingleGPU.h:24-41). */
 int culda_rebuild_all(culda_ctx *ctx);
@@ -294,10 +303,11 @@ int culda_get_stats(culda_ctx *ctx, culda_stats *out);
  * staged), summed over chunks. */
 int culda_sampler_bytes(culda_ctx *ctx, double *bytes);
 
-/* The last culda_iterate's sampler time (HIP events; params->profile = 1,
- * else 0), algorithmic bytes (as culda_sampler_bytes) and launch count,
- * without waiting for the theta rebuilds it left running (no counterpart
- * in the reference; for per-iteration measurement loops). */
+/* The last culda_iterate's (culda_iterate_n's: summed over its
+ * iterations) sampler time (HIP events; params->profile = 1, else 0),
+ * algorithmic bytes (as culda_sampler_bytes) and launch count, without
+ * waiting for the theta rebuilds it left running (no counterpart in the
+ * reference; for measurement loops). */
 int culda_last_sweeps(culda_ctx *ctx, double *sampler_ms, double *sampler_bytes, int32_t *launches);
 
 /* Autotuner result: the sampler's tokens per wave slot (and, for batches,

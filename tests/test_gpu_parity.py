@@ -303,12 +303,14 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
     chunk's rebuild.  Back-to-back iterations (nothing in between that waits)
     must give the chain of iterations each followed by a full sync, and the
     per-iteration sampler bytes must be the ones of the rebuild each sweep
-    read."""
+    read; iterate_n (iterations enqueued back to back in one call) must give
+    the same chain, LLs and summed bytes."""
     V, K, arrays = _setup(31, V=500, D=400, mean_len=90, chunks=chunks, K=128)
     alpha, beta, seed = 0.2, 0.01, 17
     live, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
     synced, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
-    for s in (live, synced):
+    batched, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
+    for s in (live, synced, batched):
         s.init_z()
         s.rebuild_all()
     ll_live, ll_synced, b_live, b_synced = [], [], [], []
@@ -321,9 +323,17 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
         synced.sync()
         b_synced.append(synced.sampler_bytes())
     assert ll_live == ll_synced and b_live == b_synced
+    # the same six iterations as one call (no host round trip between them)
+    ll_n, _ = batched.iterate_n(1, 2)
+    ll_m, _ = batched.iterate_n(3, 4)
+    ms, b, n = batched.last_sweeps()
+    assert ll_n + ll_m == ll_live
+    assert n == 4 * chunks and ms > 0 and b == sum(b_live[2:])
     for c in range(chunks):
         assert np.array_equal(live.get_z(c), synced.get_z(c))
+        assert np.array_equal(batched.get_z(c), synced.get_z(c))
         assert all(np.array_equal(x, y) for x, y in zip(live.get_theta(c), synced.get_theta(c)))
-    assert live.stats().sum_token_nnz == synced.stats().sum_token_nnz
-    live.close()
-    synced.close()
+        assert all(np.array_equal(x, y) for x, y in zip(batched.get_theta(c), synced.get_theta(c)))
+    assert live.stats().sum_token_nnz == synced.stats().sum_token_nnz == batched.stats().sum_token_nnz
+    for s in (live, synced, batched):
+        s.close()
