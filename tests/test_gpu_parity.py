@@ -335,5 +335,9 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
         assert all(np.array_equal(x, y) for x, y in zip(live.get_theta(c), synced.get_theta(c)))
         assert all(np.array_equal(x, y) for x, y in zip(batched.get_theta(c), synced.get_theta(c)))
     assert live.stats().sum_token_nnz == synced.stats().sum_token_nnz == batched.stats().sum_token_nnz
+    import culda_cgs_amd as cg
+    for first, count in ((7, 0), (7, 1025), (0, 1)):
+        with pytest.raises(cg.CuldaError):
+            batched.iterate_n(first, count)
     for s in (live, synced, batched):
         s.close()
