@@ -225,6 +225,50 @@ def test_long_doc_with_more_than_1024_theta_entries():
     s.close()
 
 
+def test_theta_counts_near_and_above_16_bits():
+    """Topic counts at and past 16 bits in the per-chunk theta rebuilds.
+    Two chunks hold docs of 70,000 and 65,535 tokens (chunk 0) and 65,536
+    tokens (chunk 1), each with every token on one topic, so their counts
+    are 70,000, 65,535 (the largest 16-bit value) and 65,536, split into
+    entries of at most 63.  Theta equals the oracle's after the full
+    rebuild and after each of two sweeps (the per-chunk rebuilds); z
+    bit-exact, LL within 1e-5."""
+    V, K = 40, 32
+    rng = np.random.default_rng(21)
+    small = random_docs(rng, V, 30, 30)
+    long_a, long_b, long_c = (rng.integers(0, V, n) for n in (70_000, 65_535, 65_536))
+    docs = small[:15] + [long_a] + small[15:] + [long_b, long_c]
+    ia, ib, ic = 15, 31, 32
+    arrays = chunks_from_docs(V, docs, 2)
+    ends, tokens = arrays[1], arrays[3]
+    assert list(ends) == [32, 33], ends
+    alpha, beta, seed = 50.0 / K, 0.01, 5
+    s, om = _pair(V, K, arrays, alpha, beta, seed)
+    s.init_z()
+    zs = []
+    for c in range(2):
+        z = s.get_z(c).copy()
+        for d, t in ((ia, 5), (ib, 6), (ic, 7)):
+            z[tokens[c] == d] = t
+        zs.append(z)
+        s.set_z(c, z)
+    om.set_z(zs)
+    s.rebuild_all()
+    om.rebuild()
+    _theta_equal(s, om, arrays)
+    for it in range(1, 3):
+        ll, _ = s.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+        for c in range(2):
+            bad = np.flatnonzero(s.get_z(c) != om.z[c])
+            assert bad.size == 0, f"sweep {it}, chunk {c}: {bad.size} tokens differ (first {bad[:5]})"
+        ref = out.ll_sum / om.num_tokens
+        assert abs(ll - ref) <= LL_RTOL * abs(ref)
+    _theta_equal(s, om, arrays)
+    s.close()
+
+
 def test_row_readers_after_lazy_rare_updates():
     """One chunk, K = 1024, mostly rare words: after each sweep only n_k and
     zp follow the new z, the rare words' rows are brought up to date when
