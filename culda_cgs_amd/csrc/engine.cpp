@@ -92,7 +92,14 @@ int64_t theta_entries_max(int64_t nd, int32_t K) {
     return std::min<int64_t>(nd, (int64_t)K + (nd + kThetaMaxCount - 1) / kThetaMaxCount);
 }
 
-static constexpr int64_t kPhiSplit = 8192;  // hot words are rebuilt in pieces
+// Hot words are rebuilt in pieces of at most phi_split(tokens) tokens per
+// unit, `tokens` being all the engine's tokens (one phi launch covers them).
+// Large launches are bound by their total work: at 97M tokens 8192-token
+// pieces took 0.367 ms, 4096- or 32768-token pieces 0.396 ms (more pieces,
+// more row atomics; or longer chains).  Small ones are bound by the longest
+// piece's chain of loads: at 1M tokens 4096-token pieces took 22 us, 8192-
+// token pieces 46 us, 2048 21 us and 1024 30 us.
+static int64_t phi_split(int64_t tokens) { return tokens <= ((int64_t)1 << 24) ? 4096 : 8192; }
 
 // Packets of at most `per` consecutive items of [first, last), all owned by `owner`.
 static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last, int32_t owner, int32_t per) {
@@ -100,21 +107,22 @@ static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last
 }
 
 // Phi list of model m over units [u0, u1): whole words, hot words as pieces
-// of at most kPhiSplit tokens per unit (rows zeroed first, atomics); items
+// of at most `split` tokens per unit (rows zeroed first, atomics); items
 // in descending token count, tokens[i] being item i's.
 static void build_phi_list(const std::vector<UnitHost> &units, int32_t m, int32_t u0, int32_t u1, int32_t V,
-                           std::vector<PhiSlot> &slots, std::vector<int64_t> &tokens, std::vector<int2> &zero) {
+                           int64_t split, std::vector<PhiSlot> &slots, std::vector<int64_t> &tokens,
+                           std::vector<int2> &zero) {
     std::vector<std::pair<int64_t, PhiSlot>> items;
     items.reserve(V);
     for (int32_t w = 0; w < V; ++w) {
         int64_t tot = 0;
         for (int32_t u = u0; u < u1; ++u) tot += units[u].word_idx_h[w + 1] - units[u].word_idx_h[w];
-        if (tot > kPhiSplit) {
+        if (tot > split) {
             zero.push_back(make_int2(m, w));
             for (int32_t u = u0; u < u1; ++u) {
                 const int64_t b = units[u].word_idx_h[w], e = units[u].word_idx_h[w + 1];
-                for (int64_t x = b; x < e; x += kPhiSplit)
-                    items.emplace_back(std::min(e, x + kPhiSplit) - x, PhiSlot{w, u, u + 1, 1, x, std::min(e, x + kPhiSplit)});
+                for (int64_t x = b; x < e; x += split)
+                    items.emplace_back(std::min(e, x + split) - x, PhiSlot{w, u, u + 1, 1, x, std::min(e, x + split)});
             }
         } else {
             items.emplace_back(tot, PhiSlot{w, u0, u1, 0, 0, 0});
@@ -310,6 +318,9 @@ void Engine::upload() {
     d_units = upload_vec(*this, ud);
     // phi work lists: fused per model (and all models concatenated), and per unit
     {
+        int64_t all_tokens = 0;
+        for (const ModelHost &m : models) all_tokens += m.num_tokens;
+        const int64_t split = phi_split(all_tokens);
         std::vector<PhiSlot> all_slots, sparse_slots;
         std::vector<Packet> all_packets;
         std::vector<int2> all_zero;
@@ -321,7 +332,7 @@ void Engine::upload() {
             std::vector<PhiSlot> sl;
             std::vector<int64_t> tok;
             std::vector<int2> zr;
-            build_phi_list(units, mi, m.unit_begin, m.unit_end, m.V, sl, tok, zr);
+            build_phi_list(units, mi, m.unit_begin, m.unit_end, m.V, split, sl, tok, zr);
             // the zp-based update skips words without tokens and the words
             // of the rare tiles
             std::vector<char> covered(m.V, 0);
@@ -347,7 +358,7 @@ void Engine::upload() {
                 std::vector<PhiSlot> s1;
                 std::vector<int64_t> t1;
                 std::vector<int2> z1;
-                build_phi_list(units, mi, ui, ui + 1, m.V, s1, t1, z1);
+                build_phi_list(units, mi, ui, ui + 1, m.V, split, s1, t1, z1);
                 const int32_t per1 =
                     std::max<int32_t>(4, (int32_t)((s1.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
                 std::vector<Packet> p1;
@@ -563,8 +574,7 @@ void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStr
     const PhiList &L = unit < 0 ? m.phi_fused : units[unit].phi_single;
     if (unit < 0) accumulate = false;
     if (!accumulate) {
-        hip_check(launch_fill_u32((uint32_t *)m.nk, m.K, 0, s), "fill n_k");
-        hip_check(launch_zero_rows(d_models, L.zero_rows, L.num_zero_rows, s), "zero rows");
+        hip_check(launch_phi_clear((uint32_t *)m.nk, m.K, d_models, L.zero_rows, L.num_zero_rows, s), "phi clear");
     }
     PhiArgs a{L.slots, L.packets, d_units, d_models, L.num_packets, accumulate ? 1 : 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
@@ -597,8 +607,9 @@ void Engine::fresh_rows(hipStream_t s) {
 
 void Engine::update_phi(hipStream_t s) {
     if (zp_valid && sparse_phi) {
-        hip_check(launch_fill_u32((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, 0, s), "fill n_k");
-        hip_check(launch_zero_rows(d_models, phi_all.zero_rows, phi_all.num_zero_rows, s), "zero rows");
+        hip_check(launch_phi_clear((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, d_models, phi_all.zero_rows,
+                                   phi_all.num_zero_rows, s),
+                  "phi clear");
         PhiArgs a{phi_sparse.slots, phi_sparse.packets, d_units, d_models, phi_sparse.num_packets, 2};
         hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
         RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
@@ -618,8 +629,9 @@ void Engine::update_phi(hipStream_t s) {
 }
 
 void Engine::rebuild_phi_all(hipStream_t s) {
-    hip_check(launch_fill_u32((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, 0, s), "fill n_k");
-    hip_check(launch_zero_rows(d_models, phi_all.zero_rows, phi_all.num_zero_rows, s), "zero rows");
+    hip_check(launch_phi_clear((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, d_models, phi_all.zero_rows,
+                               phi_all.num_zero_rows, s),
+              "phi clear");
     PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
     phi_ready = false;

@@ -9,7 +9,9 @@ namespace culda {
 hipError_t launch_init_z(const InitArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s);
-hipError_t launch_zero_rows(const ModelDev *models, const int2 *rows, int32_t n, hipStream_t s);
+// fill[0, n_fill) = 0 and the n listed (model, word) n_wk rows zeroed, one launch
+hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n,
+                            hipStream_t s);
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s);
 hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
 hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s);

@@ -50,9 +50,12 @@ __global__ void add_u32_kernel(uint32_t *p, uint32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) p[0] += v;
 }
 
-// rows[r] = (model, word): zero that model's n_wk row (hot words, before
-// their pieces are added with atomics)
-__global__ void zero_rows_kernel(const ModelDev *models, const int2 *rows, int32_t n) {
+// Clear before a count rebuild: fill[0, n_fill) = 0 (the n_k vectors), and
+// for rows[r] = (model, word) that model's n_wk row (hot words, before
+// their pieces are added with atomics).
+__global__ void phi_clear_kernel(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_fill; i += (int64_t)gridDim.x * blockDim.x)
+        fill[i] = 0;
     for (int r = blockIdx.x; r < n; r += gridDim.x) {
         const int2 mw = rows[r];
         const ModelDev &m = models[mw.x];
@@ -334,6 +337,30 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 }
 
 // ------------------------------------------------------------ theta rebuild
+// Adds the workgroup's entry statistics (Σ entries, Σ entries × doc length)
+// to the unit's counters: one pair of atomics per workgroup, not per wave
+// (all of a workgroup's docs belong to one unit).  Every wave calls it.
+__device__ __forceinline__ void flush_theta_stats(unsigned long long *stats, unsigned long long nnz,
+                                                  unsigned long long tok, int wave, int lane) {
+    __shared__ unsigned long long st[kWavesPerGroup][2];
+    if (lane == 0) {
+        st[wave][0] = nnz;
+        st[wave][1] = tok;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long n = 0, t = 0;
+        for (int w = 0; w < kWavesPerGroup; ++w) {
+            n += st[w][0];
+            t += st[w][1];
+        }
+        if (n) {
+            atomicAdd(&stats[0], n);
+            atomicAdd(&stats[1], t);
+        }
+    }
+}
+
 // One workgroup per packet (a run of docs of one unit), one wave per doc:
 // LDS histogram over the doc's tokens (gathered through rev_idx), then an
 // ascending compaction of the nonzero topics into 16-bit entries; a count
@@ -400,10 +427,7 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
         s_tok += (unsigned long long)base * (e - b);
         wave_sync();
     }
-    if (lane == 0 && s_nnz) {
-        atomicAdd(&un.stats[0], s_nnz);
-        atomicAdd(&un.stats[1], s_tok);
-    }
+    flush_theta_stats(un.stats, s_nnz, s_tok, wave, lane);
 }
 
 // The same rebuild for a launch that runs alone (a single-chunk model,
@@ -476,10 +500,7 @@ __global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
         s_tok += (unsigned long long)base * (e - b);
         wave_sync();
     }
-    if (lane == 0 && s_nnz) {
-        atomicAdd(&un.stats[0], s_nnz);
-        atomicAdd(&un.stats[1], s_tok);
-    }
+    flush_theta_stats(un.stats, s_nnz, s_tok, wave, lane);
 }
 
 // Bitonic sort of the keys of lanes [0, M) of each M-lane group into
@@ -930,9 +951,11 @@ hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero_rows(const ModelDev *models, const int2 *rows, int32_t n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(n, 1, 4096)), dim3(256), 0, s, models, rows, n);
+hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n,
+                            hipStream_t s) {
+    if (n_fill <= 0 && n <= 0) return hipSuccess;
+    const int32_t grid = std::max(n_fill > 0 ? grid_for(n_fill, 256, 4096) : 1, n > 0 ? grid_for(n, 1, 4096) : 1);
+    hipLaunchKernelGGL(phi_clear_kernel, dim3(grid), dim3(256), 0, s, fill, n_fill, models, rows, n);
     return hipGetLastError();
 }
 
