@@ -611,9 +611,8 @@ void Engine::update_phi(hipStream_t s) {
                                    phi_all.num_zero_rows, s),
                   "phi clear");
         PhiArgs a{phi_sparse.slots, phi_sparse.packets, d_units, d_models, phi_sparse.num_packets, 2};
-        hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
         RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
-        hip_check(launch_phi_rare(r, s), "rare phi launch");
+        hip_check(launch_phi_update(a, r, s), "phi update launch");
         rare_stale = num_rare_packets > 0;
         phi_ready = false;
     } else if (prm.fused_phi || units.size() == models.size()) {

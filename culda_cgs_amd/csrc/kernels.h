@@ -13,6 +13,8 @@ hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s);
 hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n,
                             hipStream_t s);
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s);
+// the word-slot rebuild p and the rare tiles' count pass r (r.mode 0) in one launch
+hipError_t launch_phi_update(const PhiArgs &p, const RareArgs &r, hipStream_t s);
 hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
 hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s);
 hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, int32_t num_words, int32_t k,

@@ -101,10 +101,10 @@ __device__ __forceinline__ void count_range(const uint16_t *z, uint16_t *zp, int
     }
 }
 
-__global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
-    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
-    __shared__ uint32_t nk_s[kMaxK];
-    const Packet pk = a.packets[blockIdx.x];
+// Body of the phi rebuild for workgroup packet `packet`, on the caller's LDS.
+__device__ __forceinline__ void phi_rebuild_body(const PhiArgs &a, int packet, uint32_t (*hist_s)[kMaxK],
+                                                 uint32_t *nk_s) {
+    const Packet pk = a.packets[packet];
     const ModelDev &m = a.models[pk.owner];
     const int K = m.k;
     const bool keep_zp = a.accumulate == 2;
@@ -244,6 +244,12 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    __shared__ uint32_t nk_s[kMaxK];
+    phi_rebuild_body(a, blockIdx.x, hist_s, nk_s);
+}
+
 // ------------------------------------------------------------ rare tiles
 // Rare words of one-unit models: their n_wk rows are brought up to date
 // only when something reads them.  After a sweep (kMaterialize false) one
@@ -256,9 +262,8 @@ __global__ __launch_bounds__(256) void phi_rebuild_kernel(PhiArgs a) {
 // that no lane keeps is cleared -- the cleared and the stored entries are
 // disjoint, so no ordering between the stores; zm = zp.
 template <bool kMaterialize>
-__global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
-    __shared__ uint32_t nk_s[kMaxK];
-    const Packet pk = a.packets[blockIdx.x];
+__device__ __forceinline__ void phi_rare_body(const RareArgs &a, int packet, uint32_t *nk_s) {
+    const Packet pk = a.packets[packet];
     const ModelDev &m = a.models[pk.owner];
     const int K = m.k;
     if (!kMaterialize) {
@@ -313,6 +318,25 @@ __global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
             if (v) atomicAdd(m.nk + k, (int)v);
         }
     }
+}
+
+template <bool kMaterialize>
+__global__ __launch_bounds__(256) void phi_rare_kernel(RareArgs a) {
+    __shared__ uint32_t nk_s[kMaxK];
+    phi_rare_body<kMaterialize>(a, blockIdx.x, nk_s);
+}
+
+// The count update after a sweep of one-unit models in one launch: the first
+// p.num_packets workgroups rebuild word slots, the rest take the rare tiles'
+// n_k / zp pass.  The two touch disjoint words and zp ranges, and their n_k
+// atomics commute, so the counts equal those of the two launches in turn.
+__global__ __launch_bounds__(256) void phi_update_kernel(PhiArgs p, RareArgs r) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    __shared__ uint32_t nk_s[kMaxK];
+    if ((int)blockIdx.x < p.num_packets)
+        phi_rebuild_body(p, blockIdx.x, hist_s, nk_s);
+    else
+        phi_rare_body<false>(r, (int)blockIdx.x - p.num_packets, nk_s);
 }
 
 // den_k = fl(fl(n_k) + fl(beta * fl(V))) for every model of the table
@@ -962,6 +986,15 @@ hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *mode
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
     hipLaunchKernelGGL(phi_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_phi_update(const PhiArgs &p, const RareArgs &r, hipStream_t s) {
+    if (r.mode) return hipErrorInvalidValue;  // the count pass only
+    const int64_t grid = (int64_t)std::max(p.num_packets, 0) + std::max(r.num_packets, 0);
+    if (grid <= 0) return hipSuccess;
+    if (grid > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(phi_update_kernel, dim3((uint32_t)grid), dim3(256), 0, s, p, r);
     return hipGetLastError();
 }
 
