@@ -29,7 +29,7 @@ class _OcCorpus(C.Structure):
 
 class SweepOut(C.Structure):
     _fields_ = [("ll_sum", C.c_double), ("ll_sum_exact", C.c_double), ("boundary", C.c_int64),
-                ("mismatch", C.c_int64), ("s_bucket", C.c_int64)]
+                ("mismatch", C.c_int64), ("s_bucket", C.c_int64), ("tokens", C.c_int64)]
 
 
 THETA_MAX_COUNT = 63  # largest count of one theta entry
@@ -56,6 +56,10 @@ def lib():
                                P(C.c_int32), P(C.c_int64), P(C.c_int32), P(C.c_int16), P(P(C.c_uint16)),
                                P(P(C.c_uint8)), P(SweepOut)]
         L.oc_sweep.restype = None
+        L.oc_sweep_range.argtypes = [P(_OcCorpus), C.c_float, C.c_float, C.c_uint64, C.c_uint32, P(C.c_int32),
+                                     P(C.c_int32), P(C.c_int64), P(C.c_int32), P(C.c_int16), C.c_int32, C.c_int32,
+                                     C.c_int, P(P(C.c_uint16)), P(P(C.c_uint8)), P(SweepOut)]
+        L.oc_sweep_range.restype = None
         L.oc_sweep_fast.argtypes = [P(_OcCorpus), C.c_float, C.c_float, C.c_uint64, C.c_uint32, P(C.c_int32),
                                     P(C.c_int32), P(C.c_int64), P(C.c_int32), P(C.c_int16), P(C.c_double)]
         L.oc_sweep_fast.restype = None
@@ -141,6 +145,10 @@ class OracleModel:
         self.token_nnz = L.oc_rebuild_theta(C.byref(self._c), _p(self.th_off, C.c_int64),
                                             _p(self.th_cnt, C.c_int32), _p(self.th_top, C.c_int16))
 
+    def rebuild_phi(self):
+        """n_wk / n_k only."""
+        lib().oc_rebuild_phi(C.byref(self._c), _p(self.nwk, C.c_int32), _p(self.nk, C.c_int32))
+
     def rebuild_theta(self):
         """Theta only (n_wk / n_k left as they are: fold-in with phi fixed)."""
         self.token_nnz = lib().oc_rebuild_theta(C.byref(self._c), _p(self.th_off, C.c_int64),
@@ -183,11 +191,24 @@ class OracleModel:
         return out, ex, fl
 
     def sweep_fast(self, alpha, beta, seed, sweep):
+        """The same FP32 sweep without the FP64 shadow (timing); returns the
+        FP64 sum of the LL terms."""
         ll = C.c_double()
         lib().oc_sweep_fast(C.byref(self._c), alpha, beta, seed, sweep, _p(self.nwk, C.c_int32),
                             _p(self.nk, C.c_int32), _p(self.th_off, C.c_int64), _p(self.th_cnt, C.c_int32),
                             _p(self.th_top, C.c_int16), C.byref(ll))
         return ll.value
+
+    def sweep_words(self, alpha, beta, seed, sweep, w_lo, w_hi, exact=False):
+        """The sweep restricted to the tokens of words [w_lo, w_hi) (other
+        tokens keep their z); exact adds the FP64 shadow (boundary draws
+        counted).  Returns SweepOut (tokens = tokens swept)."""
+        out = SweepOut()
+        lib().oc_sweep_range(C.byref(self._c), alpha, beta, seed, sweep, _p(self.nwk, C.c_int32),
+                             _p(self.nk, C.c_int32), _p(self.th_off, C.c_int64), _p(self.th_cnt, C.c_int32),
+                             _p(self.th_top, C.c_int16), int(w_lo), int(w_hi), 1 if exact else 0, None, None,
+                             C.byref(out))
+        return out
 
     def loglik_joint(self, alpha, beta):
         return lib().oc_loglik_joint(C.byref(self._c), alpha, beta, _p(self.nwk, C.c_int32),

@@ -135,17 +135,27 @@ void oc_init_z(const oc_corpus *c, uint64_t seed) {
 }
 
 /* ------------------------------------------------------------------ counts */
-/* n_wk, n_k summed over chunks (kernel/lda_phi_kernel.cu:94-116, :162-183). */
+/* n_wk, n_k summed over chunks (kernel/lda_phi_kernel.cu:94-116, :162-183).
+ * Rows are independent (one word each); n_k is summed per thread and then
+ * added, integer sums, so the result is the same for any thread count. */
 void oc_rebuild_phi(const oc_corpus *c, int32_t *nwk, int32_t *nk) {
     const int32_t K = c->K;
-    memset(nwk, 0, sizeof(int32_t) * (size_t)c->V * K);
     memset(nk, 0, sizeof(int32_t) * K);
-    for (int32_t k = 0; k < c->C; ++k)
-        for (int32_t w = 0; w < c->V; ++w)
-            for (int64_t i = c->word_idx[k][w]; i < c->word_idx[k][w + 1]; ++i) {
-                nwk[(size_t)w * K + c->z[k][i]]++;
-                nk[c->z[k][i]]++;
-            }
+#pragma omp parallel
+    {
+        int32_t *lk = (int32_t *)calloc((size_t)K, sizeof(int32_t));
+#pragma omp for schedule(dynamic, 64)
+        for (int32_t w = 0; w < c->V; ++w) {
+            int32_t *row = nwk + (size_t)w * K;
+            memset(row, 0, sizeof(int32_t) * K);
+            for (int32_t k = 0; k < c->C; ++k)
+                for (int64_t i = c->word_idx[k][w]; i < c->word_idx[k][w + 1]; ++i) row[c->z[k][i]]++;
+            for (int32_t k = 0; k < K; ++k) lk[k] += row[k];
+        }
+#pragma omp critical
+        for (int32_t k = 0; k < K; ++k) nk[k] += lk[k];
+        free(lk);
+    }
 }
 
 #define OC_THETA_MAX_COUNT 63
@@ -278,18 +288,24 @@ typedef struct oc_sweep_out {
     int64_t boundary;     /* draws within 1e-6 (relative) of a boundary     */
     int64_t mismatch;     /* faithful topic != exact topic                  */
     int64_t s_bucket;     /* draws that took the S bucket                   */
+    int64_t tokens;       /* tokens swept                                   */
 } oc_sweep_out;
 
-/* One sweep with frozen counts (nwk, nk, theta rows).  Writes the faithful
- * FP32 topics into c->z (the sweep never reads z).  exact_z (optional, C
- * arrays) receives the FP64 inverse-CDF topics, flags (optional) 1 for
- * boundary draws. */
-void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32_t sweep, const int32_t *nwk,
-              const int32_t *nk, const int64_t *th_off, const int32_t *th_cnt, const int16_t *th_top,
-              uint16_t *const *exact_z, uint8_t *const *flags, oc_sweep_out *out) {
+/* One sweep with frozen counts (nwk, nk, theta rows) over the tokens of
+ * words [w_lo, w_hi) (every word: 0, V).  Writes the faithful FP32 topics
+ * into c->z (the sweep never reads z).  With want_exact, every draw is also
+ * evaluated in FP64 by a plain linear inverse CDF: exact_z (optional, C
+ * arrays) receives those topics, flags (optional) 1 for boundary draws, and
+ * out counts them; without it only the FP32 path runs (the CPU baseline). */
+void oc_sweep_range(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32_t sweep, const int32_t *nwk,
+                    const int32_t *nk, const int64_t *th_off, const int32_t *th_cnt, const int16_t *th_top,
+                    int32_t w_lo, int32_t w_hi, int want_exact, uint16_t *const *exact_z, uint8_t *const *flags,
+                    oc_sweep_out *out) {
     const int32_t K = c->K, V = c->V, nbq = K / 16;
     const float bv = beta * (float)V;
     const double k_alpha = (double)K * (double)alpha;
+    if (w_lo < 0) w_lo = 0;
+    if (w_hi > V) w_hi = V;
     float *den = (float *)malloc(sizeof(float) * K);
     double *den_d = (double *)malloc(sizeof(double) * K);
     for (int32_t k = 0; k < K; ++k) {
@@ -307,26 +323,28 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
         }
     }
     double ll = 0, lle = 0;
-    int64_t nbound = 0, nmis = 0, ns = 0;
+    int64_t nbound = 0, nmis = 0, ns = 0, ntot = 0;
     const int64_t nbmax = max_blocks(c, th_off);
-#pragma omp parallel reduction(+ : ll, lle, nbound, nmis, ns)
+#pragma omp parallel reduction(+ : ll, lle, nbound, nmis, ns, ntot)
     {
         float *ph = (float *)malloc(sizeof(float) * K);
         double *phd = (double *)malloc(sizeof(double) * K);
         float rq[64];
         float *rs = (float *)malloc(sizeof(float) * (size_t)(nbmax + 1)); /* S prefix after each block */
 #pragma omp for schedule(dynamic, 16)
-        for (int32_t w = 0; w < V; ++w) {
+        for (int32_t w = w_lo; w < w_hi; ++w) {
             int64_t ntok = 0;
             for (int32_t k = 0; k < c->C; ++k) ntok += c->word_idx[k][w + 1] - c->word_idx[k][w];
             if (!ntok) continue;
+            ntot += ntok;
             /* head row and Q table */
             double Qd = 0;
-            for (int32_t k = 0; k < K; ++k) {
-                ph[k] = ((float)nwk[(size_t)w * K + k] + beta) / den[k];
-                phd[k] = ((double)nwk[(size_t)w * K + k] + (double)beta) / den_d[k];
-                Qd += (double)alpha * phd[k];
-            }
+            for (int32_t k = 0; k < K; ++k) ph[k] = ((float)nwk[(size_t)w * K + k] + beta) / den[k];
+            if (want_exact)
+                for (int32_t k = 0; k < K; ++k) {
+                    phd[k] = ((double)nwk[(size_t)w * K + k] + (double)beta) / den_d[k];
+                    Qd += (double)alpha * phd[k];
+                }
             for (int b = 0; b < 64; ++b) {
                 float v[16];
                 if (b < nbq) {
@@ -418,6 +436,7 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
                     }
                     c->z[ck][i] = (uint16_t)topic;
                     ll += log((double)SQ / ((double)doc_len(c, d) + k_alpha));
+                    if (!want_exact) continue;
 
                     /* exact FP64 inverse CDF over the concatenated S then Q masses */
                     double Sd = 0;
@@ -469,142 +488,29 @@ void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32
     free(den_d);
     if (out) {
         out->ll_sum = ll;
-        out->ll_sum_exact = lle;
+        out->ll_sum_exact = want_exact ? lle : 0.0;
         out->boundary = nbound;
         out->mismatch = nmis;
         out->s_bucket = ns;
+        out->tokens = ntot;
     }
 }
 
-/* FP32-only sweep for timing: the faithful path without the FP64 shadow. */
+/* Every word, FP32 choice plus the FP64 shadow. */
+void oc_sweep(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32_t sweep, const int32_t *nwk,
+              const int32_t *nk, const int64_t *th_off, const int32_t *th_cnt, const int16_t *th_top,
+              uint16_t *const *exact_z, uint8_t *const *flags, oc_sweep_out *out) {
+    oc_sweep_range(c, alpha, beta, seed, sweep, nwk, nk, th_off, th_cnt, th_top, 0, c->V, 1, exact_z, flags, out);
+}
+
+/* Every word, the FP32 path only (timing): the same code as oc_sweep with
+ * the FP64 shadow switched off. */
 void oc_sweep_fast(const oc_corpus *c, float alpha, float beta, uint64_t seed, uint32_t sweep, const int32_t *nwk,
                    const int32_t *nk, const int64_t *th_off, const int32_t *th_cnt, const int16_t *th_top,
                    double *ll_sum) {
-    const int32_t K = c->K, V = c->V, nbq = K / 16;
-    const float bv = beta * (float)V;
-    const double k_alpha = (double)K * (double)alpha;
-    float *den = (float *)malloc(sizeof(float) * K);
-    for (int32_t k = 0; k < K; ++k) den[k] = (float)nk[k] + bv;
-    int64_t *pre = word_base(c);
-    int64_t *cbase = (int64_t *)malloc(sizeof(int64_t) * (size_t)c->C * V);
-    for (int32_t w = 0; w < V; ++w) {
-        int64_t r = pre[w];
-        for (int32_t k = 0; k < c->C; ++k) {
-            cbase[(size_t)k * V + w] = r;
-            r += c->word_idx[k][w + 1] - c->word_idx[k][w];
-        }
-    }
-    double ll = 0;
-    const int64_t nbmax = max_blocks(c, th_off);
-#pragma omp parallel reduction(+ : ll)
-    {
-        float *ph = (float *)malloc(sizeof(float) * K);
-        float rq[64];
-        float *rs = (float *)malloc(sizeof(float) * (size_t)(nbmax + 1)); /* S prefix after each block */
-#pragma omp for schedule(dynamic, 16)
-        for (int32_t w = 0; w < V; ++w) {
-            int64_t ntok = 0;
-            for (int32_t k = 0; k < c->C; ++k) ntok += c->word_idx[k][w + 1] - c->word_idx[k][w];
-            if (!ntok) continue;
-            for (int32_t k = 0; k < K; ++k) ph[k] = ((float)nwk[(size_t)w * K + k] + beta) / den[k];
-            for (int b = 0; b < 64; ++b) {
-                float v[16];
-                if (b < nbq) {
-                    for (int m = 0; m < 16; ++m) v[m] = alpha * ph[b * 16 + m];
-                    ks16(v);
-                    rq[b] = v[15];
-                } else {
-                    rq[b] = 0.0f;
-                }
-            }
-            ks64(rq);
-            const float Q = rq[nbq - 1];
-            for (int32_t ck = 0; ck < c->C; ++ck) {
-                const int64_t b0 = c->word_idx[ck][w], e0 = c->word_idx[ck][w + 1];
-                for (int64_t i = b0; i < e0; ++i) {
-                    const int32_t d = c->tokens[ck][i];
-                    const int64_t o = th_off[d], nnz = th_off[d + 1] - o;
-                    const uint64_t g = (uint64_t)(cbase[(size_t)ck * V + w] + (i - b0));
-                    const float u = (float)((uint32_t)(sweep_word(seed, g, sweep) >> 40) + 1u) * 5.9604644775390625e-8f;
-                    const int64_t nb = (nnz + 15) / 16;
-                    float R = 0.0f;
-                    for (int64_t b = 0; b < nb; ++b) {
-                        float p[16];
-                        for (int m = 0; m < 16; ++m) {
-                            const int64_t j = b * 16 + m;
-                            p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
-                        }
-                        ks16(p);
-                        R = R + p[15];
-                        rs[b + 1] = R;
-                    }
-                    const float S = R, SQ = S + Q, tu = u * SQ;
-                    int32_t topic;
-                    if (tu < S) {
-                        const float t = tu;
-                        int64_t bsel = -1;
-                        for (int64_t b = 0; b < nb; ++b)
-                            if (t < rs[b + 1]) {
-                                bsel = b;
-                                break;
-                            }
-                        if (bsel < 0) {
-                            topic = th_top[o + nnz - 1];
-                        } else {
-                            float p[16];
-                            for (int m = 0; m < 16; ++m) {
-                                const int64_t j = bsel * 16 + m;
-                                p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
-                            }
-                            pfx16(p);
-                            const int64_t nvalid = nnz - bsel * 16 < 16 ? nnz - bsel * 16 : 16;
-                            int64_t msel = nvalid - 1;
-                            for (int64_t m = 0; m < nvalid; ++m) {
-                                const float C = bsel == 0 ? p[m] : rs[bsel] + p[m];
-                                if (t < C) {
-                                    msel = m;
-                                    break;
-                                }
-                            }
-                            topic = th_top[o + bsel * 16 + msel];
-                        }
-                    } else {
-                        const float t = tu - S;
-                        int bsel = -1;
-                        for (int b = 0; b < nbq; ++b)
-                            if (t < rq[b]) {
-                                bsel = b;
-                                break;
-                            }
-                        if (bsel < 0) {
-                            topic = K - 1;
-                        } else {
-                            float v[16];
-                            for (int m = 0; m < 16; ++m) v[m] = alpha * ph[bsel * 16 + m];
-                            pfx16(v);
-                            int msel = 15;
-                            for (int m = 0; m < 16; ++m) {
-                                const float C = bsel == 0 ? v[m] : rq[bsel - 1] + v[m];
-                                if (t < C) {
-                                    msel = m;
-                                    break;
-                                }
-                            }
-                            topic = bsel * 16 + msel;
-                        }
-                    }
-                    c->z[ck][i] = (uint16_t)topic;
-                    ll += log((double)SQ / ((double)doc_len(c, d) + k_alpha));
-                }
-            }
-        }
-        free(ph);
-        free(rs);
-    }
-    free(cbase);
-    free(pre);
-    free(den);
-    *ll_sum = ll;
+    oc_sweep_out o;
+    oc_sweep_range(c, alpha, beta, seed, sweep, nwk, nk, th_off, th_cnt, th_top, 0, c->V, 0, NULL, NULL, &o);
+    *ll_sum = o.ll_sum;
 }
 
 /* Joint log p(w, z) in FP64 (Griffiths & Steyvers 2004, eq. 2-3 collapsed

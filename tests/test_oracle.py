@@ -267,3 +267,93 @@ def test_theta_entries_split_counts_above_63():
         ll += np.log(mass.sum() / (lens[arrays[3][0][i]] + K * alpha))
     assert abs(out.ll_sum_exact - ll) <= 1e-10 * abs(ll)
     assert abs(out.ll_sum - ll) <= 1e-5 * abs(ll)
+
+
+def _random_config(seed):
+    """The seeded configurations of the GPU parity tests: K in multiples of
+    16 up to 1024, 1-3 chunks, shuffled word ids, empty docs, one long doc
+    (counts above 63 split theta entries) on odd seeds."""
+    from helpers import random_docs as rd
+    rng = np.random.default_rng(1000 + seed)
+    K = int(rng.choice([16, 48, 96, 256, 640, 1024]))
+    V = int(rng.integers(40, 2500))
+    D = int(rng.integers(5, 120))
+    chunks = int(rng.integers(1, 4))
+    perm = rng.permutation(V)
+    docs = [perm[d] for d in rd(rng, V, D, float(rng.uniform(5, 80)), empty_docs=int(rng.integers(0, 3)),
+                                zipf=float(rng.uniform(0.6, 1.3)))]
+    if seed % 2:
+        docs.append(perm[rng.integers(0, min(V, 30), int(rng.integers(2000, 9000)))])
+    arrays = chunks_from_docs(V, docs, min(chunks, len(docs)))
+    alpha, beta = float(rng.uniform(0.01, 0.5)), float(rng.uniform(0.005, 0.1))
+    return V, K, arrays, alpha, beta, int(rng.integers(1, 1 << 30))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_sweep_fast_equals_pinned_sweep(seed):
+    """The timing sweep (no FP64 shadow) and the word-range sweep give the
+    pinned sweep's z and LL, three sweeps in a row from a common state."""
+    V, K, arrays, alpha, beta, s_seed = _random_config(seed)
+    a = oracle.OracleModel(V, K, *arrays)
+    a.init_z(s_seed)
+    b = oracle.OracleModel(V, K, *arrays, z=a.z)
+    c = oracle.OracleModel(V, K, *arrays, z=a.z)
+    for m in (a, b, c):
+        m.rebuild()
+    for it in range(1, 4):
+        out, _, _ = a.sweep(alpha, beta, s_seed, it)
+        ll_fast = b.sweep_fast(alpha, beta, s_seed, it)
+        # c: the words in three ranges
+        cuts = [0, V // 3, (2 * V) // 3, V]
+        ll_r, ntok = 0.0, 0
+        for lo, hi in zip(cuts[:-1], cuts[1:]):
+            o = c.sweep_words(alpha, beta, s_seed, it, lo, hi)
+            ll_r += o.ll_sum
+            ntok += o.tokens
+        assert ntok == a.num_tokens
+        for ch in range(a.C):
+            assert np.array_equal(a.z[ch], b.z[ch]), f"K={K} sweep {it} chunk {ch}: fast sweep differs"
+            assert np.array_equal(a.z[ch], c.z[ch]), f"K={K} sweep {it} chunk {ch}: range sweep differs"
+        assert abs(ll_fast - out.ll_sum) <= 1e-12 * abs(out.ll_sum)
+        assert abs(ll_r - out.ll_sum) <= 1e-12 * abs(out.ll_sum)
+        for m in (a, b, c):
+            m.rebuild()
+        assert np.array_equal(a.nwk, b.nwk) and np.array_equal(a.th_top, b.th_top)
+
+
+def test_sweep_fast_equals_pinned_sweep_on_the_small_corpus():
+    """Two sweeps of the 1M-token LDA corpus (V=10k Zipf(1.07), D=10k
+    Poisson(100), K=1024): the timing sweep equals the pinned sweep."""
+    import culda_cgs_amd as cg
+    corpus = cg.Corpus.generate(10_000, 10_000, 100, 1, len_mean=100.0, doc_topic_prior=0.1,
+                                topic_word_prior=0.05, zipf_s=1.07, seed=1, num_threads=8)
+    info = corpus.info()
+    _, end, wi, wt = corpus.chunk(0)
+    K, alpha, beta, seed = 1024, 50.0 / 1024, 0.01, 1
+    a = oracle.OracleModel(info.num_words, K, corpus.doc_cum(), [end], [wi], [wt])
+    a.init_z(seed)
+    b = oracle.OracleModel(info.num_words, K, corpus.doc_cum(), [end], [wi], [wt], z=a.z)
+    a.rebuild()
+    b.rebuild()
+    for it in (1, 2):
+        out, _, _ = a.sweep(alpha, beta, seed, it)
+        ll = b.sweep_fast(alpha, beta, seed, it)
+        assert np.array_equal(a.z[0], b.z[0]), f"sweep {it}"
+        assert abs(ll - out.ll_sum) <= 1e-12 * abs(out.ll_sum)
+        a.rebuild()
+        b.rebuild()
+
+
+def test_phi_rebuild_independent_of_thread_count():
+    V, K, arrays, alpha, beta, s_seed = _random_config(3)
+    ref = None
+    for threads in (1, 3, 8):
+        oracle.set_threads(threads)
+        m = oracle.OracleModel(V, K, *arrays)
+        m.init_z(s_seed)
+        m.rebuild()
+        got = (m.nwk.copy(), m.nk.copy())
+        if ref is None:
+            ref = got
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    oracle.set_threads(oracle.max_threads())
