@@ -97,6 +97,7 @@ _SIGS = {
     "culda_create": (C.c_int, [C.POINTER(Params), C.POINTER(_P)]),
     "culda_destroy": (None, [_P]),
     "culda_upload": (C.c_int, [_P, _P]),
+    "culda_check_corpus": (C.c_int, [_P, C.c_int32]),
     "culda_init_z": (C.c_int, [_P]),
     "culda_set_z": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint16), C.c_int64]),
     "culda_get_z": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_uint16), C.c_int64]),
@@ -249,6 +250,17 @@ class Corpus:
         n = self.info().num_docs
         return np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.zeros(0, np.int64)
 
+    def check(self, k):
+        """Raise CuldaError if the corpus exceeds a documented limit for k topics
+        (no device needed; upload runs the same checks)."""
+        _check(lib().culda_check_corpus(self._h, k))
+
+    def chunk_tokens(self, c):
+        """Number of tokens of chunk c (nothing copied)."""
+        v = ChunkView()
+        _check(lib().culda_corpus_chunk(self._h, c, C.byref(v)))
+        return int(v.num_tokens)
+
     def chunk(self, c):
         """(docid_start, docid_end, word_indices[V+1], word_tokens[N_c]) as copies."""
         v = ChunkView()
@@ -300,7 +312,7 @@ class Sampler:
         _check(lib().culda_upload(self._h, corpus._h))
         self.corpus = corpus
         self._info = corpus.info()
-        self._chunk_tokens = [corpus.chunk(c)[3].size for c in range(self._info.num_chunks)]
+        self._chunk_tokens = [corpus.chunk_tokens(c) for c in range(self._info.num_chunks)]
 
     def init_z(self):
         _check(lib().culda_init_z(self._h))
@@ -370,8 +382,9 @@ class Sampler:
 
     def get_theta(self, chunk):
         """Compact CSR over the chunk's docs: (offsets, counts, topics)."""
-        s, e, _, _ = self.corpus.chunk(chunk)
-        off = np.empty(e - s + 1, np.int64)
+        v = ChunkView()
+        _check(lib().culda_corpus_chunk(self.corpus._h, chunk, C.byref(v)))
+        off = np.empty(v.docid_end - v.docid_start + 1, np.int64)
         _check(lib().culda_get_theta(self._h, chunk, _ptr(off, C.c_int64), None, None))
         n = int(off[-1])
         cnt = np.empty(max(n, 1), np.int32)
@@ -580,8 +593,7 @@ class Batch:
         return v.value
 
     def get_z(self, model, chunk):
-        n = self.corpora[model].chunk(chunk)[3].size
-        z = np.empty(n, np.uint16)
+        z = np.empty(self.corpora[model].chunk_tokens(chunk), np.uint16)
         _check(lib().culda_batch_get_z(self._h, model, chunk, _ptr(z, C.c_uint16), z.size))
         return z
 

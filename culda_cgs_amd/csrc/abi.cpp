@@ -120,7 +120,8 @@ void copy_z(Engine &e, int32_t unit, uint16_t *z, int64_t n, bool to_device) {
         for (int64_t i = 0; i < n; ++i)
             if (z[i] >= K) fail(CULDA_ERR_ARG, "z[%lld] = %u is not below k", (long long)i, z[i]);
         hip_check(hipMemcpy(u.z, z, n * 2, hipMemcpyHostToDevice), "z copy");
-        e.theta_ready = e.phi_ready = false;
+        e.theta_ok[unit] = 0;  // this chunk's rows no longer match its z
+        e.phi_ready = false;
     } else {
         hip_check(hipMemcpy(z, u.z, n * 2, hipMemcpyDeviceToHost), "z copy");
     }
@@ -189,6 +190,14 @@ extern "C" void culda_destroy(culda_ctx *ctx) {
     delete ctx;
 }
 
+extern "C" int culda_check_corpus(const culda_corpus *corpus, int32_t k) {
+    return guarded([&] {
+        if (!corpus) fail(CULDA_ERR_ARG, "null corpus");
+        check_corpus_limits(corpus_ref(corpus), k);
+        return CULDA_OK;
+    });
+}
+
 extern "C" int culda_upload(culda_ctx *ctx, const culda_corpus *corpus) {
     CTX_CALL({
         if (!corpus) fail(CULDA_ERR_ARG, "null corpus");
@@ -228,14 +237,14 @@ extern "C" int culda_rebuild_theta(culda_ctx *ctx, int32_t chunk) {
     CTX_CALL({
         check_unit(e, chunk);
         e.rebuild_theta_unit(chunk, e.stream);
-        e.theta_ready = true;
     });
 }
 
 extern "C" int culda_sample(culda_ctx *ctx, int32_t chunk, int32_t iteration) {
     CTX_CALL({
         check_unit(e, chunk);
-        if (!e.theta_ready || !e.phi_ready) fail(CULDA_ERR_STATE, "sample before the phi/theta rebuild");
+        if (!e.phi_ready) fail(CULDA_ERR_STATE, "sample before the phi rebuild and head (culda_rebuild_phi, culda_phi_head)");
+        if (!e.theta_ok[chunk]) fail(CULDA_ERR_STATE, "sample of chunk %d before its theta rebuild (culda_rebuild_theta)", chunk);
         if (iteration < 1) fail(CULDA_ERR_ARG, "sweep index must be >= 1 (0 keys the z initialisation)");
         hip_check(hipStreamSynchronize(e.stream2), "sync");
         e.set_sweep((uint32_t)iteration, e.stream);
@@ -286,7 +295,7 @@ extern "C" int culda_autotune(culda_ctx *ctx, culda_tune *out) {
 extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
     CTX_CALL({
         if (!out) fail(CULDA_ERR_ARG, "null output");
-        if (!e.theta_ready) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
+        if (!e.theta_ready()) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
         e.fresh_rows(e.stream);
         e.sync();
         const ModelHost &m = e.models[0];
@@ -361,26 +370,28 @@ extern "C" int culda_save_phi(culda_ctx *ctx, const char *prefix) {
         if (!prefix) fail(CULDA_ERR_ARG, "null prefix");
         HostPhi h = download_phi(e, 0);
         const int32_t V = e.models[0].V, K = e.models[0].K;
-        FILE *f = open_or_fail(std::string(prefix) + ".phi.bin", "wb");
-        fwrite(&V, 4, 1, f);
-        fwrite(&K, 4, 1, f);
-        if (!h.nwk.empty()) fwrite(h.nwk.data(), 4, h.nwk.size(), f);
-        fwrite(h.nk.data(), 4, h.nk.size(), f);
-        fclose(f);
-        f = open_or_fail(std::string(prefix) + ".phi.txt", "w");
+        {
+            OutFile f(std::string(prefix) + ".phi.bin", "wb");
+            f.write(&V, 4, 1);
+            f.write(&K, 4, 1);
+            f.write(h.nwk.data(), 4, h.nwk.size());
+            f.write(h.nk.data(), 4, h.nk.size());
+            f.commit();
+        }
+        OutFile f(std::string(prefix) + ".phi.txt", "w");
         for (int32_t w = 0; w < V; ++w) {
-            fprintf(f, "Word id:%d, ", w);
+            f.printf("Word id:%d, ", w);
             for (int32_t k = 0; k < K; ++k) {
-                if (k % 32 == 0) fprintf(f, "\n%2d:", k / 32);
+                if (k % 32 == 0) f.printf("\n%2d:", k / 32);
                 const int32_t v = h.nwk[(size_t)w * K + k];
                 if (v == 0)
-                    fprintf(f, "_,");
+                    f.printf("_,");
                 else
-                    fprintf(f, "%d,", v);
+                    f.printf("%d,", v);
             }
-            fprintf(f, "\n");
+            f.printf("\n");
         }
-        fclose(f);
+        f.commit();
     });
 }
 
@@ -406,15 +417,17 @@ extern "C" int culda_save_theta(culda_ctx *ctx, const char *prefix) {
             gtop.insert(gtop.end(), top.begin(), top.end());
         }
         for (int32_t d = 0; d < D; ++d) goff[d + 1] += goff[d];
-        FILE *f = open_or_fail(std::string(prefix) + ".theta.bin", "wb");
-        fwrite(&D, 4, 1, f);
-        fwrite(&K, 4, 1, f);
-        fwrite(goff.data(), 8, goff.size(), f);
-        if (!gcnt.empty()) fwrite(gcnt.data(), 4, gcnt.size(), f);
-        if (!gtop.empty()) fwrite(gtop.data(), 2, gtop.size(), f);
-        fclose(f);
-        f = open_or_fail(std::string(prefix) + ".theta.txt", "w");
-        fprintf(f, "Doc, total, top topics\n");
+        {
+            OutFile f(std::string(prefix) + ".theta.bin", "wb");
+            f.write(&D, 4, 1);
+            f.write(&K, 4, 1);
+            f.write(goff.data(), 8, goff.size());
+            f.write(gcnt.data(), 4, gcnt.size());
+            f.write(gtop.data(), 2, gtop.size());
+            f.commit();
+        }
+        OutFile f(std::string(prefix) + ".theta.txt", "w");
+        f.printf("Doc, total, top topics\n");
         for (int32_t d = 0; d < D; ++d) {
             std::vector<std::pair<int32_t, int32_t>> row;
             int64_t tot = 0;
@@ -423,11 +436,11 @@ extern "C" int culda_save_theta(culda_ctx *ctx, const char *prefix) {
                 tot += gcnt[j];
             }
             std::stable_sort(row.begin(), row.end(), [](auto &a, auto &b) { return a.first > b.first; });
-            fprintf(f, "%4d| %lld ", d, (long long)tot);
-            for (auto &pr : row) fprintf(f, "(%d,%d), ", pr.second, pr.first);
-            fprintf(f, "\n");
+            f.printf("%4d| %lld ", d, (long long)tot);
+            for (auto &pr : row) f.printf("(%d,%d), ", pr.second, pr.first);
+            f.printf("\n");
         }
-        fclose(f);
+        f.commit();
     });
 }
 
@@ -438,20 +451,21 @@ extern "C" int culda_save_checkpoint(culda_ctx *ctx, const char *path, int32_t n
         if (!path) fail(CULDA_ERR_ARG, "null path");
         if (!e.uploaded) fail(CULDA_ERR_STATE, "no corpus uploaded");
         e.sync();
-        FILE *f = open_or_fail(path, "wb");
+        // written to path.tmp and renamed: an earlier checkpoint survives a failed save
+        OutFile f(path, "wb");
         const int32_t C = (int32_t)e.units.size(), K = e.prm.k;
-        fwrite(kCkptMagic, 1, 8, f);
-        fwrite(&C, 4, 1, f);
-        fwrite(&K, 4, 1, f);
-        fwrite(&ctx->seed, 8, 1, f);
-        fwrite(&next_iteration, 4, 1, f);
+        f.write(kCkptMagic, 1, 8);
+        f.write(&C, 4, 1);
+        f.write(&K, 4, 1);
+        f.write(&ctx->seed, 8, 1);
+        f.write(&next_iteration, 4, 1);
         for (auto &u : e.units) {
             std::vector<uint16_t> z(u.num_tokens);
             if (u.num_tokens) hip_check(hipMemcpy(z.data(), u.z, u.num_tokens * 2, hipMemcpyDeviceToHost), "z");
-            fwrite(&u.num_tokens, 8, 1, f);
-            if (!z.empty()) fwrite(z.data(), 2, z.size(), f);
+            f.write(&u.num_tokens, 8, 1);
+            f.write(z.data(), 2, z.size());
         }
-        fclose(f);
+        f.commit();
     });
 }
 
@@ -492,7 +506,8 @@ extern "C" int culda_load_checkpoint(culda_ctx *ctx, const char *path, int32_t *
         for (int32_t k = 0; k < C; ++k)
             if (!zs[k].empty())
                 hip_check(hipMemcpy(e.units[k].z, zs[k].data(), zs[k].size() * 2, hipMemcpyHostToDevice), "z");
-        e.theta_ready = e.phi_ready = false;
+        e.set_theta_ready(false);
+        e.phi_ready = false;
         *next_iteration = it;
     });
 }
@@ -624,7 +639,6 @@ extern "C" int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, 
 extern "C" int culda_mh_enable(culda_ctx *ctx, const culda_mh_params *params) {
     CTX_CALL({
         if (!params) fail(CULDA_ERR_ARG, "null params");
-        if (e.units.size() != 1) fail(CULDA_ERR_ARG, "learned MH needs a one-chunk corpus");
         e.mh_enable(*params);
     });
 }
@@ -713,7 +727,7 @@ extern "C" int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, c
     CTX_CALL({
         MHState &s = e.mh_state();
         if (!logits || n < 0 || n > s.prm.infer_batch) fail(CULDA_ERR_ARG, "n must be in [0, infer_batch]");
-        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        if (!e.theta_ready()) fail(CULDA_ERR_STATE, "needs rebuilt counts");
         e.sync();
         upload_queries(e, s, n, docs, words);
         e.mh_sketch_current(e.stream);
@@ -730,7 +744,7 @@ extern "C" int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs
         MHState &s = e.mh_state();
         if (n < 1 || n > s.prm.batch) fail(CULDA_ERR_ARG, "n must be in [1, batch]");
         if (!targets) fail(CULDA_ERR_ARG, "null targets");
-        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        if (!e.theta_ready()) fail(CULDA_ERR_STATE, "needs rebuilt counts");
         for (int32_t i = 0; i < n; ++i)
             if (targets[i] < 0 || targets[i] >= s.K) fail(CULDA_ERR_ARG, "target %d out of range", targets[i]);
         e.sync();
@@ -778,10 +792,14 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
     CTX_CALL({
         MHState &s = e.mh_state();
         if (!weights || !thresholds || !aliases) fail(CULDA_ERR_ARG, "null output");
-        const UnitHost &u = e.units[0];
-        if (n < 0 || n > s.prm.infer_batch || first < 0 || first + n > u.num_tokens)
-            fail(CULDA_ERR_ARG, "token range out of bounds");
-        if (!e.theta_ready || !e.phi_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        // first counts tokens over the chunks in order; the range stays in one chunk
+        int32_t ui = 0;
+        while (ui + 1 < (int32_t)e.units.size() && first >= s.unit_base[ui + 1]) ++ui;
+        const UnitHost &u = e.units[ui];
+        const int64_t base = s.unit_base[ui];
+        if (n < 0 || n > s.prm.infer_batch || first < 0 || first + n > base + u.num_tokens)
+            fail(CULDA_ERR_ARG, "token range out of bounds (it must lie in one chunk)");
+        if (!e.theta_ready() || !e.phi_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
         e.sync();
         const size_t cells = (size_t)n * s.K;
         int32_t *dw = (int32_t *)e.dalloc(std::max<size_t>(cells, 1) * 4);
@@ -793,16 +811,16 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
         const ModelHost &m = e.models[0];
         MHArgs a{};
         a.logits = s.Li;
-        a.word = s.tok_word;
-        a.gdelta = s.gdelta[0];
-        a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, e.d_stats, u.model,
-                         u.doc_end - u.doc_start};
+        a.word = s.tok_word + base;
+        a.gdelta = s.gdelta[ui];
+        a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, e.d_stats + 2 * ui,
+                         u.model, u.doc_end - u.doc_start};
         a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V};
         a.accepted = s.accepted;
         a.w_out = dw;
         a.f_out = df;
         a.a_out = da;
-        a.start = first;
+        a.start = first - base;
         a.n = n;
         a.proposals = 0;
         a.sweep = 0;
@@ -821,7 +839,7 @@ extern "C" int culda_mh_sketch(culda_ctx *ctx, uint16_t *out) {
     CTX_CALL({
         MHState &s = e.mh_state();
         if (!out) fail(CULDA_ERR_ARG, "null output");
-        if (!e.theta_ready) fail(CULDA_ERR_STATE, "needs rebuilt counts");
+        if (!e.theta_ready()) fail(CULDA_ERR_STATE, "needs rebuilt counts");
         e.sync();
         e.mh_sketch_current(e.stream);
         hip_check(hipMemcpyAsync(out, s.sk_cur, (size_t)s.D * kSketch * 2, hipMemcpyDeviceToHost, e.stream), "copy");

@@ -158,44 +158,35 @@ void write_corpus(const Corpus &c, const std::string &prefix) {
     c.validate();
     const int32_t C = (int32_t)c.chunks.size();
     {
-        FILE *f = fopen((prefix + ".vocab").c_str(), "w");
-        if (!f) fail(CULDA_ERR_IO, "%s.vocab open failed", prefix.c_str());
+        OutFile f(prefix + ".vocab", "w");
         for (int32_t w = 0; w < c.num_words; ++w) {
             if (c.vocab.empty())
-                fprintf(f, "w%d %d\n", w, w);
+                f.printf("w%d %d\n", w, w);
             else
-                fprintf(f, "%s %d\n", c.vocab[w].c_str(), w);
+                f.printf("%s %d\n", c.vocab[w].c_str(), w);
         }
-        fclose(f);
+        f.commit();
     }
     {
-        FILE *f = fopen((prefix + ".doc.idx").c_str(), "w");
-        if (!f) fail(CULDA_ERR_IO, "%s.doc.idx open failed", prefix.c_str());
-        for (int32_t d = 0; d < c.num_docs(); ++d) fprintf(f, "%d %lld\n", d, (long long)c.doc_cum[d]);
-        fclose(f);
+        OutFile f(prefix + ".doc.idx", "w");
+        for (int32_t d = 0; d < c.num_docs(); ++d) f.printf("%d %lld\n", d, (long long)c.doc_cum[d]);
+        f.commit();
     }
     {
-        FILE *f = fopen((prefix + ".chunk").c_str(), "w");
-        if (!f) fail(CULDA_ERR_IO, "%s.chunk open failed", prefix.c_str());
-        for (int32_t k = 0; k < C; ++k) fprintf(f, "%d %d\n", k, c.chunks[k].doc_end);
-        fclose(f);
+        OutFile f(prefix + ".chunk", "w");
+        for (int32_t k = 0; k < C; ++k) f.printf("%d %d\n", k, c.chunks[k].doc_end);
+        f.commit();
     }
     for (int32_t k = 0; k < C; ++k) {
         const Chunk &ch = c.chunks[k];
-        std::string ip = chunk_name(prefix, ".word.idx", k, C);
-        FILE *f = fopen(ip.c_str(), "w");
-        if (!f) fail(CULDA_ERR_IO, "%s open failed", ip.c_str());
-        for (int32_t w = 0; w < c.num_words; ++w) fprintf(f, "%d %lld\n", w, (long long)ch.word_idx[w + 1]);
-        fclose(f);
-        std::string dp = chunk_name(prefix, ".word.data", k, C);
-        f = fopen(dp.c_str(), "wb");
-        if (!f) fail(CULDA_ERR_IO, "%s open failed", dp.c_str());
-        if (!ch.word_tokens.empty() &&
-            fwrite(ch.word_tokens.data(), 4, ch.word_tokens.size(), f) != ch.word_tokens.size()) {
-            fclose(f);
-            fail(CULDA_ERR_IO, "short write on %s", dp.c_str());
+        {
+            OutFile f(chunk_name(prefix, ".word.idx", k, C), "w");
+            for (int32_t w = 0; w < c.num_words; ++w) f.printf("%d %lld\n", w, (long long)ch.word_idx[w + 1]);
+            f.commit();
         }
-        fclose(f);
+        OutFile f(chunk_name(prefix, ".word.data", k, C), "wb");
+        f.write(ch.word_tokens.data(), 4, ch.word_tokens.size());
+        f.commit();
     }
 }
 

@@ -13,15 +13,47 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
+#include <mutex>
 #include <numeric>
+#include <string>
 
 #include "kernels.h"
 #include "mh_state.h"
 
 namespace culda {
 
+// Fault injection for tests of the error paths (the reference aborts the
+// process on any CUDA error, culda_argument.h:40-48).  CULDA_FAIL_HIP =
+// "<text>" or "<text>:<n>" makes the n-th (default: first) checked HIP call
+// whose description contains <text> report a failure after the call itself
+// succeeded, so every kernel still runs and exits normally; the count
+// restarts when the variable's value changes.
+static bool injected_failure(const char *what) {
+    static std::mutex mu;
+    static std::string spec_seen, text;
+    static long target = 0, seen = 0;
+    const char *env = std::getenv("CULDA_FAIL_HIP");
+    if (!env || !*env) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (spec_seen != env) {
+        spec_seen = env;
+        text = spec_seen;
+        target = 1;
+        const size_t colon = text.rfind(':');
+        if (colon != std::string::npos) {
+            target = std::max(1L, std::strtol(text.c_str() + colon + 1, nullptr, 10));
+            text.resize(colon);
+        }
+        seen = 0;
+    }
+    if (text.empty() || !std::strstr(what, text.c_str())) return false;
+    return ++seen == target;
+}
+
 void hip_check(hipError_t e, const char *what) {
     if (e != hipSuccess) fail(CULDA_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+    if (injected_failure(what)) fail(CULDA_ERR_DEVICE, "%s: injected failure (CULDA_FAIL_HIP)", what);
 }
 
 Engine::~Engine() {
@@ -90,6 +122,39 @@ static T *upload_vec(Engine &e, const std::vector<T> &v) {
 // per token, and at most one per topic plus one per kThetaMaxCount tokens.
 int64_t theta_entries_max(int64_t nd, int32_t K) {
     return std::min<int64_t>(nd, (int64_t)K + (nd + kThetaMaxCount - 1) / kThetaMaxCount);
+}
+
+void check_corpus_limits(const Corpus &c, int32_t K) {
+    if (K < 16 || K > kMaxK || K % 16) fail(CULDA_ERR_ARG, "k = %d: the sampler needs a multiple of 16 in [16, %d]", K, kMaxK);
+    if (c.num_tokens() >= (int64_t)INT32_MAX)
+        fail(CULDA_ERR_LIMIT, "the corpus has %lld tokens; counts are int32 (at most %d)", (long long)c.num_tokens(),
+             INT32_MAX - 1);
+    int64_t longest = 0;
+    int32_t longest_doc = -1;
+    for (size_t k = 0; k < c.chunks.size(); ++k) {
+        const Chunk &ch = c.chunks[k];
+        if (ch.num_tokens() >= (int64_t)UINT32_MAX) fail(CULDA_ERR_LIMIT, "chunk %zu has >= 2^32 tokens", k);
+        int64_t cap = 0;
+        for (int32_t d = ch.doc_start; d < ch.doc_end; ++d) {
+            const int64_t e = theta_entries_max(c.doc_len(d), K);
+            if (e > longest) {
+                longest = e;
+                longest_doc = d;
+            }
+            cap += (e + 15) / 16 * 16;
+            // the sampler addresses a chunk's theta with 32-bit byte offsets
+            if (cap + kThetaPad >= ((int64_t)1 << 31))
+                fail(CULDA_ERR_LIMIT, "chunk %zu's theta capacity exceeds 2^31 entries: use more data_chunks", k);
+        }
+    }
+    // eight S-prefix rows of the longest theta row in the sampler's LDS
+    const int64_t lds = (int64_t)kSamplerStaticLds + 8 * 4 * (int64_t)sample_rs_stride(longest);
+    if (lds > kSamplerLdsBudget)
+        fail(CULDA_ERR_LIMIT,
+             "doc %d (%lld tokens) may need %lld theta entries at k = %d; the sampler's LDS holds rows of at most "
+             "%lld entries",
+             longest_doc, (long long)c.doc_len(longest_doc), (long long)longest, K,
+             (long long)((kSamplerLdsBudget - kSamplerStaticLds) / 32 - 8) / 4 * 4 * 16);
 }
 
 // Hot words are rebuilt in pieces of at most phi_split(tokens) tokens per
@@ -194,6 +259,7 @@ int32_t Engine::add_corpus(const Corpus &c, float alpha, float beta, uint64_t se
     if (!(alpha > 0.0f) || !std::isfinite(alpha)) fail(CULDA_ERR_ARG, "alpha must be positive and finite");
     if (!(beta > 0.0f) || !std::isfinite(beta)) fail(CULDA_ERR_ARG, "beta must be positive and finite");
     c.validate();
+    check_corpus_limits(c, prm.k);
     pending.push_back(&c);
     pending_alpha.push_back(alpha);
     pending_beta.push_back(beta);
@@ -443,7 +509,8 @@ void Engine::upload() {
     rebuild_slots(prm.slot_tokens);
     hip_check(hipDeviceSynchronize(), "upload");
     pending.clear();
-    theta_ready = phi_ready = sampled = false;
+    set_theta_ready(false);
+    phi_ready = sampled = false;
     zp_valid = false;
 }
 
@@ -499,9 +566,13 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
             for (int32_t w = 0; w < m.V; ++w) {
                 const int64_t b = u.word_idx_h[w], e = u.word_idx_h[w + 1], n = e - b;
                 if (n > 0) {
-                    const int64_t pieces = (n + slot_tokens - 1) / slot_tokens;
+                    // A rare word (fewer than K / 16 tokens, so at most 63) of
+                    // a one-unit model keeps all its tokens in one slot
+                    // whatever the slot size: the sampler may stage its row
+                    // from the tokens' zp topics, one token per lane.
+                    const int32_t rare = m.unit_end - m.unit_begin == 1 && n * 16 < m.K;
+                    const int64_t pieces = rare ? 1 : (n + slot_tokens - 1) / slot_tokens;
                     const int64_t step = (n + pieces - 1) / pieces;
-                    const int32_t rare = m.unit_end - m.unit_begin == 1 && pieces == 1 && n * 16 < m.K;
                     for (int64_t x = b; x < e; x += step)
                         sl.push_back(SlotDesc{w, ui, x, std::min(e, x + step), run[w] - b, rare, 0});
                 }
@@ -564,7 +635,8 @@ void Engine::init_z() {
     if (!uploaded) fail(CULDA_ERR_STATE, "init_z before upload");
     InitArgs a{d_slots, d_units, d_models, num_slots, 0};
     hip_check(launch_init_z(a, stream), "init_z launch");
-    theta_ready = phi_ready = false;
+    set_theta_ready(false);
+    phi_ready = false;
 }
 
 // unit < 0: model's chunks in one pass (overwrite).  unit >= 0: that unit
@@ -649,13 +721,14 @@ void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
     // a single chunk's rebuild has nothing beside it: the next sweep waits
     hip_check(launch_theta_rebuild(a, units.size() == 1, s), "theta rebuild launch");
+    theta_ok[ui] = 1;
 }
 
 void Engine::rebuild_theta_all(hipStream_t s) {
     hip_check(launch_fill_u32((uint32_t *)d_stats, 4 * (int64_t)units.size(), 0, s), "stats clear");
     ThetaArgs a{d_theta_packets_all, d_units, d_models, num_theta_packets_all, 0};
     hip_check(launch_theta_rebuild(a, false, s), "theta rebuild launch");
-    theta_ready = true;
+    set_theta_ready(true);
 }
 
 void Engine::sample_unit(int32_t ui, hipStream_t s) {
@@ -777,7 +850,7 @@ void Engine::enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_d
 }
 
 void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
-    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (!uploaded || !theta_ready() || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
     if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
     const int32_t U = (int32_t)units.size();
     const bool prof = prm.profile != 0;
@@ -816,7 +889,7 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
 }
 
 void Engine::iterate_units_n(uint32_t first, int32_t count, double *ll, double *seconds) {
-    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (!uploaded || !theta_ready() || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
     if (models.size() != 1) fail(CULDA_ERR_STATE, "per-chunk iteration serves a single corpus");
     if (count < 1 || count > kLLHistory) fail(CULDA_ERR_LIMIT, "count must be in [1, %d]", kLLHistory);
     const int32_t U = (int32_t)units.size();
@@ -899,7 +972,7 @@ void Engine::enqueue_batch_iteration(hipStream_t s, hipEvent_t before, hipEvent_
 }
 
 void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double *ll, double *seconds) {
-    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
+    if (!uploaded || !theta_ready() || !phi_ready) fail(CULDA_ERR_STATE, "iterate before the initial rebuild");
     if (first < 1 || count < 1) fail(CULDA_ERR_ARG, "first iteration must be >= 1 and count >= 1");
     if (count > kLLHistory) fail(CULDA_ERR_LIMIT, "at most %d iterations per call", kLLHistory);
     hip_check(hipStreamSynchronize(stream2), "sync");
@@ -980,7 +1053,7 @@ double Engine::sweep_bytes() const {
 // and restored, counts rebuilt), then keeps the fastest.  Results never
 // depend on the slot size, so tuning leaves the chain unchanged.
 TuneResult Engine::autotune(const std::vector<int32_t> &candidates, bool batch) {
-    if (!uploaded || !theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "autotune needs the initial rebuild");
+    if (!uploaded || !theta_ready() || !phi_ready) fail(CULDA_ERR_STATE, "autotune needs the initial rebuild");
     hip_check(hipStreamSynchronize(stream2), "sync");
     std::vector<uint16_t *> saved(units.size());
     for (size_t u = 0; u < units.size(); ++u) {

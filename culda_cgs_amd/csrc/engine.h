@@ -19,6 +19,9 @@ namespace culda {
 
 void hip_check(hipError_t e, const char *what);
 int64_t theta_entries_max(int64_t nd, int32_t K);
+// The documented size limits a corpus must meet for a model with K topics
+// (fails with CULDA_ERR_LIMIT); needs no device.
+void check_corpus_limits(const Corpus &c, int32_t K);
 
 // A phi work list with its workgroup packets and the hot-word rows to zero.
 struct PhiList {
@@ -90,7 +93,17 @@ public:
     std::vector<const Corpus *> pending;  // corpora added, not yet uploaded
     std::vector<float> pending_alpha, pending_beta;
     std::vector<uint64_t> pending_seed;
-    bool uploaded = false, theta_ready = false, phi_ready = false, sampled = false;
+    bool uploaded = false, phi_ready = false, sampled = false;
+    // theta_ok[u]: unit u's theta rows have been built from some z (an
+    // unbuilt row has no entries, which the sampler must never read)
+    std::vector<char> theta_ok;
+    bool theta_ready() const {
+        if (!uploaded) return false;
+        for (char c : theta_ok)
+            if (!c) return false;
+        return true;
+    }
+    void set_theta_ready(bool v) { theta_ok.assign(units.size(), v ? 1 : 0); }
     // n_wk / n_k equal the counts of every unit's zp: set by full count
     // rebuilds (which copy z to zp), kept by count deltas, cleared by
     // anything else that writes the counts

@@ -42,6 +42,12 @@ constexpr int kThetaCountShift = 10;    // theta entry: count in bits 10..15
 constexpr uint32_t kThetaTopicMask = 1023u;  // topic in bits 0..9 (K <= kMaxK)
 constexpr uint32_t kThetaMaxCount = 63u;     // largest count of one entry
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
+// The sampler's LDS: phi row, Q block totals, z and S+Q of a 64-token batch
+// (static), plus eight rows of S block prefixes sized by the longest theta
+// row of any doc (dynamic).  Corpora are checked against this budget (the
+// HIP per-workgroup limit) before anything is allocated.
+constexpr int kSamplerLdsBudget = 65536;
+constexpr int kSamplerStaticLds = kMaxK * 4 + kWave * 4 + kWave * 2 + kWave * 4;
 
 struct DocMeta {
     uint32_t start;  // theta slab start (entries)

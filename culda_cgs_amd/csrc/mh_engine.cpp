@@ -79,7 +79,7 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     double sum = 0;
     for (double v : part) sum += v;
     phi_ready = true;
-    theta_ready = true;
+    set_theta_ready(true);
     return sum / (double)eval.num_tokens;
 }
 
@@ -274,7 +274,7 @@ void Engine::mh_sketch_current(hipStream_t st) {
 
 int32_t Engine::mh_collect() {
     MHState &s = mh_state();
-    if (!theta_ready) fail(CULDA_ERR_STATE, "collect needs rebuilt counts");
+    if (!theta_ready()) fail(CULDA_ERR_STATE, "collect needs rebuilt counts");
     if (s.nsnap >= s.prm.max_snapshots) fail(CULDA_ERR_LIMIT, "all %d snapshots taken", s.prm.max_snapshots);
     hip_check(hipStreamSynchronize(stream2), "sync");
     const ModelHost &m = models[0];
@@ -407,7 +407,7 @@ void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, in
 
 double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
     MHState &s = mh_state();
-    if (!theta_ready || !phi_ready) fail(CULDA_ERR_STATE, "MH iteration needs rebuilt counts");
+    if (!theta_ready() || !phi_ready) fail(CULDA_ERR_STATE, "MH iteration needs rebuilt counts");
     if (proposals < 0 || proposals > kMaxProposals) fail(CULDA_ERR_ARG, "proposals must be in [0, %d]", kMaxProposals);
     hip_check(hipStreamSynchronize(stream2), "sync");
     const ModelHost &m = models[0];

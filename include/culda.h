@@ -175,6 +175,18 @@ typedef struct culda_params {
 int culda_create(const culda_params *params, culda_ctx **out);
 void culda_destroy(culda_ctx *ctx);
 
+/* Check a corpus against the documented size limits for k topics, without
+ * a device (culda_upload and culda_batch_add run the same checks first):
+ * k a multiple of 16 in [16, 1024] (CULDA_ERR_ARG); fewer than 2^31 - 1
+ * tokens (int32 counts); fewer than 2^32 tokens and 2^31 theta entries per
+ * chunk; every doc's theta row (at most min(n_d, k + ceil(n_d / 63))
+ * entries) within the sampler's LDS, 30,272 entries (about 1.8M tokens per
+ * doc at k = 1024).  Returns CULDA_ERR_LIMIT naming the first limit
+ * exceeded.  No counterpart in the reference, which sizes rows from
+ * ModelTheta::InitData (model/model_theta.cpp:44-86) and fails inside a
+ * kernel when they do not fit. */
+int culda_check_corpus(const culda_corpus *corpus, int32_t k);
+
 /* Copy the corpus to HBM and build the per-chunk layouts (word-major doc
  * stream, doc-major token index, theta slabs, wave slots).  Replaces
  * DocChunk::allocGPU/toGPU (model/doc_chunk.cpp:277-356), ModelPhi::InitData
@@ -359,11 +371,14 @@ int culda_heldout_ll(culda_ctx *train, culda_ctx *fold, int32_t fold_in_sweeps, 
  * from the integer-weight alias table of its softmax and accepts with the
  * exact MH ratio of the token's collapsed conditional (the plain sampler's
  * frozen-count conditional), so the plain sampler's per-token target is
- * left invariant.  Requires k % 64 == 0 and one data_chunk.
+ * left invariant.  Requires k % 64 == 0; a corpus of several data_chunks
+ * moves every chunk's tokens with the same frozen counts (the chunks-in-turn
+ * schedule of train/MultiChunkMultiGPUequal.h:88-172), each chunk's docs
+ * through its own theta rows.
  * ---------------------------------------------------------------------- */
 typedef struct culda_mh_params {
-    int32_t batch;          /* training rows per Adam step, multiple of 64 (0: 8192)   */
-    int32_t infer_batch;    /* tokens per proposal launch, multiple of 64 (0: 65536)  */
+    int32_t batch;          /* training rows per Adam step, multiple of 128 (0: 8192)  */
+    int32_t infer_batch;    /* tokens per proposal launch, multiple of 128 (0: 65536) */
     int32_t max_snapshots;  /* training snapshots kept (0: 21)                         */
     int32_t pad;
     float lr, beta1, beta2, eps;  /* Adam (0: 1e-3, 0.9, 0.999, 1e-8)                 */
@@ -395,7 +410,9 @@ int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, const int32_
 int culda_mh_gradients(culda_ctx *ctx, int32_t n, const int32_t *docs, const int32_t *words, const int32_t *targets,
                        int32_t train_step, double *loss, float *dw1, float *db1, float *dw2, float *db2, float *demb);
 /* Proposal tables of tokens [first, first + n) (n <= infer_batch): integer
- * weights, alias thresholds and aliases, [n][K] each; z is unchanged. */
+ * weights, alias thresholds and aliases, [n][K] each; z is unchanged.
+ * Tokens are counted over the chunks in order (chunk c's tokens follow
+ * chunk c-1's, word-major within a chunk); the range lies in one chunk. */
 int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t *weights, uint32_t *thresholds,
                     uint32_t *aliases);
 /* Current doc sketches, [D][256] BF16 bit patterns. */

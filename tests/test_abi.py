@@ -60,3 +60,27 @@ def test_cli_missing_input_reports_and_exits(tmp_path):
     assert "rounded to multiples of 32" in r.stdout
     assert "alpha       :0.048828" in r.stdout  # 50/1024, not atof's 50
     assert r.returncode == 2 and "open failed" in r.stderr
+
+
+def test_check_corpus_limits_without_a_device():
+    """culda_check_corpus (run by culda_upload before anything is allocated)
+    refuses k outside the sampler's range and a doc whose theta row would not
+    fit the sampler's LDS rows (30,272 entries: a doc of ~1.9M tokens at
+    k = 16); an ordinary corpus passes."""
+    import numpy as np
+    from helpers import chunks_from_docs, product_corpus
+    c = cg.Corpus.generate(300, 50, 5, 1, len_mean=30.0, seed=3, num_threads=2)
+    for k in (16, 64, 1024):
+        c.check(k)
+    for k in (0, 8, 1000, 2048):
+        with pytest.raises(cg.CuldaError) as e:
+            c.check(k)
+        assert e.value.status == -1
+    rng = np.random.default_rng(0)
+    V = 50
+    big = product_corpus(V, *chunks_from_docs(V, [rng.integers(0, V, 1_950_000), rng.integers(0, V, 10)], 1))
+    with pytest.raises(cg.CuldaError) as e:
+        big.check(16)
+    assert e.value.status == -7 and "LDS" in str(e.value) and "doc 0" in str(e.value)
+    ok = product_corpus(V, *chunks_from_docs(V, [rng.integers(0, V, 1_800_000), rng.integers(0, V, 10)], 1))
+    ok.check(16)

@@ -165,3 +165,24 @@ def test_heldout_split_errors():
         full.heldout_split(1)
     with pytest.raises(cg.CuldaError):
         full.heldout_split(10)  # no doc held out: the held-out corpora would be empty
+
+
+def test_write_failure_reports_io_error_and_leaves_no_partial_file(tmp_path):
+    """A save whose bytes cannot reach the disk (the temporary file is
+    /dev/full, so the close reports ENOSPC) fails with CULDA_ERR_IO, and
+    neither the temporary nor a truncated target file is left behind; an
+    existing good file survives a failed rewrite."""
+    import os
+    c = cg.Corpus.generate(50, 20, 3, 1, len_mean=10.0, seed=1, num_threads=1)
+    prefix = tmp_path / "x"
+    c.write(prefix)
+    good = (tmp_path / "x.vocab").read_text()
+    os.symlink("/dev/full", tmp_path / "x.vocab.tmp")
+    with pytest.raises(cg.CuldaError) as e:
+        c.write(prefix)
+    assert e.value.status == -2
+    assert not (tmp_path / "x.vocab.tmp").exists() and not os.path.islink(tmp_path / "x.vocab.tmp")
+    assert (tmp_path / "x.vocab").read_text() == good
+    with pytest.raises(cg.CuldaError) as e:
+        c.write(tmp_path / "missing_dir" / "x")
+    assert e.value.status == -2
