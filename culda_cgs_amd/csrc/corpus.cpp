@@ -8,11 +8,13 @@
 #include "corpus.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cerrno>
 #include <cstring>
 #include <fstream>
 #include <memory>
 #include <sstream>
+#include <thread>
 #include <unordered_map>
 
 namespace culda {
@@ -30,7 +32,6 @@ void Corpus::validate() const {
         if (doc_len(d) < 0) fail(CULDA_ERR_FORMAT, "doc.idx not cumulative at doc %d", d);
     if (chunks.empty()) fail(CULDA_ERR_FORMAT, "corpus has no chunks");
     int32_t expect_start = 0;
-    std::vector<int64_t> seen(D, 0);
     for (size_t c = 0; c < chunks.size(); ++c) {
         const Chunk &ch = chunks[c];
         if (ch.doc_start != expect_start || ch.doc_end < ch.doc_start || ch.doc_end > D)
@@ -47,17 +48,53 @@ void Corpus::validate() const {
         if (ch.word_idx[num_words] != ch.num_tokens())
             fail(CULDA_ERR_FORMAT, "chunk %zu: word.idx ends at %lld but word.data has %lld tokens",
                  c, (long long)ch.word_idx[num_words], (long long)ch.num_tokens());
-        for (int32_t d : ch.word_tokens) {
-            if (d < ch.doc_start || d >= ch.doc_end)
-                fail(CULDA_ERR_FORMAT, "chunk %zu holds doc id %d outside its range [%d,%d)", c, d,
-                     ch.doc_start, ch.doc_end);
-            seen[d]++;
-        }
     }
-    for (int32_t d = 0; d < D; ++d)
-        if (seen[d] != doc_len(d))
-            fail(CULDA_ERR_FORMAT, "doc %d has %lld tokens in word.data but length %lld in doc.idx",
-                 d, (long long)seen[d], (long long)doc_len(d));
+    if (expect_start != D) fail(CULDA_ERR_FORMAT, "chunks cover docs [0,%d) of %d", expect_start, D);
+    // every token's doc inside its chunk, and per-doc token counts equal to
+    // the doc lengths: each thread counts the docs of a range of chunks
+    // (one chunk split by docs when there are fewer chunks than threads)
+    struct Job {
+        size_t chunk;
+        int32_t da, db;
+    };
+    std::vector<Job> jobs;
+    const int T = host_threads();
+    for (size_t c = 0; c < chunks.size(); ++c) {
+        const Chunk &ch = chunks[c];
+        const int64_t parts = std::max<int64_t>(1, std::min<int64_t>(T, ch.num_tokens() / (1 << 22)));
+        for (int64_t q = 0; q < parts; ++q)
+            jobs.push_back(Job{c, (int32_t)(ch.doc_start + (int64_t)(ch.doc_end - ch.doc_start) * q / parts),
+                               (int32_t)(ch.doc_start + (int64_t)(ch.doc_end - ch.doc_start) * (q + 1) / parts)});
+    }
+    std::vector<std::string> errs(jobs.size());
+    parallel_for((int64_t)jobs.size(), T, [&](int64_t a, int64_t b) {
+        std::vector<int64_t> seen;
+        for (int64_t j = a; j < b; ++j) {
+            const Job &jb = jobs[j];
+            const Chunk &ch = chunks[jb.chunk];
+            seen.assign(jb.db - jb.da, 0);
+            char buf[256];
+            for (int32_t d : ch.word_tokens) {
+                if (d < ch.doc_start || d >= ch.doc_end) {
+                    snprintf(buf, sizeof buf, "chunk %zu holds doc id %d outside its range [%d,%d)", jb.chunk, d,
+                             ch.doc_start, ch.doc_end);
+                    errs[j] = buf;
+                    break;
+                }
+                if (d >= jb.da && d < jb.db) seen[d - jb.da]++;
+            }
+            if (!errs[j].empty()) continue;
+            for (int32_t d = jb.da; d < jb.db; ++d)
+                if (seen[d - jb.da] != doc_len(d)) {
+                    snprintf(buf, sizeof buf, "doc %d has %lld tokens in word.data but length %lld in doc.idx", d,
+                             (long long)seen[d - jb.da], (long long)doc_len(d));
+                    errs[j] = buf;
+                    break;
+                }
+        }
+    });
+    for (const std::string &e : errs)
+        if (!e.empty()) fail(CULDA_ERR_FORMAT, "%s", e.c_str());
 }
 
 // ---------------------------------------------------------------- reading
@@ -208,9 +245,21 @@ std::vector<int32_t> greedy_chunk_ends(const std::vector<int64_t> &doc_len, int3
     return ends;
 }
 
+int host_threads() {
+    const char *env = std::getenv("OMP_NUM_THREADS");
+    int n = env ? std::atoi(env) : 0;
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 64));
+}
+
+// Stable counting sort by word of each chunk's doc-ordered tokens: thread
+// t counts a contiguous token range, then writes it at its own per-word
+// offsets (after threads 0..t-1 within each word), so every word's docs
+// stay ascending whatever the thread count.
 void build_chunks_from_doc_major(Corpus &c, const std::vector<int32_t> &tok_word,
-                                 const std::vector<int32_t> &chunk_ends) {
+                                 const std::vector<int32_t> &chunk_ends, int threads) {
     const int32_t V = c.num_words, C = (int32_t)chunk_ends.size();
+    if (threads <= 0) threads = host_threads();
     c.chunks.assign(C, Chunk());
     int32_t d0 = 0;
     for (int32_t k = 0; k < C; ++k) {
@@ -220,16 +269,38 @@ void build_chunks_from_doc_major(Corpus &c, const std::vector<int32_t> &tok_word
         d0 = ch.doc_end;
         const int64_t t0 = ch.doc_start ? c.doc_cum[ch.doc_start - 1] : 0;
         const int64_t t1 = ch.doc_end ? c.doc_cum[ch.doc_end - 1] : 0;
+        const int64_t n = t1 - t0;
+        const int T = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n / 65536));
+        const int64_t step = (n + T - 1) / T;
+        std::vector<std::vector<int64_t>> cnt(T, std::vector<int64_t>(V, 0));
+        parallel_for(T, T, [&](int64_t a, int64_t b) {
+            for (int64_t t = a; t < b; ++t)
+                for (int64_t i = t0 + t * step; i < std::min(t1, t0 + (t + 1) * step); ++i) cnt[t][tok_word[i]]++;
+        });
         ch.word_idx.assign(V + 1, 0);
-        for (int64_t t = t0; t < t1; ++t) ch.word_idx[tok_word[t] + 1]++;
-        for (int32_t w = 0; w < V; ++w) ch.word_idx[w + 1] += ch.word_idx[w];
-        ch.word_tokens.resize(t1 - t0);
-        std::vector<int64_t> pos(ch.word_idx.begin(), ch.word_idx.end() - 1);
-        int32_t d = ch.doc_start;
-        for (int64_t t = t0; t < t1; ++t) {
-            while (c.doc_cum[d] <= t) ++d;  // token t belongs to the first doc whose cum exceeds t
-            ch.word_tokens[pos[tok_word[t]]++] = d;
+        for (int32_t w = 0; w < V; ++w) {
+            int64_t acc = ch.word_idx[w];
+            for (int t = 0; t < T; ++t) {
+                const int64_t x = cnt[t][w];
+                cnt[t][w] = acc;  // now thread t's write position for word w
+                acc += x;
+            }
+            ch.word_idx[w + 1] = acc;
         }
+        ch.word_tokens.resize(n);
+        parallel_for(T, T, [&](int64_t a, int64_t b) {
+            for (int64_t t = a; t < b; ++t) {
+                const int64_t i0 = t0 + t * step, i1 = std::min(t1, t0 + (t + 1) * step);
+                if (i0 >= i1) continue;
+                // token i belongs to the first doc whose cumulative length exceeds i
+                int32_t d = (int32_t)(std::upper_bound(c.doc_cum.begin(), c.doc_cum.end(), i0) - c.doc_cum.begin());
+                std::vector<int64_t> &pos = cnt[t];
+                for (int64_t i = i0; i < i1; ++i) {
+                    while (c.doc_cum[d] <= i) ++d;
+                    ch.word_tokens[pos[tok_word[i]]++] = d;
+                }
+            }
+        });
     }
 }
 
@@ -270,14 +341,34 @@ void format_text(const std::string &input, const std::string &prefix, int32_t nu
 }
 
 // Doc-major word ids of every token: the tokens of doc d, in ascending
-// word order, at [doc_cum[d-1], doc_cum[d]).
+// word order, at [doc_cum[d-1], doc_cum[d]).  Each thread owns a range of
+// docs (about equal token counts) and takes its docs' tokens from one scan
+// of the word-major streams.
 static std::vector<int32_t> doc_major_words(const Corpus &c) {
-    std::vector<int32_t> out(c.num_tokens());
-    std::vector<int64_t> cur(c.num_docs());
-    for (int32_t d = 0; d < c.num_docs(); ++d) cur[d] = d ? c.doc_cum[d - 1] : 0;
-    for (const Chunk &ch : c.chunks)
-        for (int32_t w = 0; w < c.num_words; ++w)
-            for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i) out[cur[ch.word_tokens[i]]++] = w;
+    const int32_t D = c.num_docs();
+    const int64_t N = c.num_tokens();
+    std::vector<int32_t> out(N);
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), N / 65536));
+    std::vector<int32_t> dcut(T + 1, D);
+    dcut[0] = 0;
+    for (int t = 1; t < T; ++t)
+        dcut[t] = (int32_t)(std::lower_bound(c.doc_cum.begin(), c.doc_cum.end(), N / T * t) - c.doc_cum.begin());
+    parallel_for(T, T, [&](int64_t a, int64_t b) {
+        for (int64_t t = a; t < b; ++t) {
+            const int32_t da = dcut[t], db = std::max(dcut[t], dcut[t + 1]);
+            if (da >= db) continue;
+            std::vector<int64_t> cur(db - da);
+            for (int32_t d = da; d < db; ++d) cur[d - da] = d ? c.doc_cum[d - 1] : 0;
+            for (const Chunk &ch : c.chunks) {
+                if (ch.doc_end <= da || ch.doc_start >= db) continue;
+                for (int32_t w = 0; w < c.num_words; ++w)
+                    for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i) {
+                        const int32_t d = ch.word_tokens[i];
+                        if (d >= da && d < db) out[cur[d - da]++] = w;
+                    }
+            }
+        }
+    });
     return out;
 }
 
@@ -286,28 +377,51 @@ void heldout_split(const Corpus &c, int32_t every, int32_t train_chunks, Corpus 
     if (every < 2) fail(CULDA_ERR_ARG, "held-out period must be >= 2 (every %d-th doc)", every);
     if (train_chunks < 1) fail(CULDA_ERR_ARG, "number of chunks must be >= 1");
     const std::vector<int32_t> words = doc_major_words(c);
-    std::vector<int32_t> tw, ow, ew;
-    std::vector<int64_t> tl, ol, el;
-    for (int32_t d = 0; d < c.num_docs(); ++d) {
-        const int64_t b = d ? c.doc_cum[d - 1] : 0, e = c.doc_cum[d];
+    const int32_t D = c.num_docs();
+    // output position of every doc's tokens: a training doc's in tw, a
+    // held-out doc's even positions in ow and odd ones in ew
+    std::vector<int64_t> tl, ol, el, dst(D);
+    int64_t nt = 0, no = 0, ne = 0;
+    for (int32_t d = 0; d < D; ++d) {
+        const int64_t L = c.doc_len(d);
         if (d % every == every - 1) {
-            int64_t no = 0, ne = 0;
-            for (int64_t t = b; t < e; ++t) {
-                if ((t - b) % 2 == 0) {
-                    ow.push_back(words[t]);
-                    ++no;
-                } else {
-                    ew.push_back(words[t]);
-                    ++ne;
-                }
-            }
-            ol.push_back(no);
-            el.push_back(ne);
+            dst[d] = no;  // the odd positions' start in ew: edst
+            ol.push_back((L + 1) / 2);
+            el.push_back(L / 2);
+            no += (L + 1) / 2;
+            ne += L / 2;
         } else {
-            tw.insert(tw.end(), words.begin() + b, words.begin() + e);
-            tl.push_back(e - b);
+            dst[d] = nt;
+            tl.push_back(L);
+            nt += L;
         }
     }
+    std::vector<int32_t> tw(nt), ow(no), ew(ne);
+    std::vector<int64_t> edst(D, 0);
+    {
+        int64_t e = 0;
+        for (int32_t d = 0; d < D; ++d)
+            if (d % every == every - 1) {
+                edst[d] = e;
+                e += c.doc_len(d) / 2;
+            }
+    }
+    parallel_for(D, host_threads(), [&](int64_t a, int64_t b) {
+        for (int64_t d = a; d < b; ++d) {
+            const int64_t s = d ? c.doc_cum[d - 1] : 0, e = c.doc_cum[d];
+            if (d % every == every - 1) {
+                int64_t o = dst[d], x = edst[d];
+                for (int64_t t = s; t < e; ++t) {
+                    if ((t - s) % 2 == 0)
+                        ow[o++] = words[t];
+                    else
+                        ew[x++] = words[t];
+                }
+            } else {
+                std::copy(words.begin() + s, words.begin() + e, tw.begin() + dst[d]);
+            }
+        }
+    });
     auto make = [&](Corpus &out, const std::vector<int32_t> &w, const std::vector<int64_t> &lens, int32_t chunks) {
         out = Corpus();
         out.num_words = c.num_words;
@@ -348,25 +462,31 @@ Corpus rechunk(const Corpus &src, int32_t num_chunks) {
         d0 = ends[k];
         out.chunks[k].word_idx.assign(V + 1, 0);
     }
-    for (const Chunk &ch : src.chunks)
-        for (int32_t w = 0; w < V; ++w)
-            for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i)
-                out.chunks[doc_chunk[ch.word_tokens[i]]].word_idx[w + 1]++;
-    for (Chunk &ch : out.chunks) {
-        for (int32_t w = 0; w < V; ++w) ch.word_idx[w + 1] += ch.word_idx[w];
-        ch.word_tokens.resize(ch.word_idx[V]);
-    }
-    std::vector<std::vector<int64_t>> pos(num_chunks);
-    for (int32_t k = 0; k < num_chunks; ++k)
-        pos[k].assign(out.chunks[k].word_idx.begin(), out.chunks[k].word_idx.end() - 1);
-    // word-major over source chunks in doc order keeps docs ascending per word
-    for (int32_t w = 0; w < V; ++w)
-        for (const Chunk &ch : src.chunks)
-            for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i) {
-                int32_t d = ch.word_tokens[i];
-                int32_t k = doc_chunk[d];
-                out.chunks[k].word_tokens[pos[k][w]++] = d;
+    // one thread per output chunk: it scans the source and keeps its docs'
+    // tokens; word-major over the source chunks in doc order keeps docs
+    // ascending within every word
+    parallel_for(num_chunks, std::min(num_chunks, host_threads()), [&](int64_t a, int64_t b) {
+        for (int64_t k = a; k < b; ++k) {
+            Chunk &oc = out.chunks[k];
+            for (const Chunk &ch : src.chunks) {
+                if (ch.doc_end <= oc.doc_start || ch.doc_start >= oc.doc_end) continue;
+                for (int32_t w = 0; w < V; ++w)
+                    for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i)
+                        oc.word_idx[w + 1] += doc_chunk[ch.word_tokens[i]] == k;
             }
+            for (int32_t w = 0; w < V; ++w) oc.word_idx[w + 1] += oc.word_idx[w];
+            oc.word_tokens.resize(oc.word_idx[V]);
+            int64_t o = 0;
+            for (int32_t w = 0; w < V; ++w)
+                for (const Chunk &ch : src.chunks) {
+                    if (ch.doc_end <= oc.doc_start || ch.doc_start >= oc.doc_end) continue;
+                    for (int64_t i = ch.word_idx[w]; i < ch.word_idx[w + 1]; ++i) {
+                        const int32_t d = ch.word_tokens[i];
+                        if (doc_chunk[d] == k) oc.word_tokens[o++] = d;
+                    }
+                }
+        }
+    });
     out.validate();
     return out;
 }

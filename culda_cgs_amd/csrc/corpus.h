@@ -4,8 +4,10 @@
 // DocChunk (model/doc_chunk.h:19) host data: per-doc cumulative lengths,
 // and per chunk the word-major doc-id stream with chunk-local word offsets.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -57,9 +59,31 @@ void heldout_split(const Corpus &c, int32_t every, int32_t train_chunks, Corpus 
 Corpus rechunk(const Corpus &c, int32_t num_chunks);
 
 // Build word-major chunks from per-token (word, doc) pairs given in doc
-// order (docs ascending).  Used by the generator and the text formatter.
+// order (docs ascending).  Used by the generator, the text formatter and
+// the held-out split; threads <= 0 takes host_threads().
 void build_chunks_from_doc_major(Corpus &c, const std::vector<int32_t> &tok_word,
-                                 const std::vector<int32_t> &chunk_ends);
+                                 const std::vector<int32_t> &chunk_ends, int threads = 0);
+
+// Threads for host-side layout work: OMP_NUM_THREADS when set, else the
+// hardware threads; at most 64.
+int host_threads();
+
+// f(a, b) over [0, n) cut into `threads` contiguous ranges, one thread each.
+template <class F>
+void parallel_for(int64_t n, int threads, F &&f) {
+    if (threads <= 1 || n < 2) {
+        f((int64_t)0, n);
+        return;
+    }
+    std::vector<std::thread> ts;
+    const int64_t step = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const int64_t a = t * step, b = std::min(n, a + step);
+        if (a >= b) break;
+        ts.emplace_back([&f, a, b] { f(a, b); });
+    }
+    for (auto &t : ts) t.join();
+}
 
 Corpus generate_corpus(const culda_gen_params &p);
 

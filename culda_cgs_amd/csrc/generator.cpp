@@ -4,7 +4,10 @@
 // the corpus is identical for any thread count.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <thread>
+
+#include <sys/mman.h>
 
 #include "corpus.h"
 #include "philox.h"
@@ -64,39 +67,65 @@ struct Rng {
     }
 };
 
-// Normalised CDF from log-weights.
-void cdf_from_logw(const double *lw, int n, double *cdf) {
+// Walker alias table of a distribution given by log-weights (Vose's
+// construction): bucket j keeps outcome j when 32 random bits fall below
+// the low word of cell[j], else gives the outcome in its high word.
+void alias_from_logw(const double *lw, int n, uint64_t *cell, std::vector<double> &scaled,
+                     std::vector<int32_t> &small, std::vector<int32_t> &large) {
     double m = -INFINITY;
     for (int i = 0; i < n; ++i) m = std::max(m, lw[i]);
     double acc = 0.0;
+    scaled.resize(n);
+    for (int i = 0; i < n; ++i) acc += scaled[i] = std::exp(lw[i] - m);
+    small.clear();
+    large.clear();
     for (int i = 0; i < n; ++i) {
-        acc += std::exp(lw[i] - m);
-        cdf[i] = acc;
+        scaled[i] *= (double)n / acc;
+        (scaled[i] < 1.0 ? small : large).push_back(i);
     }
-    for (int i = 0; i < n; ++i) cdf[i] /= acc;
-    cdf[n - 1] = 1.0;
+    while (!small.empty() && !large.empty()) {
+        const int32_t l = small.back(), g = large.back();
+        small.pop_back();
+        const uint64_t keep = (uint64_t)std::min(4294967295.0, std::floor(scaled[l] * 4294967296.0));
+        cell[l] = keep | ((uint64_t)(uint32_t)g << 32);
+        scaled[g] = (scaled[g] + scaled[l]) - 1.0;
+        if (scaled[g] < 1.0) {
+            large.pop_back();
+            small.push_back(g);
+        }
+    }
+    // leftovers (rounding) keep themselves
+    for (int32_t i : large) cell[i] = 0xffffffffull | ((uint64_t)(uint32_t)i << 32);
+    for (int32_t i : small) cell[i] = 0xffffffffull | ((uint64_t)(uint32_t)i << 32);
 }
 
-int search_cdf(const double *cdf, int n, double u) {
-    int i = (int)(std::upper_bound(cdf, cdf + n, u) - cdf);
-    return i < n ? i : n - 1;
-}
+// Uninitialised array in 2 MB-aligned memory advised for transparent huge
+// pages (its pages are first touched by the threads that fill it).
+template <class T>
+class HugeArray {
+public:
+    explicit HugeArray(size_t n) : n_(n) {
+        const size_t align = (size_t)2 << 20, bytes = (std::max<size_t>(n, 1) * sizeof(T) + align - 1) / align * align;
+        p_ = static_cast<T *>(std::aligned_alloc(align, bytes));
+        if (!p_) throw std::bad_alloc();
+        (void)madvise(p_, bytes, MADV_HUGEPAGE);
+    }
+    ~HugeArray() { release(); }
+    HugeArray(const HugeArray &) = delete;
+    HugeArray &operator=(const HugeArray &) = delete;
+    T &operator[](size_t i) { return p_[i]; }
+    const T &operator[](size_t i) const { return p_[i]; }
+    void release() {
+        std::free(p_);
+        p_ = nullptr;
+    }
 
-template <class F>
-void parallel_for(int64_t n, int threads, F &&f) {
-    if (threads <= 1 || n < 2) {
-        f(0, n);
-        return;
-    }
-    std::vector<std::thread> ts;
-    const int64_t step = (n + threads - 1) / threads;
-    for (int t = 0; t < threads; ++t) {
-        int64_t a = t * step, b = std::min(n, a + step);
-        if (a >= b) break;
-        ts.emplace_back([&f, a, b] { f(a, b); });
-    }
-    for (auto &t : ts) t.join();
-}
+private:
+    size_t n_;
+    T *p_ = nullptr;
+};
+
+inline uint64_t mulhi(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) >> 64); }
 
 }  // namespace
 
@@ -107,17 +136,20 @@ Corpus generate_corpus(const culda_gen_params &p) {
         fail(CULDA_ERR_ARG, "generator: Dirichlet priors must be positive");
     if (p.len_kind != 0 && p.len_kind != 1) fail(CULDA_ERR_ARG, "generator: len_kind must be 0 or 1");
     const int V = p.num_words, D = p.num_docs, KT = p.true_topics;
-    int threads = p.num_threads > 0 ? p.num_threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    threads = std::min(threads, 64);
+    const int threads = p.num_threads > 0 ? std::min(p.num_threads, 64) : host_threads();
 
-    // topic-word CDFs: Dirichlet(topic_word_prior) reweighted by (rank+1)^-s
-    std::vector<double> phi_cdf((size_t)KT * V);
+    // topic-word alias tables: Dirichlet(topic_word_prior) reweighted by (rank+1)^-s
+    // the token draws read random cells of this table (KT * V * 8 bytes,
+    // over a GB for the large corpora): back it with huge pages where the
+    // kernel offers them, so that a lookup is not also a page walk
+    HugeArray<uint64_t> phi_cell((size_t)KT * V);
     parallel_for(KT, threads, [&](int64_t a, int64_t b) {
-        std::vector<double> lw(V);
+        std::vector<double> lw(V), scaled;
+        std::vector<int32_t> small, large;
         for (int64_t k = a; k < b; ++k) {
             Rng r(p.seed, 0x100000000ULL + (uint64_t)k);
             for (int w = 0; w < V; ++w) lw[w] = r.log_gamma_variate(p.topic_word_prior) - p.zipf_s * std::log(w + 1.0);
-            cdf_from_logw(lw.data(), V, &phi_cdf[(size_t)k * V]);
+            alias_from_logw(lw.data(), V, &phi_cell[(size_t)k * V], scaled, small, large);
         }
     });
 
@@ -146,24 +178,46 @@ Corpus generate_corpus(const culda_gen_params &p) {
     }
     if (n >= (int64_t)INT32_MAX) fail(CULDA_ERR_LIMIT, "generator: %lld tokens exceed 2^31-1", (long long)n);
 
-    // tokens, doc-major
+    // tokens, doc-major.  theta_d ~ Dirichlet(a) over KT topics followed by
+    // z ~ theta_d is drawn as the Polya urn it is marginally: token i of a
+    // doc takes a new topic, uniform over KT, with probability
+    // KT a / (KT a + i), else the topic of one of the i earlier tokens,
+    // uniformly -- the Dirichlet-multinomial sequence, O(1) per token.  The
+    // word is then one alias draw from phi_z.  One Philox block per token:
+    // urn coin, urn index, alias bucket, alias coin.
     std::vector<int32_t> tok_word(n);
+    const double kta = (double)KT * p.doc_topic_prior;
     parallel_for(D, threads, [&](int64_t a, int64_t b) {
-        std::vector<double> lw(KT), th(KT);
+        constexpr int kRun = 32;  // table cells looked up together (their loads overlap)
+        std::vector<int32_t> zd;
+        size_t cell[kRun];
+        uint32_t flip[kRun];
         for (int64_t d = a; d < b; ++d) {
             Rng r(p.seed, 0x300000000ULL + (uint64_t)d);
-            for (int k = 0; k < KT; ++k) lw[k] = r.log_gamma_variate(p.doc_topic_prior);
-            cdf_from_logw(lw.data(), KT, th.data());
-            const int64_t t0 = d ? c.doc_cum[d - 1] : 0;
-            for (int64_t t = t0; t < c.doc_cum[d]; ++t) {
-                int k = search_cdf(th.data(), KT, r.uniform());
-                tok_word[t] = search_cdf(&phi_cdf[(size_t)k * V], V, r.uniform());
+            const int64_t t0 = d ? c.doc_cum[d - 1] : 0, nd = c.doc_cum[d] - t0;
+            zd.resize(nd);
+            for (int64_t i0 = 0; i0 < nd; i0 += kRun) {
+                const int m = (int)std::min<int64_t>(kRun, nd - i0);
+                for (int j = 0; j < m; ++j) {
+                    const int64_t i = i0 + j;
+                    const double coin = ((double)(r.next() >> 11) + 0.5) * 0x1.0p-53;
+                    const uint64_t pick = r.next(), bucket = r.next();
+                    flip[j] = (uint32_t)(r.next() >> 32);
+                    const int32_t k = coin * (kta + (double)i) < kta ? (int32_t)mulhi(pick, (uint64_t)KT)
+                                                                     : zd[mulhi(pick, (uint64_t)i)];
+                    zd[i] = k;
+                    cell[j] = (size_t)k * V + mulhi(bucket, (uint64_t)V);
+                    __builtin_prefetch(&phi_cell[cell[j]]);
+                }
+                for (int j = 0; j < m; ++j) {
+                    const uint64_t e = phi_cell[cell[j]];
+                    tok_word[t0 + i0 + j] = flip[j] < (uint32_t)e ? (int32_t)(cell[j] % (size_t)V) : (int32_t)(e >> 32);
+                }
             }
         }
     });
-    phi_cdf.clear();
-    phi_cdf.shrink_to_fit();
-    build_chunks_from_doc_major(c, tok_word, greedy_chunk_ends(len, p.num_chunks));
+    phi_cell.release();
+    build_chunks_from_doc_major(c, tok_word, greedy_chunk_ends(len, p.num_chunks), threads);
     return c;
 }
 

@@ -125,9 +125,10 @@ int culda_corpus_heldout_split(const culda_corpus *corpus, int32_t every, int32_
  * given parameter set, independent of the thread count).
  *   phi_k ~ Dirichlet(topic_word_prior) over V, multiplied by
  *   (rank+1)^-zipf_s and renormalised; theta_d ~ Dirichlet(doc_topic_prior)
- *   over true_topics; doc lengths Poisson(len_mean) (len_kind 0, zero
- *   lengths redrawn) or round(exp(Normal(len_mu, len_sigma))) (len_kind 1,
- *   at least 1). */
+ *   over true_topics and z ~ theta_d, drawn together as the Polya urn that
+ *   is their marginal (O(1) per token); w ~ phi_z by an alias table; doc
+ *   lengths Poisson(len_mean) (len_kind 0, zero lengths redrawn) or
+ *   round(exp(Normal(len_mu, len_sigma))) (len_kind 1, at least 1). */
 typedef struct culda_gen_params {
     int32_t num_words;
     int32_t num_docs;

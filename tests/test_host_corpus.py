@@ -186,3 +186,24 @@ def test_write_failure_reports_io_error_and_leaves_no_partial_file(tmp_path):
     with pytest.raises(cg.CuldaError) as e:
         c.write(tmp_path / "missing_dir" / "x")
     assert e.value.status == -2
+
+
+@pytest.mark.parametrize("alpha", [0.1, 0.5, 2.0])
+def test_generator_doc_topics_are_dirichlet_multinomial(alpha):
+    """Two topics whose word distributions (Dirichlet(1e-4) over two words,
+    seed 2) put nearly all their mass on different words: a doc's share of
+    word 0 is then its topic-0 share, which under theta_d ~ Dirichlet(alpha)
+    is Beta-binomial: mean 1/2 and variance (1 + (n-1)/(2 alpha + 1)) / (4n)
+    for a doc of n tokens.  The measured variance matches to a few percent
+    (a per-topic alpha taken for the total, or topics drawn iid per token,
+    misses by 30% or more)."""
+    c = cg.Corpus.generate(2, 4000, 2, 1, len_mean=50.0, doc_topic_prior=alpha, topic_word_prior=1e-4,
+                           zipf_s=0.0, seed=2, num_threads=3)
+    _, _, wi, wt = c.chunk(0)
+    lens = np.diff(np.concatenate([[0], c.doc_cum()]))
+    f = np.bincount(wt[:wi[1]], minlength=lens.size) / lens
+    assert 0.45 < wi[1] / wi[2] < 0.55  # both words in use: the topics are separated
+    pred = 0.25 * (1 + (lens - 1) / (2 * alpha + 1)) / lens
+    ratio = np.mean((f - 0.5) ** 2) / np.mean(pred)
+    assert abs(ratio - 1.0) < 0.06, ratio
+    assert abs(f.mean() - 0.5) < 0.02
