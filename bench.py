@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--fused-phi", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-ttl", action="store_true", help="skip the time-to-LL (plain vs learned MH) leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py drives exactly one GPU (the corpus is processed chunk by chunk on one device); "
@@ -93,7 +94,7 @@ def main():
     corpus = cg.Corpus.generate(cfg["V"], cfg["D"], cfg["KT"], cfg["chunks"], len_kind=cfg["len_kind"],
                                 len_mean=cfg["len_mean"], len_mu=cfg["len_mu"], len_sigma=cfg["len_sigma"],
                                 doc_topic_prior=cfg["doc_prior"], topic_word_prior=cfg["word_prior"],
-                                zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=min(16, os.cpu_count() or 1))
+                                zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=host_threads())
     info = corpus.info()
     N = info.num_tokens
     log(f"corpus {args.config}: V={info.num_words} D={info.num_docs} N={N} chunks={info.num_chunks} "
@@ -111,10 +112,14 @@ def main():
     log(f"stream copy {copy_gbps:.0f} GB/s")
 
     it = 1
+    warm_lls, z_warm = [], None
     if args.warmup:
-        s.iterate_n(it, args.warmup)
+        warm_lls, _ = s.iterate_n(it, args.warmup)
         it += args.warmup
     s.sync()
+    if not args.no_cpu and args.warmup:
+        # the state after the warm-up iterations, for the parity check below
+        z_warm = [s.get_z(c) for c in range(info.num_chunks)]
     # the timed steps: one call enqueues them back to back (no host round
     # trip between iterations); the LL of every step comes back at the end
     t_start = time.perf_counter()
@@ -139,9 +144,15 @@ def main():
         except Exception:
             traffic = None
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(s, corpus, info, K, alpha, beta, args.seed, it, args.cpu_threads)
+    time_to_ll = None
+    if not args.no_ttl:
+        s.close()
+        time_to_ll = time_to_ll_c4(cg, args, K)
+
+    cpu, parity = None, None
+    if not args.no_cpu and z_warm is not None:
+        cpu, parity = cpu_baseline(corpus, info, K, alpha, beta, args.seed, args.warmup, warm_lls, z_warm,
+                                   args.cpu_threads)
 
     line = {
         "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
@@ -161,7 +172,9 @@ def main():
                    "iterations_timed": [it - args.steps, it - 1], "ll_per_token_last": lls[-1],
                    "phi_rebuild": "fused over chunks" if args.fused_phi else "chunks in turn (+=)",
                    "stream_copy_gbps": copy_gbps,
-                   "sum_token_nnz_last": st.sum_token_nnz},
+                   "sum_token_nnz_last": st.sum_token_nnz,
+                   "time_to_ll": time_to_ll},
+        "parity": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "kernel": "sample_kernel", "avg_launch_ms": avg_launch_ms,
@@ -170,6 +183,21 @@ def main():
     }
     s.close()
     print(json.dumps(line), flush=True)
+
+
+def time_to_ll_c4(cg, args, K):
+    """The second half of the metric: time to the plain sampler's
+    200-iteration held-out LL/token, plain CGS vs learned MH, on the
+    1M-token corpus (the protocol of --config c4, run with its defaults)."""
+    try:
+        out = mh_protocol(cg, args, K)
+    except Exception as e:  # reported, never hides the tokens/s line
+        log(f"time-to-LL run failed: {e}")
+        return {"error": str(e)}
+    keys = ("target_ll", "threshold_ll", "t_plain_s", "t_mh_s", "speedup", "plain_ms_per_iteration",
+            "mh_ms_per_iteration", "mh_train_s", "mh_acceptance", "time_to_ll_by_tolerance", "curve_plain",
+            "curve_mh", "workload")
+    return {k: out[k] for k in keys}
 
 
 def run_batch(cg, args):
@@ -186,7 +214,7 @@ def run_batch(cg, args):
         corpora.append(cg.Corpus.generate(V, D, C3["KT"], 1, len_kind=0, len_mean=mean,
                                           doc_topic_prior=C3["doc_prior"], topic_word_prior=C3["word_prior"],
                                           zipf_s=C3["zipf"], seed=1000 + i,
-                                          num_threads=min(16, os.cpu_count() or 1)))
+                                          num_threads=host_threads()))
     N = sum(c.info().num_tokens for c in corpora)
     log(f"c3: {len(corpora)} corpora, N={N} ({time.time() - t0:.1f}s)")
     b = cg.Batch(k=K, profile=True, slot_tokens=args.slot_tokens)
@@ -260,7 +288,7 @@ def run_large(cg, args):
     in turn with on-device count accumulation; held-out LL/token at the end."""
     K = args.k
     alpha, beta = 50.0 / K, 0.01
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     t0 = time.time()
     full = cg.Corpus.generate(140_000, 8_200_000, 1000, 1, len_kind=0, len_mean=90.0, doc_topic_prior=0.05,
                               topic_word_prior=0.01, zipf_s=1.05, seed=5, num_threads=threads)
@@ -329,18 +357,19 @@ def run_large(cg, args):
     print(json.dumps(line), flush=True)
 
 
-def run_mh(cg, args):
+def mh_protocol(cg, args, K):
     """Time to a fixed held-out LL/token, plain CGS vs learned MH, on the
-    1M-token corpus with every 10th doc held out (document completion)."""
+    1M-token corpus with every 10th doc held out (document completion).
+    Both clocks count iterations (and, for MH, snapshot collection and
+    training), not the held-out evaluations."""
     cfg = CONFIGS["c1"]
-    K = args.k
     alpha, beta = 50.0 / K, 0.01
     full = cg.Corpus.generate(cfg["V"], cfg["D"], cfg["KT"], 1, len_kind=cfg["len_kind"], len_mean=cfg["len_mean"],
                               doc_topic_prior=cfg["doc_prior"], topic_word_prior=cfg["word_prior"],
-                              zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=min(16, os.cpu_count() or 1))
+                              zipf_s=cfg["zipf"], seed=cfg["gen_seed"], num_threads=host_threads())
     train, obs, ev = full.heldout_split(10, 1)
     N = train.info().num_tokens
-    log(f"c4: train N={N}, held-out observed {obs.info().num_tokens}, evaluated {ev.info().num_tokens}")
+    log(f"time-to-LL: train N={N}, held-out observed {obs.info().num_tokens}, evaluated {ev.info().num_tokens}")
 
     def timed(fn, *a):
         t0 = time.perf_counter()
@@ -373,7 +402,8 @@ def run_mh(cg, args):
     target = plain[-1][2]
     thr = target - 0.005 * abs(target)
     t_plain = next(t for _, t, ll in plain if ll >= thr)
-    log(f"plain: target {target:.5f} (threshold {thr:.5f}) reached at {t_plain:.3f}s; {plain_iter_ms:.3f} ms/iteration")
+    log(f"plain: target {target:.5f} (threshold {thr:.5f}) reached at {t_plain:.4f}s; "
+        f"{plain_iter_ms:.3f} ms/iteration")
 
     # learned MH: plain burn-in 1..30 collecting 10..30, training, then MH
     # iterations (a plain sweep every 10th) up to args.mh_max_iters
@@ -420,32 +450,42 @@ def run_mh(cg, args):
     t_mh = tolerances["0.005"]["t_mh_s"]
     log(f"learned MH: {mh_iter_ms} ms per MH iteration, acceptance {np.mean(rates) if rates else 0:.3f}; "
         f"time-to-LL {tolerances}")
+    return {"workload": "time to the plain sampler's 200-iteration held-out LL/token (within 0.5%), "
+                        "plain CGS vs learned MH (2 proposals per token, plain sweep every 10th iteration, "
+                        "MLP trained 2 epochs on iterations 10-30); 1M-token corpus (V=10k, D=10k "
+                        "Poisson(100)), every 10th doc held out, document completion with "
+                        f"{args.fold_in} fold-in sweeps",
+            "k": K, "alpha": alpha, "beta": beta, "num_tokens_train": N,
+            "target_ll": target, "threshold_ll": thr, "t_plain_s": t_plain, "t_mh_s": t_mh,
+            "speedup": (t_plain / t_mh) if t_mh else None,
+            "time_to_ll_by_tolerance": tolerances,
+            "plain_ms_per_iteration": plain_iter_ms, "mh_ms_per_iteration": mh_iter_ms,
+            "mh_train_s": dt_train, "mh_train_loss": loss,
+            "mh_acceptance": float(np.mean(rates)) if rates else None,
+            "curve_plain": plain, "curve_mh": mhc, "burn_in_end_s": burn_in_end}
+
+
+def run_mh(cg, args):
+    """--config c4: the time-to-LL comparison as its own JSON line."""
+    K = args.k
+    out = mh_protocol(cg, args, K)
+    N = out["num_tokens_train"]
     flops = 2.0 * (384 * 512 + 512 * K) * N
+    mh_iter_ms = out["mh_ms_per_iteration"]
     line = {
         "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
-        "value": (t_plain / t_mh) if t_mh else 0.0,
+        "value": out["speedup"] or 0.0,
         "unit": "x (plain time-to-LL / learned-MH time-to-LL)",
         "n_gpus": 1,
         "steps": args.iters,
         "warmup": 0,
-        "ms_per_step": plain_iter_ms,
+        "ms_per_step": out["plain_ms_per_iteration"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (sampler), bf16 MFMA with f32 accumulation (MLP)",
         "data": "synthetic (seeded LDA generative model)",
-        "config": {"workload": "time to the plain sampler's 200-iteration held-out LL/token (within 0.5%), "
-                               "plain CGS vs learned MH (2 proposals per token, plain sweep every 10th iteration, "
-                               "MLP trained 2 epochs on iterations 10-30); 1M-token corpus (V=10k, D=10k "
-                               "Poisson(100)), every 10th doc held out, document completion with "
-                               f"{args.fold_in} fold-in sweeps",
-                   "k": K, "alpha": alpha, "beta": beta, "num_tokens_train": N,
-                   "target_ll": target, "threshold_ll": thr, "t_plain_s": t_plain, "t_mh_s": t_mh,
-                   "time_to_ll_by_tolerance": tolerances,
-                   "plain_ms_per_iteration": plain_iter_ms, "mh_ms_per_iteration": mh_iter_ms,
-                   "mh_train_s": dt_train, "mh_train_loss": loss, "mh_acceptance": float(np.mean(rates)) if rates else None,
-                   "mlp_tflops_per_mh_iteration": flops / 1e12,
-                   "curve_plain": plain, "curve_mh": mhc},
+        "config": dict(out, mlp_tflops_per_mh_iteration=flops / 1e12),
         "roofline": {"bound": "mfma", "achieved": (flops / (mh_iter_ms * 1e-3) / 1e12) if mh_iter_ms else None,
                      "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": (flops / (mh_iter_ms * 1e-3) / 1e12 / 2500.0) if mh_iter_ms else None,
@@ -460,7 +500,7 @@ def cpu_baseline_batch(b, corpora, K, alpha, beta, seed, next_it, threads):
     from the GPU's current state."""
     import oracle
 
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_threads()
     oracle.set_threads(threads)
     n, dt = 0, 0.0
     for m in range(len(corpora)):
@@ -479,26 +519,83 @@ def cpu_baseline_batch(b, corpora, K, alpha, beta, seed, next_it, threads):
             "sample": f"one full iteration of all {len(corpora)} corpora ({n} tokens) from the GPU's state"}
 
 
-def cpu_baseline(s, corpus, info, K, alpha, beta, seed, next_it, threads):
-    """The C oracle (same schedule, OpenMP) timed on this host for one full
-    iteration from the GPU's current state."""
+def host_threads():
+    """The host's CPU share: OMP_NUM_THREADS where set (the GPU hosts set it),
+    else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(corpus, info, K, alpha, beta, seed, iters, gpu_lls, gpu_z, threads):
+    """The C oracle (the same schedule, OpenMP) on this host: iterations
+    1..iters from the same initial z (the GPU's warm-up iterations), timed
+    per phase on the host's CPU share, plus one thread on a sample of the
+    next sweep.  Also the full-size parity check: the oracle's z after those
+    iterations against the GPU's, the LL of each, and the draws of the next
+    sweep that fall within 1e-6 (relative) of an FP32 bucket boundary,
+    counted by the oracle's FP64 shadow on a sample of words."""
     import oracle
 
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_threads()
     oracle.set_threads(threads)
     C = info.num_chunks
     parts = [corpus.chunk(c) for c in range(C)]
     om = oracle.OracleModel(info.num_words, K, corpus.doc_cum(), [p[1] for p in parts], [p[2] for p in parts],
-                            [p[3] for p in parts], z=[s.get_z(c) for c in range(C)])
-    om.rebuild()
+                            [p[3] for p in parts])
+    om.init_z(seed)
     t0 = time.perf_counter()
-    om.sweep_fast(alpha, beta, seed, next_it)
     om.rebuild()
-    dt = time.perf_counter() - t0
-    log(f"cpu oracle: 1 iteration in {dt:.1f}s on {threads} threads")
-    return {"value": info.num_tokens / dt, "unit": "tokens/s", "cores": threads, "kind": "port",
-            "sample": f"one full iteration (sweep {next_it} + phi/theta rebuild) of the same corpus "
-                      f"({info.num_tokens} tokens) from the GPU's state after the timed steps"}
+    t_init = time.perf_counter() - t0
+    phases = []
+    lls = []
+    for it in range(1, iters + 1):
+        t0 = time.perf_counter()
+        ll = om.sweep_fast(alpha, beta, seed, it)
+        t1 = time.perf_counter()
+        om.rebuild_phi()
+        t2 = time.perf_counter()
+        om.rebuild_theta()
+        t3 = time.perf_counter()
+        phases.append({"iteration": it, "sweep_s": t1 - t0, "phi_s": t2 - t1, "theta_s": t3 - t2})
+        lls.append(ll / om.num_tokens)
+        log(f"cpu oracle: iteration {it} in {t3 - t0:.2f}s ({threads} threads)")
+    total = sum(p["sweep_s"] + p["phi_s"] + p["theta_s"] for p in phases)
+    z_mismatch = int(sum(np.count_nonzero(om.z[c] != gpu_z[c]) for c in range(C)))
+    ll_rel = [abs(a - b) / abs(b) for a, b in zip(gpu_lls, lls)]
+    # a word sample holding ~1/32 of the tokens, from the middle of the
+    # frequency range: one thread, then the FP64 shadow for boundary draws
+    cnt = np.zeros(info.num_words, np.int64)
+    for p in parts:
+        cnt += np.diff(p[2])
+    cum = np.cumsum(cnt)
+    w_lo = int(np.searchsorted(cum, cum[-1] // 2))
+    w_hi = int(np.searchsorted(cum, cum[-1] // 2 + cum[-1] // 32)) + 1
+    zs = [z.copy() for z in om.z]
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    one = om.sweep_words(alpha, beta, seed, iters + 1, w_lo, w_hi)
+    t_one = time.perf_counter() - t0
+    om.set_z(zs)
+    oracle.set_threads(threads)
+    ex = om.sweep_words(alpha, beta, seed, iters + 1, w_lo, w_hi, exact=True)
+    log(f"cpu oracle: {iters} iterations {total:.1f}s on {threads} threads; 1 thread {one.tokens} tokens in "
+        f"{t_one:.1f}s; parity z mismatches {z_mismatch}, boundary draws {ex.boundary} of {ex.tokens}")
+    cpu = {"value": info.num_tokens * iters / total, "unit": "tokens/s", "cores": threads, "kind": "port",
+           "sample": f"iterations 1-{iters} of the same corpus ({info.num_tokens} tokens, {C} chunks) from the same "
+                     f"initial z: sweep + phi rebuild + theta rebuild, {threads} threads",
+           "per_iteration": phases, "initial_rebuild_s": t_init,
+           "one_thread": {"value": one.tokens / t_one, "unit": "tokens/s", "cores": 1,
+                          "sample": f"the sweep of iteration {iters + 1} over words {w_lo}..{w_hi - 1} "
+                                    f"({one.tokens} tokens, ~1/32 of the corpus, mid-frequency words)"}}
+    parity = {"check": f"GPU z after iterations 1-{iters} vs the oracle's from the same initial z, every chunk; "
+                       f"LL/token of each iteration; boundary draws of sweep {iters + 1} on the word sample",
+              "z_mismatch": z_mismatch, "tokens": int(info.num_tokens), "ll_rel": ll_rel,
+              "ll_rel_max": max(ll_rel) if ll_rel else None,
+              "boundary_draws": int(ex.boundary), "boundary_sample_tokens": int(ex.tokens),
+              "fp32_vs_fp64_choice_differs": int(ex.mismatch)}
+    return cpu, parity
 
 
 if __name__ == "__main__":

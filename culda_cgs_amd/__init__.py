@@ -597,6 +597,11 @@ class Batch:
         _check(lib().culda_batch_get_z(self._h, model, chunk, _ptr(z, C.c_uint16), z.size))
         return z
 
+    def get_nk(self, model):
+        nk = np.empty(self.k, np.int32)
+        _check(lib().culda_batch_get_phi(self._h, model, None, _ptr(nk, C.c_int32)))
+        return nk
+
     def get_phi(self, model):
         V = self.corpora[model].info().num_words
         nwk = np.empty(V * self.k, np.int32)

@@ -912,7 +912,14 @@ extern "C" int culda_batch_get_z(culda_batch *batch, int32_t model, int32_t chun
 
 extern "C" int culda_batch_get_phi(culda_batch *batch, int32_t model, int32_t *n_wk, int32_t *n_k) {
     BATCH_CALL({
-        if (!n_wk || !n_k) fail(CULDA_ERR_ARG, "null output");
+        if (!n_k) fail(CULDA_ERR_ARG, "null output");
+        if (!n_wk) {  // n_k only
+            check_model(e, model);
+            e.sync();
+            hip_check(hipMemcpy(n_k, e.models[model].nk, (size_t)e.models[model].K * 4, hipMemcpyDeviceToHost),
+                      "n_k copy");
+            return CULDA_OK;
+        }
         HostPhi h = download_phi(e, model);
         std::memcpy(n_wk, h.nwk.data(), h.nwk.size() * 4);
         std::memcpy(n_k, h.nk.data(), h.nk.size() * 4);

@@ -451,6 +451,7 @@ int culda_batch_iterate(culda_batch *batch, int32_t first_iteration, int32_t cou
                         double *ll, double *seconds);
 int culda_batch_autotune(culda_batch *batch, culda_tune *out);
 int culda_batch_get_z(culda_batch *batch, int32_t model, int32_t chunk, uint16_t *z, int64_t n);
+/* n_wk [V*k] (may be NULL: n_k only) and n_k [k] of one model. */
 int culda_batch_get_phi(culda_batch *batch, int32_t model, int32_t *n_wk, int32_t *n_k);
 int culda_batch_info(culda_batch *batch, int32_t *num_models, int64_t *num_tokens, int64_t *num_slots);
 /* With params->profile: sampler_ms = the batched sampler launch of the last

@@ -10,7 +10,9 @@
   MH steps given the same weights, counts and Philox stream;
 * training is deterministic and lowers the cross-entropy.
 
-K = 64 runs the 64 x 64-tile GEMM, K = 128 the 128 x 128-tile one.
+K = 64 runs the 64 x 64-tile GEMM, K = 128 and K = 1024 the 128 x 128-tile
+one; K = 1024 also fills all 16 weight and theta registers per lane of the
+MH step.  A corpus of three data_chunks checks the per-chunk token ids.
 """
 import numpy as np
 import pytest
@@ -21,12 +23,12 @@ from oracle import mh
 pytestmark = pytest.mark.gpu
 
 ALPHA, BETA, SEED = 0.1, 0.05, 5
-KS = [64, 128]
+KS = [64, 128, 1024]
 M = 1 << mh.ALIAS_BITS
 
 
-def _corpus():
-    return cg.Corpus.generate(300, 200, 8, 1, len_mean=40.0, doc_topic_prior=0.2, topic_word_prior=0.05,
+def _corpus(chunks=1):
+    return cg.Corpus.generate(300, 200, 8, chunks, len_mean=40.0, doc_topic_prior=0.2, topic_word_prior=0.05,
                               zipf_s=1.0, seed=4, num_threads=4)
 
 
@@ -167,7 +169,8 @@ def _tables_ok(W, F, A, logits, K):
     diff = np.abs(W - np.maximum(soft, 1))
     diff[np.arange(len(W)), np.argmax(W, axis=1)] = 0  # takes the rounding remainder
     assert diff.max() <= np.maximum(2, 1e-5 * W).max()
-    for i in range(len(W)):
+    step = max(1, len(W) // 400)  # the Python sweep at K = 1024 costs ~5 ms per row
+    for i in range(0, len(W), step):
         f, a = mh.alias_sweep(W[i])
         assert np.array_equal(F[i], f) and np.array_equal(A[i], a), f"alias table of row {i}"
 
@@ -274,4 +277,53 @@ def test_mh_refuses_rows_longer_than_it_holds():
     s.rebuild_all()
     with pytest.raises(cg.CuldaError):
         s.mh_enable()
+    s.close()
+
+
+def test_mh_iteration_over_chunks_matches_oracle():
+    """Three data_chunks: every chunk's tokens move with the same frozen
+    counts and their own chunk's theta rows; the Philox stream is keyed by
+    the global token id (chunk-invariant), so each token's new z equals the
+    oracle's replay."""
+    import oracle
+    K, C = 128, 3
+    c = _corpus(C)
+    s = _sampler(c, iters=3, K=K)
+    rng = np.random.default_rng(7)
+    s.mh_set_weights(_random_weights(s, rng, 4.0))
+    info = c.info()
+    D, V = info.num_docs, info.num_words
+    assert info.num_chunks == C
+    chunks = [c.chunk(k) for k in range(C)]
+    # global id of each token: word-major over the one-chunk layout
+    cnt = np.array([np.diff(ch[2]) for ch in chunks])  # [C][V]
+    pre = np.concatenate([[0], np.cumsum(cnt.sum(0))])
+    nwk, nk = s.get_phi()
+    th = np.zeros((D, K), np.int64)
+    for k in range(C):
+        off, ct, tp = s.get_theta(k)
+        ds = chunks[k][0]
+        for d in range(chunks[k][1] - ds):
+            th[ds + d, tp[off[d]:off[d + 1]]] = ct[off[d]:off[d + 1]]
+    z0 = [s.get_z(k) for k in range(C)]
+    tables, base = [], 0
+    for k in range(C):
+        n = chunks[k][3].size
+        tables.append(s.mh_tables(base, n))
+        base += n
+    with pytest.raises(cg.CuldaError):
+        s.mh_tables(chunks[0][3].size - 5, 10)  # a range across two chunks
+    it, props = 4, 2
+    s.mh_iterate(it, props)
+    for k in range(C):
+        W, F, A = tables[k]
+        wi, wt = chunks[k][2], chunks[k][3]
+        words = np.repeat(np.arange(V), np.diff(wi))
+        z1 = s.get_z(k)
+        for i in range(wt.size):
+            d, w = int(wt[i]), int(words[i])
+            g = int(pre[w] + cnt[:k, w].sum() + (i - wi[w]))
+            p_of = lambda t: mh.target(th[d, t], nwk[w, t], nk[t], V, ALPHA, BETA)
+            ref = mh.mh_token(W[i], F[i], A[i], p_of, z0[k][i], SEED, it, g, props)
+            assert z1[i] == ref, f"chunk {k} token {i}"
     s.close()
