@@ -109,6 +109,7 @@ _SIGS = {
     "culda_iterate": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "culda_rebuild_all": (C.c_int, [_P]),
     "culda_loglik_joint": (C.c_int, [_P, C.POINTER(C.c_double)]),
+    "culda_loglik_joint_noalloc": (C.c_int, [_P, C.POINTER(C.c_double)]),
     "culda_sync": (C.c_int, [_P]),
     "culda_get_phi": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "culda_get_phi_head": (C.c_int, [_P, C.POINTER(C.c_float)]),
@@ -355,6 +356,12 @@ class Sampler:
     def loglik_joint(self):
         v = C.c_double()
         _check(lib().culda_loglik_joint(self._h, C.byref(v)))
+        return v.value
+
+    def loglik_joint_noalloc(self):
+        """The same value without any device allocation (static scratch)."""
+        v = C.c_double()
+        _check(lib().culda_loglik_joint_noalloc(self._h, C.byref(v)))
         return v.value
 
     # -- state transfer ---------------------------------------------------

@@ -326,6 +326,40 @@ extern "C" int culda_loglik_joint(culda_ctx *ctx, double *out) {
     });
 }
 
+// The joint LL with no allocation: the partials go to the library's static
+// device scratch (its address looked up at culda_create), the final sum is
+// formed on the device and 8 bytes come back.  Calls that overlap on one
+// device share that scratch and race.
+extern "C" int culda_loglik_joint_noalloc(culda_ctx *ctx, double *out) {
+    CTX_CALL({
+        if (!out) fail(CULDA_ERR_ARG, "null output");
+        if (!e.theta_ready()) fail(CULDA_ERR_STATE, "joint log-likelihood needs rebuilt counts");
+        const int32_t U = (int32_t)e.units.size();
+        if (U > kJointUnitsMax)
+            fail(CULDA_ERR_LIMIT, "the allocation-free joint log-likelihood serves at most %d chunks (this corpus has %d)",
+                 kJointUnitsMax, U);
+        double *scr = joint_scratch_ptr(e.prm.device);
+        if (!scr) fail(CULDA_ERR_DEVICE, "static scratch not set up on device %d", e.prm.device);
+        const ModelHost &m = e.models[0];
+        const int32_t K = m.K, V = m.V;
+        e.fresh_rows(e.stream);
+        hip_check(launch_joint_ll_words(m.nwk, V, K, std::lgamma((double)m.beta), m.beta, scr, kJointGrid, e.stream),
+                  "joint");
+        for (int32_t k = 0; k < U; ++k) {
+            const UnitHost &u = e.units[k];
+            hip_check(launch_joint_ll_docs(u.meta, u.theta, u.doc_end - u.doc_start, m.alpha,
+                                           scr + (size_t)kJointGrid * (1 + k), kJointGrid, e.stream),
+                      "joint");
+        }
+        double *res = scr + (size_t)kJointGrid * (1 + kJointUnitsMax);
+        hip_check(launch_joint_final(scr, kJointGrid * (1 + U), m.nk, K, (double)V * (double)m.beta, e.d_units,
+                                     m.unit_begin, m.unit_end, (double)K * (double)m.alpha, res, e.stream),
+                  "joint");
+        hip_check(hipMemcpyAsync(out, res, sizeof(double), hipMemcpyDeviceToHost, e.stream), "copy");
+        hip_check(hipStreamSynchronize(e.stream), "sync");
+    });
+}
+
 extern "C" int culda_get_phi(culda_ctx *ctx, int32_t *n_wk, int32_t *n_k) {
     CTX_CALL({
         if (!n_wk || !n_k) fail(CULDA_ERR_ARG, "null output");

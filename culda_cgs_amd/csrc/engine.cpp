@@ -91,6 +91,8 @@ void Engine::init_device(const culda_params &p) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device) != hipSuccess) cus = 0;
     cu_count = cus > 0 ? cus : 256;
+    // the allocation-free joint LL's static scratch: its address on this device
+    hip_check(joint_scratch_init(p.device), "scratch symbol lookup");
 }
 
 void *Engine::dalloc(size_t bytes) {

@@ -66,4 +66,14 @@ hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint16_t *theta, int3
                                 double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_copy(const float4 *src, float4 *dst, int64_t n, hipStream_t s);
 
+// Allocation-free joint LL: a static device array of kJointGrid * (1 +
+// kJointUnitsMax) partials plus the result; joint_scratch_init (once per
+// device, that device current) finds its address, joint_scratch_ptr
+// returns it (null before the lookup).
+hipError_t joint_scratch_init(int device);
+double *joint_scratch_ptr(int device);
+hipError_t launch_joint_final(const double *partial, int32_t num_partials, const int32_t *nk, int32_t k, double vb,
+                              const UnitDev *units, int32_t unit_begin, int32_t unit_end, double ka, double *out,
+                              hipStream_t s);
+
 }  // namespace culda

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "device.h"
 #include "kernels.h"
@@ -945,6 +946,38 @@ __global__ __launch_bounds__(256) void joint_docs_kernel(const DocMeta *meta, co
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
+// ------------------------------------------------ allocation-free joint LL
+// scratch: the word partials (kJointGrid), one row of kJointGrid doc
+// partials per unit (at most kJointUnitsMax units), then the result.  One
+// copy per device, held until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns (joint_scratch_ptr).
+static __device__ __align__(256) double scratch[kJointGrid * (1 + kJointUnitsMax) + 32];
+
+// One workgroup: the partials in index order, the n_k terms
+// lgamma(V beta) - lgamma(n_k + V beta) and the doc-length terms
+// lgamma(K alpha) - lgamma(n_d + K alpha) of every unit's docs, each lane's
+// share in a fixed order and then a fixed tree; out[0] = the total.
+__global__ __launch_bounds__(256) void joint_final_kernel(const double *partial, int32_t num_partials, const int32_t *nk,
+                                                          int32_t k, double vb, const UnitDev *units, int32_t unit_begin,
+                                                          int32_t unit_end, double ka, double *out) {
+    __shared__ double s[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < num_partials; i += 256) acc += partial[i];
+    const double lg_vb = lgamma(vb), lg_ka = lgamma(ka);
+    for (int t = threadIdx.x; t < k; t += 256) acc += lg_vb - lgamma((double)nk[t] + vb);
+    for (int u = unit_begin; u < unit_end; ++u) {
+        const DocMeta *meta = units[u].meta;
+        for (int d = threadIdx.x; d < units[u].num_docs; d += 256) acc += lg_ka - lgamma((double)meta[d].nd + ka);
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
 __global__ void copy_kernel(const float4 *src, float4 *dst, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -1107,6 +1140,36 @@ hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t 
 hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, float alpha,
                                 double *partial, int32_t grid, hipStream_t s) {
     hipLaunchKernelGGL(joint_docs_kernel, dim3(grid), dim3(256), 0, s, meta, theta, num_docs, alpha, partial);
+    return hipGetLastError();
+}
+
+// Per-device address of `scratch`, looked up once per device (with that
+// device current) by joint_scratch_init.
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t joint_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return hipSuccess;
+}
+
+double *joint_scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+hipError_t launch_joint_final(const double *partial, int32_t num_partials, const int32_t *nk, int32_t k, double vb,
+                              const UnitDev *units, int32_t unit_begin, int32_t unit_end, double ka, double *out,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(joint_final_kernel, dim3(1), dim3(256), 0, s, partial, num_partials, nk, k, vb, units,
+                       unit_begin, unit_end, ka, out);
     return hipGetLastError();
 }
 

@@ -41,6 +41,9 @@ constexpr int kThetaPad = 64;      // theta entries past the last slab (sampler 
 constexpr int kThetaCountShift = 10;    // theta entry: count in bits 10..15
 constexpr uint32_t kThetaTopicMask = 1023u;  // topic in bits 0..9 (K <= kMaxK)
 constexpr uint32_t kThetaMaxCount = 63u;     // largest count of one entry
+constexpr int kJointGrid = 1024;     // workgroups (partials) of each joint-LL launch
+constexpr int kJointUnitsMax = 63;   // chunks the allocation-free joint LL serves (its static scratch)
+constexpr int kMaxDevices = 64;
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 // The sampler's LDS: phi row, Q block totals, z and S+Q of a 64-token batch
 // (static), plus eight rows of S block prefixes sized by the longest theta

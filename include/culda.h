@@ -258,6 +258,17 @@ int culda_rebuild_all(culda_ctx *ctx);
 /* Joint log p(w, z) in FP64 with lgamma over the current counts. */
 int culda_loglik_joint(culda_ctx *ctx, double *out);
 
+/* culda_loglik_joint without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (serving
+ * loops, code under HIP graph capture, harnesses that forbid hipMalloc).
+ * The partial sums go to a static device array of the library (one per
+ * device, 64 * 1024 doubles plus the result, held until the process ends)
+ * instead of a buffer allocated per call, and the total is formed on the
+ * device.  At most 63 chunks (else CULDA_ERR_LIMIT).  Every call on a
+ * device writes all the scratch it reads; calls that overlap on one device
+ * share that array and race, so serialise them. */
+int culda_loglik_joint_noalloc(culda_ctx *ctx, double *out);
+
 int culda_sync(culda_ctx *ctx);
 
 /* Counts to host.  n_wk [V*k] row-major by word, n_k [k].  Replaces
