@@ -119,17 +119,18 @@ __device__ __forceinline__ float row8_shr(float x, int m) {
     return dpp_f<0x110 + S>(m >= 8 - S ? 0.0f : x);
 }
 
-// In-block prefixes of 16 values held two per lane by an 8-lane row (lane
-// m: values 2m and 2m + 1): pair sums q_m = fl(x_2m + x_2m+1), their
-// inclusive Kogge-Stone scan P_m (offsets 1, 2, 4), then C_2m+1 = P_m and
-// C_2m = fl(P_m-1 + x_2m) (P_-1 = 0, and 0 + x = x for x >= 0).  C_15 = P_7
-// is the balanced tree of the 16 values, the block total.
-__device__ __forceinline__ void prefix16x2(float &lo, float &hi, int m) {
-    float P = lo + hi;
+// In-block prefixes of 16 products held two per lane by an 8-lane row
+// (lane m: product 2m as its factors a * b, product 2m + 1 as the rounded
+// value hi): pair sums q_m = fma(a, b, hi) (one rounding), their inclusive
+// Kogge-Stone scan P_m (offsets 1, 2, 4), then C_2m+1 = P_m and C_2m =
+// fma(a, b, P_m-1) with P_-1 = 0 (so C_0 = fl(a * b)).  C_15 = P_7 is the
+// balanced tree of the 8 pair sums, the block total.
+__device__ __forceinline__ void prefix16x2(float a, float b, float &lo, float &hi, int m) {
+    float P = __builtin_fmaf(a, b, hi);
     P = P + row8_shr<1>(P, m);
     P = P + row8_shr<2>(P, m);
     P = P + row8_shr<4>(P, m);
-    lo = row8_shr<1>(P, m) + lo;
+    lo = __builtin_fmaf(a, b, row8_shr<1>(P, m));
     hi = P;
 }
 

@@ -4,13 +4,15 @@
 // correctly rounded IEEE operation in the order written, so the CPU oracle
 // can replay it bit for bit.  The order is the normative one:
 //   phi_k   = fl(fl(fl(n_wk) + beta) / den_k),  den_k = fl(fl(n_k) + fl(beta * fl(V)))
-//   a_k     = fl(alpha * phi_k)                              (Q bucket)
-//   p_j     = fl(fl(count_j) * phi_{topic_j})                (S bucket)
-//   blocks of 16 values; a block total T_b is the balanced tree of its
-//   values; S: block totals carried sequentially R_{b+1} = fl(R_b + T_b);
-//   Q: 64-lane Kogge-Stone scan of the block totals.  In-block prefixes:
-//   pair sums q_m = fl(v_2m + v_2m+1), their Kogge-Stone scan P_m (offsets
-//   1, 2, 4), prefix_2m+1 = P_m, prefix_2m = fl(P_m-1 + v_2m).  t = fl(u * fl(S + Q)); S bucket iff t < S (target t), else
+//   Q bucket products a_k * phi_k with a_k = alpha; S bucket products
+//   count_j * phi_{topic_j} (count_j exact in FP32);
+//   blocks of 16 products; pair sums q_m = fma(x_2m, y_2m, fl(x_2m+1 *
+//   y_2m+1)) (the even product fused, one rounding); a block total T_b is
+//   the balanced tree of its 8 pair sums; S: block totals carried
+//   sequentially R_{b+1} = fl(R_b + T_b); Q: 64-lane Kogge-Stone scan of
+//   the block totals.  In-block prefixes: the pair sums' Kogge-Stone scan
+//   P_m (offsets 1, 2, 4), prefix_2m+1 = P_m, prefix_2m = fma(x_2m, y_2m,
+//   P_m-1) (P_-1 = 0).  t = fl(u * fl(S + Q)); S bucket iff t < S (target t), else
 //   Q bucket (target fl(t - S)).  Search: first block whose inclusive
 //   prefix exceeds the target, then first element of that block whose
 //   fl(prefix_before_block + in-block prefix) exceeds it; fallbacks take
@@ -637,9 +639,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         const float *v = ph + lane * kBlock;
         float q[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) q[i] = alpha * v[i];
-        const float t01 = q[0] + q[1], t23 = q[2] + q[3], t45 = q[4] + q[5], t67 = q[6] + q[7];
-        const float t89 = q[8] + q[9], tab = q[10] + q[11], tcd = q[12] + q[13], tef = q[14] + q[15];
+        for (int i = 1; i < 16; i += 2) q[i] = alpha * v[i];
+        // pair sums fma(alpha, v_2m, fl(alpha v_2m+1)), as every block total
+        const float t01 = __builtin_fmaf(alpha, v[0], q[1]), t23 = __builtin_fmaf(alpha, v[2], q[3]);
+        const float t45 = __builtin_fmaf(alpha, v[4], q[5]), t67 = __builtin_fmaf(alpha, v[6], q[7]);
+        const float t89 = __builtin_fmaf(alpha, v[8], q[9]), tab = __builtin_fmaf(alpha, v[10], q[11]);
+        const float tcd = __builtin_fmaf(alpha, v[12], q[13]), tef = __builtin_fmaf(alpha, v[14], q[15]);
         const float u03 = t01 + t23, u47 = t45 + t67, u8b = t89 + tab, ucf = tcd + tef;
         T = (u03 + u47) + (u8b + ucf);
     }
@@ -718,8 +723,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // S = sum over the doc's theta entries, 16 per block; every
                 // block lies inside the doc's 16-aligned slab, entries past
                 // nnz are zero (the slab tail), so their products are exact
-                // zeros.  A block total is the balanced tree of its 16
-                // products: lane m holds entries
+                // zeros.  A block total is the balanced tree of its 8 pair
+                // sums (each an FMA of the even product into the odd one):
+                // lane m holds entries
                 // 4m .. 4m + 3 of a group of two blocks (one 8-byte load;
                 // block m >> 2), forms the tree's first two levels in
                 // registers, and DPP row_shr 1 and 2 finish the two totals
@@ -738,7 +744,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 auto group = [&](u32x2 d, float R0, float &r0, float &r1) {
                     const float f0 = PHI_LO(d.x), f1 = PHI_HI(d.x), f2 = PHI_LO(d.y), f3 = PHI_HI(d.y);
                     const float c0 = COUNT_LO(d.x), c1 = COUNT_HI(d.x), c2 = COUNT_LO(d.y), c3 = COUNT_HI(d.y);
-                    float y = (c0 * f0 + c1 * f1) + (c2 * f2 + c3 * f3);
+                    float y = __builtin_fmaf(c0, f0, c1 * f1) + __builtin_fmaf(c2, f2, c3 * f3);
                     y = y + dpp_f<0x111>(y);
                     y = y + dpp_f<0x112>(y);
                     r0 = R0 + dpp_f<0x114>(y);
@@ -810,17 +816,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                     // another order, may round above the last entry's
                     const uint32_t j = ((uint32_t)bsel << 4) + 2u * (uint32_t)lr;
                     uint32_t e2 = 0;
-                    float vlo, vhi;
+                    float alo, blo, vlo, vhi;
                     if (sb) {
                         e2 = load_global_at(theta, (start + j) * 2u);
-                        vlo = COUNT_LO(e2) * PHI_LO(e2);
+                        alo = COUNT_LO(e2);
+                        blo = PHI_LO(e2);
                         vhi = COUNT_HI(e2) * PHI_HI(e2);
                     } else {
                         const float2 f = *reinterpret_cast<const float2 *>(ph + j);
-                        vlo = alpha * f.x;
+                        alo = alpha;
+                        blo = f.x;
                         vhi = alpha * f.y;
                     }
-                    prefix16x2(vlo, vhi, lr);
+                    prefix16x2(alo, blo, vlo, vhi, lr);
                     float Clo = vlo, Chi = vhi;
                     if (bsel != 0) {
                         const float rb = rsel[bsel - 1];

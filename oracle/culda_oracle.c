@@ -21,10 +21,11 @@
  *     instead of the reference's clock-seeded curand state (a sweep draw
  *     takes word g & 3 of the block of counter g >> 2);
  *   - FP32 head (n_wk + beta) / (n_k + beta V) (kernel/lda_phi_kernel.cu:26);
- *   - FP32 bucket arithmetic in a fixed order: blocks of 16 whose totals
- *     are balanced trees (last element of a Kogge-Stone scan, ks16), in-block
- *     prefixes from pair sums scanned Kogge-Stone (pfx16), S block totals
- *     carried sequentially,
+ *   - FP32 bucket arithmetic in a fixed order: blocks of 16 products
+ *     whose pair sums fuse the even product into the odd one (fmaf, one
+ *     rounding, pairs16) and whose totals are the balanced trees of the 8
+ *     pair sums (block16), in-block prefixes from the pair sums scanned
+ *     Kogge-Stone (pfx16), S block totals carried sequentially,
  *     Q block totals scanned Kogge-Stone over 64 lanes; t = u * (S + Q)
  *     picks the S bucket when t < S (target t) and otherwise the Q bucket
  *     (target t - S) -- the reference's u < S/(S+Q) test and
@@ -38,8 +39,9 @@
  * to S+Q) of a bucket boundary; the FP32 ("faithful") mode is what the
  * device reproduces bit for bit.
  *
- * Build: gcc -O2 -fopenmp -ffp-contract=off (x86-64 SSE arithmetic, so every
- * float operation is rounded to FP32).
+ * Build: gcc -O2 -mfma -fopenmp -ffp-contract=off (x86-64 SSE arithmetic, so
+ * every float operation is rounded to FP32; fmaf is the hardware's single-
+ * rounding FMA, and nothing is contracted into one implicitly).
  */
 #include <math.h>
 #include <stdint.h>
@@ -246,30 +248,41 @@ void oc_phi_head(int32_t V, int32_t K, float beta, const int32_t *nwk, const int
 }
 
 /* ------------------------------------------------------------------ sweep */
-/* Inclusive Kogge-Stone scan of 16 values, offsets 1, 2, 4, 8. */
-static void ks16(float *x) {
-    for (int d = 1; d < 16; d <<= 1) {
-        float y[16];
-        for (int m = 0; m < 16; ++m) y[m] = m >= d ? x[m] + x[m - d] : x[m];
-        memcpy(x, y, sizeof y);
+/* Pair sums of 16 products x_i * y_i: q_m = fma(x_2m, y_2m, fl(x_2m+1 *
+ * y_2m+1)) -- the even product fused into the odd one, one rounding. */
+static void pairs16(const float *x, const float *y, float *q) {
+    for (int m = 0; m < 8; ++m) q[m] = fmaf(x[2 * m], y[2 * m], x[2 * m + 1] * y[2 * m + 1]);
+}
+
+/* Inclusive Kogge-Stone scan of 8 pair sums, offsets 1, 2, 4. */
+static void ks8(float *P) {
+    for (int d = 1; d < 8; d <<= 1) {
+        float z[8];
+        for (int m = 0; m < 8; ++m) z[m] = m >= d ? P[m] + P[m - d] : P[m];
+        memcpy(P, z, sizeof z);
     }
 }
 
-/* In-block prefixes of 16 products, in place: pair sums q_m = p_2m + p_2m+1,
- * their inclusive Kogge-Stone scan P_m (offsets 1, 2, 4), then
- * C_2m+1 = P_m and C_2m = P_m-1 + p_2m (P_-1 = 0).  C_15 = P_7 is the balanced
- * tree of the 16 products, the same value as ks16's last element. */
-static void pfx16(float *x) {
+/* Block total of 16 products: the balanced tree of the 8 pair sums
+ * ((q0 + q1) + (q2 + q3)) + ((q4 + q5) + (q6 + q7)), the last element of
+ * their Kogge-Stone scan. */
+static float block16(const float *x, const float *y) {
+    float q[8];
+    pairs16(x, y, q);
+    ks8(q);
+    return q[7];
+}
+
+/* In-block prefixes of 16 products into c: the pair sums' scan P_m,
+ * c_2m+1 = P_m and c_2m = fma(x_2m, y_2m, P_m-1) (P_-1 = 0).  c_15 is the
+ * block total. */
+static void pfx16(const float *x, const float *y, float *c) {
     float P[8];
-    for (int m = 0; m < 8; ++m) P[m] = x[2 * m] + x[2 * m + 1];
-    for (int d = 1; d < 8; d <<= 1) {
-        float y[8];
-        for (int m = 0; m < 8; ++m) y[m] = m >= d ? P[m] + P[m - d] : P[m];
-        memcpy(P, y, sizeof y);
-    }
+    pairs16(x, y, P);
+    ks8(P);
     for (int m = 0; m < 8; ++m) {
-        x[2 * m] = m ? P[m - 1] + x[2 * m] : x[2 * m];
-        x[2 * m + 1] = P[m];
+        c[2 * m] = fmaf(x[2 * m], y[2 * m], m ? P[m - 1] : 0.0f);
+        c[2 * m + 1] = P[m];
     }
 }
 
@@ -345,16 +358,9 @@ void oc_sweep_range(const oc_corpus *c, float alpha, float beta, uint64_t seed, 
                     phd[k] = ((double)nwk[(size_t)w * K + k] + (double)beta) / den_d[k];
                     Qd += (double)alpha * phd[k];
                 }
-            for (int b = 0; b < 64; ++b) {
-                float v[16];
-                if (b < nbq) {
-                    for (int m = 0; m < 16; ++m) v[m] = alpha * ph[b * 16 + m];
-                    ks16(v);
-                    rq[b] = v[15];
-                } else {
-                    rq[b] = 0.0f;
-                }
-            }
+            float av[16];
+            for (int m = 0; m < 16; ++m) av[m] = alpha;
+            for (int b = 0; b < 64; ++b) rq[b] = b < nbq ? block16(av, ph + b * 16) : 0.0f;
             ks64(rq);
             const float Q = rq[nbq - 1];
             for (int32_t ck = 0; ck < c->C; ++ck) {
@@ -369,13 +375,13 @@ void oc_sweep_range(const oc_corpus *c, float alpha, float beta, uint64_t seed, 
                     const int64_t nb = (nnz + 15) / 16;
                     float R = 0.0f;
                     for (int64_t b = 0; b < nb; ++b) {
-                        float p[16];
+                        float x[16], y[16];
                         for (int m = 0; m < 16; ++m) {
                             const int64_t j = b * 16 + m;
-                            p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
+                            x[m] = j < nnz ? (float)th_cnt[o + j] : 0.0f; /* padding: exact zero products */
+                            y[m] = j < nnz ? ph[th_top[o + j]] : 0.0f;
                         }
-                        ks16(p);
-                        R = R + p[15];
+                        R = R + block16(x, y);
                         rs[b + 1] = R;
                     }
                     const float S = R, SQ = S + Q, tu = u * SQ;
@@ -392,12 +398,13 @@ void oc_sweep_range(const oc_corpus *c, float alpha, float beta, uint64_t seed, 
                         if (bsel < 0) {
                             topic = th_top[o + nnz - 1];
                         } else {
-                            float p[16];
+                            float x[16], y[16], p[16];
                             for (int m = 0; m < 16; ++m) {
                                 const int64_t j = bsel * 16 + m;
-                                p[m] = j < nnz ? (float)th_cnt[o + j] * ph[th_top[o + j]] : 0.0f;
+                                x[m] = j < nnz ? (float)th_cnt[o + j] : 0.0f;
+                                y[m] = j < nnz ? ph[th_top[o + j]] : 0.0f;
                             }
-                            pfx16(p);
+                            pfx16(x, y, p);
                             const int64_t nvalid = nnz - bsel * 16 < 16 ? nnz - bsel * 16 : 16;
                             int64_t msel = nvalid - 1;
                             for (int64_t m = 0; m < nvalid; ++m) {
@@ -421,8 +428,7 @@ void oc_sweep_range(const oc_corpus *c, float alpha, float beta, uint64_t seed, 
                             topic = K - 1;
                         } else {
                             float v[16];
-                            for (int m = 0; m < 16; ++m) v[m] = alpha * ph[bsel * 16 + m];
-                            pfx16(v);
+                            pfx16(av, ph + bsel * 16, v);
                             int msel = 15;
                             for (int m = 0; m < 16; ++m) {
                                 const float C = bsel == 0 ? v[m] : rq[bsel - 1] + v[m];
