@@ -553,8 +553,7 @@ extern "C" int culda_get_stats(culda_ctx *ctx, culda_stats *out) {
         for (auto &m : e.models) out->num_tokens += m.num_tokens;
         out->num_slots = e.num_slots;
         for (auto &u : e.units) out->theta_capacity += u.theta_cap;
-        for (int64_t v : e.unit_doc_nnz) out->sum_doc_nnz += v;
-        for (int64_t v : e.unit_token_nnz) out->sum_token_nnz += v;
+        e.theta_stats(out->sum_doc_nnz, out->sum_token_nnz);
         out->sampler_ms = e.sampler_ms;
         out->phi_ms = e.phi_ms;
         out->theta_ms = e.theta_ms;
@@ -847,8 +846,7 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
         a.logits = s.Li;
         a.word = s.tok_word + base;
         a.gdelta = s.gdelta[ui];
-        a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, e.d_stats + 2 * ui,
-                         u.model, u.doc_end - u.doc_start};
+        a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, u.model, u.doc_end - u.doc_start};
         a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V};
         a.accepted = s.accepted;
         a.w_out = dw;
@@ -967,8 +965,7 @@ extern "C" int culda_batch_get_stats(culda_batch *batch, culda_stats *out) {
         for (auto &m : e.models) out->num_tokens += m.num_tokens;
         out->num_slots = e.num_slots;
         for (auto &u : e.units) out->theta_capacity += u.theta_cap;
-        for (int64_t v : e.unit_doc_nnz) out->sum_doc_nnz += v;
-        for (int64_t v : e.unit_token_nnz) out->sum_token_nnz += v;
+        e.theta_stats(out->sum_doc_nnz, out->sum_token_nnz);
         out->sampler_ms = e.sampler_ms;
         out->other_ms = e.iter_ms;
         out->sampler_launches = e.sampler_launches;

@@ -67,7 +67,6 @@ Engine::~Engine() {
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t &e : ev_theta_unit)
         if (e) (void)hipEventDestroy(e);
-    if (h_stats_ring) (void)hipHostFree(h_stats_ring);
     for (hipEvent_t &e : ev_iter)
         if (e) (void)hipEventDestroy(e);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -358,10 +357,6 @@ void Engine::upload() {
     }
     d_nk_all = (int32_t *)dalloc((size_t)M * K * sizeof(int32_t));
     for (int32_t mi = 0; mi < M; ++mi) models[mi].nk = d_nk_all + (size_t)mi * K;
-    d_stats = (unsigned long long *)dalloc(2 * sizeof(unsigned long long) * U);
-    hip_check(hipMemset(d_stats, 0, 2 * sizeof(unsigned long long) * U), "stats clear");
-    unit_doc_nnz.assign(U, 0);
-    unit_token_nnz.assign(U, 0);
     // device tables
     std::vector<ModelDev> md(M);
     for (int32_t mi = 0; mi < M; ++mi) {
@@ -380,7 +375,7 @@ void Engine::upload() {
     std::vector<UnitDev> ud(U);
     for (int32_t ui = 0; ui < U; ++ui) {
         const UnitHost &u = units[ui];
-        ud[ui] = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+        ud[ui] = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, u.model,
                          u.doc_end - u.doc_start, u.zm};
     }
     d_units = upload_vec(*this, ud);
@@ -500,13 +495,12 @@ void Engine::upload() {
     hip_check(hipMemset(d_sweep, 0, 16), "sweep clear");
     d_ll_hist = (double *)dalloc(sizeof(double) * kLLHistory * M);
     hip_check(hipMemset(d_ll_hist, 0, sizeof(double) * kLLHistory * M), "ll clear");
+    d_nnz_hist = (double *)dalloc(sizeof(double) * kLLHistory * M);
+    hip_check(hipMemset(d_nnz_hist, 0, sizeof(double) * kLLHistory * M), "history clear");
     ev_sampled.assign(U, nullptr);
     for (hipEvent_t &e : ev_sampled) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     ev_theta_unit.assign(U, nullptr);
     for (hipEvent_t &e : ev_theta_unit) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-    hip_check(hipHostMalloc((void **)&h_stats_ring, (size_t)kStatsSlots * 2 * sizeof(unsigned long long) * U,
-                            hipHostMallocDefault),
-              "stats buffer");
     uploaded = true;
     rebuild_slots(prm.slot_tokens);
     hip_check(hipDeviceSynchronize(), "upload");
@@ -549,6 +543,8 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     dfree(d_ll_packets);
     dfree(d_model_packets);
     dfree(d_ll_scratch);
+    dfree(d_nnz_partial);
+    dfree(d_nnz_scratch);
     const int64_t slot_tokens = slot_tokens_in > 0 ? slot_tokens_in : default_slot_tokens();
     std::vector<SlotDesc> all;
     std::vector<Packet> llp;
@@ -622,6 +618,9 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     d_ll_packets = upload_vec(*this, llp);
     d_model_packets = upload_vec(*this, mp);
     d_ll_scratch = (double *)dalloc(sizeof(double) * std::max<size_t>(llp.size(), 1));
+    d_nnz_partial = (double *)dalloc(sizeof(double) * std::max<size_t>(all.size(), 1));
+    hip_check(hipMemset(d_nnz_partial, 0, sizeof(double) * std::max<size_t>(all.size(), 1)), "partials clear");
+    d_nnz_scratch = (double *)dalloc(sizeof(double) * std::max<size_t>(llp.size(), 1));
     num_ll_packets = (int32_t)llp.size();
     prm.slot_tokens = (int32_t)slot_tokens;
     // the copies and the memset above run on the null stream, which does not
@@ -643,12 +642,14 @@ void Engine::init_z() {
 
 // unit < 0: model's chunks in one pass (overwrite).  unit >= 0: that unit
 // alone, added to the counts when accumulate (chunks in turn).
-void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStream_t s) {
+void Engine::rebuild_phi_model(int32_t mi, int32_t unit, bool accumulate, hipStream_t s, const LLArgs *ll) {
     ModelHost &m = models[mi];
     const PhiList &L = unit < 0 ? m.phi_fused : units[unit].phi_single;
     if (unit < 0) accumulate = false;
     if (!accumulate) {
-        hip_check(launch_phi_clear((uint32_t *)m.nk, m.K, d_models, L.zero_rows, L.num_zero_rows, s), "phi clear");
+        hip_check(launch_phi_clear((uint32_t *)m.nk, m.K, d_models, L.zero_rows, L.num_zero_rows, s, ll), "phi clear");
+    } else if (ll) {
+        hip_check(launch_phi_clear(nullptr, 0, d_models, nullptr, 0, s, ll), "phi clear");
     }
     PhiArgs a{L.slots, L.packets, d_units, d_models, L.num_packets, accumulate ? 1 : 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
@@ -679,10 +680,10 @@ void Engine::fresh_rows(hipStream_t s) {
     rare_stale = false;
 }
 
-void Engine::update_phi(hipStream_t s) {
+void Engine::update_phi(hipStream_t s, const LLArgs *ll) {
     if (zp_valid && sparse_phi) {
         hip_check(launch_phi_clear((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, d_models, phi_all.zero_rows,
-                                   phi_all.num_zero_rows, s),
+                                   phi_all.num_zero_rows, s, ll),
                   "phi clear");
         PhiArgs a{phi_sparse.slots, phi_sparse.packets, d_units, d_models, phi_sparse.num_packets, 2};
         RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
@@ -690,20 +691,20 @@ void Engine::update_phi(hipStream_t s) {
         rare_stale = num_rare_packets > 0;
         phi_ready = false;
     } else if (prm.fused_phi || units.size() == models.size()) {
-        rebuild_phi_all(s);
+        rebuild_phi_all(s, ll);
     } else {
         for (size_t mi = 0; mi < models.size(); ++mi)
             for (int32_t ui = models[mi].unit_begin; ui < models[mi].unit_end; ++ui)
-                rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, s);
+                rebuild_phi_model((int32_t)mi, ui, ui > models[mi].unit_begin, s, mi == 0 && ui == 0 ? ll : nullptr);
         if (sparse_phi) copy_zp(-1, s);
         zp_valid = sparse_phi;
         rare_stale = false;
     }
 }
 
-void Engine::rebuild_phi_all(hipStream_t s) {
+void Engine::rebuild_phi_all(hipStream_t s, const LLArgs *ll) {
     hip_check(launch_phi_clear((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, d_models, phi_all.zero_rows,
-                               phi_all.num_zero_rows, s),
+                               phi_all.num_zero_rows, s, ll),
               "phi clear");
     PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 0};
     hip_check(launch_phi_rebuild(a, s), "phi rebuild launch");
@@ -713,35 +714,47 @@ void Engine::rebuild_phi_all(hipStream_t s) {
     rare_stale = false;
 }
 
-void Engine::phi_head(hipStream_t s) {
-    hip_check(launch_den(d_models, (int32_t)models.size(), s), "den launch");
+void Engine::phi_head(hipStream_t s, const LLArgs *ll, bool advance) {
+    hip_check(launch_den(d_models, (int32_t)models.size(), s, ll, advance ? d_sweep : nullptr), "den launch");
     phi_ready = true;
 }
 
+LLArgs Engine::ll_args() const {
+    return LLArgs{d_ll_partial,    d_ll_packets,  d_model_packets, d_ll_scratch,   d_ll_hist,
+                  d_nnz_partial,   d_nnz_scratch, d_nnz_hist,      d_sweep,        num_ll_packets,
+                  (int32_t)models.size()};
+}
+
+// workgroups per CU of a chunk's theta rebuild beside a sweep
+static constexpr int32_t kThetaBesideGroupsPerCU = 1;
+
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
-    hip_check(launch_fill_u32((uint32_t *)(d_stats + 2 * ui), 4, 0, s), "stats clear");
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
-    // a single chunk's rebuild has nothing beside it: the next sweep waits
-    hip_check(launch_theta_rebuild(a, units.size() == 1, s), "theta rebuild launch");
+    // a single chunk's rebuild has nothing beside it: the next sweep waits;
+    // a chunk's rebuild beside the next chunk's sweep holds at most one
+    // workgroup per CU, so that it takes the CUs the sweep leaves idle
+    // rather than the sweep's own (the sweeps are the critical path)
+    const bool alone = units.size() == 1;
+    hip_check(launch_theta_rebuild(a, alone, alone ? 0 : kThetaBesideGroupsPerCU * cu_count, s), "theta rebuild launch");
     theta_ok[ui] = 1;
 }
 
 void Engine::rebuild_theta_all(hipStream_t s) {
-    hip_check(launch_fill_u32((uint32_t *)d_stats, 4 * (int64_t)units.size(), 0, s), "stats clear");
     ThetaArgs a{d_theta_packets_all, d_units, d_models, num_theta_packets_all, 0};
-    hip_check(launch_theta_rebuild(a, false, s), "theta rebuild launch");
+    hip_check(launch_theta_rebuild(a, false, 0, s), "theta rebuild launch");
     set_theta_ready(true);
 }
 
 void Engine::sample_unit(int32_t ui, hipStream_t s) {
     UnitHost &u = units[ui];
-    SampleArgs a{d_slots + u.slot_begin, nullptr, d_units, d_models, d_ll_partial + u.slot_begin, d_sweep, nullptr,
+    SampleArgs a{d_slots + u.slot_begin, nullptr, d_units, d_models, d_ll_partial + u.slot_begin,
+                 d_nnz_partial + u.slot_begin, d_sweep, nullptr,
                  u.slot_end - u.slot_begin, (zp_valid && sparse_phi) ? 1 : 0, rs_stride, 0};
     hip_check(launch_sample(a, 0, s), "sampler launch");
 }
 
 void Engine::sample_all(hipStream_t s) {
-    SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_sweep, nullptr, num_slots,
+    SampleArgs a{d_slots, d_order, d_units, d_models, d_ll_partial, d_nnz_partial, d_sweep, nullptr, num_slots,
                  (zp_valid && sparse_phi) ? 1 : 0, rs_stride, 0};
     int32_t grid = 0;
     if (persistent_groups > 0) {
@@ -754,33 +767,29 @@ void Engine::sample_all(hipStream_t s) {
 }
 
 void Engine::ll_reduce(hipStream_t s) {
-    LLArgs a{d_ll_partial, d_ll_packets, d_model_packets, d_ll_scratch, d_ll_hist, d_sweep, num_ll_packets,
-             (int32_t)models.size()};
-    hip_check(launch_ll_reduce(a, s), "ll reduce launch");
+    hip_check(launch_ll_reduce(ll_args(), s), "ll reduce launch");
 }
 
-void Engine::read_stats() {
-    std::vector<unsigned long long> h(2 * units.size());
-    hip_check(hipMemcpyAsync(h.data(), d_stats, h.size() * 8, hipMemcpyDeviceToHost, stream), "stats copy");
+void Engine::theta_stats(int64_t &doc_nnz, int64_t &token_nnz) {
+    doc_nnz = token_nnz = 0;
+    if (!uploaded) return;
+    sync();
+    unsigned long long *d = (unsigned long long *)dalloc(2 * sizeof(unsigned long long));
+    hip_check(launch_fill_u32((uint32_t *)d, 4, 0, stream), "stats clear");
+    for (const UnitHost &u : units)
+        hip_check(launch_meta_stats(u.meta, u.doc_end - u.doc_start, d, stream), "theta statistics");
+    unsigned long long h[2] = {0, 0};
+    hip_check(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, stream), "stats copy");
     hip_check(hipStreamSynchronize(stream), "stats sync");
-    take_stats(h.data());
-}
-
-void Engine::take_stats(const unsigned long long *h) {
-    for (size_t u = 0; u < units.size(); ++u) {
-        unit_doc_nnz[u] = (int64_t)h[2 * u];
-        unit_token_nnz[u] = (int64_t)h[2 * u + 1];
-    }
+    dfree(d);
+    doc_nnz = (int64_t)h[0];
+    token_nnz = (int64_t)h[1];
 }
 
 void Engine::settle() {
     if (stream2_busy) {
         hip_check(hipStreamSynchronize(stream2), "sync");
         stream2_busy = false;
-    }
-    if (stats_pending >= 0) {
-        take_stats(stats_slot(stats_pending));
-        stats_pending = -1;
     }
 }
 
@@ -799,7 +808,7 @@ void Engine::rebuild_all() {
     }
     phi_head(stream);
     rebuild_theta_all(stream);
-    read_stats();
+    sync();
 }
 
 double Engine::model_ll(int32_t mi, uint32_t sweep) {
@@ -819,9 +828,9 @@ double Engine::model_ll(int32_t mi, uint32_t sweep) {
 // ev_theta_unit[c].  A theta rebuild reads z of its own chunk, which no
 // other launch of the iteration writes.  The rebuilds' statistics go to
 // the pinned stats_dst; ev_a / ev_b (profiling) bracket the sweeps.
-void Engine::enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_dst, hipEvent_t ev_a, hipEvent_t ev_b) {
+void Engine::enqueue_units_iteration(uint32_t sweep, bool set, hipEvent_t ev_a, hipEvent_t ev_b) {
     const int32_t U = (int32_t)units.size();
-    set_sweep(sweep, stream);
+    if (set) set_sweep(sweep, stream);
     const bool after_iteration = stream2_busy;
     for (int32_t u = 0; u < U; ++u) {
         if (after_iteration) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[u], 0), "wait");
@@ -833,16 +842,14 @@ void Engine::enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_d
         rebuild_theta_unit(u, stream2);
         if (u + 1 < U) hip_check(hipEventRecord(ev_theta_unit[u], stream2), "event");
     }
-    // the statistics are copied before the last unit's event: the next
-    // iteration's last sweep waits for it, so they are complete when that
-    // iteration's stream is synchronised
-    hip_check(hipMemcpyAsync(stats_dst, d_stats, 2 * sizeof(unsigned long long) * U, hipMemcpyDeviceToHost, stream2),
-              "stats copy");
     hip_check(hipEventRecord(ev_theta_unit[U - 1], stream2), "event");
     if (ev_b) hip_check(hipEventRecord(ev_b, stream), "event");
-    ll_reduce(stream);
-    update_phi(stream);
-    phi_head(stream);
+    // the LL reduction rides in the count update's clear launch (stage 1)
+    // and in the head-denominator launch (stage 2), which then moves the
+    // sweep counter on to the next sweep
+    const LLArgs ll = ll_args();
+    update_phi(stream, &ll);
+    phi_head(stream, &ll, true);
     // all rebuilds but the last one before the next iteration's sweeps:
     // they run beside the sweeps and may fall behind (many chunks, short
     // sweeps); here they catch up while nothing else competes, so their
@@ -859,22 +866,19 @@ void Engine::iterate_units(uint32_t sweep, double *ll, double *seconds) {
     if (prof && !ev[0])
         for (hipEvent_t &e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     auto t0 = std::chrono::steady_clock::now();
-    const int32_t slot = stats_next;
-    stats_next = (stats_next + 1) % kStatsSlots;
-    enqueue_units_iteration(sweep, stats_slot(slot), prof ? ev[0] : nullptr, prof ? ev[1] : nullptr);
+    enqueue_units_iteration(sweep, true, prof ? ev[0] : nullptr, prof ? ev[1] : nullptr);
     if (prof) hip_check(hipEventRecord(ev[2], stream), "event");
-    double h_ll = 0;
-    hip_check(hipMemcpyAsync(&h_ll, d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
+    double h[2] = {0, 0};  // LL sum, theta entries read
+    hip_check(hipMemcpyAsync(&h[0], d_ll_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
+                             stream),
+              "ll copy");
+    hip_check(hipMemcpyAsync(&h[1], d_nnz_hist + (size_t)(sweep % kLLHistory), sizeof(double), hipMemcpyDeviceToHost,
                              stream),
               "ll copy");
     hip_check(hipStreamSynchronize(stream), "iteration sync");
     auto t1 = std::chrono::steady_clock::now();
-    // statistics of the rebuild this sweep read: the previous iteration's
-    // (complete: this iteration's last sweep waited for their copy), or
-    // current already
-    if (stats_pending >= 0) take_stats(stats_slot(stats_pending));
-    const double bytes = sweep_bytes();
-    stats_pending = slot;
+    const double h_ll = h[0];
+    const double bytes = sweep_bytes(h[1]);
     if (prof) {
         float a = 0, b = 0;
         hip_check(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
@@ -902,36 +906,26 @@ void Engine::iterate_units_n(uint32_t first, int32_t count, double *ll, double *
             hip_check(hipEventCreate(&e), "hipEventCreate");
             ev_iter.push_back(e);
         }
-    std::vector<int32_t> slots(count);
     auto t0 = std::chrono::steady_clock::now();
-    for (int32_t i = 0; i < count; ++i) {
-        slots[i] = stats_next;
-        stats_next = (stats_next + 1) % kStatsSlots;
-        enqueue_units_iteration(first + (uint32_t)i, stats_slot(slots[i]), prof ? ev_iter[2 * i] : nullptr,
+    for (int32_t i = 0; i < count; ++i)
+        enqueue_units_iteration(first + (uint32_t)i, i == 0, prof ? ev_iter[2 * i] : nullptr,
                                 prof ? ev_iter[2 * i + 1] : nullptr);
-    }
-    std::vector<double> hist(kLLHistory);
+    std::vector<double> hist(kLLHistory), nnz(kLLHistory);
     hip_check(hipMemcpyAsync(hist.data(), d_ll_hist, kLLHistory * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "ll copy");
+    hip_check(hipMemcpyAsync(nnz.data(), d_nnz_hist, kLLHistory * sizeof(double), hipMemcpyDeviceToHost, stream),
               "ll copy");
     hip_check(hipStreamSynchronize(stream), "iteration sync");
     auto t1 = std::chrono::steady_clock::now();
-    // sweep i read the rebuild of iteration i - 1 (complete: iteration i's
-    // last sweep waited for its copy), sweep 0 the state before the call
     double bytes = 0, ms = 0;
     for (int32_t i = 0; i < count; ++i) {
-        if (i == 0) {
-            if (stats_pending >= 0) take_stats(stats_slot(stats_pending));
-        } else {
-            take_stats(stats_slot(slots[i - 1]));
-        }
-        bytes += sweep_bytes();
+        bytes += sweep_bytes(sweep_entries(nnz, first + (uint32_t)i));
         if (prof) {
             float a = 0;
             hip_check(hipEventElapsedTime(&a, ev_iter[2 * i], ev_iter[2 * i + 1]), "elapsed");
             ms += a;
         }
     }
-    stats_pending = slots[count - 1];
     if (prof) {
         sampler_ms = ms;
         phi_ms = 0;
@@ -966,11 +960,14 @@ void Engine::enqueue_batch_iteration(hipStream_t s, hipEvent_t before, hipEvent_
     hip_check(hipStreamWaitEvent(stream2, ev_fork, 0), "wait");
     rebuild_theta_all(stream2);
     hip_check(hipEventRecord(ev_join, stream2), "event");
-    ll_reduce(s);
-    update_phi(s);
-    phi_head(s);
+    // LL stage 1 in the count update's clear launch, stage 2 in the head
+    // launch, which also advances the sweep counter for a single model
+    const LLArgs ll = ll_args();
+    const bool one = models.size() == 1;
+    update_phi(s, &ll);
+    phi_head(s, &ll, one);
     hip_check(hipStreamWaitEvent(s, ev_join, 0), "wait");
-    hip_check(launch_add_u32(d_sweep, 1, s), "sweep advance");
+    if (!one) hip_check(launch_add_u32(d_sweep, 1, s), "sweep advance");
 }
 
 void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double *ll, double *seconds) {
@@ -1004,7 +1001,6 @@ void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double
     if (prof) hip_check(hipEventRecord(ev[0], stream), "event");
     // the last eager iteration's sampler launch is timed on its own
     const bool time_sampler = prof && !use_graph;
-    if (time_sampler) last_sampler_bytes = sweep_bytes();
     for (int32_t i = 0; i < count; ++i) {
         if (use_graph)
             hip_check(hipGraphLaunch(graph, stream), "graph launch");
@@ -1026,25 +1022,34 @@ void Engine::iterate_batch(uint32_t first, int32_t count, bool use_graph, double
         sampler_launches = time_sampler ? 1 : 0;
     }
     sampled = true;
+    const size_t M = models.size();
+    if (time_sampler) {
+        std::vector<double> nnz((size_t)kLLHistory * M);
+        hip_check(hipMemcpy(nnz.data(), d_nnz_hist, nnz.size() * 8, hipMemcpyDeviceToHost), "history copy");
+        last_sampler_bytes = sweep_bytes(sweep_entries(nnz, first + (uint32_t)count - 1));
+    }
     if (ll) {
-        const size_t M = models.size();
         std::vector<double> hist((size_t)kLLHistory * M);
         hip_check(hipMemcpy(hist.data(), d_ll_hist, hist.size() * 8, hipMemcpyDeviceToHost), "ll copy");
         for (int32_t i = 0; i < count; ++i)
             for (size_t mi = 0; mi < M; ++mi)
                 ll[(size_t)i * M + mi] = hist[(size_t)((first + i) % kLLHistory) * M + mi] / (double)models[mi].num_tokens;
     }
-    read_stats();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
 // Per token: doc id, doc meta, z write, the doc's theta nonzeros; per slot
 // the word's n_wk row, or for a rare slot staged from zp (when the counts
 // allow) zp, the count and the denominator of each token's topic.
-double Engine::sweep_bytes() const {
-    int64_t n = 0, nnz = 0;
+double Engine::sweep_entries(const std::vector<double> &nnz_hist, uint32_t sweep) const {
+    double e = 0;
+    for (size_t mi = 0; mi < models.size(); ++mi) e += nnz_hist[(size_t)(sweep % kLLHistory) * models.size() + mi];
+    return e;
+}
+
+double Engine::sweep_bytes(double nnz) const {
+    int64_t n = 0;
     for (auto &m : models) n += m.num_tokens;
-    for (int64_t v : unit_token_nnz) nnz += v;
     const bool zp_stage = zp_valid && sparse_phi;
     const double rows = (double)(num_slots - (zp_stage ? rare_slots : 0)) * 4.0 * (double)prm.k;
     const double rare = zp_stage ? 6.0 * (double)rare_tokens : 0.0;  // zp topic + den_k

@@ -129,6 +129,9 @@ public:
     int32_t *d_model_packets = nullptr;
     double *d_ll_scratch = nullptr;
     double *d_ll_hist = nullptr;
+    // theta entries read per slot, per packet and per (sweep, model): the
+    // sampler's algorithmic bytes, summed like the LL
+    double *d_nnz_partial = nullptr, *d_nnz_scratch = nullptr, *d_nnz_hist = nullptr;
     int32_t num_ll_packets = 0;
     PhiList phi_all;  // every model, fused over its units
     // the zp-based update (update_phi when zp_valid): words with tokens that
@@ -141,7 +144,6 @@ public:
     Packet *d_theta_packets_all = nullptr;
     int32_t num_theta_packets_all = 0;
     int32_t *d_nk_all = nullptr;  // M*K
-    unsigned long long *d_stats = nullptr;  // 2 per unit
     uint32_t *d_sweep = nullptr;
     uint32_t *d_queue = nullptr;
     int32_t persistent_groups = 0;  // 0 = per-slot grid
@@ -153,21 +155,14 @@ public:
     std::vector<hipEvent_t> ev_sampled;
     std::vector<hipEvent_t> ev_theta_unit;  // per-chunk iteration: theta of unit u rebuilt (stream2)
     // The per-chunk iteration returns with the last chunk's theta rebuild
-    // still running on stream2; each iteration's rebuild statistics land in
-    // the next slot of a pinned ring (written by a copy on stream2) and are
-    // read one iteration later or by settle().
-    static constexpr int32_t kStatsSlots = kLLHistory + 2;  // > iterations of one call + the pending slot
+    // still running on stream2 (settle() waits for it).
     bool stream2_busy = false;
-    int32_t stats_pending = -1;  // ring slot holding stats not yet read, or -1
-    int32_t stats_next = 0;      // ring slot of the next iteration's copy
-    unsigned long long *h_stats_ring = nullptr;  // kStatsSlots x 2U
     std::vector<hipEvent_t> ev_iter;             // profiling: two per iteration of iterate_units_n
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // batched iteration: theta on stream2
     hipEvent_t ev[4] = {};
     double sampler_ms = 0, phi_ms = 0, theta_ms = 0, iter_ms = 0;
     int32_t sampler_launches = 0;
     double last_sampler_bytes = 0;
-    std::vector<int64_t> unit_doc_nnz, unit_token_nnz;  // after the last theta rebuild
 
     ~Engine();
     void init_device(const culda_params &p);
@@ -180,24 +175,28 @@ public:
 
     void set_sweep(uint32_t t, hipStream_t s);
     void init_z();
-    void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s);
-    void rebuild_phi_all(hipStream_t s);
+    // ll (optional): the LL stage 1 rides in the count-clear launch
+    void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s, const LLArgs *ll = nullptr);
+    void rebuild_phi_all(hipStream_t s, const LLArgs *ll = nullptr);
     void copy_zp(int32_t model, hipStream_t s);  // zp (and zm) = z for the model's units (all when model < 0)
     void fresh_rows(hipStream_t s);  // n_wk rows current (rare-tile words moved from zm to zp)
     // counts of every model after a sweep: a rebuild that rewrites only the
     // touched entries of rare words when zp_valid, else a full rebuild
     // (fused, or chunks in turn for a single corpus)
-    void update_phi(hipStream_t s);
-    void phi_head(hipStream_t s);
+    void update_phi(hipStream_t s, const LLArgs *ll = nullptr);
+    // head denominators; ll: also the LL stage 2; advance (one model): the
+    // sweep counter then moves on to the next sweep
+    void phi_head(hipStream_t s, const LLArgs *ll = nullptr, bool advance = false);
+    LLArgs ll_args() const;
     void rebuild_theta_unit(int32_t unit, hipStream_t s);
     void rebuild_theta_all(hipStream_t s);
     void sample_unit(int32_t unit, hipStream_t s);
     void sample_all(hipStream_t s);
     void ll_reduce(hipStream_t s);
-    void read_stats();
-    void take_stats(const unsigned long long *h);
-    // wait for the theta rebuilds a per-chunk iteration left on stream2 and
-    // read their statistics (before anything else touches device state)
+    // sum_d entries(d) and sum_d n_d * entries(d) over every unit's current theta rows
+    void theta_stats(int64_t &doc_nnz, int64_t &token_nnz);
+    // wait for the theta rebuilds a per-chunk iteration left on stream2
+    // (before anything else touches device state)
     void settle();
     void rebuild_all();
 
@@ -207,12 +206,16 @@ public:
     // `count` such iterations from sweep `first` with no host round trip
     // between them; ll[i] = LL/token of sweep first + i
     void iterate_units_n(uint32_t first, int32_t count, double *ll, double *seconds);
-    void enqueue_units_iteration(uint32_t sweep, unsigned long long *stats_dst, hipEvent_t ev_a, hipEvent_t ev_b);
-    unsigned long long *stats_slot(int32_t i) { return h_stats_ring + (size_t)i * 2 * units.size(); }
+    // set_sweep: write the sweep index first (else the previous iteration's
+    // head launch has moved the counter on to it)
+    void enqueue_units_iteration(uint32_t sweep, bool set_sweep, hipEvent_t ev_a, hipEvent_t ev_b);
     // culda_batch iteration: one launch per step for every model
     void enqueue_batch_iteration(hipStream_t s, hipEvent_t before = nullptr, hipEvent_t after = nullptr);
-    // algorithmic bytes of one sweep over every unit with the current theta
-    double sweep_bytes() const;
+    // algorithmic bytes of one sweep over every unit that read `entries`
+    // theta entries in all
+    double sweep_bytes(double entries) const;
+    // theta entries read by sweep `sweep` of every model (from the history)
+    double sweep_entries(const std::vector<double> &nnz_hist, uint32_t sweep) const;
     void iterate_batch(uint32_t first, int32_t count, bool use_graph, double *ll, double *seconds);
     double model_ll(int32_t model, uint32_t sweep);  // per token, from the history
     TuneResult autotune(const std::vector<int32_t> &candidates, bool batch);

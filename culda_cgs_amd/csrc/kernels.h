@@ -9,18 +9,24 @@ namespace culda {
 hipError_t launch_init_z(const InitArgs &a, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s);
-// fill[0, n_fill) = 0 and the n listed (model, word) n_wk rows zeroed, one launch
+// fill[0, n_fill) = 0 and the n listed (model, word) n_wk rows zeroed, one
+// launch; with ll, that launch also sums the LL packets (stage 1)
 hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n,
-                            hipStream_t s);
+                            hipStream_t s, const LLArgs *ll = nullptr);
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s);
 // the word-slot rebuild p and the rare tiles' count pass r (r.mode 0) in one launch
 hipError_t launch_phi_update(const PhiArgs &p, const RareArgs &r, hipStream_t s);
 hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
-hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s);
+// head denominators; with ll also the LL stage 2, with advance (one model)
+// the sweep counter then moves on by one
+hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s, const LLArgs *ll = nullptr,
+                      uint32_t *advance = nullptr);
 hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, int32_t num_words, int32_t k,
                            float beta, hipStream_t s);
-// alone: nothing else runs beside it (a single-chunk model), see theta_rebuild_alone_kernel
-hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, hipStream_t s);
+// alone: nothing else runs beside it (a single-chunk model), see
+// theta_rebuild_alone_kernel; else at most max_groups workgroups (0: one
+// per packet) take the packets in turn
+hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, int32_t max_groups, hipStream_t s);
 // grid is used in persistent mode (a.queue != null) only
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s);
@@ -65,6 +71,8 @@ hipError_t launch_joint_ll_words(const int32_t *nwk, int32_t num_words, int32_t 
 hipError_t launch_joint_ll_docs(const DocMeta *meta, const uint16_t *theta, int32_t num_docs, float alpha,
                                 double *partial, int32_t grid, hipStream_t s);
 hipError_t launch_copy(const float4 *src, float4 *dst, int64_t n, hipStream_t s);
+// out[0] += sum entries, out[1] += sum n_d * entries over a unit's docs
+hipError_t launch_meta_stats(const DocMeta *meta, int32_t num_docs, unsigned long long *out, hipStream_t s);
 
 // Allocation-free joint LL: a static device array of kJointGrid * (1 +
 // kJointUnitsMax) partials plus the result; joint_scratch_init (once per

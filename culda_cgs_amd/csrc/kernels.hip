@@ -56,10 +56,18 @@ __global__ void add_u32_kernel(uint32_t *p, uint32_t v) {
 // Clear before a count rebuild: fill[0, n_fill) = 0 (the n_k vectors), and
 // for rows[r] = (model, word) that model's n_wk row (hot words, before
 // their pieces are added with atomics).
-__global__ void phi_clear_kernel(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_fill; i += (int64_t)gridDim.x * blockDim.x)
+// Workgroups from clear_groups on sum the LL packets of the sweeps before
+// (stage 1, see ll_packet_sum), which the clear does not touch.
+__device__ __forceinline__ void ll_packet_sum(const LLArgs &a, int p);
+__global__ __launch_bounds__(256) void phi_clear_kernel(uint32_t *fill, int64_t n_fill, const ModelDev *models,
+                                                        const int2 *rows, int32_t n, int32_t clear_groups, LLArgs ll) {
+    if ((int)blockIdx.x >= clear_groups) {
+        ll_packet_sum(ll, blockIdx.x - clear_groups);
+        return;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_fill; i += (int64_t)clear_groups * blockDim.x)
         fill[i] = 0;
-    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+    for (int r = blockIdx.x; r < n; r += clear_groups) {
         const int2 mw = rows[r];
         const ModelDev &m = models[mw.x];
         int32_t *row = m.nwk + (int64_t)mw.y * m.k;
@@ -344,7 +352,15 @@ __global__ __launch_bounds__(256) void phi_update_kernel(PhiArgs p, RareArgs r) 
 
 // den_k = fl(fl(n_k) + fl(beta * fl(V))) for every model of the table
 // (kernel/lda_phi_kernel.cu:26); one workgroup per model.
-__global__ void den_kernel(const ModelDev *models, int32_t num_models) {
+// Head denominators; with ll.num_packets > 0 also the LL stage 2 of every
+// model, and with advance (one workgroup) the sweep counter moves to the next
+// sweep after the LL of this one is stored.
+__device__ __forceinline__ void ll_model_sum(const LLArgs &a, int mi);
+__global__ __launch_bounds__(256) void den_kernel(const ModelDev *models, int32_t num_models, LLArgs ll,
+                                                  uint32_t *advance) {
+    if (ll.num_packets > 0 && threadIdx.x == 0)
+        for (int mi = blockIdx.x; mi < ll.num_models; mi += gridDim.x) ll_model_sum(ll, mi);
+    if (advance && threadIdx.x == 0) *advance = *advance + 1u;  // one workgroup: after its LL store
     for (int mi = blockIdx.x; mi < num_models; mi += gridDim.x) {
         const ModelDev &m = models[mi];
         const float bv = m.beta * (float)m.num_words;
@@ -364,30 +380,6 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 }
 
 // ------------------------------------------------------------ theta rebuild
-// Adds the workgroup's entry statistics (Σ entries, Σ entries × doc length)
-// to the unit's counters: one pair of atomics per workgroup, not per wave
-// (all of a workgroup's docs belong to one unit).  Every wave calls it.
-__device__ __forceinline__ void flush_theta_stats(unsigned long long *stats, unsigned long long nnz,
-                                                  unsigned long long tok, int wave, int lane) {
-    __shared__ unsigned long long st[kWavesPerGroup][2];
-    if (lane == 0) {
-        st[wave][0] = nnz;
-        st[wave][1] = tok;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long n = 0, t = 0;
-        for (int w = 0; w < kWavesPerGroup; ++w) {
-            n += st[w][0];
-            t += st[w][1];
-        }
-        if (n) {
-            atomicAdd(&stats[0], n);
-            atomicAdd(&stats[1], t);
-        }
-    }
-}
-
 // One workgroup per packet (a run of docs of one unit), one wave per doc:
 // LDS histogram over the doc's tokens (gathered through rev_idx), then an
 // ascending compaction of the nonzero topics into 16-bit entries; a count
@@ -397,13 +389,15 @@ __device__ __forceinline__ void flush_theta_stats(unsigned long long *stats, uns
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
-    const Packet pk = a.packets[blockIdx.x];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *h = hist_s[wave];
+    // a grid smaller than the packet list takes the packets in turn (a
+    // rebuild beside a sweep holds a bounded share of the CUs)
+    for (int pi = blockIdx.x; pi < a.num_packets; pi += gridDim.x) {
+    const Packet pk = a.packets[pi];
     const UnitDev &un = a.units[pk.owner];
     const ModelDev &m = a.models[un.model];
     const int K = m.k;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint32_t *h = hist_s[wave];
-    unsigned long long s_nnz = 0, s_tok = 0;
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
         for (int k = lane; k < K; k += kWave) h[k] = 0;
         wave_sync();
@@ -450,11 +444,9 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
         if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
         if (lane == 0) un.meta[d].nnz = base;
-        s_nnz += base;
-        s_tok += (unsigned long long)base * (e - b);
         wave_sync();
     }
-    flush_theta_stats(un.stats, s_nnz, s_tok, wave, lane);
+    }
 }
 
 // The same rebuild for a launch that runs alone (a single-chunk model,
@@ -471,7 +463,6 @@ __global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
     for (int k = 4 * lane; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0, 0, 0, 0);
-    unsigned long long s_nnz = 0, s_tok = 0;
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
         wave_sync();
         const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
@@ -523,11 +514,8 @@ __global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
         if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
         if (lane == 0) un.meta[d].nnz = base;
-        s_nnz += base;
-        s_tok += (unsigned long long)base * (e - b);
         wave_sync();
     }
-    flush_theta_stats(un.stats, s_nnz, s_tok, wave, lane);
 }
 
 // Bitonic sort of the keys of lanes [0, M) of each M-lane group into
@@ -662,6 +650,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
     // aligned: two groups' four prefixes are one store)
     float *rr = rs_s + rrow * a.rs_stride + 4;
     double ll = 0.0;
+    uint32_t gathered = 0;  // theta entries this lane's tokens read (the slot's total fits 32 bits)
     // Philox blocks [pc0, pc0 + 64), block pc0 + l's four words (top 24
     // bits) in lane l: token g draws from word g & 3 of block g >> 2, so one
     // evaluation per lane serves up to four batches of the slot
@@ -859,12 +848,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             store_global_u16(un.z + base + tok, (uint16_t)zres[lane]);
             // FP32 term log(fl(SQ / fl(fl(n_d) + fl(K alpha)))), summed in FP64
             ll += (double)logf(sqres[lane] / ((float)pnd + (float)md.k_alpha));
+            gathered += pnnz;
         }
         wave_sync();
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
+    gathered = wave_add_scan_u32(gathered);
     if (lane == 0) a.ll_partial[slot] = ll;
+    if (lane == kWave - 1) a.nnz_partial[slot] = (double)gathered;
 #undef PHI_LO
 #undef PHI_HI
 #undef COUNT_LO
@@ -879,33 +871,54 @@ __global__ void copy_u16_kernel(const uint16_t *src, uint16_t *dst, int64_t n) {
         dst[i] = src[i];
 }
 
-// Fixed-order FP64 sums of the per-slot LL partials, per model, in two
-// launches: stage 1 sums each packet (a contiguous slot range of one
-// model), stage 2 sums a model's packets in order and stores the total at
+// Fixed-order FP64 sums of the per-slot LL partials, per model: stage 1
+// sums each packet (a contiguous slot range of one model), stage 2 sums a
+// model's packets in order and stores the total at
 // history[sweep % kLLHistory][model].  The order depends only on the slot
-// lists, so every run adds the same values in the same order.
+// lists, so every run adds the same values in the same order.  Inside an
+// iteration stage 1 runs as extra workgroups of the count-clear launch and
+// stage 2 inside the head-denominator launch (no launches of their own).
 // Reference: LDATrainPerplexityReduce1/2 (kernel/lda_train_kernel.cu:273-321),
 // which truncated partials to int.
-__global__ __launch_bounds__(256) void ll_stage1_kernel(LLArgs a) {
-    __shared__ double s[256];
-    const Packet pk = a.packets[blockIdx.x];
-    double acc = 0.0;
-    for (int i = pk.first + threadIdx.x; i < pk.last; i += 256) acc += a.partial[i];
+__device__ __forceinline__ void ll_packet_sum(const LLArgs &a, int p) {
+    __shared__ double s[256], g[256];
+    const Packet pk = a.packets[p];
+    double acc = 0.0, nz = 0.0;
+    for (int i = pk.first + threadIdx.x; i < pk.last; i += 256) {
+        acc += a.partial[i];
+        nz += a.nnz_partial[i];
+    }
     s[threadIdx.x] = acc;
+    g[threadIdx.x] = nz;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        if ((int)threadIdx.x < w) {
+            s[threadIdx.x] += s[threadIdx.x + w];
+            g[threadIdx.x] += g[threadIdx.x + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) a.scratch[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) {
+        a.scratch[p] = s[0];
+        a.nnz_scratch[p] = g[0];
+    }
 }
 
+__device__ __forceinline__ void ll_model_sum(const LLArgs &a, int mi) {
+    double acc = 0.0, nz = 0.0;
+    for (int p = a.model_packets[2 * mi]; p < a.model_packets[2 * mi + 1]; ++p) {
+        acc += a.scratch[p];
+        nz += a.nnz_scratch[p];
+    }
+    const size_t h = (size_t)(*a.sweep % kLLHistory) * a.num_models + mi;
+    a.history[h] = acc;
+    a.nnz_history[h] = nz;
+}
+
+__global__ __launch_bounds__(256) void ll_stage1_kernel(LLArgs a) { ll_packet_sum(a, blockIdx.x); }
+
 __global__ __launch_bounds__(64) void ll_stage2_kernel(LLArgs a) {
-    const int mi = blockIdx.x;
-    if (threadIdx.x != 0 || mi >= a.num_models) return;
-    double acc = 0.0;
-    for (int p = a.model_packets[2 * mi]; p < a.model_packets[2 * mi + 1]; ++p) acc += a.scratch[p];
-    a.history[(size_t)(*a.sweep % kLLHistory) * a.num_models + mi] = acc;
+    if (threadIdx.x == 0 && (int)blockIdx.x < a.num_models) ll_model_sum(a, blockIdx.x);
 }
 
 // Joint log-likelihood pieces: sum over nonzero n_wk of
@@ -986,6 +999,34 @@ __global__ __launch_bounds__(256) void joint_final_kernel(const double *partial,
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// Theta statistics of one unit's current rows: out[0] += sum_d entries(d),
+// out[1] += sum_d n_d * entries(d) (integer sums, one atomic pair per
+// workgroup).
+__global__ __launch_bounds__(256) void meta_stats_kernel(const DocMeta *meta, int32_t num_docs,
+                                                         unsigned long long *out) {
+    __shared__ unsigned long long s[2][256];
+    unsigned long long n = 0, t = 0;
+    for (int d = blockIdx.x * 256 + threadIdx.x; d < num_docs; d += gridDim.x * 256) {
+        const DocMeta m = meta[d];
+        n += m.nnz;
+        t += (unsigned long long)m.nnz * m.nd;
+    }
+    s[0][threadIdx.x] = n;
+    s[1][threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(&out[0], s[0][0]);
+        atomicAdd(&out[1], s[1][0]);
+    }
+}
+
 __global__ void copy_kernel(const float4 *src, float4 *dst, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -1017,10 +1058,13 @@ hipError_t launch_add_u32(uint32_t *p, uint32_t v, hipStream_t s) {
 }
 
 hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *models, const int2 *rows, int32_t n,
-                            hipStream_t s) {
-    if (n_fill <= 0 && n <= 0) return hipSuccess;
+                            hipStream_t s, const LLArgs *ll) {
+    const int32_t llp = ll ? std::max(ll->num_packets, 0) : 0;
+    if (n_fill <= 0 && n <= 0 && llp == 0) return hipSuccess;
     const int32_t grid = std::max(n_fill > 0 ? grid_for(n_fill, 256, 4096) : 1, n > 0 ? grid_for(n, 1, 4096) : 1);
-    hipLaunchKernelGGL(phi_clear_kernel, dim3(grid), dim3(256), 0, s, fill, n_fill, models, rows, n);
+    LLArgs none{};
+    hipLaunchKernelGGL(phi_clear_kernel, dim3(grid + llp), dim3(256), 0, s, fill, n_fill, models, rows, n, grid,
+                       llp ? *ll : none);
     return hipGetLastError();
 }
 
@@ -1048,9 +1092,13 @@ hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s) {
+hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s, const LLArgs *ll,
+                      uint32_t *advance) {
     if (num_models <= 0) return hipSuccess;
-    hipLaunchKernelGGL(den_kernel, dim3(grid_for(num_models, 1, 4096)), dim3(256), 0, s, models, num_models);
+    const int32_t grid = grid_for(num_models, 1, 4096);
+    if (advance && grid != 1) return hipErrorInvalidValue;  // the advance follows the one workgroup's LL store
+    LLArgs none{};
+    hipLaunchKernelGGL(den_kernel, dim3(grid), dim3(256), 0, s, models, num_models, ll ? *ll : none, advance);
     return hipGetLastError();
 }
 
@@ -1062,12 +1110,14 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
     return hipGetLastError();
 }
 
-hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, hipStream_t s) {
+hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, int32_t max_groups, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
     if (alone)
         hipLaunchKernelGGL(theta_rebuild_alone_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL(theta_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(theta_rebuild_kernel,
+                           dim3(max_groups > 0 ? std::min(a.num_packets, max_groups) : a.num_packets), dim3(256), 0, s,
+                           a);
     return hipGetLastError();
 }
 
@@ -1178,6 +1228,12 @@ hipError_t launch_joint_final(const double *partial, int32_t num_partials, const
                               hipStream_t s) {
     hipLaunchKernelGGL(joint_final_kernel, dim3(1), dim3(256), 0, s, partial, num_partials, nk, k, vb, units,
                        unit_begin, unit_end, ka, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_meta_stats(const DocMeta *meta, int32_t num_docs, unsigned long long *out, hipStream_t s) {
+    if (num_docs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(meta_stats_kernel, dim3(grid_for(num_docs, 256, 1024)), dim3(256), 0, s, meta, num_docs, out);
     return hipGetLastError();
 }
 

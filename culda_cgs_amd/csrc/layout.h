@@ -81,7 +81,6 @@ struct UnitDev {
     const uint32_t *rev_idx;
     DocMeta *meta;
     uint16_t *theta;
-    unsigned long long *stats;  // [0] += sum entries, [1] += sum n_d * entries (last rebuild)
     int32_t model;
     int32_t num_docs;
     uint16_t *zm;               // z the rare-word rows of n_wk hold the counts of (one-unit models)
@@ -128,6 +127,7 @@ struct SampleArgs {
     const UnitDev *units;
     const ModelDev *models;
     double *ll_partial;     // one per slot index
+    double *nnz_partial;    // one per slot index: theta entries its tokens read (exact in FP64)
     const uint32_t *sweep;  // sweep index t >= 1, read on the device
     uint32_t *queue;        // persistent mode: next list position; else null
     int32_t num_slots;
@@ -188,12 +188,17 @@ struct ThetaArgs {
     int32_t pad;
 };
 
+// The per-slot LL partials and gathered-entry counts summed per model, both
+// in the same fixed order.
 struct LLArgs {
     const double *partial;  // per slot index
     const Packet *packets;  // owner = model, [first, last) = slot indices
     const int32_t *model_packets;  // per model: [begin, end) into packets (2 ints)
     double *scratch;        // one per packet
     double *history;        // [kLLHistory][num_models]
+    const double *nnz_partial;  // per slot index
+    double *nnz_scratch;        // one per packet
+    double *nnz_history;        // [kLLHistory][num_models]: theta entries read by that sweep
     const uint32_t *sweep;
     int32_t num_packets;
     int32_t num_models;

@@ -75,7 +75,7 @@ double Engine::heldout_ll(Engine &train, int32_t sweeps) {
     std::vector<double> part(eval.num_slots);
     hip_check(hipMemcpyAsync(part.data(), eval.partial, part.size() * 8, hipMemcpyDeviceToHost, stream), "copy");
     hip_check(hipStreamSynchronize(stream), "held-out sync");
-    read_stats();
+    sync();
     double sum = 0;
     for (double v : part) sum += v;
     phi_ready = true;
@@ -425,7 +425,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             a.logits = s.Li;
             a.word = s.tok_word + base;
             a.gdelta = s.gdelta[ui - m.unit_begin];
-            a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, d_stats + 2 * ui, u.model,
+            a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, u.model,
                              u.doc_end - u.doc_start};
             a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K,
                                m.V};
@@ -444,7 +444,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
     unsigned long long acc = 0;
     hip_check(hipMemcpyAsync(&acc, s.accepted, 8, hipMemcpyDeviceToHost, stream), "copy");
     hip_check(hipStreamSynchronize(stream), "mh iteration");
-    read_stats();
+    sync();
     sampled = true;
     const double tried = (double)s.N * (double)proposals;
     return tried > 0 ? (double)acc / tried : 0.0;
