@@ -702,6 +702,31 @@ void Engine::update_phi(hipStream_t s, const LLArgs *ll) {
     }
 }
 
+void Engine::update_counts_one_unit(hipStream_t s, const LLArgs *ll) {
+    const ThetaArgs t{units[0].theta_packets, d_units, d_models, units[0].num_theta_packets, 0};
+    const RareArgs none{d_rare_tiles, d_rare_packets, d_units, d_models, 0, 0};
+    if (zp_valid && sparse_phi) {
+        hip_check(launch_phi_clear((uint32_t *)d_nk_all, prm.k, d_models, phi_all.zero_rows, phi_all.num_zero_rows, s,
+                                   ll),
+                  "phi clear");
+        const PhiArgs a{phi_sparse.slots, phi_sparse.packets, d_units, d_models, phi_sparse.num_packets, 2};
+        const RareArgs r{d_rare_tiles, d_rare_packets, d_units, d_models, num_rare_packets, 0};
+        hip_check(launch_count_update(a, r, t, s), "count update launch");
+        rare_stale = num_rare_packets > 0;
+    } else {
+        hip_check(launch_phi_clear((uint32_t *)d_nk_all, prm.k, d_models, phi_all.zero_rows, phi_all.num_zero_rows, s,
+                                   ll),
+                  "phi clear");
+        const PhiArgs a{phi_all.slots, phi_all.packets, d_units, d_models, phi_all.num_packets, 0};
+        hip_check(launch_count_update(a, none, t, s), "count update launch");
+        if (sparse_phi) copy_zp(-1, s);
+        zp_valid = sparse_phi;
+        rare_stale = false;
+    }
+    phi_ready = false;
+    theta_ok[0] = 1;
+}
+
 void Engine::rebuild_phi_all(hipStream_t s, const LLArgs *ll) {
     hip_check(launch_phi_clear((uint32_t *)d_nk_all, (int64_t)models.size() * prm.k, d_models, phi_all.zero_rows,
                                phi_all.num_zero_rows, s, ll),
@@ -831,6 +856,19 @@ double Engine::model_ll(int32_t mi, uint32_t sweep) {
 void Engine::enqueue_units_iteration(uint32_t sweep, bool set, hipEvent_t ev_a, hipEvent_t ev_b) {
     const int32_t U = (int32_t)units.size();
     if (set) set_sweep(sweep, stream);
+    if (U == 1) {
+        // one chunk: sweep, then the count update and the theta rebuild in
+        // one launch, all on one stream (the next sweep needs both)
+        if (stream2_busy) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[0], 0), "wait");
+        stream2_busy = false;
+        if (ev_a) hip_check(hipEventRecord(ev_a, stream), "event");
+        sample_unit(0, stream);
+        if (ev_b) hip_check(hipEventRecord(ev_b, stream), "event");
+        const LLArgs ll = ll_args();
+        update_counts_one_unit(stream, &ll);
+        phi_head(stream, &ll, true);
+        return;
+    }
     const bool after_iteration = stream2_busy;
     for (int32_t u = 0; u < U; ++u) {
         if (after_iteration) hip_check(hipStreamWaitEvent(stream, ev_theta_unit[u], 0), "wait");

@@ -178,6 +178,9 @@ public:
     // ll (optional): the LL stage 1 rides in the count-clear launch
     void rebuild_phi_model(int32_t model, int32_t unit, bool accumulate, hipStream_t s, const LLArgs *ll = nullptr);
     void rebuild_phi_all(hipStream_t s, const LLArgs *ll = nullptr);
+    // a one-unit model after a sweep: count update and theta rebuild in one
+    // launch (after the count clear, which carries ll's stage 1)
+    void update_counts_one_unit(hipStream_t s, const LLArgs *ll);
     void copy_zp(int32_t model, hipStream_t s);  // zp (and zm) = z for the model's units (all when model < 0)
     void fresh_rows(hipStream_t s);  // n_wk rows current (rare-tile words moved from zm to zp)
     // counts of every model after a sweep: a rebuild that rewrites only the

@@ -17,6 +17,9 @@ hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s);
 // the word-slot rebuild p and the rare tiles' count pass r (r.mode 0) in one launch
 hipError_t launch_phi_update(const PhiArgs &p, const RareArgs &r, hipStream_t s);
 hipError_t launch_phi_rare(const RareArgs &a, hipStream_t s);
+// one launch: theta rebuild t (a one-unit model, the single-chunk kernel),
+// phi slots p and rare tiles r (r.mode 0)
+hipError_t launch_count_update(const PhiArgs &p, const RareArgs &r, const ThetaArgs &t, hipStream_t s);
 // head denominators; with ll also the LL stage 2, with advance (one model)
 // the sweep counter then moves on by one
 hipError_t launch_den(const ModelDev *models, int32_t num_models, hipStream_t s, const LLArgs *ll = nullptr,

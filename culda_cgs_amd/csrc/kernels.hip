@@ -454,9 +454,8 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // doc's compaction reads all of a lane's counters in one LDS round trip and
 // clears them -- shorter per doc, but at 78 VGPRs it does not fit beside
 // the sampler's waves, as the per-chunk rebuilds of larger models must.
-__global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
-    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
-    const Packet pk = a.packets[blockIdx.x];
+__device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet, uint32_t (*hist_s)[kMaxK]) {
+    const Packet pk = a.packets[packet];
     const UnitDev &un = a.units[pk.owner];
     const ModelDev &m = a.models[un.model];
     const int K = m.k;
@@ -516,6 +515,27 @@ __global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
         if (lane == 0) un.meta[d].nnz = base;
         wave_sync();
     }
+}
+
+__global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    theta_alone_body(a, blockIdx.x, hist_s);
+}
+
+// A one-unit model's count update and theta rebuild in one launch (both
+// read only the new z): workgroups [0, t.num_packets) rebuild theta, then
+// the phi slot packets, then the rare tiles (r.mode 0).  The next sweep
+// depends on both, so neither needs a stream of its own.
+__global__ __launch_bounds__(256) void count_update_kernel(PhiArgs p, RareArgs r, ThetaArgs t) {
+    __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    __shared__ uint32_t nk_s[kMaxK];
+    const int b = blockIdx.x;
+    if (b < t.num_packets)
+        theta_alone_body(t, b, hist_s);
+    else if (b < t.num_packets + p.num_packets)
+        phi_rebuild_body(p, b - t.num_packets, hist_s, nk_s);
+    else
+        phi_rare_body<false>(r, b - t.num_packets - p.num_packets, nk_s);
 }
 
 // Bitonic sort of the keys of lanes [0, M) of each M-lane group into
@@ -1071,6 +1091,16 @@ hipError_t launch_phi_clear(uint32_t *fill, int64_t n_fill, const ModelDev *mode
 hipError_t launch_phi_rebuild(const PhiArgs &a, hipStream_t s) {
     if (a.num_packets <= 0) return hipSuccess;
     hipLaunchKernelGGL(phi_rebuild_kernel, dim3(a.num_packets), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_update(const PhiArgs &p, const RareArgs &r, const ThetaArgs &t, hipStream_t s) {
+    if (r.mode) return hipErrorInvalidValue;  // the count pass only
+    const int64_t grid =
+        (int64_t)std::max(t.num_packets, 0) + std::max(p.num_packets, 0) + std::max(r.num_packets, 0);
+    if (grid <= 0) return hipSuccess;
+    if (grid > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(count_update_kernel, dim3((uint32_t)grid), dim3(256), 0, s, p, r, t);
     return hipGetLastError();
 }
 
