@@ -944,10 +944,46 @@ void Engine::iterate_units_n(uint32_t first, int32_t count, double *ll, double *
             hip_check(hipEventCreate(&e), "hipEventCreate");
             ev_iter.push_back(e);
         }
+    // One chunk: the iteration (sweep, count clear, count update + theta,
+    // head) is captured once as a graph and replayed; it reads the sweep
+    // index on the device and advances it.  The capture depends on whether
+    // the counts follow zp (the sparse update), so a change of that state
+    // captures again.  A replayed graph has no per-sweep timing events: the
+    // call then reports a sampler time of 0 (culda_iterate times one).
+    const bool use_graph = U == 1;
+    const bool key = zp_valid && sparse_phi;
+    if (use_graph && (!graph || graph_key != key)) {
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        settle();
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            enqueue_units_iteration(first, false, nullptr, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "end capture");
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "graph instantiate");
+        graph_key = key;
+        graph_rare_stale = rare_stale;
+    }
     auto t0 = std::chrono::steady_clock::now();
-    for (int32_t i = 0; i < count; ++i)
-        enqueue_units_iteration(first + (uint32_t)i, i == 0, prof ? ev_iter[2 * i] : nullptr,
-                                prof ? ev_iter[2 * i + 1] : nullptr);
+    if (use_graph) {
+        set_sweep(first, stream);
+        for (int32_t i = 0; i < count; ++i) hip_check(hipGraphLaunch(graph, stream), "graph launch");
+        rare_stale = graph_rare_stale;
+    } else {
+        for (int32_t i = 0; i < count; ++i)
+            enqueue_units_iteration(first + (uint32_t)i, i == 0, prof ? ev_iter[2 * i] : nullptr,
+                                    prof ? ev_iter[2 * i + 1] : nullptr);
+    }
     std::vector<double> hist(kLLHistory), nnz(kLLHistory);
     hip_check(hipMemcpyAsync(hist.data(), d_ll_hist, kLLHistory * sizeof(double), hipMemcpyDeviceToHost, stream),
               "ll copy");
@@ -958,7 +994,7 @@ void Engine::iterate_units_n(uint32_t first, int32_t count, double *ll, double *
     double bytes = 0, ms = 0;
     for (int32_t i = 0; i < count; ++i) {
         bytes += sweep_bytes(sweep_entries(nnz, first + (uint32_t)i));
-        if (prof) {
+        if (prof && !use_graph) {
             float a = 0;
             hip_check(hipEventElapsedTime(&a, ev_iter[2 * i], ev_iter[2 * i + 1]), "elapsed");
             ms += a;

@@ -111,7 +111,8 @@ public:
     // the n_wk rows of the rare-tile words hold the counts of zm, not of zp;
     // fresh_rows() brings them up to date before anything reads n_wk
     bool rare_stale = false;
-    bool graph_rare_stale = false;  // a replay of the captured batched iteration leaves them stale
+    bool graph_rare_stale = false;  // a replay of the captured iteration leaves them stale
+    bool graph_key = false;         // one-chunk graph: captured with the zp-based count update
     // rebuilds touch only the entries of rare words (at least a quarter of
     // the words below K / 16 tokens); otherwise every row is rewritten
     bool sparse_phi = false;

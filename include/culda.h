@@ -248,7 +248,9 @@ int culda_iterate(culda_ctx *ctx, int32_t iteration, double *ll, double *seconds
  * NULL), seconds = wall time until the last LL is known.  culda_last_sweeps
  * then reports the sums over the count iterations.  Same chain as count
  * culda_iterate calls (no counterpart in the reference: its loop reads the
- * LL back every iteration). */
+ * LL back every iteration).  A one-chunk corpus replays its iteration as a
+ * HIP graph (captured on the first call): no per-sweep timing then, and
+ * culda_last_sweeps reports a sampler time of 0. */
 int culda_iterate_n(culda_ctx *ctx, int32_t first_iteration, int32_t count, double *ll, double *seconds);
 
 /* Rebuild phi, head and theta from the current z (initial model state,

@@ -348,7 +348,8 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
     must give the chain of iterations each followed by a full sync, and the
     per-iteration sampler bytes must be the ones of the rebuild each sweep
     read; iterate_n (iterations enqueued back to back in one call) must give
-    the same chain, LLs and summed bytes."""
+    the same chain, LLs and summed bytes (a one-chunk corpus replays a
+    captured graph there, with no per-sweep timing)."""
     V, K, arrays = _setup(31, V=500, D=400, mean_len=90, chunks=chunks, K=128)
     alpha, beta, seed = 0.2, 0.01, 17
     live, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
@@ -372,7 +373,8 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
     ll_m, _ = batched.iterate_n(3, 4)
     ms, b, n = batched.last_sweeps()
     assert ll_n + ll_m == ll_live
-    assert n == 4 * chunks and ms > 0 and b == sum(b_live[2:])
+    assert n == 4 * chunks and b == sum(b_live[2:])
+    assert (ms == 0) if chunks == 1 else (ms > 0)
     for c in range(chunks):
         assert np.array_equal(live.get_z(c), synced.get_z(c))
         assert np.array_equal(batched.get_z(c), synced.get_z(c))
