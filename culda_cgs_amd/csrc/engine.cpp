@@ -878,9 +878,8 @@ void Engine::enqueue_units_iteration(uint32_t sweep, bool set, hipEvent_t ev_a, 
         hip_check(hipEventRecord(ev_sampled[u], stream), "event");
         hip_check(hipStreamWaitEvent(stream2, ev_sampled[u], 0), "wait");
         rebuild_theta_unit(u, stream2);
-        if (u + 1 < U) hip_check(hipEventRecord(ev_theta_unit[u], stream2), "event");
+        hip_check(hipEventRecord(ev_theta_unit[u], stream2), "event");
     }
-    hip_check(hipEventRecord(ev_theta_unit[U - 1], stream2), "event");
     if (ev_b) hip_check(hipEventRecord(ev_b, stream), "event");
     // the LL reduction rides in the count update's clear launch (stage 1)
     // and in the head-denominator launch (stage 2), which then moves the

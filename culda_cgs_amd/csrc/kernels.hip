@@ -353,13 +353,37 @@ __global__ __launch_bounds__(256) void phi_update_kernel(PhiArgs p, RareArgs r) 
 // den_k = fl(fl(n_k) + fl(beta * fl(V))) for every model of the table
 // (kernel/lda_phi_kernel.cu:26); one workgroup per model.
 // Head denominators; with ll.num_packets > 0 also the LL stage 2 of every
-// model, and with advance (one workgroup) the sweep counter moves to the next
-// sweep after the LL of this one is stored.
-__device__ __forceinline__ void ll_model_sum(const LLArgs &a, int mi);
+// model (the packet sums fetched by all threads at once, then added in
+// packet order by one), and with advance (one workgroup) the sweep counter
+// moves to the next sweep after the LL of this one is stored.
 __global__ __launch_bounds__(256) void den_kernel(const ModelDev *models, int32_t num_models, LLArgs ll,
                                                   uint32_t *advance) {
-    if (ll.num_packets > 0 && threadIdx.x == 0)
-        for (int mi = blockIdx.x; mi < ll.num_models; mi += gridDim.x) ll_model_sum(ll, mi);
+    __shared__ double ps[256], pn[256];
+    if (ll.num_packets > 0) {
+        const uint32_t sweep = *ll.sweep;
+        for (int mi = blockIdx.x; mi < ll.num_models; mi += gridDim.x) {
+            const int p0 = ll.model_packets[2 * mi], p1 = ll.model_packets[2 * mi + 1];
+            double acc = 0.0, nz = 0.0;
+            for (int c = p0; c < p1; c += 256) {
+                __syncthreads();
+                if (c + (int)threadIdx.x < p1) {
+                    ps[threadIdx.x] = ll.scratch[c + threadIdx.x];
+                    pn[threadIdx.x] = ll.nnz_scratch[c + threadIdx.x];
+                }
+                __syncthreads();
+                if (threadIdx.x == 0)
+                    for (int j = 0; j < 256 && c + j < p1; ++j) {
+                        acc += ps[j];
+                        nz += pn[j];
+                    }
+            }
+            if (threadIdx.x == 0) {
+                const size_t h = (size_t)(sweep % kLLHistory) * ll.num_models + mi;
+                ll.history[h] = acc;
+                ll.nnz_history[h] = nz;
+            }
+        }
+    }
     if (advance && threadIdx.x == 0) *advance = *advance + 1u;  // one workgroup: after its LL store
     for (int mi = blockIdx.x; mi < num_models; mi += gridDim.x) {
         const ModelDev &m = models[mi];
