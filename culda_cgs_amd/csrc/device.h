@@ -31,6 +31,13 @@ __device__ __forceinline__ u32x2 load_global2(const uint32_t *p) { return *(cons
 __device__ __forceinline__ u32x4 load_global4(const void *p) { return *(const CULDA_AS1 u32x4 *)p; }
 __device__ __forceinline__ uint32_t load_global_u16(const uint16_t *p) { return *(const CULDA_AS1 uint16_t *)p; }
 __device__ __forceinline__ void store_global_u16(uint16_t *p, uint16_t v) { *(CULDA_AS1 uint16_t *)p = v; }
+// Global-address-space view of a generic pointer, and a relaxed
+// device-scope atomic add through it (a global_atomic, not a flat one).
+template <class T>
+__device__ __forceinline__ CULDA_AS1 T *glb(T *p) { return (CULDA_AS1 T *)p; }
+__device__ __forceinline__ void atomic_add_global(int32_t *p, int32_t v) {
+    __hip_atomic_fetch_add(glb(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // The same at base + a 32-bit byte offset (base wave-uniform): the address
 // is the scalar base plus a 32-bit vector offset, no 64-bit vector math.
 __device__ __forceinline__ u32x2 load_global2_at(const void *base, uint32_t off) {
@@ -41,6 +48,9 @@ __device__ __forceinline__ uint32_t load_global_at(const void *base, uint32_t of
 }
 __device__ __forceinline__ uint32_t load_global_u16_at(const void *base, uint32_t off) {
     return *(const CULDA_AS1 uint16_t *)((const char *)base + off);
+}
+__device__ __forceinline__ void store_global_u16_at(void *base, uint32_t off, uint16_t v) {
+    *(CULDA_AS1 uint16_t *)((char *)base + off) = v;
 }
 
 // DPP move with zero for lanes whose source is outside the 16-lane row.

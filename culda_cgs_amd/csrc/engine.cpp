@@ -384,6 +384,10 @@ void Engine::upload() {
         int64_t all_tokens = 0;
         for (const ModelHost &m : models) all_tokens += m.num_tokens;
         const int64_t split = phi_split(all_tokens);
+        // whole-word lists in packets of one resident round of the rebuild
+        // launch (a second round of a few workgroups would double its time)
+        const int32_t per_cu = phi_blocks_per_cu();
+        const size_t phi_groups = (size_t)cu_count * (size_t)(per_cu > 0 ? per_cu : 8);
         std::vector<PhiSlot> all_slots, sparse_slots;
         std::vector<Packet> all_packets;
         std::vector<int2> all_zero;
@@ -404,7 +408,7 @@ void Engine::upload() {
             model_tiles[mi + 1] = (int64_t)tiles.size();
             for (size_t i = 0; i < sl.size(); ++i)
                 if (sl[i].split || (tok[i] > 0 && !covered[sl[i].word])) model_sparse[mi].push_back(sl[i]);
-            const int32_t per = std::max<int32_t>(4, (int32_t)((sl.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
+            const int32_t per = std::max<int32_t>(4, (int32_t)((sl.size() + phi_groups - 1) / phi_groups));
             std::vector<Packet> pk;
             deal_packets(sl, mi, per, 0, pk);
             m.phi_fused.slots = upload_vec(*this, sl);
@@ -423,7 +427,7 @@ void Engine::upload() {
                 std::vector<int2> z1;
                 build_phi_list(units, mi, ui, ui + 1, m.V, split, s1, t1, z1);
                 const int32_t per1 =
-                    std::max<int32_t>(4, (int32_t)((s1.size() + (size_t)cu_count * 8 - 1) / ((size_t)cu_count * 8)));
+                    std::max<int32_t>(4, (int32_t)((s1.size() + phi_groups - 1) / phi_groups));
                 std::vector<Packet> p1;
                 deal_packets(s1, mi, per1, 0, p1);
                 UnitHost &u = units[ui];

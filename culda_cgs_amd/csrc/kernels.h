@@ -35,6 +35,7 @@ hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s);
 // workgroups of the persistent sampler resident per CU (0 if unknown)
 int32_t sample_blocks_per_cu(int32_t rs_stride);
+int32_t phi_blocks_per_cu();  // resident phi-rebuild workgroups per CU (0 if unknown)
 int32_t sample_rs_stride(int64_t max_entries);
 
 // mh.hip

@@ -87,7 +87,6 @@ __global__ __launch_bounds__(256) void phi_clear_kernel(uint32_t *fill, int64_t 
 // The counts are the same either way.
 // Reference: LDAUpdatePhiKernel (kernel/lda_phi_kernel.cu:76-117) and the
 // chunk reduction LDAUpdatePhiReduceKernel* (:162-183).
-constexpr int kSmallWordUnits = 16;  // units whose loads the small-word path keeps in flight together
 
 // Histogram of z[b, e) into the wave's LDS bins h; zp = z when keep_zp.
 // Each lane issues eight loads before their LDS atomics, so a long word
@@ -99,20 +98,83 @@ __device__ __forceinline__ void count_range(const uint16_t *z, uint16_t *zp, int
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int64_t x = i + j * kWave;
-            t[j] = x < e ? z[x] : 0;
+            t[j] = x < e ? (uint16_t)load_global_u16(z + x) : 0;
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int64_t x = i + j * kWave;
             if (x < e) {
                 atomicAdd(&h[t[j]], 1u);
-                if (keep_zp) zp[x] = t[j];
+                if (keep_zp) store_global_u16(zp + x, t[j]);
             }
         }
     }
 }
 
+// Histogram into the wave's LDS bins h of a word's tokens in units j < nu
+// <= 64, lane j holding unit j's z and zp pointers and range [wb, we): the
+// 64-token blocks of all those units in unit order, kGather blocks' loads
+// in flight before their LDS atomics; zp = z when kKeepZp (fewer blocks in
+// flight then: their zp addresses take registers too).  A word of up to
+// kGather blocks waits for one memory round trip whatever the number of
+// its units.  The cursor (unit j, its next block's address, the tokens left
+// in it) is wave-uniform, so a block costs a few scalar operations and one
+// vector load at a scalar base plus the lane's byte offset.
+template <bool kKeepZp, int kGather>
+__device__ __forceinline__ void count_blocks(const uint16_t *zl, uint16_t *zpl, int nu, int64_t wb, int64_t we,
+                                             int lane, uint32_t *h) {
+    constexpr uint32_t kNone = 0xffffffffu;  // no token in this lane of the block
+    int j = 0;
+    const char *zb = nullptr;
+    char *zpb = nullptr;
+    int64_t left = 0;
+    auto seek = [&]() {  // first unit from j on with tokens left
+        for (; j < nu; ++j) {
+            const int64_t p = readlane64(wb, j);
+            left = readlane64(we, j) - p;
+            if (left > 0) {
+                zb = reinterpret_cast<const char *>(readlane64((int64_t)zl, j)) + 2 * p;
+                if (kKeepZp) zpb = reinterpret_cast<char *>(readlane64((int64_t)zpl, j)) + 2 * p;
+                return;
+            }
+        }
+    };
+    seek();
+    const uint32_t off = 2u * (uint32_t)lane;
+    while (j < nu) {
+        uint32_t t[kGather];
+        char *zq[kGather];
+#pragma unroll
+        for (int q = 0; q < kGather; ++q) {
+            t[q] = kNone;
+            zq[q] = nullptr;
+            if (j < nu) {
+                if ((int64_t)lane < left) t[q] = load_global_u16_at(zb, off);
+                if (kKeepZp) zq[q] = zpb;
+                zb += 2 * kWave;
+                zpb += 2 * kWave;
+                left -= kWave;
+                if (left <= 0) {
+                    ++j;
+                    seek();
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kGather; ++q)
+            if (t[q] != kNone) {
+                atomicAdd(&h[t[q]], 1u);
+                if (kKeepZp) store_global_u16_at(zq[q], off, (uint16_t)t[q]);
+            }
+    }
+}
+
 // Body of the phi rebuild for workgroup packet `packet`, on the caller's LDS.
+// A wave loads its next item's slot descriptor and word ranges while it
+// counts the current one (the descriptor two items ahead); the ranges of a
+// slot's first 64 units come through per-lane word_idx pointers, loaded
+// again only when the slot's unit range changes (every whole-word item of a
+// packet has the same one).
 __device__ __forceinline__ void phi_rebuild_body(const PhiArgs &a, int packet, uint32_t (*hist_s)[kMaxK],
                                                  uint32_t *nk_s) {
     const Packet pk = a.packets[packet];
@@ -121,137 +183,167 @@ __device__ __forceinline__ void phi_rebuild_body(const PhiArgs &a, int packet, u
     const bool keep_zp = a.accumulate == 2;
     for (int k = threadIdx.x; k < K; k += blockDim.x) nk_s[k] = 0;
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
-    for (int si = pk.first + wave; si < pk.last; si += kWavesPerGroup) {
-        const PhiSlot sl = a.slots[si];
+    int ulo = -1, uhi = -1;
+    // lane j: word_idx, z and zp of unit ulo + j
+    const int64_t *wil = nullptr;
+    const uint16_t *zl = nullptr;
+    uint16_t *zpl = nullptr;
+    // lane j < min(64, unit count): the slot word's range in unit unit_lo + j
+    auto load_ranges = [&](const PhiSlot &s, int64_t &b, int64_t &e) {
+        b = e = 0;
+        if (s.split) return;
+        if (s.unit_lo != ulo || s.unit_hi != uhi) {
+            ulo = s.unit_lo;
+            uhi = s.unit_hi;
+            if (lane < uhi - ulo) {
+                const CULDA_AS1 UnitDev &u = *glb(a.units + ulo + lane);
+                wil = u.word_idx;
+                zl = u.z;
+                zpl = u.zp;
+            }
+        }
+        if (lane < uhi - ulo) {
+            b = *glb(wil + s.word);
+            e = *glb(wil + s.word + 1);
+        }
+    };
+    constexpr int step = kWavesPerGroup;
+    int si = pk.first + wave;
+    PhiSlot s1{}, s2{};
+    int64_t b1 = 0, e1 = 0;
+    if (si < pk.last) {
+        s1 = a.slots[si];
+        load_ranges(s1, b1, e1);
+    }
+    if (si + step < pk.last) s2 = a.slots[si + step];
+    for (; si < pk.last; si += step) {
+        const PhiSlot sl = s1;
+        const int64_t wb = b1, we = e1;
+        if (si + step < pk.last) {
+            s1 = s2;
+            load_ranges(s1, b1, e1);
+            if (si + 2 * step < pk.last) s2 = a.slots[si + 2 * step];
+        }
         int32_t *row = m.nwk + (int64_t)sl.word * K;
         // fn(unit, b, e) over the word's token range [b, e) in every unit of
-        // the slot, in unit order: the ranges of up to 64 units are loaded
-        // at once, one unit per lane, and read back from registers, so a
-        // word costs one round trip for its ranges instead of one per unit
+        // the slot, in unit order (ranges of units past the first 64 loaded
+        // here, one unit per lane)
         auto each_unit = [&](auto &&fn) {
             for (int g0 = sl.unit_lo; g0 < sl.unit_hi; g0 += kWave) {
                 const int cnt = sl.unit_hi - g0 < kWave ? sl.unit_hi - g0 : kWave;
-                int64_t wb = 0, we = 0;
-                if (lane < cnt) {
-                    const int64_t *wi = a.units[g0 + lane].word_idx;
-                    wb = wi[sl.word];
-                    we = wi[sl.word + 1];
+                int64_t b = wb, e = we;
+                if (g0 != sl.unit_lo) {
+                    b = e = 0;
+                    if (lane < cnt) {
+                        const int64_t *wi = glb(a.units + g0 + lane)->word_idx;
+                        b = *glb(wi + sl.word);
+                        e = *glb(wi + sl.word + 1);
+                    }
                 }
-                for (int j = 0; j < cnt; ++j) fn(g0 + j, readlane64(wb, j), readlane64(we, j));
+                for (int j = 0; j < cnt; ++j) fn(g0 + j, readlane64(b, j), readlane64(e, j));
             }
         };
-        bool sparse = false;
+        const int nu = sl.unit_hi - sl.unit_lo;
         if (!sl.split) {
             int64_t n = 0;
             each_unit([&](int, int64_t b, int64_t e) { n += e - b; });
             if (n == 0 && a.accumulate != 0) continue;  // nothing to add, row already right
-            sparse = keep_zp && n * 16 < K;
+            if (keep_zp && n * 16 < K) {
+                // Every loop runs over the word's tokens, not over K.  The
+                // row holds the counts of zp: entries of old topics that
+                // have no new token are cleared, entries of new topics get
+                // their new counts -- disjoint stores, so no ordering
+                // between them.
+                each_unit([&](int u, int64_t b, int64_t e) {
+                    const uint16_t *z = a.units[u].z, *zp = a.units[u].zp;
+                    for (int64_t i = b + lane; i < e; i += kWave) {
+                        h[load_global_u16(z + i)] = 0;
+                        h[load_global_u16(zp + i)] = 0;
+                    }
+                });
+                wave_sync();
+                each_unit([&](int u, int64_t b, int64_t e) {
+                    const uint16_t *z = a.units[u].z;
+                    for (int64_t i = b + lane; i < e; i += kWave) {
+                        const uint32_t t = load_global_u16(z + i);
+                        atomicAdd(&h[t], 1u);
+                        atomicAdd(&nk_s[t], 1u);
+                    }
+                });
+                wave_sync();
+                each_unit([&](int u, int64_t b, int64_t e) {
+                    const uint16_t *z = a.units[u].z;
+                    uint16_t *zp = a.units[u].zp;
+                    for (int64_t i = b + lane; i < e; i += kWave) {
+                        const uint16_t t = (uint16_t)load_global_u16(z + i), p = (uint16_t)load_global_u16(zp + i);
+                        if (h[p] == 0) *glb(row + p) = 0;
+                        *glb(row + t) = (int)h[t];
+                        store_global_u16(zp + i, t);
+                    }
+                });
+                wave_sync();
+                continue;
+            }
         }
-        if (sparse) {
-            // Every loop runs over the word's tokens, not over K.  The row
-            // holds the counts of zp: entries of old topics that have no
-            // new token are cleared, entries of new topics get their new
-            // counts -- disjoint stores, so no ordering between them.
-            each_unit([&](int u, int64_t b, int64_t e) {
-                const uint16_t *z = a.units[u].z, *zp = a.units[u].zp;
-                for (int64_t i = b + lane; i < e; i += kWave) {
-                    h[z[i]] = 0;
-                    h[zp[i]] = 0;
-                }
-            });
-            wave_sync();
-            each_unit([&](int u, int64_t b, int64_t e) {
-                const uint16_t *z = a.units[u].z;
-                for (int64_t i = b + lane; i < e; i += kWave) {
-                    const uint16_t t = z[i];
-                    atomicAdd(&h[t], 1u);
-                    atomicAdd(&nk_s[t], 1u);
-                }
-            });
-            wave_sync();
-            each_unit([&](int u, int64_t b, int64_t e) {
-                const uint16_t *z = a.units[u].z;
-                uint16_t *zp = a.units[u].zp;
-                for (int64_t i = b + lane; i < e; i += kWave) {
-                    const uint16_t t = z[i], p = zp[i];
-                    if (h[p] == 0) row[p] = 0;
-                    row[t] = (int)h[t];
-                    zp[i] = t;
-                }
-            });
-            wave_sync();
-            continue;
-        }
-        for (int k = lane; k < K; k += kWave) h[k] = 0;
+        for (int k = lane * 4; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0u, 0u, 0u, 0u);
         wave_sync();
-        if (!sl.split) {
-            const int nu = sl.unit_hi - sl.unit_lo;
-            int64_t wb = 0, we = 0;
-            if (lane < nu && nu <= kSmallWordUnits) {
-                const int64_t *wi = a.units[sl.unit_lo + lane].word_idx;
-                wb = wi[sl.word];
-                we = wi[sl.word + 1];
-            }
-            if (nu <= kSmallWordUnits && __ballot(we - wb > kWave) == 0ull) {
-                // at most 64 of the word's tokens in every unit: one load
-                // per unit per lane, all in flight at once
-                uint16_t t[kSmallWordUnits];
-                uint32_t have = 0;  // bit j: this lane holds a token of unit j
-#pragma unroll
-                for (int j = 0; j < kSmallWordUnits; ++j) {
-                    t[j] = 0;
-                    if (j < nu) {
-                        const int64_t i = readlane64(wb, j) + lane;
-                        if (i < readlane64(we, j)) {
-                            t[j] = a.units[sl.unit_lo + j].z[i];
-                            have |= 1u << j;
-                        }
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < kSmallWordUnits; ++j)
-                    if ((have >> j) & 1u) {
-                        atomicAdd(&h[t[j]], 1u);
-                        if (keep_zp) a.units[sl.unit_lo + j].zp[readlane64(wb, j) + lane] = t[j];
-                    }
-            } else {
-                each_unit([&](int u, int64_t b, int64_t e) { count_range(a.units[u].z, a.units[u].zp, b, e, lane, h, keep_zp); });
-            }
+        const uint16_t *cz = zl;
+        uint16_t *czp = zpl;
+        int cn = nu;
+        int64_t cb = wb, ce = we;
+        if (sl.split) {
+            const CULDA_AS1 UnitDev &u = *glb(a.units + sl.unit_lo);
+            cz = u.z;
+            czp = u.zp;
+            cn = 1;
+            cb = sl.begin;
+            ce = sl.end;
+        }
+        if (cn <= kWave) {
+            if (keep_zp)
+                count_blocks<true, 8>(cz, czp, cn, cb, ce, lane, h);
+            else
+                count_blocks<false, 16>(cz, czp, cn, cb, ce, lane, h);
         } else {
-            count_range(a.units[sl.unit_lo].z, a.units[sl.unit_lo].zp, sl.begin, sl.end, lane, h, keep_zp);
+            each_unit([&](int u, int64_t b, int64_t e) { count_range(a.units[u].z, a.units[u].zp, b, e, lane, h, keep_zp); });
         }
         wave_sync();
         if (!sl.split) {
+            // whole row with 16-byte stores; n_k from the same bins
             for (int k = lane * 4; k < K; k += 4 * kWave) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(h + k);
                 int4 o = make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+                CULDA_AS1 int4 *dst = reinterpret_cast<CULDA_AS1 int4 *>(glb(row + k));
                 if (a.accumulate == 1) {
-                    const int4 old = *reinterpret_cast<const int4 *>(row + k);
+                    const int4 old = *dst;
                     o.x += old.x;
                     o.y += old.y;
                     o.z += old.z;
                     o.w += old.w;
                 }
-                *reinterpret_cast<int4 *>(row + k) = o;
+                *dst = o;
+                if (v.x) atomicAdd(&nk_s[k], v.x);
+                if (v.y) atomicAdd(&nk_s[k + 1], v.y);
+                if (v.z) atomicAdd(&nk_s[k + 2], v.z);
+                if (v.w) atomicAdd(&nk_s[k + 3], v.w);
             }
         } else {
             for (int k = lane; k < K; k += kWave) {
                 const uint32_t v = h[k];
-                if (v) atomicAdd(row + k, (int)v);
+                if (v) {
+                    atomic_add_global(row + k, (int)v);
+                    atomicAdd(&nk_s[k], v);
+                }
             }
-        }
-        for (int k = lane; k < K; k += kWave) {
-            const uint32_t v = h[k];
-            if (v) atomicAdd(&nk_s[k], v);
         }
         wave_sync();
     }
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += blockDim.x) {
         const uint32_t v = nk_s[k];
-        if (v) atomicAdd(m.nk + k, (int)v);
+        if (v) atomic_add_global(m.nk + k, (int)v);
     }
 }
 
@@ -326,7 +418,7 @@ __device__ __forceinline__ void phi_rare_body(const RareArgs &a, int packet, uin
         __syncthreads();
         for (int k = threadIdx.x; k < K; k += blockDim.x) {
             const uint32_t v = nk_s[k];
-            if (v) atomicAdd(m.nk + k, (int)v);
+            if (v) atomic_add_global(m.nk + k, (int)v);
         }
     }
 }
@@ -1227,6 +1319,19 @@ int32_t sample_blocks_per_cu(int32_t rs_stride) {
                              ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 6>, kWave, lds)
                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 7>, kWave, lds);
     return e == hipSuccess ? n : 0;
+}
+
+// Resident workgroups per CU of the phi rebuild (256 threads, static LDS):
+// a launch of at most this many packets per CU runs in one round.
+int32_t phi_blocks_per_cu() {
+    static int32_t cached = 0;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, phi_rebuild_kernel, 256, 0) == hipSuccess && n > 0)
+            cached = n;
+    });
+    return cached;
 }
 
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s) {
