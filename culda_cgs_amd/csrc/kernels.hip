@@ -690,6 +690,8 @@ __device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
 // kWaves waves per SIMD bound the registers (7: at most 72 VGPRs, 6: 80):
 // the per-token chains of dependent loads need the occupancy more than the
 // registers, unless the S-prefix rows in LDS already hold it below seven.
+constexpr int kStageSteps = kMaxK / (4 * kWave);  // 16-byte loads per lane that stage a row
+
 template <bool kPersistent, int kWaves>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) void sample_kernel(SampleArgs a) {
     // one wave per workgroup: every LDS array sits at a fixed address, so
@@ -736,22 +738,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         // of them, one per lane): the zero-count row (fl(beta / den_k), the
         // same value) plus, at each zp topic, its count among the lanes --
         // the row itself need not be current.
-        for (int k = lane * 4; k < K; k += 4 * kWave)
-            *reinterpret_cast<float4 *>(ph + k) = *reinterpret_cast<const float4 *>(md.zero_phi + k);
         const int n = (int)(s.end - s.begin);
-        const uint32_t t = lane < n ? un.zp[s.begin + lane] : 0xffffffffu;
+        const uint32_t t = lane < n ? load_global_u16(un.zp + s.begin + lane) : 0xffffffffu;
+        for (int k = lane * 4; k < K; k += 4 * kWave)
+            *reinterpret_cast<u32x4 *>(ph + k) = load_global4(md.zero_phi + k);
         uint32_t c = 0;
         for (int j = 0; j < n; ++j) c += (uint32_t)__builtin_amdgcn_readlane((int)t, j) == t;
         wave_sync();
-        if (lane < n) ph[t] = ((float)c + beta) / md.den[t];
+        if (lane < n) ph[t] = ((float)c + beta) / __uint_as_float(load_global((const uint32_t *)md.den + t));
     } else {
-        for (int k = lane * 4; k < K; k += 4 * kWave) {
-            const int4 c = *reinterpret_cast<const int4 *>(row + k);
-            const float4 dn = *reinterpret_cast<const float4 *>(md.den + k);
-            ph[k + 0] = ((float)c.x + beta) / dn.x;
-            ph[k + 1] = ((float)c.y + beta) / dn.y;
-            ph[k + 2] = ((float)c.z + beta) / dn.z;
-            ph[k + 3] = ((float)c.w + beta) / dn.w;
+        // the loads of kBatch 256-topic steps of the row and of den issued
+        // before their first use (all of them: one memory round trip per
+        // row; the persistent loop keeps more registers live, so one)
+        constexpr int kBatch = kPersistent ? 1 : kStageSteps;
+#pragma unroll
+        for (int i0 = 0; i0 < kStageSteps; i0 += kBatch) {
+            if (lane * 4 + i0 * 4 * kWave >= K) break;
+            u32x4 cr[kBatch], dr[kBatch];
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const int k = lane * 4 + (i0 + i) * 4 * kWave;
+                if (k < K) {
+                    cr[i] = load_global4(row + k);
+                    dr[i] = load_global4(md.den + k);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kBatch; ++i) {
+                const int k = lane * 4 + (i0 + i) * 4 * kWave;
+                if (k < K) {
+                    ph[k + 0] = ((float)(int32_t)cr[i].x + beta) / __uint_as_float(dr[i].x);
+                    ph[k + 1] = ((float)(int32_t)cr[i].y + beta) / __uint_as_float(dr[i].y);
+                    ph[k + 2] = ((float)(int32_t)cr[i].z + beta) / __uint_as_float(dr[i].z);
+                    ph[k + 3] = ((float)(int32_t)cr[i].w + beta) / __uint_as_float(dr[i].w);
+                }
+            }
         }
     }
     wave_sync();
