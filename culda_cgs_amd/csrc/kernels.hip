@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void phi_clear_kernel(uint32_t *fill, int64_t 
         const int2 mw = rows[r];
         const ModelDev &m = models[mw.x];
         int32_t *row = m.nwk + (int64_t)mw.y * m.k;
-        for (int j = threadIdx.x; j < m.k; j += blockDim.x) row[j] = 0;
+        for (int j = threadIdx.x; j < m.k; j += blockDim.x) *glb(row + j) = 0;
     }
 }
 
@@ -380,16 +380,16 @@ __device__ __forceinline__ void phi_rare_body(const RareArgs &a, int packet, uin
         const bool live = lane < t.count;
         if (!kMaterialize) {
             if (live) {
-                const uint16_t zt = u.z[t.begin + lane];
-                u.zp[t.begin + lane] = zt;
+                const uint16_t zt = (uint16_t)load_global_u16(u.z + t.begin + lane);
+                store_global_u16(u.zp + t.begin + lane, zt);
                 atomicAdd(&nk_s[zt], 1u);
             }
             continue;
         }
         uint32_t zt = 0, zo = 0;
         if (live) {
-            zt = u.zp[t.begin + lane];
-            zo = u.zm[t.begin + lane];
+            zt = load_global_u16(u.zp + t.begin + lane);
+            zo = load_global_u16(u.zm + t.begin + lane);
         }
         // word of present-word rank q: lane r (word w0 + r present) pushes
         // w0 + r to lane rank(r); absent words push to lane 63, which no
@@ -409,9 +409,9 @@ __device__ __forceinline__ void phi_rare_body(const RareArgs &a, int packet, uin
         }
         if (live) {
             int32_t *row = m.nwk + (int64_t)word * K;
-            if (!kept) row[zo] = 0;
-            row[zt] = (int32_t)c;
-            u.zm[t.begin + lane] = (uint16_t)zt;
+            if (!kept) *glb(row + zo) = 0;
+            *glb(row + zt) = (int32_t)c;
+            store_global_u16(u.zm + t.begin + lane, (uint16_t)zt);
         }
     }
     if (!kMaterialize) {
@@ -481,9 +481,9 @@ __global__ __launch_bounds__(256) void den_kernel(const ModelDev *models, int32_
         const ModelDev &m = models[mi];
         const float bv = m.beta * (float)m.num_words;
         for (int i = threadIdx.x; i < m.k; i += blockDim.x) {
-            const float d = (float)m.nk[i] + bv;
-            m.den[i] = d;
-            m.zero_phi[i] = m.beta / d;  // = fl(fl(0 + beta) / d)
+            const float d = (float)*glb(m.nk + i) + bv;
+            *glb(m.den + i) = d;
+            *glb(m.zero_phi + i) = m.beta / d;  // = fl(fl(0 + beta) / d)
         }
     }
 }
@@ -580,16 +580,16 @@ __device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet,
     for (int k = 4 * lane; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0, 0, 0, 0);
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
         wave_sync();
-        const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
-        const uint32_t start = un.meta[d].start;  // issued before the histogram's waits
+        const uint32_t b = load_global(un.rev_off + d), e = load_global(un.rev_off + d + 1);
+        const uint32_t start = glb(un.meta + d)->start;  // issued before the histogram's waits
         // four index loads, then four z gathers, in flight per lane
         for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
             uint32_t t[4];
             uint16_t zz[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? un.rev_idx[i0 + q * kWave] : 0xffffffffu;
+            for (int q = 0; q < 4; ++q) t[q] = i0 + q * kWave < e ? load_global(un.rev_idx + i0 + q * kWave) : 0xffffffffu;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? un.z[t[q]] : (uint16_t)0;
+            for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? (uint16_t)load_global_u16(un.z + t[q]) : (uint16_t)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
@@ -613,7 +613,7 @@ __device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet,
             if (__ballot(v > kThetaMaxCount) == 0ull) {
                 const unsigned long long msk = __ballot(v != 0);
                 const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                if (v) un.theta[start + pos] = (uint16_t)((v << kThetaCountShift) | (uint32_t)k);
+                if (v) store_global_u16(un.theta + start + pos, (uint16_t)((v << kThetaCountShift) | (uint32_t)k));
                 base += (uint32_t)__popcll(msk);
             } else {
                 const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
@@ -621,14 +621,14 @@ __device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet,
                 const uint32_t pos = base + incl - ne;
                 for (uint32_t q = 0; q < ne; ++q) {
                     const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
-                    un.theta[start + pos + q] = (uint16_t)((c << kThetaCountShift) | (uint32_t)k);
+                    store_global_u16(un.theta + start + pos + q, (uint16_t)((c << kThetaCountShift) | (uint32_t)k));
                 }
                 base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
             }
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
-        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
-        if (lane == 0) un.meta[d].nnz = base;
+        if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) store_global_u16(un.theta + start + base + lane, 0u);
+        if (lane == 0) glb(un.meta + d)->nnz = base;
         wave_sync();
     }
 }
