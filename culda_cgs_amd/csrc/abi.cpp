@@ -765,8 +765,13 @@ extern "C" int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, c
         upload_queries(e, s, n, docs, words);
         e.mh_sketch_current(e.stream);
         e.mh_query_logits(s.q_doc, s.q_word, n, s.Li, e.stream);
-        hip_check(hipMemcpyAsync(logits, s.Li, (size_t)n * s.K * 4, hipMemcpyDeviceToHost, e.stream), "copy");
+        std::vector<uint16_t> h((size_t)n * s.K);
+        if (n) hip_check(hipMemcpyAsync(h.data(), s.Li, h.size() * 2, hipMemcpyDeviceToHost, e.stream), "copy");
         hip_check(hipStreamSynchronize(e.stream), "sync");
+        for (size_t i = 0; i < h.size(); ++i) {
+            const uint32_t u = (uint32_t)h[i] << 16;  // BF16 -> FP32, exact
+            std::memcpy(logits + i, &u, 4);
+        }
     });
 }
 

@@ -239,7 +239,7 @@ public:
     double mh_train(int32_t epochs);
     double mh_iterate(uint32_t sweep, int32_t proposals);
     void mh_sketch_current(hipStream_t s);
-    void mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, float *d_logits, hipStream_t s);
+    void mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, uint16_t *d_logits, hipStream_t s);
     MHState &mh_state();
 };
 

@@ -44,6 +44,7 @@ constexpr uint32_t kThetaMaxCount = 63u;     // largest count of one entry
 constexpr int kJointGrid = 1024;     // workgroups (partials) of each joint-LL launch
 constexpr int kJointUnitsMax = 63;   // chunks the allocation-free joint LL serves (its static scratch)
 constexpr int kMaxDevices = 64;
+constexpr uint32_t kXcds = 8;      // accelerator dies of an MI355X; workgroup i runs on die i mod 8
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 // The sampler's LDS: phi row, Q block totals, z and S+Q of a 64-token batch
 // (static), plus eight rows of S block prefixes sized by the longest theta
@@ -277,7 +278,7 @@ struct MHTokenInfo {
 
 // Inputs of the MH step for a run of tokens of one unit.
 struct MHArgs {
-    const float *logits;        // [n][K] proposal logits (rows = tokens start .. start+n-1)
+    const uint16_t *logits;     // [n][K] BF16 proposal logits (rows = tokens start .. start+n-1)
     const uint32_t *word;       // word id per token of the unit
     const int64_t *gdelta;      // per word: global id = unit-local index + gdelta[w]
     UnitDev unit;               // tokens (doc ids), z, meta, theta

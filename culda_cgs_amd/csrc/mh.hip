@@ -67,18 +67,23 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
     return (uint16_t)(u >> 16);
 }
 
-// tanh-approximated GELU and its derivative.
+// tanh-approximated GELU and its derivative, through the logistic
+// function: 0.5 (1 + tanh(u)) = s = 1 / (1 + e^(-2u)), u = c (x + a x^3),
+// so gelu = x s and gelu' = s + 2 x s (1 - s) c (1 + 3 a x^2) -- one
+// hardware exp and one reciprocal instead of a tanh.
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluA = 0.044715f;
 
-__device__ __forceinline__ float gelu(float x) {
-    const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
-    return 0.5f * x * (1.0f + t);
+__device__ __forceinline__ float gelu_logistic(float x) {
+    const float u = kGeluC * (x + kGeluA * x * x * x);
+    return __frcp_rn(1.0f + __expf(-2.0f * u));
 }
 
+__device__ __forceinline__ float gelu(float x) { return x * gelu_logistic(x); }
+
 __device__ __forceinline__ float gelu_grad(float x) {
-    const float t = tanhf(kGeluC * (x + kGeluA * x * x * x));
-    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
+    const float sg = gelu_logistic(x);
+    return sg + 2.0f * x * sg * (1.0f - sg) * kGeluC * (1.0f + 3.0f * kGeluA * x * x);
 }
 
 __device__ __forceinline__ float wave_sum_f(float x) {
@@ -193,19 +198,39 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(GemmArgs g) {
             }
 }
 
+// Four FP32 values to BF16 (round to nearest even), packed for one 8-byte store.
+__device__ __forceinline__ uint2 f2bf4(float a, float b, float c, float d) {
+    return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+}
+
 // 128 x 128 C tile per workgroup of four waves (2 x 2, each 64 x 64 =
 // 4 x 4 MFMA 16x16x32 tiles) for A stored [m][k] and B stored [n][k]
 // (both k-contiguous).  32-deep k stages, double-buffered in LDS: the next
 // stage's global loads are issued before the current stage's MFMAs and
-// stored to the other buffer after them.  Same epilogues as the 64-tile
-// kernel.
-__global__ __launch_bounds__(256) void gemm_bf16_nt128_kernel(GemmArgs g) {
+// stored to the other buffer after them.  The MFMA takes B's fragment as
+// its first operand, so it forms C^T: lane l holds C[row l % 16][columns
+// 4 (l / 16) .. +3] of each 16 x 16 tile, and the epilogue stores four
+// consecutive columns at once (16 bytes FP32, 8 bytes BF16).  Tiles are
+// dealt to the eight XCDs in contiguous runs (workgroup i runs on XCD
+// i mod 8), so the column tiles of one row block share an L2.  Same
+// epilogues as the 64-tile kernel.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void gemm_bf16_nt128_kernel(GemmArgs g) {
     constexpr int T = 128, KS = kGemmK, LD = KS + 8;
     __shared__ __attribute__((aligned(16))) uint16_t As[2][T][LD];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][T][LD];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t m0 = (int64_t)blockIdx.y * T, n0 = (int64_t)blockIdx.x * T;
+    int64_t tm = blockIdx.y, tn = blockIdx.x;
+    {
+        const uint32_t nt = gridDim.x, tiles = gridDim.x * gridDim.y;
+        if (tiles % kXcds == 0) {
+            const uint32_t id = blockIdx.y * nt + blockIdx.x;
+            const uint32_t tile = (id % kXcds) * (tiles / kXcds) + id / kXcds;
+            tm = tile / nt;
+            tn = tile % nt;
+        }
+    }
+    const int64_t m0 = tm * T, n0 = tn * T;
     const int64_t kz = (int64_t)blockIdx.z * g.k;
     const int lr = t >> 2, lc = (t & 3) * 8;  // rows lr and lr + 64, k offset lc
     const uint16_t *pa = g.a + (m0 + lr) * g.lda + kz + lc;
@@ -244,7 +269,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt128_kernel(GemmArgs g) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], af[i], acc[i][j], 0, 0, 0);
         if (more) {
             const int nx = cur ^ 1;
             *reinterpret_cast<uint4 *>(&As[nx][lr][lc]) = ra0;
@@ -255,22 +280,38 @@ __global__ __launch_bounds__(256) void gemm_bf16_nt128_kernel(GemmArgs g) {
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+        const int64_t col = n0 + wn * 64 + j * 16 + 4 * (lane >> 4);
+        float4 bias = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (g.bias && g.epilogue != kEpiGeluGrad) bias = *reinterpret_cast<const float4 *>(g.bias + col);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + r;
-                const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
-                float x = acc[i][j][r];
-                if (g.epilogue == kEpiGeluGrad) {
-                    g.c16[row * g.ldc16 + col] = f2bf(x * gelu_grad(g.aux[row * g.ldaux + col]));
-                } else {
-                    if (g.bias) x += g.bias[col];
-                    if (g.c) g.c[blockIdx.z * g.split_stride + row * g.ldc + col] = x;
-                    if (g.c16) g.c16[row * g.ldc16 + col] = f2bf(g.epilogue == kEpiBiasGelu ? gelu(x) : x);
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = m0 + wm * 64 + i * 16 + (lane & 15);
+            float x0 = acc[i][j][0], x1 = acc[i][j][1], x2 = acc[i][j][2], x3 = acc[i][j][3];
+            if (g.epilogue == kEpiGeluGrad) {
+                const float4 p = *reinterpret_cast<const float4 *>(g.aux + row * g.ldaux + col);
+                *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) =
+                    f2bf4(x0 * gelu_grad(p.x), x1 * gelu_grad(p.y), x2 * gelu_grad(p.z), x3 * gelu_grad(p.w));
+            } else {
+                x0 += bias.x;
+                x1 += bias.y;
+                x2 += bias.z;
+                x3 += bias.w;
+                if (g.c)
+                    *reinterpret_cast<float4 *>(g.c + blockIdx.z * g.split_stride + row * g.ldc + col) =
+                        make_float4(x0, x1, x2, x3);
+                if (g.c16) {
+                    if (g.epilogue == kEpiBiasGelu) {
+                        x0 = gelu(x0);
+                        x1 = gelu(x1);
+                        x2 = gelu(x2);
+                        x3 = gelu(x3);
+                    }
+                    *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) = f2bf4(x0, x1, x2, x3);
                 }
             }
+        }
+    }
 }
 
 // BF16 transpose through 64 x 64 LDS tiles: dst[c][r] = src[r][c] for
@@ -559,12 +600,12 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
         const uint32_t j = lane + kWave * t;  // entries 2j, 2j + 1 (2j + 1 past nnz is a zero of the slab tail)
         tr[t] = 2 * j < dm.nnz ? trow[j] : 0u;
     }
-    const float *lg = a.logits + (int64_t)r * K;
+    const uint16_t *lg = a.logits + (int64_t)r * K + lane * KL;
     float e[16];
     float mx = -INFINITY;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        e[j] = j < KL ? lg[lane * KL + j] : -INFINITY;
+        e[j] = j < KL ? __uint_as_float(load_global_u16(lg + j) << 16) : -INFINITY;
         mx = fmaxf(mx, e[j]);
     }
     mx = wave_max_all_f(mx);
@@ -739,7 +780,12 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits) {
         ((uintptr_t)g.a & 15) || ((uintptr_t)g.b & 15) || g.m / kGemmTile > 65535)
         return hipErrorInvalidValue;
     if (g.epilogue == kEpiGeluGrad && (!g.c16 || !g.aux)) return hipErrorInvalidValue;
-    if (g.m % 128 == 0 && g.n % 128 == 0)
+    // the 128-tile kernel stores four consecutive columns at once: 16-byte
+    // aligned FP32 rows and bias, 8-byte aligned BF16 rows
+    const bool vec = g.ldc % 4 == 0 && g.ldc16 % 4 == 0 && g.ldaux % 4 == 0 && g.split_stride % 4 == 0 &&
+                     ((uintptr_t)g.c & 15) == 0 && ((uintptr_t)g.aux & 15) == 0 && ((uintptr_t)g.bias & 15) == 0 &&
+                     ((uintptr_t)g.c16 & 7) == 0;
+    if (g.m % 128 == 0 && g.n % 128 == 0 && vec)
         hipLaunchKernelGGL(gemm_bf16_nt128_kernel, dim3(g.n / 128, g.m / 128, splits), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL(gemm_bf16_kernel, dim3(g.n / kGemmTile, g.m / kGemmTile, splits), dim3(256), 0, s, g);

@@ -240,7 +240,7 @@ void Engine::mh_enable(const culda_mh_params &p_in) {
     const int64_t IB = p.infer_batch;
     s.Xi = (uint16_t *)dalloc(IB * kIn * 2);
     s.Hi = (uint16_t *)dalloc(IB * kHidden * 2);
-    s.Li = falloc(IB * K);
+    s.Li = (uint16_t *)dalloc(IB * K * 2);
     s.accepted = (unsigned long long *)dalloc(8);
     {
         const int64_t w2 = (int64_t)mh_splits(kHidden, (int32_t)K, (int32_t)B) * kHidden * K;
@@ -390,9 +390,10 @@ double Engine::mh_train(int32_t epochs) {
     return (double)(long long)fx / 16777216.0 / (double)T;
 }
 
-// Logits [n][K] (rows padded to 64 inside) for explicit (global doc, word)
-// queries against s.sk_cur.
-void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, float *d_logits,
+// BF16 logits [n][K] (rows padded to 128 inside) for explicit (global doc,
+// word) queries against s.sk_cur: the proposal needs no more precision,
+// and the product writes (and the MH step reads) half the bytes of FP32.
+void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, int64_t n, uint16_t *d_logits,
                              hipStream_t st) {
     MHState &s = mh_state();
     const int64_t np = round128(n);
@@ -400,7 +401,7 @@ void Engine::mh_query_logits(const uint32_t *d_docs, const uint32_t *d_words, in
     GemmArgs f1{s.Xi, s.w1t, nullptr, s.Hi, s.b1, nullptr, kIn, kIn, 0, kHidden, 0, (int32_t)np, kHidden, kIn,
                 kEpiBiasGelu};
     hip_check(launch_gemm_bf16(f1, st), "gemm x.w1");
-    GemmArgs f2{s.Hi, s.w2t, d_logits, nullptr, s.b2, nullptr, kHidden, kHidden, s.K, 0, 0, (int32_t)np, s.K, kHidden,
+    GemmArgs f2{s.Hi, s.w2t, nullptr, d_logits, s.b2, nullptr, kHidden, kHidden, 0, s.K, 0, (int32_t)np, s.K, kHidden,
                 kEpiStore};
     hip_check(launch_gemm_bf16(f2, st), "gemm h.w2");
 }

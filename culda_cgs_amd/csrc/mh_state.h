@@ -37,7 +37,7 @@ struct MHState {
     uint32_t *q_doc = nullptr, *q_word = nullptr;  // explicit query rows
     // inference buffers
     uint16_t *Xi = nullptr, *Hi = nullptr;
-    float *Li = nullptr;
+    uint16_t *Li = nullptr;  // BF16 logits [infer_batch][K], as the proposal uses them
     unsigned long long *accepted = nullptr;
     float *part = nullptr;  // split-K and column-sum partials
 };

@@ -416,7 +416,8 @@ int culda_mh_get_weights(culda_ctx *ctx, float *w1, float *b1, float *w2, float 
 int culda_mh_set_weights(culda_ctx *ctx, const float *w1, const float *b1, const float *w2, const float *b2,
                          const float *emb);
 /* Logits [n][K] for (doc, word) queries, doc sketches from the current
- * counts. */
+ * counts: FP32 values of the BF16 logits the proposal uses (the last
+ * product stores BF16). */
 int culda_mh_logits(culda_ctx *ctx, int64_t n, const int32_t *docs, const int32_t *words, float *logits);
 /* Gradients of the mean cross-entropy over n rows (n <= batch) of
  * (doc, word, target) at the current weights, without updating them;
