@@ -43,6 +43,12 @@ __device__ __forceinline__ void atomic_add_global(int32_t *p, int32_t v) {
 __device__ __forceinline__ u32x2 load_global2_at(const void *base, uint32_t off) {
     return *(const CULDA_AS1 u32x2 *)((const char *)base + off);
 }
+// ... plus a constant byte offset, added after the 32-bit offset (so it
+// can ride in the instruction's immediate field)
+template <int IMM>
+__device__ __forceinline__ u32x2 load_global2_at_imm(const void *base, uint32_t off) {
+    return *(const CULDA_AS1 u32x2 *)((const char *)base + off + IMM);
+}
 __device__ __forceinline__ uint32_t load_global_at(const void *base, uint32_t off) {
     return *(const CULDA_AS1 uint32_t *)((const char *)base + off);
 }

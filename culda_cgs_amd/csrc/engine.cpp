@@ -155,7 +155,7 @@ void check_corpus_limits(const Corpus &c, int32_t K) {
              "doc %d (%lld tokens) may need %lld theta entries at k = %d; the sampler's LDS holds rows of at most "
              "%lld entries",
              longest_doc, (long long)c.doc_len(longest_doc), (long long)longest, K,
-             (long long)((kSamplerLdsBudget - kSamplerStaticLds) / 32 - 8) / 4 * 4 * 16);
+             (long long)((kSamplerLdsBudget - kSamplerStaticLds) / 32 / 4 * 4 - 1) * 16);
 }
 
 // Hot words are rebuilt in pieces of at most phi_split(tokens) tokens per

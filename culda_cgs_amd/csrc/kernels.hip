@@ -654,24 +654,6 @@ __global__ __launch_bounds__(256) void count_update_kernel(PhiArgs p, RareArgs r
         phi_rare_body<false>(r, b - t.num_packets - p.num_packets, nk_s);
 }
 
-// Bitonic sort of the keys of lanes [0, M) of each M-lane group into
-// descending order (the last stage's direction is the same for every lane
-// below M).
-template <int M>
-__device__ __forceinline__ uint32_t bitonic_desc(uint32_t key, int lane) {
-#pragma unroll
-    for (int kk = 2; kk <= M; kk <<= 1) {
-#pragma unroll
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            const uint32_t other = __shfl_xor(key, j, kWave);
-            const bool up = (lane & kk) == 0, lower = (lane & j) == 0;
-            const uint32_t mx = key > other ? key : other, mn = key > other ? other : key;
-            key = (lower == up) ? mx : mn;
-        }
-    }
-    return key;
-}
-
 // ------------------------------------------------------------ sampler
 // One wave per slot (one word's token range of one unit).  The wave stages
 // the word's head row phi_k in LDS (computed from n_wk), builds the
@@ -697,13 +679,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
     // one wave per workgroup: every LDS array sits at a fixed address, so
     // the phi lookup is the entry's topic times 4 plus a constant offset
     __shared__ __attribute__((aligned(16))) float ph[kMaxK];
-    __shared__ float rq[kWave];
+    __shared__ __attribute__((aligned(16))) float rq[kWave];
     // eight rows of a.rs_stride floats: S block prefixes of each row's doc
     extern __shared__ __attribute__((aligned(16))) float rs_s[];
-    __shared__ uint16_t zres[kWave];
-    __shared__ float sqres[kWave];
     const int lane = threadIdx.x;
     const uint32_t sweep = *a.sweep;
+    const int rrow = lane >> 3, lr = lane & 7;
+    // rr[b] = inclusive S prefix after block b of this row's doc (16-byte
+    // aligned: two groups' four prefixes are one store)
+    float *const rr = rs_s + rrow * a.rs_stride;
     int pos = blockIdx.x;
     uint32_t next = 0, claim_end = 0;  // persistent: claimed positions [next, claim_end)
     while (true) {
@@ -798,14 +782,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         const float y = __shfl_up(T, d, kWave);
         if (lane >= d) T = T + y;
     }
-    rq[lane] = T;
+    // past the last block: -inf, which no target is below (the block
+    // search reads four prefixes at a time without bounds per prefix)
+    rq[lane] = lane < nbq ? T : -__builtin_inff();
     const float Q = __shfl(T, nbq - 1, kWave);
     wave_sync();
 
-    const int rrow = lane >> 3, lr = lane & 7;
-    // rr[b] = inclusive S prefix after block b of this row's doc (16-byte
-    // aligned: two groups' four prefixes are one store)
-    float *rr = rs_s + rrow * a.rs_stride + 4;
     double ll = 0.0;
     uint32_t gathered = 0;  // theta entries this lane's tokens read (the slot's total fits 32 bits)
     // Philox blocks [pc0, pc0 + 64), block pc0 + l's four words (top 24
@@ -843,28 +825,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             mnnz = m.y;
             mnd = m.z;
         }
-        // bitonic sort of (valid, nnz, lane), descending: positions
-        // 0..ntok-1 hold the valid tokens, longest theta rows first.  The
-        // valid tokens are lanes 0..ntok-1, so sorting the first 16 or 32
-        // lanes suffices for short batches, and a batch of at most 8 tokens
-        // is one step whatever its order.
-        uint32_t key = (valid ? (0x80000000u | ((mnnz < 0xffffu ? mnnz : 0xffffu) << 6)) : 0u) | (uint32_t)lane;
-        if (ntok > 32)
-            key = bitonic_desc<64>(key, lane);
-        else if (ntok > 16)
-            key = bitonic_desc<32>(key, lane);
-        else if (ntok > 8)
-            key = bitonic_desc<16>(key, lane);
-        const int tok = (int)(key & 63u);
-        const uint32_t pstart = __shfl(mstart, tok, kWave);
-        const uint32_t pnnz = __shfl(mnnz, tok, kWave);
-        const uint32_t pnd = __shfl(mnd, tok, kWave);
-        const float pu = __shfl(u, tok, kWave);
+        // Processing order: a counting sort of the valid tokens (lanes
+        // 0..ntok-1) by the number of 32-entry S groups of their doc,
+        // descending, so that the eight rows of a step run S loops of
+        // similar length.  The histogram sits in the first 64 words of the
+        // S-prefix rows (free between batches): lane l's count of tokens
+        // with key l, then the number of tokens with a larger key.  Equal
+        // keys take the order of the LDS atomics (the order never changes a
+        // result).  The batch's data moves between lanes by permutes, not
+        // through LDS arrays: LDS bounds the waves per CU.
+        const uint32_t nbk = (mnnz + 15u) >> 4;
+        const uint32_t key = min((nbk + 1u) >> 1, 63u);
+        uint32_t *const hist = reinterpret_cast<uint32_t *>(rs_s);
+        hist[lane] = 0u;
+        wave_sync();
+        uint32_t pe = 0;
+        if (valid) pe = atomicAdd(hist + key, 1u);
+        wave_sync();
+        hist[lane] = (uint32_t)ntok - wave_add_scan_u32(hist[lane]);
+        wave_sync();
+        // valid lanes take positions 0..ntok-1, the others keep their own
+        const uint32_t rank = valid ? hist[key] + pe : (uint32_t)lane;
+        wave_sync();
+        // lane p holds the token drawn at position p (a forward permute)
+        const uint32_t pstart = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank << 2), (int)mstart);
+        const uint32_t pnnz = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank << 2), (int)mnnz);
+        const float pu = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute((int)(rank << 2), __builtin_bit_cast(int, u)));
+        uint32_t ztop = 0;  // lane p: topic and S + Q of position p
+        float zsq = 0.0f;
         for (int r = 0; r < ntok; r += 8) {
             const int p = r + rrow;
             const uint32_t start = __shfl(pstart, p, kWave);
             const uint32_t nnz = __shfl(pnnz, p, kWave);
             const float uu = __shfl(pu, p, kWave);
+            uint32_t topic = 0;
+            float SQ = 0.0f;
             if (p < ntok) {
                 // S = sum over the doc's theta entries, 16 per block; every
                 // block lies inside the doc's 16-aligned slab, entries past
@@ -900,18 +895,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // summed (no register copy between passes)
                 u32x2 ca = load_global2_at(theta, tp);
                 float R = 0.0f, r0, r1;
-                uint32_t g = 0;
-                for (; g + 2 < ng; g += 2) {
-                    const u32x2 cb = load_global2_at(theta, tp + ((g + 1) << 6));
+                uint32_t g = 0, o = tp;  // o: byte offset of group g's entries
+                for (; g + 2 < ng; g += 2, o += 128u) {
+                    const u32x2 cb = load_global2_at_imm<64>(theta, o);
                     float r2, r3;
                     group(ca, R, r0, r1);
-                    ca = load_global2_at(theta, tp + ((g + 2) << 6));
+                    ca = load_global2_at_imm<128>(theta, o);
                     group(cb, r1, r2, r3);
                     if (lr == 7) *reinterpret_cast<float4 *>(rr + (g << 1)) = make_float4(r0, r1, r2, r3);
                     R = r3;
                 }
                 if (g + 1 < ng) {
-                    const u32x2 cb = load_global2_at(theta, tp + ((g + 1) << 6));
+                    const u32x2 cb = load_global2_at_imm<64>(theta, o);
                     group(ca, R, r0, r1);
                     if (lr == 7) *reinterpret_cast<float2 *>(rr + (g << 1)) = make_float2(r0, r1);
                     R = r1;
@@ -925,14 +920,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 const float S = rr[nb - 1];  // nb >= 1: a token's doc has an entry
                 // t = fl(u * (S + Q)); the S bucket takes t < S with target
                 // t, the Q bucket target fl(t - S)
-                const float SQ = S + Q;
+                SQ = S + Q;
                 const float tu = uu * SQ;
                 const bool sb = tu < S;
                 const float t = sb ? tu : tu - S;
                 const float *rsel = sb ? rr : rq;  // inclusive prefix after each block
                 const int nbb = sb ? (int)nb : nbq;
                 // first block b with t < rsel[b]: lane m tests blocks
-                // q0 + 4m .. q0 + 4m + 3, the row takes the least hit
+                // q0 + 4m .. q0 + 4m + 3, the row takes the least hit.
+                // Prefixes read past the last block (at most three) need no
+                // test of their own: rq holds -inf there, and in the S
+                // bucket t < S = rr[nb - 1], so a hit at or before nb - 1
+                // exists and any later one loses the minimum.
                 int bsel = -1;
                 for (int q0 = 0; q0 < nbb; q0 += 32) {
                     const int b = q0 + 4 * lr;
@@ -940,10 +939,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                     if (b < nbb) {
                         // one 16-byte read, conditions without short-circuit
                         const float4 v = *reinterpret_cast<const float4 *>(rsel + b);
-                        const int nv = nbb - b;
-                        hit = ((nv > 3) & (t < v.w)) ? 4u * lr + 3u : hit;
-                        hit = ((nv > 2) & (t < v.z)) ? 4u * lr + 2u : hit;
-                        hit = ((nv > 1) & (t < v.y)) ? 4u * lr + 1u : hit;
+                        hit = (t < v.w) ? 4u * lr + 3u : hit;
+                        hit = (t < v.z) ? 4u * lr + 2u : hit;
+                        hit = (t < v.y) ? 4u * lr + 1u : hit;
                         hit = (t < v.x) ? 4u * lr : hit;
                     }
                     hit = row8_min(hit);
@@ -952,7 +950,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                         break;
                     }
                 }
-                uint32_t topic;
                 if (bsel < 0) {
                     topic = sb ? (load_global_u16_at(theta, (start + nnz - 1u) * 2u) & kThetaTopicMask) : (uint32_t)(K - 1);
                 } else {
@@ -975,12 +972,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                         vhi = alpha * f.y;
                     }
                     prefix16x2(alo, blo, vlo, vhi, lr);
-                    float Clo = vlo, Chi = vhi;
-                    if (bsel != 0) {
-                        const float rb = rsel[bsel - 1];
-                        Clo = rb + vlo;
-                        Chi = rb + vhi;
-                    }
+                    // prefix before the block (0 for the first: 0 + v = v,
+                    // the products and sums being nonnegative)
+                    const float rb = bsel ? (sb ? rr : rq)[bsel - 1] : 0.0f;
+                    const float Clo = rb + vlo, Chi = rb + vhi;
                     const uint32_t lim = sb ? nnz : 0xffffffffu;
                     uint32_t hit = ((j + 1u < lim) & (t < Chi)) ? 2u * lr + 1u : 0xffu;
                     hit = row8_min(((j < lim) & (t < Clo)) ? 2u * lr : hit);
@@ -994,20 +989,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                     const uint32_t ew = __shfl(e2, (rrow << 3) + (msel >> 1), kWave);
                     topic = sb ? (((msel & 1) ? (ew >> 16) : ew) & kThetaTopicMask) : (uint32_t)((bsel << 4) + msel);
                 }
-                if (lr == 0) {
-                    zres[p] = (uint16_t)topic;
-                    sqres[p] = SQ;
-                }
+            }
+            // position r + m's result is in the first lane of row m
+            const int src = ((lane - r) & 7) << 3;
+            const uint32_t zt = __shfl(topic, src, kWave);
+            const float zs = __shfl(SQ, src, kWave);
+            if ((unsigned)(lane - r) < 8u) {
+                ztop = zt;
+                zsq = zs;
             }
         }
-        wave_sync();
-        if (lane < ntok) {
-            store_global_u16(un.z + base + tok, (uint16_t)zres[lane]);
+        // this lane's token was drawn at position rank of the batch
+        const uint32_t mytopic = __shfl(ztop, rank, kWave);
+        const float mysq = __shfl(zsq, rank, kWave);
+        if (valid) {
+            store_global_u16(un.z + base + lane, (uint16_t)mytopic);
             // FP32 term log(fl(SQ / fl(fl(n_d) + fl(K alpha)))), summed in FP64
-            ll += (double)logf(sqres[lane] / ((float)pnd + (float)md.k_alpha));
-            gathered += pnnz;
+            ll += (double)logf(mysq / ((float)mnd + (float)md.k_alpha));
+            gathered += mnnz;
         }
-        wave_sync();
+        wave_sync();  // the next batch's histogram reuses the S-prefix rows
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) ll += __shfl_xor(ll, d, kWave);
@@ -1289,12 +1290,14 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, int32_t max_grou
 }
 
 // Row stride of the sampler's S block prefixes for rows of at most
-// max_entries theta entries (nb = ceil(max_entries / 16) blocks): rr = row
-// + 4 is written at [0, nb] and read at [0, nb + 2].  72 floats while no row
-// has more than 1024 entries.
+// max_entries theta entries (nb = ceil(max_entries / 16) blocks): a row is
+// written at [0, nb] and read, four at a time from multiples of four, below
+// 4 * ceil(nb / 4); at least 8 (the rows also hold a batch's 64-word
+// histogram).  68 floats while no row has more than 1072 entries.  Every
+// 512 bytes of LDS a wave holds costs waves per CU.
 int32_t sample_rs_stride(int64_t max_entries) {
     const int64_t nb = (std::max<int64_t>(max_entries, 1) + 15) / 16;
-    return (int32_t)(4 * ((nb + 3) / 4) + 8);
+    return (int32_t)std::max<int64_t>(8, 4 * ((nb + 4) / 4));
 }
 
 // Whether the sampler's LDS (dynamic part: eight S-prefix rows) keeps it

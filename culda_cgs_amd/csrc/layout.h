@@ -46,12 +46,12 @@ constexpr int kJointUnitsMax = 63;   // chunks the allocation-free joint LL serv
 constexpr int kMaxDevices = 64;
 constexpr uint32_t kXcds = 8;      // accelerator dies of an MI355X; workgroup i runs on die i mod 8
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
-// The sampler's LDS: phi row, Q block totals, z and S+Q of a 64-token batch
-// (static), plus eight rows of S block prefixes sized by the longest theta
-// row of any doc (dynamic).  Corpora are checked against this budget (the
-// HIP per-workgroup limit) before anything is allocated.
+// The sampler's LDS: phi row and Q block prefixes (static), plus eight
+// rows of S block prefixes sized by the longest theta row of any doc
+// (dynamic).  Corpora are checked against this budget (the HIP
+// per-workgroup limit) before anything is allocated.
 constexpr int kSamplerLdsBudget = 65536;
-constexpr int kSamplerStaticLds = kMaxK * 4 + kWave * 4 + kWave * 2 + kWave * 4;
+constexpr int kSamplerStaticLds = kMaxK * 4 + kWave * 4;
 
 struct DocMeta {
     uint32_t start;  // theta slab start (entries)

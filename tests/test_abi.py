@@ -65,7 +65,7 @@ def test_cli_missing_input_reports_and_exits(tmp_path):
 def test_check_corpus_limits_without_a_device():
     """culda_check_corpus (run by culda_upload before anything is allocated)
     refuses k outside the sampler's range and a doc whose theta row would not
-    fit the sampler's LDS rows (30,272 entries: a doc of ~1.9M tokens at
+    fit the sampler's LDS rows (30,576 entries: a doc of ~1.93M tokens at
     k = 16); an ordinary corpus passes."""
     import numpy as np
     from helpers import chunks_from_docs, product_corpus
