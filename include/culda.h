@@ -181,8 +181,8 @@ void culda_destroy(culda_ctx *ctx);
  * k a multiple of 16 in [16, 1024] (CULDA_ERR_ARG); fewer than 2^31 - 1
  * tokens (int32 counts); fewer than 2^32 tokens and 2^31 theta entries per
  * chunk; every doc's theta row (at most min(n_d, k + ceil(n_d / 63))
- * entries) within the sampler's LDS, 30,272 entries (about 1.8M tokens per
- * doc at k = 1024).  Returns CULDA_ERR_LIMIT naming the first limit
+ * entries) within the sampler's LDS, 30,576 entries (about 1.86M tokens
+ * per doc at k = 1024).  Returns CULDA_ERR_LIMIT naming the first limit
  * exceeded.  No counterpart in the reference, which sizes rows from
  * ModelTheta::InitData (model/model_theta.cpp:44-86) and fails inside a
  * kernel when they do not fit. */
