@@ -70,6 +70,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-ttl", action="store_true", help="skip the time-to-LL (plain vs learned MH) leg")
+    ap.add_argument("--mh", action="store_true",
+                    help="c5: also the bounded learned-MH leg (branched from the plain chain's state)")
+    ap.add_argument("--mh-branch-iters", type=int, default=5, help="c5 --mh: iterations of each branch")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py drives exactly one GPU (the corpus is processed chunk by chunk on one device); "
@@ -335,6 +338,7 @@ def run_large(cg, args):
     hll = fold.heldout_ll(s, args.fold_in)
     log(f"{N * args.steps / elapsed / 1e6:.1f} M tokens/s, {elapsed / args.steps * 1e3:.1f} ms/iteration, "
         f"held-out LL/token {ll0:.4f} -> {hll:.4f} after {it - 1} iterations")
+    mh = c5_mh_leg(args, s, fold, it, hll) if args.mh else None
     line = {
         "metric": "tokens/sec at K=1024 + time to fixed LL/token: plain CGS vs learned-MH",
         "value": N * args.steps / elapsed,
@@ -353,7 +357,7 @@ def run_large(cg, args):
                    "k": K, "alpha": alpha, "beta": beta, "num_tokens": N, "chunks": info.num_chunks,
                    "iterations_timed": [it - args.steps, it - 1], "ll_per_token_last": lls[-1],
                    "heldout_ll_per_token_initial": ll0, "heldout_ll_per_token_last": hll,
-                   "fold_in_sweeps": args.fold_in},
+                   "fold_in_sweeps": args.fold_in, "learned_mh": mh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "sample_kernel",
                      "avg_launch_ms": avg_launch_ms, "algorithmic_bytes_per_launch": bytes_per_launch},
@@ -362,6 +366,66 @@ def run_large(cg, args):
     fold.close()
     s.close()
     print(json.dumps(line), flush=True)
+
+
+def c5_mh_leg(args, s, fold, it, hll):
+    """Learned MH on the large corpus in bounded form, branched from the
+    plain chain's state after iteration it - 1 (a checkpoint): the plain
+    branch runs n more plain iterations; the MH branch takes two training
+    snapshots (the branch state and one plain iteration later), trains the
+    proposal MLP one epoch on them, then runs n - 1 MH iterations (2
+    proposals per token).  Held-out LL/token after every iteration of both
+    branches, each against its own wall clock (evaluations excluded)."""
+    import tempfile
+    n = args.mh_branch_iters
+    ck = Path(tempfile.mkdtemp(prefix="culda_c5_")) / "state.ck"
+    s.save_checkpoint(ck, it)
+
+    def timed(fn, *a):
+        t0 = time.perf_counter()
+        out = fn(*a)
+        s.sync()
+        return out, time.perf_counter() - t0
+
+    clock, plain = 0.0, [(0, 0.0, hll)]
+    for j in range(n):
+        _, dt = timed(s.iterate, it + j)
+        clock += dt
+        plain.append((j + 1, clock, fold.heldout_ll(s, args.fold_in)))
+    log(f"c5 plain branch: {plain}")
+    nxt = s.load_checkpoint(ck)
+    ck.unlink()
+    ck.parent.rmdir()
+    s.rebuild_all()
+    s.sync()
+    _, t_enable = timed(lambda: s.mh_enable(max_snapshots=2, init_seed=args.seed))
+    clock, mhc = 0.0, [(0, 0.0, fold.heldout_ll(s, args.fold_in))]
+    _, dt = timed(s.mh_collect)
+    clock += dt
+    _, dt = timed(s.iterate, nxt)
+    clock += dt
+    _, dt2 = timed(s.mh_collect)
+    clock += dt2
+    mhc.append((1, clock, fold.heldout_ll(s, args.fold_in)))
+    (loss, _), dt_train = timed(s.mh_train, 1)
+    clock += dt_train
+    log(f"c5 MH: 2 snapshots, trained 1 epoch in {dt_train:.1f}s, loss {loss:.4f}")
+    mh_ms, rates = [], []
+    for j in range(1, n):
+        (rate, _), dt = timed(s.mh_iterate, nxt + j, 2)
+        clock += dt
+        mh_ms.append(dt * 1e3)
+        rates.append(rate)
+        mhc.append((j + 1, clock, fold.heldout_ll(s, args.fold_in)))
+    log(f"c5 MH branch: {mhc}; {np.median(mh_ms) if mh_ms else 0:.0f} ms per MH iteration, "
+        f"acceptance {np.mean(rates) if rates else 0:.3f}")
+    return {"protocol": "both branches from the plain chain's state after iteration "
+                        f"{it - 1}: {n} plain iterations vs 1 plain iteration + 2 snapshots + 1 training "
+                        f"epoch + {n - 1} MH iterations (2 proposals per token); held-out LL/token after "
+                        f"each iteration ({args.fold_in} fold-in sweeps), clocks exclude evaluations",
+            "curve_plain": plain, "curve_mh": mhc, "mh_enable_s": t_enable, "mh_train_s": dt_train,
+            "mh_train_loss": loss, "mh_ms_per_iteration": float(np.median(mh_ms)) if mh_ms else None,
+            "mh_acceptance": float(np.mean(rates)) if rates else None}
 
 
 def mh_protocol(cg, args, K):
