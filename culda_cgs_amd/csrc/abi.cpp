@@ -852,7 +852,7 @@ extern "C" int culda_mh_tables(culda_ctx *ctx, int64_t first, int32_t n, int32_t
         a.word = s.tok_word + base;
         a.gdelta = s.gdelta[ui];
         a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, u.model, u.doc_end - u.doc_start};
-        a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V};
+        a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.rden, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K, m.V};
         a.accepted = s.accepted;
         a.w_out = dw;
         a.f_out = df;

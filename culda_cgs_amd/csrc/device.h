@@ -59,6 +59,16 @@ __device__ __forceinline__ void store_global_u16_at(void *base, uint32_t off, ui
     *(CULDA_AS1 uint16_t *)((char *)base + off) = v;
 }
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// fl(x / d) for FP32 x, d > 0 from r = RN64(1 / d): RN32(RN64(x * r)).  x / d
+// is never a midpoint between two FP32 numbers and lies at least 2^-49
+// (relative) from every midpoint (x - d m is a nonzero multiple of 2^-47
+// for x, d in [1, 2) and a 25-bit m), while RN64(x * r) is within 2^-52
+// of x / d, so both round to the same FP32 number -- the correctly rounded
+// quotient.  One FP64 multiply instead of an FP32 division sequence.
+__device__ __forceinline__ float phi_of(float x, double r) { return (float)((double)x * r); }
+
 // DPP move with zero for lanes whose source is outside the 16-lane row.
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float x) {

@@ -334,6 +334,7 @@ void Engine::upload() {
         m.nwk = (int32_t *)dalloc((size_t)m.V * K * sizeof(int32_t));
         m.den = (float *)dalloc(K * sizeof(float));
         m.zero_phi = (float *)dalloc(K * sizeof(float));
+        m.rden = (double *)dalloc(K * sizeof(double));
         models.push_back(std::move(m));
     }
     const int32_t M = (int32_t)models.size(), U = (int32_t)units.size();
@@ -361,7 +362,7 @@ void Engine::upload() {
     std::vector<ModelDev> md(M);
     for (int32_t mi = 0; mi < M; ++mi) {
         const ModelHost &m = models[mi];
-        md[mi] = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K,
+        md[mi] = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.rden, m.seed, (double)K * (double)m.alpha, m.alpha, m.beta, K,
                           m.V};
     }
     d_models = upload_vec(*this, md);

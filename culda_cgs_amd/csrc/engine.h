@@ -42,6 +42,7 @@ struct ModelHost {
     int32_t *nwk = nullptr, *nk = nullptr;
     float *den = nullptr;
     float *zero_phi = nullptr;
+    double *rden = nullptr;
     PhiList phi_fused;  // all units of the model in one pass
 };
 

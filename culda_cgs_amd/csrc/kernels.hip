@@ -484,6 +484,7 @@ __global__ __launch_bounds__(256) void den_kernel(const ModelDev *models, int32_
             const float d = (float)*glb(m.nk + i) + bv;
             *glb(m.den + i) = d;
             *glb(m.zero_phi + i) = m.beta / d;  // = fl(fl(0 + beta) / d)
+            *glb(m.rden + i) = 1.0 / (double)d;
         }
     }
 }
@@ -714,7 +715,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
 #define COUNT_LO(d) ((float)(((d) >> kThetaCountShift) & kThetaMaxCount))
 #define COUNT_HI(d) ((float)((d) >> (16 + kThetaCountShift)))
 
-    // stage phi_k = fl(fl(n_wk) + beta) / den_k
+    // stage phi_k = fl(fl(fl(n_wk) + beta) / den_k), formed as the FP32
+    // rounding of an FP64 product with 1 / den_k (see phi_of)
     const int32_t *row = md.nwk + (int64_t)s.word * K;
     if (a.use_zp && s.rare) {
         // A rare word's counts are those of its tokens' zp topics, and
@@ -729,7 +731,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         uint32_t c = 0;
         for (int j = 0; j < n; ++j) c += (uint32_t)__builtin_amdgcn_readlane((int)t, j) == t;
         wave_sync();
-        if (lane < n) ph[t] = ((float)c + beta) / __uint_as_float(load_global((const uint32_t *)md.den + t));
+        if (lane < n) ph[t] = phi_of((float)c + beta, *glb(md.rden + t));
     } else {
         // the loads of kBatch 256-topic steps of the row and of den issued
         // before their first use (all of them: one memory round trip per
@@ -738,23 +740,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
 #pragma unroll
         for (int i0 = 0; i0 < kStageSteps; i0 += kBatch) {
             if (lane * 4 + i0 * 4 * kWave >= K) break;
-            u32x4 cr[kBatch], dr[kBatch];
+            u32x4 cr[kBatch];
+            f64x2 ra[kBatch], rb[kBatch];
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
                 const int k = lane * 4 + (i0 + i) * 4 * kWave;
                 if (k < K) {
                     cr[i] = load_global4(row + k);
-                    dr[i] = load_global4(md.den + k);
+                    ra[i] = *glb(reinterpret_cast<const f64x2 *>(md.rden + k));
+                    rb[i] = *glb(reinterpret_cast<const f64x2 *>(md.rden + k + 2));
                 }
             }
 #pragma unroll
             for (int i = 0; i < kBatch; ++i) {
                 const int k = lane * 4 + (i0 + i) * 4 * kWave;
                 if (k < K) {
-                    ph[k + 0] = ((float)(int32_t)cr[i].x + beta) / __uint_as_float(dr[i].x);
-                    ph[k + 1] = ((float)(int32_t)cr[i].y + beta) / __uint_as_float(dr[i].y);
-                    ph[k + 2] = ((float)(int32_t)cr[i].z + beta) / __uint_as_float(dr[i].z);
-                    ph[k + 3] = ((float)(int32_t)cr[i].w + beta) / __uint_as_float(dr[i].w);
+                    ph[k + 0] = phi_of((float)(int32_t)cr[i].x + beta, ra[i].x);
+                    ph[k + 1] = phi_of((float)(int32_t)cr[i].y + beta, ra[i].y);
+                    ph[k + 2] = phi_of((float)(int32_t)cr[i].z + beta, rb[i].x);
+                    ph[k + 3] = phi_of((float)(int32_t)cr[i].w + beta, rb[i].y);
                 }
             }
         }

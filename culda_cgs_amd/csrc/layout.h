@@ -65,6 +65,7 @@ struct ModelDev {
     int32_t *nk;
     float *den;
     float *zero_phi;  // phi_k of a zero count, fl(beta / den_k)
+    double *rden;     // 1 / den_k in FP64: fl(fl(x * rden_k)) = fl(x / den_k) for every FP32 x
     uint64_t seed;
     double k_alpha;  // (double)K * (double)alpha
     float alpha;

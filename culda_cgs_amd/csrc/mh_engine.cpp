@@ -428,7 +428,7 @@ double Engine::mh_iterate(uint32_t sweep, int32_t proposals) {
             a.gdelta = s.gdelta[ui - m.unit_begin];
             a.unit = UnitDev{u.tokens, u.z, u.zp, u.word_idx, u.rev_off, u.rev_idx, u.meta, u.theta, u.model,
                              u.doc_end - u.doc_start};
-            a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K,
+            a.model = ModelDev{m.nwk, m.nk, m.den, m.zero_phi, m.rden, m.seed, (double)m.K * (double)m.alpha, m.alpha, m.beta, m.K,
                                m.V};
             a.accepted = s.accepted;
             a.start = start;
