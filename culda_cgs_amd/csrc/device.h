@@ -160,6 +160,43 @@ __device__ __forceinline__ void prefix16x2(float a, float b, float &lo, float &h
     hi = P;
 }
 
+// ---- 4-lane rows (four per 16-lane DPP row); q = lane & 3.
+
+// Minimum over each 4-lane row, in every lane of it.
+__device__ __forceinline__ uint32_t quad_min(uint32_t x) {
+    x = min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xf, 0xf, false));
+    return min(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xf, 0xf, false));
+}
+
+// Value of the lane S places below in the same 4-lane row, 0 for q < S
+// (sources that would cross into the next row zeroed first).
+template <int S>
+__device__ __forceinline__ float quad_shr(float x, int q) {
+    return dpp_f<0x110 + S>(q >= 4 - S ? 0.0f : x);
+}
+
+// The in-block prefixes of prefix16x2 with the 16 products held four per
+// lane by a 4-lane row (lane q: products 4q and 4q + 2 as factors a * b,
+// 4q + 1 and 4q + 3 as rounded values v): pair sums fma(a, b, v) (pairs 2q
+// and 2q + 1 in lane q), their inclusive Kogge-Stone scan with offsets 1, 2,
+// 4 (one pair within the lane and the last pair of lane q - 1; lane q - 1;
+// lane q - 2), C_odd = P, C_even = fma(a, b, P of the pair before) -- the
+// same operations on the same operands as the 8-lane form.
+__device__ __forceinline__ void prefix16x4(float a0, float b0, float v1, float a2, float b2, float v3, float &c0,
+                                           float &c1, float &c2, float &c3, int q) {
+    const float q0 = __builtin_fmaf(a0, b0, v1), q1 = __builtin_fmaf(a2, b2, v3);
+    float P0 = q0 + quad_shr<1>(q1, q);
+    float P1 = q1 + q0;
+    P0 = P0 + quad_shr<1>(P0, q);
+    P1 = P1 + quad_shr<1>(P1, q);
+    P0 = P0 + quad_shr<2>(P0, q);
+    P1 = P1 + quad_shr<2>(P1, q);
+    c0 = __builtin_fmaf(a0, b0, quad_shr<1>(P1, q));
+    c1 = P0;
+    c2 = __builtin_fmaf(a2, b2, P0);
+    c3 = P1;
+}
+
 }  // namespace
 
 namespace {

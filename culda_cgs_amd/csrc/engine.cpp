@@ -149,7 +149,7 @@ void check_corpus_limits(const Corpus &c, int32_t K) {
         }
     }
     // eight S-prefix rows of the longest theta row in the sampler's LDS
-    const int64_t lds = (int64_t)kSamplerStaticLds + 8 * 4 * (int64_t)sample_rs_stride(longest);
+    const int64_t lds = (int64_t)kSamplerStaticLds + (int64_t)sample_dyn_lds(sample_rs_stride(longest));
     if (lds > kSamplerLdsBudget)
         fail(CULDA_ERR_LIMIT,
              "doc %d (%lld tokens) may need %lld theta entries at k = %d; the sampler's LDS holds rows of at most "

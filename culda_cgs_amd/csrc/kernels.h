@@ -37,6 +37,7 @@ hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStr
 int32_t sample_blocks_per_cu(int32_t rs_stride);
 int32_t phi_blocks_per_cu();  // resident phi-rebuild workgroups per CU (0 if unknown)
 int32_t sample_rs_stride(int64_t max_entries);
+size_t sample_dyn_lds(int32_t rs_stride);
 
 // mh.hip
 hipError_t launch_heldout(const HeldoutArgs &a, hipStream_t s);

@@ -689,6 +689,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
     // rr[b] = inclusive S prefix after block b of this row's doc (16-byte
     // aligned: two groups' four prefixes are one store)
     float *const rr = rs_s + rrow * a.rs_stride;
+    // sixteen 4-lane rows of a.rs_stride4 floats over the same LDS, for docs
+    // of at most qmax blocks
+    const int qrow = lane >> 2, ql = lane & 3;
+    float *const rq4 = rs_s + qrow * a.rs_stride4;
+    const uint32_t qmax = (uint32_t)min(a.rs_stride4 - 1, 62);
     int pos = blockIdx.x;
     uint32_t next = 0, claim_end = 0;  // persistent: claimed positions [next, claim_end)
     while (true) {
@@ -830,16 +835,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             mnd = m.z;
         }
         // Processing order: a counting sort of the valid tokens (lanes
-        // 0..ntok-1) by the number of 32-entry S groups of their doc,
-        // descending, so that the eight rows of a step run S loops of
-        // similar length.  The histogram sits in the first 64 words of the
+        // 0..ntok-1) by the number of 16-entry S blocks of their doc
+        // (capped at 63), descending, so that the rows of a step run S loops
+        // of similar length.  The histogram sits in the first 64 words of the
         // S-prefix rows (free between batches): lane l's count of tokens
         // with key l, then the number of tokens with a larger key.  Equal
         // keys take the order of the LDS atomics (the order never changes a
         // result).  The batch's data moves between lanes by permutes, not
         // through LDS arrays: LDS bounds the waves per CU.
-        const uint32_t nbk = (mnnz + 15u) >> 4;
-        const uint32_t key = min((nbk + 1u) >> 1, 63u);
+        const uint32_t key = min((mnnz + 15u) >> 4, 63u);
         uint32_t *const hist = reinterpret_cast<uint32_t *>(rs_s);
         hist[lane] = 0u;
         wave_sync();
@@ -857,7 +861,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         const float pu = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute((int)(rank << 2), __builtin_bit_cast(int, u)));
         uint32_t ztop = 0;  // lane p: topic and S + Q of position p
         float zsq = 0.0f;
-        for (int r = 0; r < ntok; r += 8) {
+        // Steps of eight 8-lane rows while position r's doc has more blocks
+        // than a 4-lane row's prefix row holds (the longest docs come
+        // first), then steps of sixteen 4-lane rows, which share the
+        // per-step work among twice the tokens.
+        int r = 0;
+        for (; r < ntok; r += 8) {
+            if ((((uint32_t)__builtin_amdgcn_readlane((int)pnnz, r) + 15u) >> 4) <= qmax) break;
             const int p = r + rrow;
             const uint32_t start = __shfl(pstart, p, kWave);
             const uint32_t nnz = __shfl(pnnz, p, kWave);
@@ -999,6 +1009,144 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             const uint32_t zt = __shfl(topic, src, kWave);
             const float zs = __shfl(SQ, src, kWave);
             if ((unsigned)(lane - r) < 8u) {
+                ztop = zt;
+                zsq = zs;
+            }
+        }
+        for (; r < ntok; r += 16) {
+            const int p = r + qrow;  // >= 64 only past ntok
+            const uint32_t start = __shfl(pstart, p, kWave);
+            const uint32_t nnz = __shfl(pnnz, p, kWave);
+            const float uu = __shfl(pu, p, kWave);
+            uint32_t topic = 0;
+            float SQ = 0.0f;
+            if (p < ntok) {
+                // S over the doc's blocks: lane q of the 4-lane row holds
+                // entries 4q .. 4q + 3 of block b (one 8-byte load), forms
+                // the first two tree levels, and DPP row_shr 1 and 2 finish
+                // the block total at lane 3 -- the same balanced tree as
+                // the 8-lane rows -- which carries R_{b+1} = fl(R_b + T_b).
+                // Blocks are loaded one ahead; none past nb.
+                const uint32_t nb = (nnz + 15u) >> 4;
+                const uint32_t tp = (start + 4u * (uint32_t)ql) * 2u;
+                auto block = [&](u32x2 d, float R0) {
+                    const float f0 = PHI_LO(d.x), f1 = PHI_HI(d.x), f2 = PHI_LO(d.y), f3 = PHI_HI(d.y);
+                    const float c0 = COUNT_LO(d.x), c1 = COUNT_HI(d.x), c2 = COUNT_LO(d.y), c3 = COUNT_HI(d.y);
+                    float y = __builtin_fmaf(c0, f0, c1 * f1) + __builtin_fmaf(c2, f2, c3 * f3);
+                    y = y + dpp_f<0x111>(y);
+                    y = y + dpp_f<0x112>(y);
+                    return R0 + y;
+                };
+                u32x2 ca = load_global2_at(theta, tp);
+                float R = 0.0f;
+                uint32_t b = 0, o = tp;  // o: byte offset of block b's entries
+                for (; b + 2 < nb; b += 2, o += 64u) {
+                    const u32x2 cb = load_global2_at_imm<32>(theta, o);
+                    const float r0 = block(ca, R);
+                    ca = load_global2_at_imm<64>(theta, o);
+                    const float r1 = block(cb, r0);
+                    if (ql == 3) *reinterpret_cast<float2 *>(rq4 + b) = make_float2(r0, r1);
+                    R = r1;
+                }
+                if (b + 1 < nb) {
+                    const u32x2 cb = load_global2_at_imm<32>(theta, o);
+                    const float r0 = block(ca, R);
+                    const float r1 = block(cb, r0);
+                    if (ql == 3) *reinterpret_cast<float2 *>(rq4 + b) = make_float2(r0, r1);
+                } else {
+                    const float r0 = block(ca, R);
+                    if (ql == 3) rq4[b] = r0;
+                }
+                wave_sync();
+                const float S = rq4[nb - 1];
+                SQ = S + Q;
+                const float tu = uu * SQ;
+                const bool sb = tu < S;
+                const float t = sb ? tu : tu - S;
+                const float *rsel = sb ? rq4 : rq;
+                const int nbb = sb ? (int)nb : nbq;
+                // first block b with t < rsel[b]: lane q tests blocks q0 + 8q
+                // .. q0 + 8q + 7 (two 16-byte reads), the row takes the least
+                // hit (reads past the last block: as for the 8-lane rows)
+                int bsel = -1;
+                for (int q0 = 0; q0 < nbb; q0 += 32) {
+                    const int bb = q0 + 8 * ql;
+                    uint32_t hit = 0xffu;
+                    if (bb < nbb) {
+                        if (bb + 4 < nbb) {
+                            const float4 w = *reinterpret_cast<const float4 *>(rsel + bb + 4);
+                            hit = (t < w.w) ? 8u * ql + 7u : hit;
+                            hit = (t < w.z) ? 8u * ql + 6u : hit;
+                            hit = (t < w.y) ? 8u * ql + 5u : hit;
+                            hit = (t < w.x) ? 8u * ql + 4u : hit;
+                        }
+                        const float4 v = *reinterpret_cast<const float4 *>(rsel + bb);
+                        hit = (t < v.w) ? 8u * ql + 3u : hit;
+                        hit = (t < v.z) ? 8u * ql + 2u : hit;
+                        hit = (t < v.y) ? 8u * ql + 1u : hit;
+                        hit = (t < v.x) ? 8u * ql : hit;
+                    }
+                    hit = quad_min(hit);
+                    if (hit != 0xffu) {
+                        bsel = q0 + (int)hit;
+                        break;
+                    }
+                }
+                if (bsel < 0) {
+                    topic = sb ? (load_global_u16_at(theta, (start + nnz - 1u) * 2u) & kThetaTopicMask) : (uint32_t)(K - 1);
+                } else {
+                    // the block's 16 products, four per lane (entries j .. j + 3)
+                    const uint32_t j = ((uint32_t)bsel << 4) + 4u * (uint32_t)ql;
+                    u32x2 e = u32x2{0u, 0u};
+                    float a0, b0, v1, a2, b2, v3;
+                    if (sb) {
+                        e = load_global2_at(theta, (start + j) * 2u);
+                        a0 = COUNT_LO(e.x);
+                        b0 = PHI_LO(e.x);
+                        v1 = COUNT_HI(e.x) * PHI_HI(e.x);
+                        a2 = COUNT_LO(e.y);
+                        b2 = PHI_LO(e.y);
+                        v3 = COUNT_HI(e.y) * PHI_HI(e.y);
+                    } else {
+                        const float4 f = *reinterpret_cast<const float4 *>(ph + j);
+                        a0 = alpha;
+                        b0 = f.x;
+                        v1 = alpha * f.y;
+                        a2 = alpha;
+                        b2 = f.z;
+                        v3 = alpha * f.w;
+                    }
+                    float C0, C1, C2, C3;
+                    prefix16x4(a0, b0, v1, a2, b2, v3, C0, C1, C2, C3, ql);
+                    const float rb = bsel ? (sb ? rq4 : rq)[bsel - 1] : 0.0f;
+                    C0 = rb + C0;
+                    C1 = rb + C1;
+                    C2 = rb + C2;
+                    C3 = rb + C3;
+                    const uint32_t lim = sb ? nnz : 0xffffffffu;
+                    uint32_t hit = ((j + 3u < lim) & (t < C3)) ? 4u * ql + 3u : 0xffu;
+                    hit = ((j + 2u < lim) & (t < C2)) ? 4u * ql + 2u : hit;
+                    hit = ((j + 1u < lim) & (t < C1)) ? 4u * ql + 1u : hit;
+                    hit = quad_min(((j < lim) & (t < C0)) ? 4u * ql : hit);
+                    int msel;
+                    if (hit != 0xffu) {
+                        msel = (int)hit;
+                    } else {
+                        const int nvalid = (int)nnz - (bsel << 4);
+                        msel = sb ? (nvalid < 16 ? nvalid : 16) - 1 : 15;
+                    }
+                    // entry msel: lane msel >> 2 of the row, dword (msel >> 1) & 1
+                    const int srcl = (qrow << 2) + (msel >> 2);
+                    const uint32_t ex = __shfl(e.x, srcl, kWave), ey = __shfl(e.y, srcl, kWave);
+                    const uint32_t ew = (msel & 2) ? ey : ex;
+                    topic = sb ? (((msel & 1) ? (ew >> 16) : ew) & kThetaTopicMask) : (uint32_t)((bsel << 4) + msel);
+                }
+            }
+            // position r + m's result is in the first lane of row m
+            const int src = ((lane - r) & 15) << 2;
+            const uint32_t zt = __shfl(topic, src, kWave);
+            const float zs = __shfl(SQ, src, kWave);
+            if ((unsigned)(lane - r) < 16u) {
                 ztop = zt;
                 zsq = zs;
             }
@@ -1304,7 +1452,19 @@ int32_t sample_rs_stride(int64_t max_entries) {
     return (int32_t)std::max<int64_t>(8, 4 * ((nb + 4) / 4));
 }
 
-// Whether the sampler's LDS (dynamic part: eight S-prefix rows) keeps it
+// Row stride of the sixteen 4-lane rows: the 8-lane stride while sixteen
+// such rows take at most 1.5 KB, else half of it (the same LDS as the
+// eight 8-lane rows).
+int32_t sample_rs_stride4(int32_t rs_stride) {
+    return rs_stride * 16 * 4 <= 1536 ? rs_stride : std::max(8, rs_stride / 8 * 4);
+}
+
+// Dynamic LDS of the sampler: the larger of the two row layouts.
+size_t sample_dyn_lds(int32_t rs_stride) {
+    return (size_t)std::max(8 * rs_stride, 16 * sample_rs_stride4(rs_stride)) * sizeof(float);
+}
+
+// Whether the sampler's LDS (dynamic part: the S-prefix rows) keeps it
 // below seven waves per SIMD, in which case the six-wave build, with more
 // registers, runs instead.
 static bool sample_lds_bound(int32_t rs_stride) {
@@ -1313,7 +1473,7 @@ static bool sample_lds_bound(int32_t rs_stride) {
     if (rs_stride != last_stride) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 7>, kWave,
-                                                         (size_t)8 * rs_stride * sizeof(float)) != hipSuccess)
+                                                         sample_dyn_lds(rs_stride)) != hipSuccess)
             n = 0;
         last = n > 0 && n < 7 * 4;
         last_stride = rs_stride;
@@ -1321,10 +1481,12 @@ static bool sample_lds_bound(int32_t rs_stride) {
     return last;
 }
 
-hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
-    if (a.num_slots <= 0) return hipSuccess;
-    if (a.rs_stride < 8 || a.rs_stride % 4) return hipErrorInvalidValue;
-    const size_t lds = (size_t)8 * a.rs_stride * sizeof(float);
+hipError_t launch_sample(const SampleArgs &a_in, int32_t grid, hipStream_t s) {
+    if (a_in.num_slots <= 0) return hipSuccess;
+    if (a_in.rs_stride < 8 || a_in.rs_stride % 4) return hipErrorInvalidValue;
+    SampleArgs a = a_in;
+    a.rs_stride4 = sample_rs_stride4(a.rs_stride);
+    const size_t lds = sample_dyn_lds(a.rs_stride);
     const bool six = sample_lds_bound(a.rs_stride);
     if (a.queue) {
         if (six)
@@ -1342,7 +1504,7 @@ hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s) {
 
 int32_t sample_blocks_per_cu(int32_t rs_stride) {
     int n = 0;
-    const size_t lds = (size_t)8 * rs_stride * sizeof(float);
+    const size_t lds = sample_dyn_lds(rs_stride);
     const hipError_t e = sample_lds_bound(rs_stride)
                              ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 6>, kWave, lds)
                              : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sample_kernel<true, 7>, kWave, lds);

@@ -135,7 +135,7 @@ struct SampleArgs {
     int32_t num_slots;
     int32_t use_zp;         // counts equal counts(zp): rare slots stage phi from zp
     int32_t rs_stride;      // floats per 8-lane row of S block prefixes (dynamic LDS), sample_rs_stride()
-    int32_t pad;
+    int32_t rs_stride4;     // floats per 4-lane row (set by launch_sample)
 };
 
 struct InitArgs {
