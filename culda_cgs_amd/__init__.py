@@ -162,10 +162,12 @@ def lib():
     """Load libculda.so (built by __graft_entry__.build()); raise if absent."""
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c "
+        # CULDA_LIBRARY: another build of the same library (the sanitized host build)
+        path = Path(os.environ.get("CULDA_LIBRARY", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it first (python -c "
                                "'import __graft_entry__ as g; g.build()')")
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(str(path))
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
             fn.restype = res

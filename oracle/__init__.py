@@ -11,6 +11,7 @@ file:line it follows).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -38,9 +39,11 @@ THETA_MAX_COUNT = 63  # largest count of one theta entry
 def lib():
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-        L = C.CDLL(str(LIB_PATH))
+        # CULDA_ORACLE_LIBRARY: another build of the oracle (the sanitized one)
+        path = Path(os.environ.get("CULDA_ORACLE_LIBRARY", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(str(path))
         P = C.POINTER
         L.oc_philox4x64_10.argtypes = [P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]
         L.oc_philox4x64_10.restype = None
