@@ -595,18 +595,34 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     const DocMeta dm = a.unit.meta[d];
     uint32_t tr[16];
     const uint32_t *trow = reinterpret_cast<const uint32_t *>(a.unit.theta + dm.start);
+    // dwords of the row per lane actually holding entries (128 entries per
+    // dword index t); the rest stay 0 and are skipped below
+    const int tmax = (int)((dm.nnz + 127u) >> 7);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
         const uint32_t j = lane + kWave * t;  // entries 2j, 2j + 1 (2j + 1 past nnz is a zero of the slab tail)
-        tr[t] = 2 * j < dm.nnz ? trow[j] : 0u;
+        tr[t] = (t < tmax && 2 * j < dm.nnz) ? trow[j] : 0u;
     }
     const uint16_t *lg = a.logits + (int64_t)r * K + lane * KL;
     float e[16];
     float mx = -INFINITY;
+    if (KL == 16) {
+        // the lane's 16 logits (32 bytes, aligned): two 16-byte loads
+        const u32x4 l0 = load_global4(lg), l1 = load_global4(lg + 8);
+        const uint32_t lw[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        e[j] = j < KL ? __uint_as_float(load_global_u16(lg + j) << 16) : -INFINITY;
-        mx = fmaxf(mx, e[j]);
+        for (int j = 0; j < 8; ++j) {
+            e[2 * j] = __uint_as_float(lw[j] << 16);
+            e[2 * j + 1] = __uint_as_float(lw[j] & 0xffff0000u);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) mx = fmaxf(mx, e[j]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            e[j] = j < KL ? __uint_as_float(load_global_u16(lg + j) << 16) : -INFINITY;
+            mx = fmaxf(mx, e[j]);
+        }
     }
     mx = wave_max_all_f(mx);
     float s = 0.f;
@@ -649,12 +665,14 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
         if (j < KL && (uint32_t)(lane * KL + j) == kmax) W[j] = (uint32_t)((int32_t)W[j] + rem);
     // lane-local light / heavy counts and sums, then wave prefixes
     uint32_t cL = 0, sDef = 0, cH = 0, sExc = 0;
+    uint32_t lmask = 0;  // bit j: element j of this lane is a light
 #pragma unroll
     for (int j = 0; j < 16; ++j)
         if (j < KL) {
             if (W[j] < M) {
                 ++cL;
                 sDef += M - W[j];
+                lmask |= 1u << j;
             } else {
                 ++cH;
                 sExc += W[j] - M;
@@ -717,17 +735,27 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
     uint32_t cand[kMaxProposals + 1];
     double ud[kMaxProposals];
     cand[0] = z0;
+    // every proposal's Philox block in one evaluation: lane p computes
+    // proposal p's (the counters differ only in p)
+    const u64x4 xl = philox4x64_10(gid, a.sweep, kPurposeMH, (uint64_t)(lane < np ? lane : 0), a.model.seed, 0);
 #pragma unroll
     for (int p = 0; p < kMaxProposals; ++p) {
         if (p < np) {
-            const u64x4 x = philox4x64_10(gid, a.sweep, kPurposeMH, (uint64_t)p, a.model.seed, 0);
-            const uint32_t j = (uint32_t)mulhi64(x.v[0], (uint64_t)K);
-            const uint32_t rr = (uint32_t)(x.v[1] >> (64 - kAliasBits));
-            ud[p] = (double)(x.v[2] >> 11) * 0x1.0p-53;
-            // the rank of bucket j, from the lane that holds it
-            const uint32_t rk = (uint32_t)__shfl((int)rank_of(j % (uint32_t)KL), (int)(j / (uint32_t)KL), kWave);
+            const uint64_t x0 = (uint64_t)readlane64((int64_t)xl.v[0], p), x1 = (uint64_t)readlane64((int64_t)xl.v[1], p);
+            const uint64_t x2 = (uint64_t)readlane64((int64_t)xl.v[2], p);
+            const uint32_t j = (uint32_t)mulhi64(x0, (uint64_t)K);
+            const uint32_t rr = (uint32_t)(x1 >> (64 - kAliasBits));
+            ud[p] = (double)(x2 >> 11) * 0x1.0p-53;
+            // the rank of bucket j among the lights or the heavies, from the
+            // lane that holds it: its first rank plus the lights (heavies)
+            // before element jj of that lane
+            const uint32_t wj = sW[j], jj = j % (uint32_t)KL;
+            const uint32_t below = (1u << jj) - 1u;
+            const uint32_t own = wj < M ? aL0 + (uint32_t)__builtin_popcount(lmask & below)
+                                        : bH0 + (uint32_t)__builtin_popcount(~lmask & below);
+            const uint32_t rk = (uint32_t)__shfl((int)own, (int)(j / (uint32_t)KL), kWave);
             uint32_t f, al;
-            alias_entry(j, sW[j], rk, sDE, sHI, nL, nH, f, al);
+            alias_entry(j, wj, rk, sDE, sHI, nL, nH, f, al);
             cand[p + 1] = rr < f ? j : al;
         } else {
             ud[p] = 0.0;
@@ -746,6 +774,7 @@ __global__ __launch_bounds__(64) void mh_kernel(MHArgs a) {
         uint32_t cnt = 0;
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
+            if (t >= tmax) break;  // wave-uniform
             if ((tr[t] & kThetaTopicMask) == k) cnt += (tr[t] >> kThetaCountShift) & kThetaMaxCount;
             if (((tr[t] >> 16) & kThetaTopicMask) == k) cnt += tr[t] >> (16 + kThetaCountShift);
         }
