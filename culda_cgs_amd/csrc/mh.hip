@@ -70,13 +70,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // tanh-approximated GELU and its derivative, through the logistic
 // function: 0.5 (1 + tanh(u)) = s = 1 / (1 + e^(-2u)), u = c (x + a x^3),
 // so gelu = x s and gelu' = s + 2 x s (1 - s) c (1 + 3 a x^2) -- one
-// hardware exp and one reciprocal instead of a tanh.
+// hardware exp and one hardware reciprocal (v_rcp_f32, 1 ulp; 1 / inf = 0)
+// instead of a tanh or a division.
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kGeluA = 0.044715f;
 
 __device__ __forceinline__ float gelu_logistic(float x) {
     const float u = kGeluC * (x + kGeluA * x * x * x);
-    return __frcp_rn(1.0f + __expf(-2.0f * u));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
 __device__ __forceinline__ float gelu(float x) { return x * gelu_logistic(x); }
@@ -203,6 +204,33 @@ __device__ __forceinline__ uint2 f2bf4(float a, float b, float c, float d) {
     return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
 }
 
+// Epilogue of four consecutive columns col .. col + 3 of C row `row` (the
+// four values of one MFMA lane), for the 128- and 256-row tile kernels.
+__device__ __forceinline__ void gemm_epilogue4(const GemmArgs &g, int64_t row, int64_t col, float4 bias, float x0,
+                                               float x1, float x2, float x3) {
+    if (g.epilogue == kEpiGeluGrad) {
+        const float4 p = *reinterpret_cast<const float4 *>(g.aux + row * g.ldaux + col);
+        *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) =
+            f2bf4(x0 * gelu_grad(p.x), x1 * gelu_grad(p.y), x2 * gelu_grad(p.z), x3 * gelu_grad(p.w));
+        return;
+    }
+    x0 += bias.x;
+    x1 += bias.y;
+    x2 += bias.z;
+    x3 += bias.w;
+    if (g.c)
+        *reinterpret_cast<float4 *>(g.c + blockIdx.z * g.split_stride + row * g.ldc + col) = make_float4(x0, x1, x2, x3);
+    if (g.c16) {
+        if (g.epilogue == kEpiBiasGelu) {
+            x0 = gelu(x0);
+            x1 = gelu(x1);
+            x2 = gelu(x2);
+            x3 = gelu(x3);
+        }
+        *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) = f2bf4(x0, x1, x2, x3);
+    }
+}
+
 // 128 x 128 C tile per workgroup of four waves (2 x 2, each 64 x 64 =
 // 4 x 4 MFMA 16x16x32 tiles) for A stored [m][k] and B stored [n][k]
 // (both k-contiguous).  32-deep k stages, double-buffered in LDS: the next
@@ -287,30 +315,152 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int64_t row = m0 + wm * 64 + i * 16 + (lane & 15);
-            float x0 = acc[i][j][0], x1 = acc[i][j][1], x2 = acc[i][j][2], x3 = acc[i][j][3];
-            if (g.epilogue == kEpiGeluGrad) {
-                const float4 p = *reinterpret_cast<const float4 *>(g.aux + row * g.ldaux + col);
-                *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) =
-                    f2bf4(x0 * gelu_grad(p.x), x1 * gelu_grad(p.y), x2 * gelu_grad(p.z), x3 * gelu_grad(p.w));
-            } else {
-                x0 += bias.x;
-                x1 += bias.y;
-                x2 += bias.z;
-                x3 += bias.w;
-                if (g.c)
-                    *reinterpret_cast<float4 *>(g.c + blockIdx.z * g.split_stride + row * g.ldc + col) =
-                        make_float4(x0, x1, x2, x3);
-                if (g.c16) {
-                    if (g.epilogue == kEpiBiasGelu) {
-                        x0 = gelu(x0);
-                        x1 = gelu(x1);
-                        x2 = gelu(x2);
-                        x3 = gelu(x3);
-                    }
-                    *reinterpret_cast<uint2 *>(g.c16 + row * g.ldc16 + col) = f2bf4(x0, x1, x2, x3);
-                }
+            gemm_epilogue4(g, row, col, bias, acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
+        }
+    }
+}
+
+// Large products (the proposal queries: tens of thousands of rows): C tile
+// 128 x 256 per workgroup of eight waves (2 x 4, each 64 x 64 = 4 x 4 MFMA
+// 16x16x32 tiles), two workgroups per CU.  The operands stream through a
+// ring of three LDS slots of 32-deep k slices filled by direct
+// global-to-LDS loads (global_load_lds_dwordx4: each wave instruction writes
+// 16 tile rows of 64 bytes, lane i the 16 bytes at i * 16), two slices in
+// flight while the third is multiplied; one barrier per slice both publishes
+// the slice and frees the slot read before it.  Tile rows (128 A rows, then
+// 256 B rows) are 64 bytes, their four 16-byte k chunks XOR-swizzled by the
+// row (ring_chunk) so that the fragment reads of 16 consecutive rows hit
+// distinct LDS banks.  When BF16 is the only output, the epilogue stages the
+// C tile through the idle ring (16-byte chunks swizzled by the row) and
+// writes whole 512-byte rows; otherwise it is the 128-tile kernel's.
+// Operand and fragment layouts and the XCD-aware tile order are the 128-tile
+// kernel's.
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+typedef __attribute__((address_space(1))) void *global_void_ptr;
+constexpr int kRingTM = 128, kRingTN = 256, kRingWaves = 8, kRingSlots = 3;
+
+// physical 16-byte chunk of logical k chunk c (0..3) in tile row r
+__device__ __forceinline__ int ring_chunk(int r, int c) { return c ^ ((r >> 1) & 3); }
+
+// s_waitcnt immediate: wait until at most n vector memory operations of this
+// wave are outstanding (other counters unconstrained)
+#define CULDA_VMCNT(n) (0x0F70 | ((n)&15) | (((n) >> 4) << 14))
+
+__global__ __launch_bounds__(64 * kRingWaves) __attribute__((amdgpu_waves_per_eu(4))) void gemm_bf16_ring_kernel(
+    GemmArgs g, int staged) {
+    constexpr int TM = kRingTM, TN = kRingTN, WN = 4, NW = kRingWaves, NB = kRingSlots, KS = kGemmK;
+    constexpr int FM = TM / 2 / 16, FN = TN / WN / 16;  // 4 x 4 MFMA tiles per wave
+    constexpr int ROWS = TM + TN, PER = ROWS / 16 / NW;  // 1 KB loads per wave per slice
+    static_assert(KS == 32 && ROWS % (16 * NW) == 0 && TM * TN <= NB * ROWS * KS, "ring tile");
+    __shared__ __attribute__((aligned(1024))) uint16_t ring[NB][ROWS * KS];
+    const int t = threadIdx.x, lane = t & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wm = wave / WN, wn = wave % WN;
+    int64_t tm = blockIdx.y, tn = blockIdx.x;
+    {
+        const uint32_t nt = gridDim.x, tiles = gridDim.x * gridDim.y;
+        if (tiles % kXcds == 0) {
+            const uint32_t id = blockIdx.y * nt + blockIdx.x;
+            const uint32_t tile = (id % kXcds) * (tiles / kXcds) + id / kXcds;
+            tm = tile / nt;
+            tn = tile % nt;
+        }
+    }
+    const int64_t m0 = tm * TM, n0 = tn * TN;
+    const int64_t kz = (int64_t)blockIdx.z * g.k;
+    // this lane's source of each of its wave's PER loads of a slice: load q
+    // covers tile rows 16 j .. 16 j + 15, j = q * NW + wave
+    const uint16_t *src[PER];
+    {
+        const int r16 = lane >> 2, c = ring_chunk(r16, lane & 3);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int row = (q * NW + wave) * 16 + r16;
+            src[q] = row < TM ? g.a + (m0 + row) * g.lda + kz + 8 * c : g.b + (n0 + row - TM) * g.ldb + kz + 8 * c;
+        }
+    }
+    auto issue = [&](int kt, int slot) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            __builtin_amdgcn_global_load_lds((global_void_ptr)(src[q] + (int64_t)kt * KS),
+                                             (lds_void_ptr)(&ring[slot][(q * NW + wave) * 16 * KS]), 16, 0, 0);
+    };
+    // this lane's fragment: tile row (lane & 15) of a 16-row group, k chunk lane >> 4
+    const int fo = (lane & 15) * KS + 8 * ring_chunk(lane & 15, lane >> 4);
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nk = g.k / KS;
+#pragma unroll
+    for (int st = 0; st < NB - 1; ++st)
+        if (st < nk) issue(st, st);
+    for (int kt = 0; kt < nk; ++kt) {
+        // slice kt landed for this wave (the next may stay in flight) ...
+        if (kt + NB - 2 < nk)
+            __builtin_amdgcn_s_waitcnt(CULDA_VMCNT(PER * (NB - 2)));
+        else
+            __builtin_amdgcn_s_waitcnt(CULDA_VMCNT(0));
+        // ... and for every wave, which have also finished reading slice kt - 1
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        if (kt + NB - 1 < nk) issue(kt + NB - 1, (kt + NB - 1) % NB);
+        const uint16_t *base = ring[kt % NB];
+        bf16x8 bfr[FN];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            bfr[j] = *reinterpret_cast<const bf16x8 *>(base + (TM + wn * (TN / WN) + j * 16) * KS + fo);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const bf16x8 af = *reinterpret_cast<const bf16x8 *>(base + (wm * (TM / 2) + i * 16) * KS + fo);
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], af, acc[i][j], 0, 0, 0);
+        }
+    }
+    if (!staged) {
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int64_t col = n0 + wn * (TN / WN) + j * 16 + 4 * (lane >> 4);
+            float4 bias = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (g.bias && g.epilogue != kEpiGeluGrad) bias = *reinterpret_cast<const float4 *>(g.bias + col);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int64_t row = m0 + wm * (TM / 2) + i * 16 + (lane & 15);
+                gemm_epilogue4(g, row, col, bias, acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
             }
         }
+        return;
+    }
+    // staged BF16 epilogue (kEpiStore or kEpiBiasGelu, c16 only): tile [TM][TN]
+    constexpr int CPR = TN / 8, RPI = 64 / CPR;  // 16-byte chunks per row, rows per wave store
+    uint16_t *tile = &ring[0][0];
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+        const int col = wn * (TN / WN) + j * 16 + 4 * (lane >> 4);
+        float4 bias = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (g.bias) bias = *reinterpret_cast<const float4 *>(g.bias + n0 + col);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int row = wm * (TM / 2) + i * 16 + (lane & 15);
+            float x0 = acc[i][j][0] + bias.x, x1 = acc[i][j][1] + bias.y, x2 = acc[i][j][2] + bias.z,
+                  x3 = acc[i][j][3] + bias.w;
+            if (g.epilogue == kEpiBiasGelu) {
+                x0 = gelu(x0);
+                x1 = gelu(x1);
+                x2 = gelu(x2);
+                x3 = gelu(x3);
+            }
+            const int ch = (col >> 3) ^ (row & (CPR - 1));
+            *reinterpret_cast<uint2 *>(tile + row * TN + ch * 8 + (col & 4)) = f2bf4(x0, x1, x2, x3);
+        }
+    }
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < TM / RPI / NW; ++q) {
+        const int row = (q * NW + wave) * RPI + lane / CPR, ch = lane % CPR;
+        const uint4 v = *reinterpret_cast<const uint4 *>(tile + row * TN + (ch ^ (row & (CPR - 1))) * 8);
+        *reinterpret_cast<uint4 *>(g.c16 + (m0 + row) * g.ldc16 + n0 + ch * 8) = v;
     }
 }
 
@@ -814,7 +964,14 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits) {
     const bool vec = g.ldc % 4 == 0 && g.ldc16 % 4 == 0 && g.ldaux % 4 == 0 && g.split_stride % 4 == 0 &&
                      ((uintptr_t)g.c & 15) == 0 && ((uintptr_t)g.aux & 15) == 0 && ((uintptr_t)g.bias & 15) == 0 &&
                      ((uintptr_t)g.c16 & 7) == 0;
-    if (g.m % 128 == 0 && g.n % 128 == 0 && vec)
+    // large products (at least two tiles per CU): the ring kernel
+    if (g.m % kRingTM == 0 && g.n % kRingTN == 0 && vec &&
+        (int64_t)(g.m / kRingTM) * (g.n / kRingTN) * splits >= 512) {
+        const int staged = g.c16 && !g.c && (g.epilogue == kEpiStore || g.epilogue == kEpiBiasGelu) &&
+                           g.ldc16 % 8 == 0 && ((uintptr_t)g.c16 & 15) == 0;
+        hipLaunchKernelGGL(gemm_bf16_ring_kernel, dim3(g.n / kRingTN, g.m / kRingTM, splits), dim3(64 * kRingWaves), 0,
+                           s, g, staged);
+    } else if (g.m % 128 == 0 && g.n % 128 == 0 && vec)
         hipLaunchKernelGGL(gemm_bf16_nt128_kernel, dim3(g.n / 128, g.m / 128, splits), dim3(256), 0, s, g);
     else
         hipLaunchKernelGGL(gemm_bf16_kernel, dim3(g.n / kGemmTile, g.m / kGemmTile, splits), dim3(256), 0, s, g);

@@ -11,8 +11,9 @@
 * training is deterministic and lowers the cross-entropy.
 
 K = 64 runs the 64 x 64-tile GEMM, K = 128 and K = 1024 the 128 x 128-tile
-one; K = 1024 also fills all 16 weight and theta registers per lane of the
-MH step.  A corpus of three data_chunks checks the per-chunk token ids.
+one, a full 65,536-query batch at K = 1024 the ring kernel; K = 1024 also
+fills all 16 weight and theta registers per lane of the MH step.  A corpus
+of three data_chunks checks the per-chunk token ids.
 """
 import numpy as np
 import pytest
@@ -117,6 +118,34 @@ def test_logits_match_torch(K):
     assert got.shape == (n, K)
     err = np.abs(got - ref).max()
     assert err <= 1e-2 * max(1.0, np.abs(ref).max()), err
+    s.close()
+
+
+def test_logits_full_batch_match_torch():
+    """65,536 queries at K = 1024: both products (65,536 x 384 x 512 and
+    65,536 x 512 x 1024) take the ring kernel with the staged BF16 epilogue.
+    Sampled rows against torch; the first 12,800 rows bit-exact against a
+    12,800-query call, whose products are too small for the ring kernel and
+    take the 128-tile kernel (same MFMA k order and epilogue arithmetic)."""
+    K = 1024
+    c = _corpus()
+    s = _sampler(c, K=K)
+    rng = np.random.default_rng(2)
+    w = _random_weights(s, rng, 3.0)
+    s.mh_set_weights(w)
+    D, V = c.info().num_docs, c.info().num_words
+    n, n_small = 65536, 12800
+    docs = rng.integers(0, D, n).astype(np.int32)
+    words = rng.integers(0, V, n).astype(np.int32)
+    got = s.mh_logits(docs, words)
+    assert got.shape == (n, K)
+    sel = np.sort(rng.choice(n, 512, replace=False))
+    ref, _ = _torch_forward(s, w, docs[sel], words[sel])
+    ref = ref.detach().numpy()
+    err = np.abs(got[sel] - ref).max()
+    assert err <= 1e-2 * max(1.0, np.abs(ref).max()), err
+    small = s.mh_logits(docs[:n_small], words[:n_small])
+    assert np.array_equal(small, got[:n_small])
     s.close()
 
 
