@@ -880,6 +880,7 @@ void Engine::enqueue_units_iteration(uint32_t sweep, bool set, hipEvent_t ev_a, 
         // the sweeps' time starts once the first one may run
         if (u == 0 && ev_a) hip_check(hipEventRecord(ev_a, stream), "event");
         sample_unit(u, stream);
+        if (u + 1 == U) break;  // the last chunk's rebuild follows the count update (below)
         hip_check(hipEventRecord(ev_sampled[u], stream), "event");
         hip_check(hipStreamWaitEvent(stream2, ev_sampled[u], 0), "wait");
         rebuild_theta_unit(u, stream2);
@@ -892,6 +893,14 @@ void Engine::enqueue_units_iteration(uint32_t sweep, bool set, hipEvent_t ev_a, 
     const LLArgs ll = ll_args();
     update_phi(stream, &ll);
     phi_head(stream, &ll, true);
+    // the last chunk's theta rebuild is needed only by that chunk's next
+    // sweep: started after the count update (on the critical path, and a
+    // gather-bound rebuild beside it slows it), it runs beside the next
+    // iteration's first sweeps instead
+    hip_check(hipEventRecord(ev_sampled[U - 1], stream), "event");
+    hip_check(hipStreamWaitEvent(stream2, ev_sampled[U - 1], 0), "wait");
+    rebuild_theta_unit(U - 1, stream2);
+    hip_check(hipEventRecord(ev_theta_unit[U - 1], stream2), "event");
     // all rebuilds but the last one before the next iteration's sweeps:
     // they run beside the sweeps and may fall behind (many chunks, short
     // sweeps); here they catch up while nothing else competes, so their
