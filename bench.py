@@ -57,7 +57,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4", "c5"])
-    ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (c1/c2, c3)")
     ap.add_argument("--iters", type=int, default=200, help="c4: plain iterations that fix the target")
     ap.add_argument("--mh-max-iters", type=int, default=100, help="c4: learned-MH chain length (iterations)")
     ap.add_argument("--eval-every", type=int, default=5, help="c4: held-out evaluation period")
@@ -120,6 +120,14 @@ def main():
         warm_lls, _ = s.iterate_n(it, args.warmup)
         it += args.warmup
     s.sync()
+    tune = None
+    if not args.no_tune:
+        # the autotuner (slot size from timed sweeps; with several chunks, the
+        # theta rebuild's workgroups per CU from timed iterations) on the
+        # warmed-up state, which it leaves unchanged; not timed
+        t0 = time.time()
+        tune = s.autotune()
+        log(f"autotune ({time.time() - t0:.1f}s): slot_tokens={tune.slot_tokens} theta_groups={tune.theta_groups}")
     if not args.no_cpu and args.warmup:
         # the state after the warm-up iterations, for the parity check below
         z_warm = [s.get_z(c) for c in range(info.num_chunks)]
@@ -183,6 +191,9 @@ def main():
                    "phi_rebuild": "fused over chunks" if args.fused_phi else "chunks in turn (+=)",
                    "stream_copy_gbps": copy_gbps,
                    "sum_token_nnz_last": st.sum_token_nnz,
+                   "autotune": None if tune is None else {
+                       "slot_tokens": tune.slot_tokens, "theta_groups_per_cu": tune.theta_groups,
+                       "sweep_ms_by_slot_tokens": tune.tried(), "two_iterations_ms_by_theta_groups": tune.theta_tried()},
                    "time_to_ll": time_to_ll},
         "parity": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",

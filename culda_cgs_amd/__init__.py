@@ -70,11 +70,15 @@ class MHParams(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("slot_tokens", C.c_int32), ("persistent_groups", C.c_int32), ("num_tried", C.c_int32),
-                ("pad", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 32),
-                ("tried_ms", C.c_double * 32)]
+                ("theta_groups", C.c_int32), ("best_ms", C.c_double), ("tried_slot_tokens", C.c_int32 * 32),
+                ("tried_ms", C.c_double * 32), ("theta_tried_ms", C.c_double * 8)]
 
     def tried(self):
         return [(self.tried_slot_tokens[i], self.tried_ms[i]) for i in range(self.num_tried)]
+
+    def theta_tried(self):
+        """(theta workgroups per CU, ms of two iterations) of the timed candidates."""
+        return [(g, self.theta_tried_ms[i]) for i, g in enumerate((1, 2, 3, 4, 6, 8)) if self.theta_tried_ms[i] > 0]
 
 
 # name -> (restype, argtypes); every int-returning call is checked.
@@ -519,7 +523,9 @@ class Sampler:
         return out
 
     def autotune(self):
-        """Pick the sampler's tokens per slot by timing one sweep per size."""
+        """Pick the sampler's tokens per slot by timing one sweep per size and,
+        with several chunks, the theta rebuild's workgroups per CU by timing
+        whole iterations (``theta_groups``); the chain's state is kept."""
         t = Tune()
         _check(lib().culda_autotune(self._h, C.byref(t)))
         return t

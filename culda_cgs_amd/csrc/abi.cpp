@@ -130,6 +130,8 @@ void copy_z(Engine &e, int32_t unit, uint16_t *z, int64_t n, bool to_device) {
 void fill_tune(const TuneResult &r, culda_tune *out) {
     std::memset(out, 0, sizeof *out);
     out->slot_tokens = r.slot_tokens;
+    out->theta_groups = r.theta_groups;
+    for (size_t i = 0; i < r.theta_tried.size() && i < 8; ++i) out->theta_tried_ms[i] = r.theta_tried[i].second;
     out->best_ms = r.best_ms;
     out->num_tried = (int32_t)std::min<size_t>(r.tried.size(), 32);
     for (int32_t i = 0; i < out->num_tried; ++i) {

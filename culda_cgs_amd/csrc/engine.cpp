@@ -472,28 +472,24 @@ void Engine::upload() {
         num_rare_tiles = (int64_t)tiles.size();
     }
     // theta packets.  Per unit (a chunk's rebuild, which runs beside the
-    // next chunk's sweep): about two workgroups per CU, so that it keeps
-    // pace with the sweeps without taking the CUs from them.  All units at
-    // once (batches, full rebuilds): eight workgroups per CU over the total.
+    // next chunk's sweep): about two workgroups per CU by default, so that
+    // it keeps pace with the sweeps without taking the CUs from them (the
+    // autotuner may pick another count).  All units at once (batches, full
+    // rebuilds): eight workgroups per CU over the total.
     {
         int64_t total_docs = 0;
         for (const UnitHost &u : units) total_docs += u.doc_end - u.doc_start;
         const int64_t per_all = std::max<int64_t>(16, (total_docs + (int64_t)cu_count * 8 - 1) / ((int64_t)cu_count * 8));
         std::vector<Packet> all;
         for (int32_t ui = 0; ui < U; ++ui) {
-            UnitHost &u = units[ui];
-            const int64_t Du = u.doc_end - u.doc_start;
-            const int64_t per_u = std::max<int64_t>(16, (Du + (int64_t)cu_count * 2 - 1) / ((int64_t)cu_count * 2));
-            std::vector<Packet> pk;
-            append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
-            u.theta_packets = upload_vec(*this, pk);
-            u.num_theta_packets = (int32_t)pk.size();
+            const int64_t Du = units[ui].doc_end - units[ui].doc_start;
             std::vector<Packet> pa;
             append_packets(pa, 0, (int32_t)Du, ui, (int32_t)per_all);
             all.insert(all.end(), pa.begin(), pa.end());
         }
         d_theta_packets_all = upload_vec(*this, all);
         num_theta_packets_all = (int32_t)all.size();
+        build_theta_packets(theta_groups_per_cu);
     }
     d_sweep = (uint32_t *)dalloc(16);
     d_queue = (uint32_t *)dalloc(16);
@@ -757,6 +753,21 @@ LLArgs Engine::ll_args() const {
 
 // workgroups per CU of a chunk's theta rebuild beside a sweep
 static constexpr int32_t kThetaBesideGroupsPerCU = 1;
+
+void Engine::build_theta_packets(int32_t groups_per_cu) {
+    const int64_t groups = (int64_t)cu_count * std::max<int32_t>(groups_per_cu, 1);
+    for (int32_t ui = 0; ui < (int32_t)units.size(); ++ui) {
+        UnitHost &u = units[ui];
+        const int64_t Du = u.doc_end - u.doc_start;
+        const int64_t per_u = std::max<int64_t>(16, (Du + groups - 1) / groups);
+        std::vector<Packet> pk;
+        append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
+        dfree(u.theta_packets);
+        u.theta_packets = upload_vec(*this, pk);
+        u.num_theta_packets = (int32_t)pk.size();
+    }
+    theta_groups_per_cu = groups_per_cu;
+}
 
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
@@ -1183,6 +1194,40 @@ TuneResult Engine::autotune(const std::vector<int32_t> &candidates, bool batch) 
     }
     rebuild_slots(r.slot_tokens);
     hip_check(hipStreamSynchronize(stream), "sync");
+    if (!batch && models.size() == 1 && units.size() >= 2) {
+        // a chunk's theta rebuild runs beside the next chunk's sweep, so its
+        // workgroup count is timed over whole iterations (two, back to back,
+        // each candidate from the same z and freshly rebuilt counts), not alone
+        const int32_t U = (int32_t)units.size();
+        double best = 1e30;
+        for (int32_t g : {1, 2, 3, 4, 6, 8}) {
+            build_theta_packets(g);
+            for (size_t u = 0; u < units.size(); ++u)
+                hip_check(hipMemcpyAsync(units[u].z, saved[u], units[u].num_tokens * 2, hipMemcpyDeviceToDevice, stream),
+                          "z restore");
+            rebuild_all();
+            stream2_busy = false;
+            hip_check(hipEventRecord(e0, stream), "event");
+            enqueue_units_iteration(1, true, nullptr, nullptr);
+            enqueue_units_iteration(2, true, nullptr, nullptr);
+            hip_check(hipStreamWaitEvent(stream, ev_theta_unit[U - 1], 0), "wait");
+            hip_check(hipEventRecord(e1, stream), "event");
+            hip_check(hipEventSynchronize(e1), "event sync");
+            stream2_busy = false;
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            r.theta_tried.emplace_back(g, ms);
+            if (ms < best) {
+                best = ms;
+                r.theta_groups = g;
+            }
+        }
+        build_theta_packets(r.theta_groups);
+        for (size_t u = 0; u < units.size(); ++u)
+            hip_check(hipMemcpyAsync(units[u].z, saved[u], units[u].num_tokens * 2, hipMemcpyDeviceToDevice, stream),
+                      "z restore");
+        rebuild_all();
+    }
     for (uint16_t *p : saved) dfree(p);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

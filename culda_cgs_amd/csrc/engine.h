@@ -81,6 +81,11 @@ struct TuneResult {
     int32_t slot_tokens = 0;
     double best_ms = 0;
     std::vector<std::pair<int32_t, double>> tried;  // (slot tokens, ms per sweep)
+    // multi-chunk corpora: workgroups per CU of a chunk's theta rebuild
+    // (docs per workgroup = the chunk's docs / (that * CUs)), timed over
+    // whole iterations; 0 when not tuned
+    int32_t theta_groups = 0;
+    std::vector<std::pair<int32_t, double>> theta_tried;  // (groups per CU, ms per two iterations)
 };
 
 class Engine {
@@ -194,6 +199,9 @@ public:
     void phi_head(hipStream_t s, const LLArgs *ll = nullptr, bool advance = false);
     LLArgs ll_args() const;
     void rebuild_theta_unit(int32_t unit, hipStream_t s);
+    // per-chunk theta packets: about groups_per_cu workgroups per CU, at least 16 docs each
+    void build_theta_packets(int32_t groups_per_cu);
+    int32_t theta_groups_per_cu = 2;
     void rebuild_theta_all(hipStream_t s);
     void sample_unit(int32_t unit, hipStream_t s);
     void sample_all(hipStream_t s);

@@ -344,14 +344,21 @@ typedef struct culda_tune {
     int32_t slot_tokens;          /* chosen tokens per wave slot             */
     int32_t persistent_groups;    /* batches: chosen workgroups (0 = one per slot) */
     int32_t num_tried;
-    int32_t pad;
+    int32_t theta_groups;         /* several chunks: workgroups per CU of a chunk's theta
+                                     rebuild (docs per workgroup = its docs / (this * CUs)),
+                                     timed over whole iterations; 0 = not tuned */
     double  best_ms;              /* sampler time of the chosen candidate    */
     int32_t tried_slot_tokens[32];
     double  tried_ms[32];
+    double  theta_tried_ms[8];    /* two iterations' time per theta_groups candidate
+                                     1, 2, 3, 4, 6, 8 (0 where not tried)       */
 } culda_tune;
 
-/* Time slot sizes 128..4096 for this corpus and keep the fastest.  Needs
- * the initial rebuild (culda_rebuild_all). */
+/* Time slot sizes 128..4096 for this corpus and keep the fastest; with
+ * several chunks, then time two whole iterations for each theta-rebuild
+ * workgroup count (1..8 per CU) and keep the fastest.  z, the counts and
+ * theta are left as they were.  Needs the initial rebuild
+ * (culda_rebuild_all). */
 int culda_autotune(culda_ctx *ctx, culda_tune *out);
 
 /* STREAM-style device copy of n_bytes (multiple of 16) between two device

@@ -135,6 +135,8 @@ def test_autotune_leaves_chain_unchanged(corpora):
         s.rebuild_all()
     t = tuned.autotune()
     assert t.num_tried == 6 and t.slot_tokens in [x for x, _ in t.tried()]
+    assert sp[3] > 1 and t.theta_groups in (1, 2, 3, 4, 6, 8)  # docs per theta workgroup, tuned
+    assert len(t.theta_tried()) == 6 and all(ms > 0 for _, ms in t.theta_tried())
     assert all(ms > 0 for _, ms in t.tried())
     # the slot partition changes how the FP64 LL partials group: same value
     # up to rounding; z and the counts are identical
@@ -148,7 +150,7 @@ def test_autotune_leaves_chain_unchanged(corpora):
     eager = _batch(corpora)
     tb = _batch(corpora)
     t = tb.autotune()
-    assert t.num_tried == 18 and t.persistent_groups >= 0
+    assert t.num_tried == 18 and t.persistent_groups >= 0 and t.theta_groups == 0
     for it in range(1, 4):
         assert np.allclose(eager.iterate(it)[0], tb.iterate(it, use_graph=True)[0], rtol=1e-12, atol=0)
     for m, (sp, _, _) in enumerate(corpora):
