@@ -203,20 +203,31 @@ static void build_phi_list(const std::vector<UnitHost> &units, int32_t m, int32_
     }
 }
 
-// Deals a list in descending token order round-robin into packets of about
-// `per` items (packet p takes items p, p + P, p + 2P, ...), so every packet
-// gets a like share of long and short items; reorders `slots` to match and
-// appends the packets, their ranges offset by `base`.
+// Deals a list in descending token order into packets of about `per`
+// items, in snake order (round r gives packet p the item of rank r P + p,
+// or r P + P - 1 - p when r is odd), so every packet gets a like share of
+// long and short items and no packet the longest of every round; within a
+// packet, the waves of the rebuild take items w, w + W, w + 2W, ... (W
+// waves), so every other run of W items is reversed for the same reason.
+// Reorders `slots` to match and appends the packets, their ranges offset
+// by `base`.
 static void deal_packets(std::vector<PhiSlot> &slots, int32_t owner, int32_t per, int32_t base,
                          std::vector<Packet> &packets) {
     const int32_t n = (int32_t)slots.size();
     if (n == 0) return;
-    const int32_t P = (n + per - 1) / per;
-    std::vector<PhiSlot> out;
+    const int32_t P = (n + per - 1) / per, W = kWavesPerGroup;
+    std::vector<PhiSlot> out, pk;
     out.reserve(n);
     for (int32_t p = 0; p < P; ++p) {
+        pk.clear();
+        for (int32_t r = 0; (int64_t)r * P < n; ++r) {
+            const int32_t i = r * P + ((r & 1) ? P - 1 - p : p);
+            if (i < n) pk.push_back(slots[i]);
+        }
+        for (size_t j = W; j < pk.size(); j += 2 * (size_t)W)
+            std::reverse(pk.begin() + j, pk.begin() + std::min(pk.size(), j + (size_t)W));
         const int32_t first = base + (int32_t)out.size();
-        for (int32_t i = p; i < n; i += P) out.push_back(slots[i]);
+        out.insert(out.end(), pk.begin(), pk.end());
         packets.push_back(Packet{first, base + (int32_t)out.size(), owner, 0});
     }
     slots.swap(out);
