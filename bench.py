@@ -55,7 +55,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c3", "c4", "c5"])
     ap.add_argument("--no-tune", action="store_true", help="c3: skip the autotuner")
     ap.add_argument("--tune", action="store_true", help="c1/c2: run the autotuner on the warmed-up state before timing")
@@ -117,9 +117,13 @@ def main():
 
     it = 1
     warm_lls, z_warm = [], None
-    if args.warmup:
-        warm_lls, _ = s.iterate_n(it, args.warmup)
-        it += args.warmup
+    # the parity check and the CPU baseline take the warm-up iterations but
+    # the last: z is downloaded after them, and the last warm-up iteration
+    # then runs the GPU up again after that pause, before the timed steps
+    parity_iters = args.warmup - 1 if (not args.no_cpu and args.warmup > 1) else args.warmup
+    if parity_iters:
+        warm_lls, _ = s.iterate_n(it, parity_iters)
+        it += parity_iters
     s.sync()
     tune = None
     if args.tune:
@@ -132,9 +136,13 @@ def main():
         t0 = time.time()
         tune = s.autotune()
         log(f"autotune ({time.time() - t0:.1f}s): slot_tokens={tune.slot_tokens} theta_groups={tune.theta_groups}")
-    if not args.no_cpu and args.warmup:
-        # the state after the warm-up iterations, for the parity check below
+    if not args.no_cpu and parity_iters:
+        # the state after those warm-up iterations, for the parity check below
         z_warm = [s.get_z(c) for c in range(info.num_chunks)]
+    if args.warmup > parity_iters:
+        s.iterate_n(it, args.warmup - parity_iters)
+        it += args.warmup - parity_iters
+        s.sync()
     # the timed steps: one call enqueues them back to back (no host round
     # trip between iterations); the LL of every step comes back at the end
     t_start = time.perf_counter()
@@ -173,7 +181,7 @@ def main():
 
     cpu, parity = None, None
     if not args.no_cpu and z_warm is not None:
-        cpu, parity = cpu_baseline(corpus, info, K, alpha, beta, args.seed, args.warmup, warm_lls, z_warm,
+        cpu, parity = cpu_baseline(corpus, info, K, alpha, beta, args.seed, parity_iters, warm_lls, z_warm,
                                    args.cpu_threads)
 
     line = {
