@@ -330,11 +330,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
 // the slice and frees the slot read before it.  Tile rows (128 A rows, then
 // 256 B rows) are 64 bytes, their four 16-byte k chunks XOR-swizzled by the
 // row (ring_chunk) so that the fragment reads of 16 consecutive rows hit
-// distinct LDS banks.  When BF16 is the only output, the epilogue stages the
-// C tile through the idle ring (16-byte chunks swizzled by the row) and
-// writes whole 512-byte rows; otherwise it is the 128-tile kernel's.
-// Operand and fragment layouts and the XCD-aware tile order are the 128-tile
-// kernel's.
+// distinct LDS banks.  BF16 is the only output (kEpiStore or kEpiBiasGelu
+// into c16; the launcher sends every other product to the 128-tile
+// kernel): the epilogue stages the C tile through the idle ring (16-byte
+// chunks swizzled by the row) and writes whole 512-byte rows.  Operand and
+// fragment layouts and the XCD-aware tile order are the 128-tile kernel's.
 typedef __attribute__((address_space(3))) void *lds_void_ptr;
 typedef __attribute__((address_space(1))) void *global_void_ptr;
 constexpr int kRingTM = 128, kRingTN = 256, kRingWaves = 8, kRingSlots = 3;
@@ -347,7 +347,7 @@ __device__ __forceinline__ int ring_chunk(int r, int c) { return c ^ ((r >> 1) &
 #define CULDA_VMCNT(n) (0x0F70 | ((n)&15) | (((n) >> 4) << 14))
 
 __global__ __launch_bounds__(64 * kRingWaves) __attribute__((amdgpu_waves_per_eu(4))) void gemm_bf16_ring_kernel(
-    GemmArgs g, int staged) {
+    GemmArgs g) {
     constexpr int TM = kRingTM, TN = kRingTN, WN = 4, NW = kRingWaves, NB = kRingSlots, KS = kGemmK;
     constexpr int FM = TM / 2 / 16, FN = TN / WN / 16;  // 4 x 4 MFMA tiles per wave
     constexpr int ROWS = TM + TN, PER = ROWS / 16 / NW;  // 1 KB loads per wave per slice
@@ -367,7 +367,6 @@ __global__ __launch_bounds__(64 * kRingWaves) __attribute__((amdgpu_waves_per_eu
         }
     }
     const int64_t m0 = tm * TM, n0 = tn * TN;
-    const int64_t kz = (int64_t)blockIdx.z * g.k;
     // this lane's source of each of its wave's PER loads of a slice: load q
     // covers tile rows 16 j .. 16 j + 15, j = q * NW + wave
     const uint16_t *src[PER];
@@ -376,7 +375,7 @@ __global__ __launch_bounds__(64 * kRingWaves) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int row = (q * NW + wave) * 16 + r16;
-            src[q] = row < TM ? g.a + (m0 + row) * g.lda + kz + 8 * c : g.b + (n0 + row - TM) * g.ldb + kz + 8 * c;
+            src[q] = row < TM ? g.a + (m0 + row) * g.lda + 8 * c : g.b + (n0 + row - TM) * g.ldb + 8 * c;
         }
     }
     auto issue = [&](int kt, int slot) {
@@ -417,21 +416,7 @@ __global__ __launch_bounds__(64 * kRingWaves) __attribute__((amdgpu_waves_per_eu
             for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j], af, acc[i][j], 0, 0, 0);
         }
     }
-    if (!staged) {
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int64_t col = n0 + wn * (TN / WN) + j * 16 + 4 * (lane >> 4);
-            float4 bias = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (g.bias && g.epilogue != kEpiGeluGrad) bias = *reinterpret_cast<const float4 *>(g.bias + col);
-#pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                const int64_t row = m0 + wm * (TM / 2) + i * 16 + (lane & 15);
-                gemm_epilogue4(g, row, col, bias, acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]);
-            }
-        }
-        return;
-    }
-    // staged BF16 epilogue (kEpiStore or kEpiBiasGelu, c16 only): tile [TM][TN]
+    // staged BF16 epilogue: tile [TM][TN]
     constexpr int CPR = TN / 8, RPI = 64 / CPR;  // 16-byte chunks per row, rows per wave store
     uint16_t *tile = &ring[0][0];
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
@@ -964,13 +949,13 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, hipStream_t s, int32_t splits) {
     const bool vec = g.ldc % 4 == 0 && g.ldc16 % 4 == 0 && g.ldaux % 4 == 0 && g.split_stride % 4 == 0 &&
                      ((uintptr_t)g.c & 15) == 0 && ((uintptr_t)g.aux & 15) == 0 && ((uintptr_t)g.bias & 15) == 0 &&
                      ((uintptr_t)g.c16 & 7) == 0;
-    // large products (at least two tiles per CU): the ring kernel
-    if (g.m % kRingTM == 0 && g.n % kRingTN == 0 && vec &&
-        (int64_t)(g.m / kRingTM) * (g.n / kRingTN) * splits >= 512) {
-        const int staged = g.c16 && !g.c && (g.epilogue == kEpiStore || g.epilogue == kEpiBiasGelu) &&
+    // large products with BF16 the only output (the proposal queries: at
+    // least two tiles per CU): the ring kernel
+    const bool bf16_only = g.c16 && !g.c && (g.epilogue == kEpiStore || g.epilogue == kEpiBiasGelu) &&
                            g.ldc16 % 8 == 0 && ((uintptr_t)g.c16 & 15) == 0;
-        hipLaunchKernelGGL(gemm_bf16_ring_kernel, dim3(g.n / kRingTN, g.m / kRingTM, splits), dim3(64 * kRingWaves), 0,
-                           s, g, staged);
+    if (bf16_only && splits == 1 && g.m % kRingTM == 0 && g.n % kRingTN == 0 && vec &&
+        (int64_t)(g.m / kRingTM) * (g.n / kRingTN) >= 512) {
+        hipLaunchKernelGGL(gemm_bf16_ring_kernel, dim3(g.n / kRingTN, g.m / kRingTM), dim3(64 * kRingWaves), 0, s, g);
     } else if (g.m % 128 == 0 && g.n % 128 == 0 && vec)
         hipLaunchKernelGGL(gemm_bf16_nt128_kernel, dim3(g.n / 128, g.m / 128, splits), dim3(256), 0, s, g);
     else
