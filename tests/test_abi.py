@@ -1,5 +1,6 @@
 """The C ABI library loads and exports every symbol include/culda.h declares;
-the CLI parses the reference flags.  No compute calls (no GPU needed)."""
+the ctypes mirror lays out every ABI struct as a C compiler does; the CLI
+parses the reference flags.  No compute calls (no GPU needed)."""
 import re
 import subprocess
 from pathlib import Path
@@ -84,3 +85,30 @@ def test_check_corpus_limits_without_a_device():
     assert e.value.status == -7 and "LDS" in str(e.value) and "doc 0" in str(e.value)
     ok = product_corpus(V, *chunks_from_docs(V, [rng.integers(0, V, 1_800_000), rng.integers(0, V, 10)], 1))
     ok.check(16)
+
+
+# every struct of the ABI and its ctypes mirror
+ABI_STRUCTS = [("culda_corpus_info", cg.CorpusInfo), ("culda_chunk_view", cg.ChunkView),
+               ("culda_gen_params", cg.GenParams), ("culda_params", cg.Params), ("culda_stats", cg.Stats),
+               ("culda_mh_params", cg.MHParams), ("culda_tune", cg.Tune)]
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and every field offset, as a C compiler lays out include/culda.h,
+    equal the ctypes mirror's (a field name the header lacks fails to compile)."""
+    import ctypes
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "culda.h"', "int main(void) {"]
+    for cname, cls in ABI_STRUCTS:
+        lines.append(f'printf("{cname} . %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    got = {(a, b): int(c) for a, b, c in (line.split() for line in out.splitlines())}
+    for cname, cls in ABI_STRUCTS:
+        assert got[(cname, ".")] == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert got[(cname, f)] == getattr(cls, f).offset, f"{cname}.{f}"
