@@ -340,7 +340,7 @@ def test_random_configurations_bit_exact(seed):
     s.close()
 
 
-@pytest.mark.parametrize("chunks", [1, 4])
+@pytest.mark.parametrize("chunks", [1, 2, 4])
 def test_back_to_back_iterations_equal_synchronised_ones(chunks):
     """culda_iterate returns with the last chunks' theta rebuilds still on the
     second stream; the next iteration orders each chunk's sweep after that
@@ -349,10 +349,13 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
     per-iteration sampler bytes must be the ones of the rebuild each sweep
     read; iterate_n (iterations enqueued back to back in one call) must give
     the same chain, LLs and summed bytes (a one-chunk corpus replays a
-    captured graph there, with no per-sweep timing)."""
+    captured graph there, with no per-sweep timing).  The last chunk's
+    rebuild starts after the count update; two chunks is the case where the
+    iteration's final wait is for the first chunk's rebuild.  The chain ends
+    at the oracle's z."""
     V, K, arrays = _setup(31, V=500, D=400, mean_len=90, chunks=chunks, K=128)
     alpha, beta, seed = 0.2, 0.01, 17
-    live, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
+    live, om = _pair(V, K, arrays, alpha, beta, seed, profile=True)
     synced, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
     batched, _ = _pair(V, K, arrays, alpha, beta, seed, profile=True)
     for s in (live, synced, batched):
@@ -381,6 +384,13 @@ def test_back_to_back_iterations_equal_synchronised_ones(chunks):
         assert all(np.array_equal(x, y) for x, y in zip(live.get_theta(c), synced.get_theta(c)))
         assert all(np.array_equal(x, y) for x, y in zip(batched.get_theta(c), synced.get_theta(c)))
     assert live.stats().sum_token_nnz == synced.stats().sum_token_nnz == batched.stats().sum_token_nnz
+    om.init_z(seed)
+    om.rebuild()
+    for it in range(1, 7):
+        om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+    for c in range(chunks):
+        assert np.array_equal(synced.get_z(c), om.z[c]), f"chunk {c}"
     import culda_cgs_amd as cg
     for first, count in ((7, 0), (7, 1025), (0, 1)):
         with pytest.raises(cg.CuldaError):
