@@ -762,32 +762,63 @@ LLArgs Engine::ll_args() const {
                   (int32_t)models.size()};
 }
 
-// workgroups per CU of a chunk's theta rebuild beside a sweep
-static constexpr int32_t kThetaBesideGroupsPerCU = 1;
+// Packets of `per` consecutive docs of [0, D) in the order a grid of `grid`
+// workgroups takes them (workgroup b takes packets b, b + grid, ... and runs
+// on XCD b mod kXcds): each XCD walks its own contiguous share of the docs
+// with its workgroups side by side, so the docs in flight on one XCD are
+// neighbours, and the z lines of the words they share (a word's tokens lie
+// in doc order in the word-major z) are gathered from that XCD's L2 instead
+// of the fabric.  Empty packets pad the XCDs' lists to whole rounds.
+static void append_xcd_packets(std::vector<Packet> &out, int32_t D, int32_t owner, int32_t per, int32_t grid) {
+    const int32_t X = (int32_t)kXcds;
+    if (grid < X) {
+        append_packets(out, 0, D, owner, per);
+        return;
+    }
+    std::vector<std::vector<Packet>> lists(X);
+    for (int32_t x = 0; x < X; ++x)
+        append_packets(lists[x], (int32_t)((int64_t)D * x / X), (int32_t)((int64_t)D * (x + 1) / X), owner, per);
+    const int64_t groups_per_xcd = grid / X;  // workgroups b < X * groups_per_xcd; the rest take only padding
+    int64_t rounds = 0;
+    for (const auto &l : lists) rounds = std::max<int64_t>(rounds, ((int64_t)l.size() + groups_per_xcd - 1) / groups_per_xcd);
+    for (int64_t r = 0; r < rounds; ++r)
+        for (int32_t b = 0; b < grid; ++b) {
+            const int64_t i = r * groups_per_xcd + b / X;
+            const auto &l = lists[b % X];
+            out.push_back(b < X * groups_per_xcd && i < (int64_t)l.size() ? l[i] : Packet{0, 0, owner, 0});
+        }
+}
 
 void Engine::build_theta_packets(int32_t groups_per_cu) {
+    theta_groups_per_cu = groups_per_cu;  // read by theta_beside_grid
     const int64_t groups = (int64_t)cu_count * std::max<int32_t>(groups_per_cu, 1);
     for (int32_t ui = 0; ui < (int32_t)units.size(); ++ui) {
         UnitHost &u = units[ui];
         const int64_t Du = u.doc_end - u.doc_start;
         const int64_t per_u = std::max<int64_t>(16, (Du + groups - 1) / groups);
         std::vector<Packet> pk;
-        append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
+        if (units.size() > 1)
+            append_xcd_packets(pk, (int32_t)Du, ui, kWavesPerGroup, theta_beside_grid());
+        else
+            append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
         dfree(u.theta_packets);
         u.theta_packets = upload_vec(*this, pk);
         u.num_theta_packets = (int32_t)pk.size();
     }
-    theta_groups_per_cu = groups_per_cu;
 }
+
+// workgroups of a chunk's theta rebuild beside a sweep: a bounded share of
+// the CUs, theta_groups_per_cu per CU
+int32_t Engine::theta_beside_grid() const { return std::max<int32_t>(theta_groups_per_cu, 1) * cu_count; }
 
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
     // a single chunk's rebuild has nothing beside it: the next sweep waits;
-    // a chunk's rebuild beside the next chunk's sweep holds at most one
-    // workgroup per CU, so that it takes the CUs the sweep leaves idle
-    // rather than the sweep's own (the sweeps are the critical path)
+    // a chunk's rebuild beside the next chunk's sweep holds a bounded grid
+    // (theta_beside_grid), so that the sweep keeps most of the CUs (the
+    // sweeps are the critical path)
     const bool alone = units.size() == 1;
-    hip_check(launch_theta_rebuild(a, alone, alone ? 0 : kThetaBesideGroupsPerCU * cu_count, s), "theta rebuild launch");
+    hip_check(launch_theta_rebuild(a, alone, alone ? 0 : theta_beside_grid(), s), "theta rebuild launch");
     theta_ok[ui] = 1;
 }
 

@@ -199,9 +199,13 @@ public:
     void phi_head(hipStream_t s, const LLArgs *ll = nullptr, bool advance = false);
     LLArgs ll_args() const;
     void rebuild_theta_unit(int32_t unit, hipStream_t s);
-    // per-chunk theta packets: about groups_per_cu workgroups per CU, at least 16 docs each
+    // per-chunk theta packets.  A single chunk: about groups_per_cu
+    // workgroups per CU, at least 16 docs each.  Several chunks (each
+    // rebuild beside a sweep, on a grid of groups_per_cu workgroups per
+    // CU): 4-doc packets dealt to the XCDs in contiguous runs
     void build_theta_packets(int32_t groups_per_cu);
     int32_t theta_groups_per_cu = 2;
+    int32_t theta_beside_grid() const;
     void rebuild_theta_all(hipStream_t s);
     void sample_unit(int32_t unit, hipStream_t s);
     void sample_all(hipStream_t s);

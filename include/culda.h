@@ -345,7 +345,7 @@ typedef struct culda_tune {
     int32_t persistent_groups;    /* batches: chosen workgroups (0 = one per slot) */
     int32_t num_tried;
     int32_t theta_groups;         /* several chunks: workgroups per CU of a chunk's theta
-                                     rebuild (docs per workgroup = its docs / (this * CUs)),
+                                     rebuild beside the next sweep (the launch's grid),
                                      timed over whole iterations; 0 = not tuned */
     double  best_ms;              /* sampler time of the chosen candidate    */
     int32_t tried_slot_tokens[32];

@@ -135,7 +135,7 @@ def test_autotune_leaves_chain_unchanged(corpora):
         s.rebuild_all()
     t = tuned.autotune()
     assert t.num_tried == 6 and t.slot_tokens in [x for x, _ in t.tried()]
-    assert sp[3] > 1 and t.theta_groups in (1, 2, 3, 4, 6, 8)  # docs per theta workgroup, tuned
+    assert sp[3] > 1 and t.theta_groups in (1, 2, 3, 4, 6, 8)  # theta rebuild grid, tuned
     assert len(t.theta_tried()) == 6 and all(ms > 0 for _, ms in t.theta_tried())
     assert all(ms > 0 for _, ms in t.tried())
     # the slot partition changes how the FP64 LL partials group: same value
