@@ -168,6 +168,52 @@ void check_corpus_limits(const Corpus &c, int32_t K) {
 static int64_t phi_split(int64_t tokens) { return tokens <= ((int64_t)1 << 24) ? 4096 : 8192; }
 
 // Packets of at most `per` consecutive items of [first, last), all owned by `owner`.
+// Where each word's token list (word-major, unit-local doc ids `doc`)
+// crosses into each XCD doc region: (kXcds + 1) offsets per word from the
+// word's first token, the first 0 and the last the word's token count.
+// Empty if some word's docs are not in ascending order (no region
+// affinity for that unit's slots then).
+static std::vector<uint32_t> doc_regions(const std::vector<uint32_t> &doc, const std::vector<int64_t> &word_idx,
+                                         int32_t V, int32_t D) {
+    const int32_t X = (int32_t)kXcds;
+    std::vector<uint32_t> out((size_t)V * (X + 1));
+    for (int32_t w = 0; w < V; ++w) {
+        const auto b = doc.begin() + word_idx[w], e = doc.begin() + word_idx[w + 1];
+        if (!std::is_sorted(b, e)) return {};
+        for (int32_t r = 0; r <= X; ++r)
+            out[(size_t)w * (X + 1) + r] =
+                r == X ? (uint32_t)(e - b) : (uint32_t)(std::lower_bound(b, e, (uint32_t)((int64_t)D * r / X)) - b);
+    }
+    return out;
+}
+
+// A unit's slots (indices, longest first) in launch order: position b
+// (workgroup b, which runs on XCD b mod kXcds) takes the longer of the next
+// slot of region b mod kXcds and the next untagged slot; an XCD whose own
+// and the untagged slots are used up takes the longest remaining one.
+// Each XCD still takes its slots longest first.
+static std::vector<size_t> deal_slots_to_xcds(const std::vector<size_t> &by_len, const std::vector<int8_t> &tag,
+                                              const std::vector<SlotDesc> &sl) {
+    const int X = (int)kXcds;
+    std::vector<std::vector<size_t>> q(X + 1);
+    for (size_t i : by_len) q[tag[i] < 0 ? X : tag[i]].push_back(i);
+    std::vector<size_t> head(X + 1, 0), out;
+    out.reserve(by_len.size());
+    auto len = [&](int y) { return sl[q[y][head[y]]].end - sl[q[y][head[y]]].begin; };
+    auto left = [&](int y) { return head[y] < q[y].size(); };
+    for (size_t b = 0; b < by_len.size(); ++b) {
+        const int x = (int)(b % X);
+        int pick = left(x) ? x : -1;
+        if (left(X) && (pick < 0 || len(X) > len(pick))) pick = X;
+        for (int y = 0; pick < 0 && y < X; ++y)
+            if (left(y)) pick = y;
+        for (int y = 0; pick >= 0 && pick != x && pick != X && y < X; ++y)
+            if (left(y) && len(y) > len(pick)) pick = y;
+        out.push_back(q[pick][head[pick]++]);
+    }
+    return out;
+}
+
 static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last, int32_t owner, int32_t per) {
     for (int32_t x = first; x < last; x += per) out.push_back(Packet{x, std::min(last, x + per), owner, 0});
 }
@@ -326,6 +372,7 @@ void Engine::upload() {
             }
             u.theta_cap = cap;
             u.word_idx_h = ch.word_idx;
+            u.region_h = doc_regions(tok, ch.word_idx, c.num_words, Dc);
             u.tokens = upload_vec(*this, tok);
             u.z = (uint16_t *)dalloc(std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t));
             hip_check(hipMemset(u.z, 0, std::max<int64_t>(u.num_tokens, 1) * sizeof(uint16_t)), "z clear");
@@ -572,7 +619,14 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
         std::vector<int64_t> run(pre.begin(), pre.end() - 1);
         for (int32_t ui = m.unit_begin; ui < m.unit_end; ++ui) {
             UnitHost &u = units[ui];
+            // XCD affinity where the unit's theta rows outgrow the XCDs' L2s
+            // together (smaller ones stay cached anyway, and the extra slots
+            // would only cost): doc regions of at least 64 docs, docs sorted
+            // in every word
+            const bool xcd = !u.region_h.empty() && u.doc_end - u.doc_start >= 64 * (int32_t)kXcds &&
+                             u.theta_cap * (int64_t)sizeof(uint16_t) > (int64_t)kXcds * kL2BytesPerXcd;
             std::vector<SlotDesc> sl;
+            std::vector<int8_t> tag;  // the XCD region holding all of a slot's docs, or -1
             for (int32_t w = 0; w < m.V; ++w) {
                 const int64_t b = u.word_idx_h[w], e = u.word_idx_h[w + 1], n = e - b;
                 if (n > 0) {
@@ -581,17 +635,37 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
                     // whatever the slot size: the sampler may stage its row
                     // from the tokens' zp topics, one token per lane.
                     const int32_t rare = m.unit_end - m.unit_begin == 1 && n * 16 < m.K;
-                    const int64_t pieces = rare ? 1 : (n + slot_tokens - 1) / slot_tokens;
-                    const int64_t step = (n + pieces - 1) / pieces;
-                    for (int64_t x = b; x < e; x += step)
-                        sl.push_back(SlotDesc{w, ui, x, std::min(e, x + step), run[w] - b, rare, 0});
+                    auto add = [&](int64_t x0, int64_t x1, int8_t t) {
+                        const int64_t nx = x1 - x0;
+                        const int64_t pieces = rare ? 1 : (nx + slot_tokens - 1) / slot_tokens;
+                        const int64_t step = (nx + pieces - 1) / pieces;
+                        for (int64_t x = x0; x < x1; x += step) {
+                            sl.push_back(SlotDesc{w, ui, x, std::min(x1, x + step), run[w] - b, rare, 0});
+                            tag.push_back(t);
+                        }
+                    };
+                    const uint32_t *rb = xcd ? &u.region_h[(size_t)w * (kXcds + 1)] : nullptr;
+                    if (rb && !rare && n > slot_tokens) {
+                        // a word of several slots is cut at the region
+                        // boundaries too: each piece's docs (whose theta rows
+                        // and metadata the sampler gathers) lie in one region
+                        for (uint32_t r = 0; r < kXcds; ++r)
+                            if (rb[r + 1] > rb[r]) add(b + rb[r], b + rb[r + 1], (int8_t)r);
+                    } else {
+                        int8_t t = -1;
+                        for (uint32_t r = 0; rb && r < kXcds; ++r)
+                            if (rb[r] == 0 && rb[r + 1] == (uint32_t)n) t = (int8_t)r;
+                        add(b, e, t);
+                    }
                 }
                 run[w] += n;
             }
-            std::stable_sort(sl.begin(), sl.end(),
-                             [](const SlotDesc &a, const SlotDesc &b) { return a.end - a.begin > b.end - b.begin; });
+            std::vector<size_t> by_len(sl.size());
+            std::iota(by_len.begin(), by_len.end(), 0);
+            std::stable_sort(by_len.begin(), by_len.end(),
+                             [&](size_t a, size_t b) { return sl[a].end - sl[a].begin > sl[b].end - sl[b].begin; });
             u.slot_begin = (int32_t)all.size();
-            all.insert(all.end(), sl.begin(), sl.end());
+            for (size_t i : xcd ? deal_slots_to_xcds(by_len, tag, sl) : by_len) all.push_back(sl[i]);
             u.slot_end = (int32_t)all.size();
         }
         m.slot_end = (int32_t)all.size();

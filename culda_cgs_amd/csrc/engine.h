@@ -63,6 +63,10 @@ struct UnitHost {
     Packet *theta_packets = nullptr;
     int32_t num_theta_packets = 0;
     std::vector<int64_t> word_idx_h;
+    // word w's tokens (in doc order) cross into XCD doc region r (docs
+    // [D r / kXcds, D (r + 1) / kXcds) of the unit) at word_idx_h[w] +
+    // region_h[w * (kXcds + 1) + r]; empty if some word's docs are unsorted
+    std::vector<uint32_t> region_h;
 };
 
 // Evaluated tokens of the held-out docs (document completion), kept by the

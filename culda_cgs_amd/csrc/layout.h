@@ -45,6 +45,7 @@ constexpr int kJointGrid = 1024;     // workgroups (partials) of each joint-LL l
 constexpr int kJointUnitsMax = 63;   // chunks the allocation-free joint LL serves (its static scratch)
 constexpr int kMaxDevices = 64;
 constexpr uint32_t kXcds = 8;      // accelerator dies of an MI355X; workgroup i runs on die i mod 8
+constexpr int64_t kL2BytesPerXcd = (int64_t)4 << 20;  // each die's L2
 constexpr int kQueueClaim = 16;    // slot positions a persistent sampler workgroup claims at once
 // The sampler's LDS: phi row and Q block prefixes (static), plus eight
 // rows of S block prefixes sized by the longest theta row of any doc
