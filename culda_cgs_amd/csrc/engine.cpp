@@ -681,20 +681,22 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
             rare_tokens += sd.end - sd.begin;
         }
     num_slots = (int32_t)all.size();
-    std::vector<int32_t> order(all.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    std::vector<int32_t> by_len(all.size());
+    std::iota(by_len.begin(), by_len.end(), 0);
+    std::stable_sort(by_len.begin(), by_len.end(), [&](int32_t a, int32_t b) {
         return all[a].end - all[a].begin > all[b].end - all[b].begin;
     });
+    // the one-launch order of a batch: every model on one XCD
+    const std::vector<int32_t> order = models.size() > 1 ? deal_models_to_xcds(by_len, all) : by_len;
     // persistent claims: run j of kQueueClaim positions holds the slots of
     // rank j, j + R, j + 2R, ... (R runs), so every claim mixes long and
     // short slots and the longest are still taken first
     std::vector<int32_t> claims;
-    claims.reserve(order.size());
+    claims.reserve(by_len.size());
     {
-        const size_t R = (order.size() + kQueueClaim - 1) / kQueueClaim;
+        const size_t R = (by_len.size() + kQueueClaim - 1) / kQueueClaim;
         for (size_t j = 0; j < R; ++j)
-            for (size_t i = j; i < order.size(); i += R) claims.push_back(order[i]);
+            for (size_t i = j; i < by_len.size(); i += R) claims.push_back(by_len[i]);
     }
     d_slots = upload_vec(*this, all);
     d_order = upload_vec(*this, order);
@@ -712,6 +714,45 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     // the copies and the memset above run on the null stream, which does not
     // order against the engine's non-blocking streams
     hip_check(hipDeviceSynchronize(), "slot upload");
+}
+
+std::vector<int32_t> Engine::deal_models_to_xcds(const std::vector<int32_t> &by_len,
+                                                const std::vector<SlotDesc> &all) const {
+    const int X = (int)kXcds;
+    // models to XCDs, largest first onto the least-loaded XCD (tokens)
+    std::vector<int32_t> ms(models.size());
+    std::iota(ms.begin(), ms.end(), 0);
+    std::stable_sort(ms.begin(), ms.end(),
+                     [&](int32_t a, int32_t b) { return models[a].num_tokens > models[b].num_tokens; });
+    std::vector<int64_t> load(X, 0);
+    std::vector<int32_t> xcd_of(models.size()), rank_of(models.size());
+    for (size_t i = 0; i < ms.size(); ++i) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        xcd_of[ms[i]] = x;
+        rank_of[ms[i]] = (int32_t)i;
+        load[x] += models[ms[i]].num_tokens;
+    }
+    // each XCD's slots model by model (largest model first), each model's
+    // longest first: the XCD's L2 then mostly holds one model's theta rows
+    std::vector<std::vector<int32_t>> q(X);
+    std::vector<int32_t> sorted(by_len);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int32_t a, int32_t b) {
+        return rank_of[units[all[a].unit].model] < rank_of[units[all[b].unit].model];
+    });
+    for (int32_t i : sorted) q[xcd_of[units[all[i].unit].model]].push_back(i);
+    // position b takes XCD b mod X's next slot (workgroup b runs there);
+    // an XCD with none left takes one of the XCD with the most left
+    std::vector<size_t> head(X, 0);
+    std::vector<int32_t> out;
+    out.reserve(by_len.size());
+    for (size_t b = 0; b < by_len.size(); ++b) {
+        int x = (int)(b % X);
+        if (head[x] == q[x].size())
+            for (int y = 0; y < X; ++y)
+                if (q[y].size() - head[y] > q[x].size() - head[x]) x = y;
+        out.push_back(q[x][head[x]++]);
+    }
+    return out;
 }
 
 void Engine::set_sweep(uint32_t t, hipStream_t s) {
