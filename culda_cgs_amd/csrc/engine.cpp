@@ -214,6 +214,27 @@ static std::vector<size_t> deal_slots_to_xcds(const std::vector<size_t> &by_len,
     return out;
 }
 
+// Per-XCD lists merged into one launch order: position b takes XCD
+// b mod kXcds's next item (workgroup b runs there); an XCD with none left
+// takes one of the XCD with the most left.
+template <class T>
+static std::vector<T> interleave_xcds(const std::vector<std::vector<T>> &q) {
+    const int X = (int)q.size();
+    size_t n = 0;
+    for (const auto &l : q) n += l.size();
+    std::vector<size_t> head(X, 0);
+    std::vector<T> out;
+    out.reserve(n);
+    for (size_t b = 0; b < n; ++b) {
+        int x = (int)(b % X);
+        if (head[x] == q[x].size())
+            for (int y = 0; y < X; ++y)
+                if (q[y].size() - head[y] > q[x].size() - head[x]) x = y;
+        out.push_back(q[x][head[x]++]);
+    }
+    return out;
+}
+
 static void append_packets(std::vector<Packet> &out, int32_t first, int32_t last, int32_t owner, int32_t per) {
     for (int32_t x = first; x < last; x += per) out.push_back(Packet{x, std::min(last, x + per), owner, 0});
 }
@@ -538,13 +559,15 @@ void Engine::upload() {
         int64_t total_docs = 0;
         for (const UnitHost &u : units) total_docs += u.doc_end - u.doc_start;
         const int64_t per_all = std::max<int64_t>(16, (total_docs + (int64_t)cu_count * 8 - 1) / ((int64_t)cu_count * 8));
-        std::vector<Packet> all;
+        // a batch's models each on the XCD that samples them (their z, just
+        // written there, may still be in its L2)
+        if (M > 1) assign_models_to_xcds();
+        std::vector<std::vector<Packet>> q(M > 1 ? kXcds : 1);
         for (int32_t ui = 0; ui < U; ++ui) {
             const int64_t Du = units[ui].doc_end - units[ui].doc_start;
-            std::vector<Packet> pa;
-            append_packets(pa, 0, (int32_t)Du, ui, (int32_t)per_all);
-            all.insert(all.end(), pa.begin(), pa.end());
+            append_packets(q[M > 1 ? model_xcd[units[ui].model] : 0], 0, (int32_t)Du, ui, (int32_t)per_all);
         }
+        const std::vector<Packet> all = M > 1 ? interleave_xcds(q) : q[0];
         d_theta_packets_all = upload_vec(*this, all);
         num_theta_packets_all = (int32_t)all.size();
         build_theta_packets(theta_groups_per_cu);
@@ -716,43 +739,34 @@ void Engine::rebuild_slots(int32_t slot_tokens_in) {
     hip_check(hipDeviceSynchronize(), "slot upload");
 }
 
-std::vector<int32_t> Engine::deal_models_to_xcds(const std::vector<int32_t> &by_len,
-                                                const std::vector<SlotDesc> &all) const {
-    const int X = (int)kXcds;
-    // models to XCDs, largest first onto the least-loaded XCD (tokens)
+void Engine::assign_models_to_xcds() {
+    // largest first onto the least-loaded XCD (tokens)
     std::vector<int32_t> ms(models.size());
     std::iota(ms.begin(), ms.end(), 0);
     std::stable_sort(ms.begin(), ms.end(),
                      [&](int32_t a, int32_t b) { return models[a].num_tokens > models[b].num_tokens; });
-    std::vector<int64_t> load(X, 0);
-    std::vector<int32_t> xcd_of(models.size()), rank_of(models.size());
+    std::vector<int64_t> load(kXcds, 0);
+    model_xcd.assign(models.size(), 0);
+    model_rank.assign(models.size(), 0);
     for (size_t i = 0; i < ms.size(); ++i) {
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        xcd_of[ms[i]] = x;
-        rank_of[ms[i]] = (int32_t)i;
+        model_xcd[ms[i]] = x;
+        model_rank[ms[i]] = (int32_t)i;
         load[x] += models[ms[i]].num_tokens;
     }
+}
+
+std::vector<int32_t> Engine::deal_models_to_xcds(const std::vector<int32_t> &by_len,
+                                                const std::vector<SlotDesc> &all) const {
     // each XCD's slots model by model (largest model first), each model's
     // longest first: the XCD's L2 then mostly holds one model's theta rows
-    std::vector<std::vector<int32_t>> q(X);
     std::vector<int32_t> sorted(by_len);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int32_t a, int32_t b) {
-        return rank_of[units[all[a].unit].model] < rank_of[units[all[b].unit].model];
+        return model_rank[units[all[a].unit].model] < model_rank[units[all[b].unit].model];
     });
-    for (int32_t i : sorted) q[xcd_of[units[all[i].unit].model]].push_back(i);
-    // position b takes XCD b mod X's next slot (workgroup b runs there);
-    // an XCD with none left takes one of the XCD with the most left
-    std::vector<size_t> head(X, 0);
-    std::vector<int32_t> out;
-    out.reserve(by_len.size());
-    for (size_t b = 0; b < by_len.size(); ++b) {
-        int x = (int)(b % X);
-        if (head[x] == q[x].size())
-            for (int y = 0; y < X; ++y)
-                if (q[y].size() - head[y] > q[x].size() - head[x]) x = y;
-        out.push_back(q[x][head[x]++]);
-    }
-    return out;
+    std::vector<std::vector<int32_t>> q(kXcds);
+    for (int32_t i : sorted) q[model_xcd[units[all[i].unit].model]].push_back(i);
+    return interleave_xcds(q);
 }
 
 void Engine::set_sweep(uint32_t t, hipStream_t s) {

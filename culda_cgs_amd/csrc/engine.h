@@ -210,8 +210,12 @@ public:
     void build_theta_packets(int32_t groups_per_cu);
     int32_t theta_groups_per_cu = 2;
     int32_t theta_beside_grid() const;
+    // a batch's models on the XCDs, balanced by tokens (model_rank: the
+    // models largest first); set at upload
+    void assign_models_to_xcds();
+    std::vector<int32_t> model_xcd, model_rank;
     // a batch's sampler launch order (slot indices, given longest first):
-    // each model's slots on one XCD (workgroup b runs on XCD b mod kXcds)
+    // each model's slots on its XCD (workgroup b runs on XCD b mod kXcds)
     std::vector<int32_t> deal_models_to_xcds(const std::vector<int32_t> &by_len, const std::vector<SlotDesc> &all) const;
     void rebuild_theta_all(hipStream_t s);
     void sample_unit(int32_t unit, hipStream_t s);
