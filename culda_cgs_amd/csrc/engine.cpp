@@ -385,6 +385,7 @@ void Engine::upload() {
             int64_t cap = 0;
             for (int32_t d = 0; d < Dc; ++d) {
                 const int64_t nd = m.doc_len[ch.doc_start + d];
+                u.max_doc_len = std::max(u.max_doc_len, nd);
                 meta[d] = DocMeta{(uint32_t)cap, 0u, (uint32_t)nd, 0u};
                 cap += (theta_entries_max(nd, K) + 15) / 16 * 16;
                 // the sampler addresses a chunk's theta with 32-bit byte offsets
@@ -920,14 +921,14 @@ static void append_xcd_packets(std::vector<Packet> &out, int32_t D, int32_t owne
 
 void Engine::build_theta_packets(int32_t groups_per_cu) {
     theta_groups_per_cu = groups_per_cu;  // read by theta_beside_grid
-    const int64_t groups = (int64_t)cu_count * std::max<int32_t>(groups_per_cu, 1);
+    const int64_t groups = (int64_t)cu_count * (groups_per_cu > 0 ? groups_per_cu : 2);
     for (int32_t ui = 0; ui < (int32_t)units.size(); ++ui) {
         UnitHost &u = units[ui];
         const int64_t Du = u.doc_end - u.doc_start;
         const int64_t per_u = std::max<int64_t>(16, (Du + groups - 1) / groups);
         std::vector<Packet> pk;
         if (units.size() > 1)
-            append_xcd_packets(pk, (int32_t)Du, ui, kWavesPerGroup, theta_beside_grid());
+            append_xcd_packets(pk, (int32_t)Du, ui, kWavesPerGroup, theta_beside_grid(ui));
         else
             append_packets(pk, 0, (int32_t)Du, ui, (int32_t)per_u);
         dfree(u.theta_packets);
@@ -937,17 +938,30 @@ void Engine::build_theta_packets(int32_t groups_per_cu) {
 }
 
 // workgroups of a chunk's theta rebuild beside a sweep: a bounded share of
-// the CUs, theta_groups_per_cu per CU
-int32_t Engine::theta_beside_grid() const { return std::max<int32_t>(theta_groups_per_cu, 1) * cu_count; }
+// the CUs, theta_groups_per_cu per CU (by default 4 of the one-wave kernel,
+// 2 of the four-wave one)
+int32_t Engine::theta_beside_grid(int32_t ui) const {
+    const int32_t g = theta_groups_per_cu > 0 ? theta_groups_per_cu : (theta_slim(ui) ? 4 : 2);
+    return g * cu_count;
+}
+
+bool Engine::theta_slim(int32_t ui) const {
+    return units.size() > 1 && units[ui].max_doc_len < 65536 && units[ui].num_tokens < ((int64_t)1 << 30);
+}
 
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {
     ThetaArgs a{units[ui].theta_packets, d_units, d_models, units[ui].num_theta_packets, 0};
+    if (theta_slim(ui)) {
+        hip_check(launch_theta_rebuild_slim(a, theta_beside_grid(ui), s), "theta rebuild launch");
+        theta_ok[ui] = 1;
+        return;
+    }
     // a single chunk's rebuild has nothing beside it: the next sweep waits;
     // a chunk's rebuild beside the next chunk's sweep holds a bounded grid
     // (theta_beside_grid), so that the sweep keeps most of the CUs (the
     // sweeps are the critical path)
     const bool alone = units.size() == 1;
-    hip_check(launch_theta_rebuild(a, alone, alone ? 0 : theta_beside_grid(), s), "theta rebuild launch");
+    hip_check(launch_theta_rebuild(a, alone, alone ? 0 : theta_beside_grid(ui), s), "theta rebuild launch");
     theta_ok[ui] = 1;
 }
 

@@ -50,6 +50,7 @@ struct UnitHost {
     int32_t model = 0;
     int32_t doc_start = 0, doc_end = 0;
     int64_t num_tokens = 0, theta_cap = 0;
+    int64_t max_doc_len = 0;
     int32_t slot_begin = 0, slot_end = 0;
     uint32_t *tokens = nullptr;
     uint16_t *z = nullptr;
@@ -208,8 +209,11 @@ public:
     // rebuild beside a sweep, on a grid of groups_per_cu workgroups per
     // CU): 4-doc packets dealt to the XCDs in contiguous runs
     void build_theta_packets(int32_t groups_per_cu);
-    int32_t theta_groups_per_cu = 2;
-    int32_t theta_beside_grid() const;
+    int32_t theta_groups_per_cu = 0;  // 0: each kernel's default
+    int32_t theta_beside_grid(int32_t ui) const;
+    // whether unit ui's rebuild beside a sweep runs the one-wave kernel
+    // (16-bit counts: docs below 65,536 tokens; 32-bit byte offsets)
+    bool theta_slim(int32_t ui) const;
     // a batch's models on the XCDs, balanced by tokens (model_rank: the
     // models largest first); set at upload
     void assign_models_to_xcds();

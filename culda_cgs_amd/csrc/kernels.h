@@ -30,6 +30,7 @@ hipError_t launch_phi_head(const int32_t *nwk, const float *den, float *head, in
 // theta_rebuild_alone_kernel; else at most max_groups workgroups (0: one
 // per packet) take the packets in turn
 hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, int32_t max_groups, hipStream_t s);
+hipError_t launch_theta_rebuild_slim(const ThetaArgs &a, int32_t max_groups, hipStream_t s);
 // grid is used in persistent mode (a.queue != null) only
 hipError_t launch_sample(const SampleArgs &a, int32_t grid, hipStream_t s);
 hipError_t launch_copy_u16(const uint16_t *src, uint16_t *dst, int64_t n, hipStream_t s);

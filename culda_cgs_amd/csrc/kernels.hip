@@ -566,6 +566,75 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     }
 }
 
+// The rebuild beside a sweep, sized to fit in what the sampler leaves of a
+// CU, so that it takes no wave slot a sweep could use: one wave per
+// workgroup, at most 32 VGPRs (six sampler waves of 80 hold 480 of a SIMD's
+// 512) and 2 KB of LDS, the doc's counts as 16-bit halves of 32-bit words --
+// every doc of the unit must then be shorter than 65,536 tokens, and the
+// unit below 2^30 tokens for 32-bit byte offsets (the host checks).  The
+// counters are cleared once; each doc's compaction clears the ones it
+// reads.  A packet's docs are the wave's, in turn.  Entries, their order
+// and nnz are theta_rebuild_kernel's.
+__global__ __launch_bounds__(64) void theta_rebuild_slim_kernel(ThetaArgs a) {
+    __shared__ __attribute__((aligned(16))) uint32_t h[kMaxK / 2];
+    uint16_t *const h16 = reinterpret_cast<uint16_t *>(h);
+    const int lane = threadIdx.x;
+    for (int i = lane; i < kMaxK / 2; i += kWave) h[i] = 0u;
+    for (int pi = blockIdx.x; pi < a.num_packets; pi += gridDim.x) {
+        // descriptors in scalar registers, read once per packet
+        const Packet pk = uniform_copy(a.packets + pi);
+        const UnitDev un = uniform_copy(a.units + pk.owner);
+        const int K = __builtin_amdgcn_readfirstlane(a.models[un.model].k);
+        for (int d = pk.first; d < pk.last; ++d) {
+            wave_sync();
+            const uint32_t b = __builtin_amdgcn_readfirstlane(load_global_at(un.rev_off, 4u * d));
+            const uint32_t e = __builtin_amdgcn_readfirstlane(load_global_at(un.rev_off, 4u * d + 4u));
+            // four index loads, then four z gathers, in flight per lane
+            for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
+                uint32_t t[4], zz[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    t[q] = i0 + q * kWave < e ? load_global_at(un.rev_idx, 4u * (i0 + q * kWave)) : 0xffffffffu;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? load_global_u16_at(un.z, 2u * t[q]) : 0u;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q] >> 1], 1u << ((zz[q] & 1u) * 16u));
+            }
+            wave_sync();
+            const uint32_t start = __builtin_amdgcn_readfirstlane(glb(un.meta + d)->start);
+            uint32_t base = 0;
+            for (int k0 = 0; k0 < K; k0 += kWave) {
+                const int k = k0 + lane;
+                uint32_t v = 0;
+                if (k < K) {
+                    v = h16[k];
+                    h16[k] = 0;
+                }
+                if (__ballot(v > kThetaMaxCount) == 0ull) {
+                    const unsigned long long msk = __ballot(v != 0);
+                    const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
+                    if (v) store_global_u16_at(un.theta, 2u * (start + pos), (uint16_t)((v << kThetaCountShift) | (uint32_t)k));
+                    base += (uint32_t)__popcll(msk);
+                } else {
+                    const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
+                    const uint32_t incl = wave_add_scan_u32(ne);
+                    const uint32_t pos = base + incl - ne;
+                    for (uint32_t q = 0; q < ne; ++q) {
+                        const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
+                        store_global_u16_at(un.theta, 2u * (start + pos + q),
+                                            (uint16_t)((c << kThetaCountShift) | (uint32_t)k));
+                    }
+                    base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+                }
+            }
+            // zero the slab tail up to the next multiple of 16 (read by the sampler)
+            if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) store_global_u16_at(un.theta, 2u * (start + base + lane), 0u);
+            if (lane == 0) glb(un.meta + d)->nnz = base;
+        }
+    }
+}
+
 // The same rebuild for a launch that runs alone (a single-chunk model,
 // whose next sweep waits for it): the histogram is cleared once, and each
 // doc's compaction reads all of a lane's counters in one LDS round trip and
@@ -1438,6 +1507,13 @@ hipError_t launch_theta_rebuild(const ThetaArgs &a, bool alone, int32_t max_grou
         hipLaunchKernelGGL(theta_rebuild_kernel,
                            dim3(max_groups > 0 ? std::min(a.num_packets, max_groups) : a.num_packets), dim3(256), 0, s,
                            a);
+    return hipGetLastError();
+}
+
+hipError_t launch_theta_rebuild_slim(const ThetaArgs &a, int32_t max_groups, hipStream_t s) {
+    if (a.num_packets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(theta_rebuild_slim_kernel, dim3(std::min(a.num_packets, std::max(max_groups, 1))), dim3(kWave),
+                       0, s, a);
     return hipGetLastError();
 }
 

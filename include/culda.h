@@ -345,8 +345,9 @@ typedef struct culda_tune {
     int32_t persistent_groups;    /* batches: chosen workgroups (0 = one per slot) */
     int32_t num_tried;
     int32_t theta_groups;         /* several chunks: workgroups per CU of a chunk's theta
-                                     rebuild beside the next sweep (the launch's grid),
-                                     timed over whole iterations; 0 = not tuned */
+                                     rebuild beside the next sweep (the launch's grid; one-wave
+                                     workgroups, four-wave ones where a doc has 65,536 tokens or
+                                     more), timed over whole iterations; 0 = not tuned */
     double  best_ms;              /* sampler time of the chosen candidate    */
     int32_t tried_slot_tokens[32];
     double  tried_ms[32];

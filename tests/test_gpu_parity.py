@@ -269,6 +269,55 @@ def test_theta_counts_near_and_above_16_bits():
     s.close()
 
 
+def test_theta_rebuild_kernels_beside_a_sweep_at_the_16_bit_bound():
+    """The two rebuilds that run beside a sweep.  Chunk 0's longest doc has
+    65,535 tokens, all on one topic: its rebuild takes the one-wave kernel
+    with 16-bit counts, at the largest count they hold.  Chunk 1 holds a
+    65,536-token doc, so its rebuild takes the four-wave kernel with 32-bit
+    counts.  Theta equals the oracle's after the full rebuild and after
+    each of two sweeps; z bit-exact, LL within 1e-5."""
+    V, K = 40, 32
+    rng = np.random.default_rng(22)
+    small = random_docs(rng, V, 30, 30)
+    long_b, long_c = rng.integers(0, V, 65_535), rng.integers(0, V, 65_536)
+    docs = small[:15] + [long_b] + small[15:] + [long_c]
+    ib, ic = 15, 31
+    arrays = chunks_from_docs(V, docs, 2)
+    ends, tokens = arrays[1], arrays[3]
+    assert list(ends) == [16, 32], ends
+    alpha, beta, seed = 50.0 / K, 0.01, 9
+    s, om = _pair(V, K, arrays, alpha, beta, seed)
+    s.init_z()
+    zs = []
+    for c in range(2):
+        z = s.get_z(c).copy()
+        for d, t in ((ib, 3), (ic, 11)):
+            z[tokens[c] == d] = t
+        zs.append(z)
+        s.set_z(c, z)
+    om.set_z(zs)
+    s.rebuild_all()
+    om.rebuild()
+    _theta_equal(s, om, arrays)
+    # the per-chunk rebuilds (the kernels that run beside a sweep) on the
+    # same state: counts of exactly 65,535 and 65,536
+    for c in range(2):
+        s.rebuild_theta(c)
+    s.sync()
+    _theta_equal(s, om, arrays)
+    for it in range(1, 3):
+        ll, _ = s.iterate(it)
+        out, _, _ = om.sweep(alpha, beta, seed, it)
+        om.rebuild()
+        for c in range(2):
+            bad = np.flatnonzero(s.get_z(c) != om.z[c])
+            assert bad.size == 0, f"sweep {it}, chunk {c}: {bad.size} tokens differ (first {bad[:5]})"
+        ref = out.ll_sum / om.num_tokens
+        assert abs(ll - ref) <= LL_RTOL * abs(ref)
+        _theta_equal(s, om, arrays)
+    s.close()
+
+
 def test_row_readers_after_lazy_rare_updates():
     """One chunk, K = 1024, mostly rare words: after each sweep only n_k and
     zp follow the new z, the rare words' rows are brought up to date when
