@@ -945,8 +945,17 @@ int32_t Engine::theta_beside_grid(int32_t ui) const {
     return g * cu_count;
 }
 
+// The one-wave kernel's waves take their packets' docs in turn, and a doc
+// costs a fixed compaction over K besides its tokens: a chunk of short docs
+// needs more waves than fit beside the sweep, so it keeps the four-wave
+// kernel (C5's 90-token docs: 19.3 G tokens/s with it, 17.9 with the
+// one-wave kernel at 4 per CU; C2's 323-token docs: 13.85 vs 13.93).
+static constexpr int64_t kSlimMinMeanDocLen = 192;
+
 bool Engine::theta_slim(int32_t ui) const {
-    return units.size() > 1 && units[ui].max_doc_len < 65536 && units[ui].num_tokens < ((int64_t)1 << 30);
+    const UnitHost &u = units[ui];
+    return units.size() > 1 && u.max_doc_len < 65536 && u.num_tokens < ((int64_t)1 << 30) &&
+           u.num_tokens >= kSlimMinMeanDocLen * (int64_t)(u.doc_end - u.doc_start);
 }
 
 void Engine::rebuild_theta_unit(int32_t ui, hipStream_t s) {

@@ -212,7 +212,8 @@ public:
     int32_t theta_groups_per_cu = 0;  // 0: each kernel's default
     int32_t theta_beside_grid(int32_t ui) const;
     // whether unit ui's rebuild beside a sweep runs the one-wave kernel
-    // (16-bit counts: docs below 65,536 tokens; 32-bit byte offsets)
+    // (16-bit counts: docs below 65,536 tokens; 32-bit byte offsets; docs
+    // of at least kSlimMinMeanDocLen tokens on average)
     bool theta_slim(int32_t ui) const;
     // a batch's models on the XCDs, balanced by tokens (model_rank: the
     // models largest first); set at upload
