@@ -800,10 +800,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         // the row itself need not be current.
         const int n = (int)(s.end - s.begin);
         const uint32_t t = lane < n ? load_global_u16(un.zp + s.begin + lane) : 0xffffffffu;
+        // each lane's count of its topic, by LDS atomics on the row's own
+        // words at the zp topics (cleared first), read back before the
+        // zero-count row is staged over them
+        uint32_t *const hc = reinterpret_cast<uint32_t *>(ph);
+        if (lane < n) hc[t] = 0u;
+        wave_sync();
+        if (lane < n) atomicAdd(hc + t, 1u);
+        wave_sync();
+        const uint32_t c = lane < n ? hc[t] : 0u;
+        wave_sync();
         for (int k = lane * 4; k < K; k += 4 * kWave)
             *reinterpret_cast<u32x4 *>(ph + k) = load_global4(md.zero_phi + k);
-        uint32_t c = 0;
-        for (int j = 0; j < n; ++j) c += (uint32_t)__builtin_amdgcn_readlane((int)t, j) == t;
         wave_sync();
         if (lane < n) ph[t] = phi_of((float)c + beta, *glb(md.rden + t));
     } else {
