@@ -498,16 +498,24 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 
 // ------------------------------------------------------------ theta rebuild
 // One workgroup per packet (a run of docs of one unit), one wave per doc:
-// LDS histogram over the doc's tokens (gathered through rev_idx), then an
-// ascending compaction of the nonzero topics into 16-bit entries; a count
-// above kThetaMaxCount takes several entries (kThetaMaxCount each, the
-// remainder last), placed by a wave prefix sum (a ballot when no topic of
-// the 64 needs more than one).
+// LDS histogram over the doc's tokens (gathered through rev_idx) and a
+// bitmap of the topics they touch, then the nonzero topics in ascending
+// order, 64 per step: lane r of a step takes the topic of the r-th set bit
+// (a binary search over the bitmap words' popcount prefixes, then over the
+// word's halves), so a doc costs ceil(nonzero topics / 64) steps instead of
+// K / 64.  A count above kThetaMaxCount takes several entries
+// (kThetaMaxCount each, the remainder last), placed by a wave prefix sum.
+// Counters and bitmap words are cleared as they are read.
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
+    __shared__ uint32_t bits_s[kWavesPerGroup][kMaxK / 32];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint32_t *h = hist_s[wave];
+    uint32_t *const h = hist_s[wave];
+    uint32_t *const bm = bits_s[wave];
+    for (int k = lane; k < kMaxK; k += kWave) h[k] = 0u;
+    if (lane < kMaxK / 32) bm[lane] = 0u;
+    wave_sync();
     // a grid smaller than the packet list takes the packets in turn (a
     // rebuild beside a sweep holds a bounded share of the CUs)
     for (int pi = blockIdx.x; pi < a.num_packets; pi += gridDim.x) {
@@ -515,9 +523,8 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     const UnitDev &un = a.units[pk.owner];
     const ModelDev &m = a.models[un.model];
     const int K = m.k;
+    const uint32_t nw = (uint32_t)(K + 31) / 32u;  // bitmap words, at most 32
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
-        for (int k = lane; k < K; k += kWave) h[k] = 0;
-        wave_sync();
         const uint32_t b = un.rev_off[d], e = un.rev_off[d + 1];
         // four index loads, then four z gathers, in flight per lane
         for (uint32_t i0 = b + lane; i0 < e; i0 += 4 * kWave) {
@@ -529,33 +536,63 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
             for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? un.z[t[q]] : (uint16_t)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
+                if (t[q] != 0xffffffffu) {
+                    atomicAdd(&h[zz[q]], 1u);
+                    atomicOr(&bm[zz[q] >> 5], 1u << (zz[q] & 31u));
+                }
         }
         wave_sync();
+        // lane j < nw: bitmap word j and the number of set bits before it
+        // (lanes past the last word: the total, which no rank reaches)
+        const uint32_t w = (uint32_t)lane < nw ? bm[lane] : 0u;
+        if ((uint32_t)lane < nw) bm[lane] = 0u;
+        const uint32_t pc = (uint32_t)__popc(w);
+        const uint32_t incl_w = wave_add_scan_u32(pc);
+        const uint32_t ex = incl_w - pc;
+        const uint32_t nz = (uint32_t)__builtin_amdgcn_readlane((int)incl_w, kWave - 1);
         const uint32_t start = un.meta[d].start;
         uint32_t base = 0;
-        for (int k0 = 0; k0 < K; k0 += kWave) {
-            const int k = k0 + lane;
-            const uint32_t v = k < K ? h[k] : 0u;
-            if (__ballot(v > kThetaMaxCount) == 0ull) {
-                const unsigned long long msk = __ballot(v != 0);
-                const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                if (v) un.theta[start + pos] = (uint16_t)((v << kThetaCountShift) | (uint32_t)k);
-                base += (uint32_t)__popcll(msk);
-            } else {
-                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
-                uint32_t incl = ne;
+        for (uint32_t r0 = 0; r0 < nz; r0 += kWave) {
+            // every lane searches (the permutes read from all lanes); only
+            // ranks below nz are used
+            const uint32_t r = r0 + (uint32_t)lane;
+            uint32_t j = 0;  // the last word whose prefix is at most r
 #pragma unroll
-                for (int d2 = 1; d2 < kWave; d2 <<= 1) {
-                    const uint32_t y = __shfl_up(incl, d2, kWave);
-                    if (lane >= d2) incl += y;
+            for (uint32_t step = 16; step >= 1; step >>= 1) {
+                const uint32_t ej = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((j + step) << 2), (int)ex);
+                if (ej <= r) j += step;
+            }
+            const uint32_t wj = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)w);
+            uint32_t q = r - (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)ex);
+            // bit index of the q-th set bit of wj
+            uint32_t x = wj, bit = 0;
+#pragma unroll
+            for (uint32_t width = 16; width >= 1; width >>= 1) {
+                const uint32_t low = (uint32_t)__popc(x & ((1u << width) - 1u));
+                if (q >= low) {
+                    q -= low;
+                    x >>= width;
+                    bit += width;
                 }
+            }
+            const uint32_t k = j * 32u + bit;
+            uint32_t v = 0;
+            if (r < nz) {
+                v = h[k];
+                h[k] = 0u;
+            }
+            if (__ballot(v > kThetaMaxCount) == 0ull) {
+                if (r < nz) un.theta[start + base + (uint32_t)lane] = (uint16_t)((v << kThetaCountShift) | k);
+                base += min(nz - r0, (uint32_t)kWave);
+            } else {
+                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;  // 0 past nz
+                const uint32_t incl = wave_add_scan_u32(ne);
                 const uint32_t pos = base + incl - ne;
-                for (uint32_t q = 0; q < ne; ++q) {
-                    const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
-                    un.theta[start + pos + q] = (uint16_t)((c << kThetaCountShift) | (uint32_t)k);
+                for (uint32_t c0 = 0; c0 < ne; ++c0) {
+                    const uint32_t c = c0 + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
+                    un.theta[start + pos + c0] = (uint16_t)((c << kThetaCountShift) | k);
                 }
-                base += __shfl(incl, kWave - 1, kWave);
+                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
             }
         }
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
