@@ -497,15 +497,72 @@ __global__ void phi_head_kernel(const int32_t *nwk, const float *den, float *hea
 }
 
 // ------------------------------------------------------------ theta rebuild
+// A doc's theta entries from its LDS counts and the bitmap of its topics
+// (bm: nw words, cleared here): the nonzero topics in ascending order, 64
+// per step, lane r of a step taking the topic of the r-th set bit (a binary
+// search over the words' popcount prefixes, then over the word's halves);
+// a count above kThetaMaxCount takes several entries (kThetaMaxCount each,
+// the remainder last), placed by a wave prefix sum.  take(k) returns topic
+// k's count and clears it, put(p, e) stores entry e at position p.
+// Returns the number of entries.
+template <class Take, class Put>
+__device__ __forceinline__ uint32_t theta_compact(uint32_t *bm, uint32_t nw, int lane, Take take, Put put) {
+    // lane j < nw: bitmap word j and the number of set bits before it
+    // (lanes past the last word: the total, which no rank reaches)
+    const uint32_t w = (uint32_t)lane < nw ? bm[lane] : 0u;
+    if ((uint32_t)lane < nw) bm[lane] = 0u;
+    const uint32_t pc = (uint32_t)__popc(w);
+    const uint32_t incl_w = wave_add_scan_u32(pc);
+    const uint32_t ex = incl_w - pc;
+    const uint32_t nz = (uint32_t)__builtin_amdgcn_readlane((int)incl_w, kWave - 1);
+    uint32_t base = 0;
+    for (uint32_t r0 = 0; r0 < nz; r0 += kWave) {
+        // every lane searches (the permutes read from all lanes); only
+        // ranks below nz are used
+        const uint32_t r = r0 + (uint32_t)lane;
+        uint32_t j = 0;  // the last word whose prefix is at most r
+#pragma unroll
+        for (uint32_t step = 16; step >= 1; step >>= 1) {
+            const uint32_t ej = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((j + step) << 2), (int)ex);
+            if (ej <= r) j += step;
+        }
+        const uint32_t wj = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)w);
+        uint32_t q = r - (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)ex);
+        // bit index of the q-th set bit of wj
+        uint32_t x = wj, bit = 0;
+#pragma unroll
+        for (uint32_t width = 16; width >= 1; width >>= 1) {
+            const uint32_t low = (uint32_t)__popc(x & ((1u << width) - 1u));
+            if (q >= low) {
+                q -= low;
+                x >>= width;
+                bit += width;
+            }
+        }
+        const uint32_t k = j * 32u + bit;
+        const uint32_t v = r < nz ? take(k) : 0u;
+        if (__ballot(v > kThetaMaxCount) == 0ull) {
+            if (r < nz) put(base + (uint32_t)lane, (uint16_t)((v << kThetaCountShift) | k));
+            base += min(nz - r0, (uint32_t)kWave);
+        } else {
+            const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;  // 0 past nz
+            const uint32_t incl = wave_add_scan_u32(ne);
+            const uint32_t pos = base + incl - ne;
+            for (uint32_t c0 = 0; c0 < ne; ++c0) {
+                const uint32_t c = c0 + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
+                put(pos + c0, (uint16_t)((c << kThetaCountShift) | k));
+            }
+            base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+        }
+    }
+    return base;
+}
+
 // One workgroup per packet (a run of docs of one unit), one wave per doc:
 // LDS histogram over the doc's tokens (gathered through rev_idx) and a
-// bitmap of the topics they touch, then the nonzero topics in ascending
-// order, 64 per step: lane r of a step takes the topic of the r-th set bit
-// (a binary search over the bitmap words' popcount prefixes, then over the
-// word's halves), so a doc costs ceil(nonzero topics / 64) steps instead of
-// K / 64.  A count above kThetaMaxCount takes several entries
-// (kThetaMaxCount each, the remainder last), placed by a wave prefix sum.
-// Counters and bitmap words are cleared as they are read.
+// bitmap of the topics they touch, then theta_compact: a doc costs
+// ceil(nonzero topics / 64) steps instead of K / 64.  Counters and bitmap
+// words are cleared as they are read.
 // Reference: LDAUpdateThetaKernel (kernel/lda_theta_kernel.cu:5-87).
 __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
@@ -542,59 +599,15 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
                 }
         }
         wave_sync();
-        // lane j < nw: bitmap word j and the number of set bits before it
-        // (lanes past the last word: the total, which no rank reaches)
-        const uint32_t w = (uint32_t)lane < nw ? bm[lane] : 0u;
-        if ((uint32_t)lane < nw) bm[lane] = 0u;
-        const uint32_t pc = (uint32_t)__popc(w);
-        const uint32_t incl_w = wave_add_scan_u32(pc);
-        const uint32_t ex = incl_w - pc;
-        const uint32_t nz = (uint32_t)__builtin_amdgcn_readlane((int)incl_w, kWave - 1);
         const uint32_t start = un.meta[d].start;
-        uint32_t base = 0;
-        for (uint32_t r0 = 0; r0 < nz; r0 += kWave) {
-            // every lane searches (the permutes read from all lanes); only
-            // ranks below nz are used
-            const uint32_t r = r0 + (uint32_t)lane;
-            uint32_t j = 0;  // the last word whose prefix is at most r
-#pragma unroll
-            for (uint32_t step = 16; step >= 1; step >>= 1) {
-                const uint32_t ej = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((j + step) << 2), (int)ex);
-                if (ej <= r) j += step;
-            }
-            const uint32_t wj = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)w);
-            uint32_t q = r - (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)ex);
-            // bit index of the q-th set bit of wj
-            uint32_t x = wj, bit = 0;
-#pragma unroll
-            for (uint32_t width = 16; width >= 1; width >>= 1) {
-                const uint32_t low = (uint32_t)__popc(x & ((1u << width) - 1u));
-                if (q >= low) {
-                    q -= low;
-                    x >>= width;
-                    bit += width;
-                }
-            }
-            const uint32_t k = j * 32u + bit;
-            uint32_t v = 0;
-            if (r < nz) {
-                v = h[k];
+        const uint32_t base = theta_compact(
+            bm, nw, lane,
+            [&](uint32_t k) {
+                const uint32_t v = h[k];
                 h[k] = 0u;
-            }
-            if (__ballot(v > kThetaMaxCount) == 0ull) {
-                if (r < nz) un.theta[start + base + (uint32_t)lane] = (uint16_t)((v << kThetaCountShift) | k);
-                base += min(nz - r0, (uint32_t)kWave);
-            } else {
-                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;  // 0 past nz
-                const uint32_t incl = wave_add_scan_u32(ne);
-                const uint32_t pos = base + incl - ne;
-                for (uint32_t c0 = 0; c0 < ne; ++c0) {
-                    const uint32_t c = c0 + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
-                    un.theta[start + pos + c0] = (uint16_t)((c << kThetaCountShift) | k);
-                }
-                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-            }
-        }
+                return v;
+            },
+            [&](uint32_t p, uint16_t en) { un.theta[start + p] = en; });
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
         if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) un.theta[start + base + lane] = 0u;
         if (lane == 0) un.meta[d].nnz = base;
@@ -608,15 +621,17 @@ __global__ __launch_bounds__(256) void theta_rebuild_kernel(ThetaArgs a) {
 // workgroup, at most 32 VGPRs (six sampler waves of 80 hold 480 of a SIMD's
 // 512) and 2 KB of LDS, the doc's counts as 16-bit halves of 32-bit words --
 // every doc of the unit must then be shorter than 65,536 tokens, and the
-// unit below 2^30 tokens for 32-bit byte offsets (the host checks).  The
-// counters are cleared once; each doc's compaction clears the ones it
-// reads.  A packet's docs are the wave's, in turn.  Entries, their order
-// and nnz are theta_rebuild_kernel's.
+// unit below 2^30 tokens for 32-bit byte offsets (the host checks) -- plus
+// the topic bitmap of theta_compact.  The counters are cleared once; each
+// doc's compaction clears the ones it reads.  A packet's docs are the
+// wave's, in turn.  Entries, their order and nnz are theta_rebuild_kernel's.
 __global__ __launch_bounds__(64) void theta_rebuild_slim_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t h[kMaxK / 2];
+    __shared__ uint32_t bm[kMaxK / 32];
     uint16_t *const h16 = reinterpret_cast<uint16_t *>(h);
     const int lane = threadIdx.x;
     for (int i = lane; i < kMaxK / 2; i += kWave) h[i] = 0u;
+    if (lane < kMaxK / 32) bm[lane] = 0u;
     for (int pi = blockIdx.x; pi < a.num_packets; pi += gridDim.x) {
         // descriptors in scalar registers, read once per packet
         const Packet pk = uniform_copy(a.packets + pi);
@@ -636,35 +651,21 @@ __global__ __launch_bounds__(64) void theta_rebuild_slim_kernel(ThetaArgs a) {
                 for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? load_global_u16_at(un.z, 2u * t[q]) : 0u;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q] >> 1], 1u << ((zz[q] & 1u) * 16u));
+                    if (t[q] != 0xffffffffu) {
+                        atomicAdd(&h[zz[q] >> 1], 1u << ((zz[q] & 1u) * 16u));
+                        atomicOr(&bm[zz[q] >> 5], 1u << (zz[q] & 31u));
+                    }
             }
             wave_sync();
             const uint32_t start = __builtin_amdgcn_readfirstlane(glb(un.meta + d)->start);
-            uint32_t base = 0;
-            for (int k0 = 0; k0 < K; k0 += kWave) {
-                const int k = k0 + lane;
-                uint32_t v = 0;
-                if (k < K) {
-                    v = h16[k];
+            const uint32_t base = theta_compact(
+                bm, (uint32_t)(K + 31) / 32u, lane,
+                [&](uint32_t k) {
+                    const uint32_t v = h16[k];
                     h16[k] = 0;
-                }
-                if (__ballot(v > kThetaMaxCount) == 0ull) {
-                    const unsigned long long msk = __ballot(v != 0);
-                    const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                    if (v) store_global_u16_at(un.theta, 2u * (start + pos), (uint16_t)((v << kThetaCountShift) | (uint32_t)k));
-                    base += (uint32_t)__popcll(msk);
-                } else {
-                    const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
-                    const uint32_t incl = wave_add_scan_u32(ne);
-                    const uint32_t pos = base + incl - ne;
-                    for (uint32_t q = 0; q < ne; ++q) {
-                        const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
-                        store_global_u16_at(un.theta, 2u * (start + pos + q),
-                                            (uint16_t)((c << kThetaCountShift) | (uint32_t)k));
-                    }
-                    base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-                }
-            }
+                    return v;
+                },
+                [&](uint32_t p, uint16_t en) { store_global_u16_at(un.theta, 2u * (start + p), en); });
             // zero the slab tail up to the next multiple of 16 (read by the sampler)
             if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) store_global_u16_at(un.theta, 2u * (start + base + lane), 0u);
             if (lane == 0) glb(un.meta + d)->nnz = base;
@@ -673,18 +674,20 @@ __global__ __launch_bounds__(64) void theta_rebuild_slim_kernel(ThetaArgs a) {
 }
 
 // The same rebuild for a launch that runs alone (a single-chunk model,
-// whose next sweep waits for it): the histogram is cleared once, and each
-// doc's compaction reads all of a lane's counters in one LDS round trip and
-// clears them -- shorter per doc, but at 78 VGPRs it does not fit beside
-// the sampler's waves, as the per-chunk rebuilds of larger models must.
-__device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet, uint32_t (*hist_s)[kMaxK]) {
+// whose next sweep waits for it): the histogram is cleared once per
+// workgroup and each doc's compaction (theta_compact) clears what it reads.
+// bits: kWavesPerGroup topic bitmaps of kMaxK / 32 words.
+__device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet, uint32_t (*hist_s)[kMaxK],
+                                                 uint32_t *bits) {
     const Packet pk = a.packets[packet];
     const UnitDev &un = a.units[pk.owner];
     const ModelDev &m = a.models[un.model];
     const int K = m.k;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     uint32_t *h = hist_s[wave];
+    uint32_t *const bm = bits + wave * (kMaxK / 32);
     for (int k = 4 * lane; k < K; k += 4 * kWave) *reinterpret_cast<uint4 *>(h + k) = make_uint4(0, 0, 0, 0);
+    if (lane < kMaxK / 32) bm[lane] = 0u;
     for (int d = pk.first + wave; d < pk.last; d += kWavesPerGroup) {
         wave_sync();
         const uint32_t b = load_global(un.rev_off + d), e = load_global(un.rev_off + d + 1);
@@ -699,40 +702,20 @@ __device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet,
             for (int q = 0; q < 4; ++q) zz[q] = t[q] != 0xffffffffu ? (uint16_t)load_global_u16(un.z + t[q]) : (uint16_t)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (t[q] != 0xffffffffu) atomicAdd(&h[zz[q]], 1u);
+                if (t[q] != 0xffffffffu) {
+                    atomicAdd(&h[zz[q]], 1u);
+                    atomicOr(&bm[zz[q] >> 5], 1u << (zz[q] & 31u));
+                }
         }
         wave_sync();
-        // the lane's 16 counters (topics r * 64 + lane < K; the rest of
-        // the array is never cleared) read at once, then cleared: one LDS
-        // round trip instead of one per 64 topics
-        uint32_t hv[kMaxK / kWave];
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r) hv[r] = r * kWave + lane < K ? h[r * kWave + lane] : 0u;
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r)
-            if (hv[r]) h[r * kWave + lane] = 0u;
-        uint32_t base = 0;
-#pragma unroll
-        for (int r = 0; r < kMaxK / kWave; ++r) {
-            if (r * kWave >= K) break;
-            const int k = r * kWave + lane;
-            const uint32_t v = hv[r];
-            if (__ballot(v > kThetaMaxCount) == 0ull) {
-                const unsigned long long msk = __ballot(v != 0);
-                const uint32_t pos = base + (uint32_t)__popcll(msk & ((1ull << lane) - 1ull));
-                if (v) store_global_u16(un.theta + start + pos, (uint16_t)((v << kThetaCountShift) | (uint32_t)k));
-                base += (uint32_t)__popcll(msk);
-            } else {
-                const uint32_t ne = (v + kThetaMaxCount - 1u) / kThetaMaxCount;
-                const uint32_t incl = wave_add_scan_u32(ne);
-                const uint32_t pos = base + incl - ne;
-                for (uint32_t q = 0; q < ne; ++q) {
-                    const uint32_t c = q + 1u < ne ? kThetaMaxCount : v - kThetaMaxCount * (ne - 1u);
-                    store_global_u16(un.theta + start + pos + q, (uint16_t)((c << kThetaCountShift) | (uint32_t)k));
-                }
-                base += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-            }
-        }
+        const uint32_t base = theta_compact(
+            bm, (uint32_t)(K + 31) / 32u, lane,
+            [&](uint32_t k) {
+                const uint32_t v = h[k];
+                h[k] = 0u;
+                return v;
+            },
+            [&](uint32_t p, uint16_t en) { store_global_u16(un.theta + start + p, en); });
         // zero the slab tail up to the next multiple of 16 (read by the sampler)
         if ((uint32_t)lane < ((16u - (base & 15u)) & 15u)) store_global_u16(un.theta + start + base + lane, 0u);
         if (lane == 0) glb(un.meta + d)->nnz = base;
@@ -742,7 +725,8 @@ __device__ __forceinline__ void theta_alone_body(const ThetaArgs &a, int packet,
 
 __global__ __launch_bounds__(256) void theta_rebuild_alone_kernel(ThetaArgs a) {
     __shared__ __attribute__((aligned(16))) uint32_t hist_s[kWavesPerGroup][kMaxK];
-    theta_alone_body(a, blockIdx.x, hist_s);
+    __shared__ uint32_t bits_s[kWavesPerGroup * (kMaxK / 32)];
+    theta_alone_body(a, blockIdx.x, hist_s, bits_s);
 }
 
 // A one-unit model's count update and theta rebuild in one launch (both
@@ -754,7 +738,7 @@ __global__ __launch_bounds__(256) void count_update_kernel(PhiArgs p, RareArgs r
     __shared__ uint32_t nk_s[kMaxK];
     const int b = blockIdx.x;
     if (b < t.num_packets)
-        theta_alone_body(t, b, hist_s);
+        theta_alone_body(t, b, hist_s, nk_s);  // nk_s: the theta workgroups' topic bitmaps
     else if (b < t.num_packets + p.num_packets)
         phi_rebuild_body(p, b - t.num_packets, hist_s, nk_s);
     else
