@@ -611,9 +611,11 @@ int32_t Engine::default_slot_tokens() const {
 }
 
 // Sampler slots: each unit's word ranges cut into pieces of at most
-// slot_tokens, longest first within the unit; units in order, so a model's
-// slots are contiguous (its LL partials too).  d_order lists every slot
-// longest first, for launches over all units.
+// slot_tokens (and, where the unit's theta rows outgrow the L2s, at its
+// XCD doc regions), longest first within the unit (dealt to the XCDs by
+// region there); units in order, so a model's slots are contiguous (its LL
+// partials too).  d_order is the launch order over all units: for a batch,
+// each model's slots on one XCD.
 void Engine::rebuild_slots(int32_t slot_tokens_in) {
     if (graph) {
         (void)hipGraphExecDestroy(graph);

@@ -132,7 +132,7 @@ public:
     ModelDev *d_models = nullptr;
     UnitDev *d_units = nullptr;
     SlotDesc *d_slots = nullptr;
-    int32_t *d_order = nullptr;  // slot indices, longest first over all units
+    int32_t *d_order = nullptr;  // slot indices in the launch order over all units
     int32_t *d_order_claims = nullptr;  // the same, interleaved so each kQueueClaim run spans all lengths
     int32_t num_slots = 0;
     int64_t rare_slots = 0, rare_tokens = 0;  // slots staged from zp when the counts allow
