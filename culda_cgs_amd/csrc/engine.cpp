@@ -947,11 +947,12 @@ int32_t Engine::theta_beside_grid(int32_t ui) const {
     return g * cu_count;
 }
 
-// The one-wave kernel's waves take their packets' docs in turn, and a doc
-// costs a fixed compaction over K besides its tokens: a chunk of short docs
-// needs more waves than fit beside the sweep, so it keeps the four-wave
-// kernel (C5's 90-token docs: 19.3 G tokens/s with it, 17.9 with the
-// one-wave kernel at 4 per CU; C2's 323-token docs: 13.85 vs 13.93).
+// The one-wave kernel's waves take their packets' docs in turn, and every
+// doc has a fixed cost besides its tokens: a chunk of short docs needs more
+// waves than fit beside the sweep, so it keeps the four-wave kernel (16
+// chunks of 90-token docs, 664M tokens: 20.5 G tokens/s with it, 20.2 with
+// the one-wave kernel at 4 per CU; 4 chunks of 323-token docs, 97M tokens:
+// 13.85 with the four-wave kernel, 13.93 with the one-wave one).
 static constexpr int64_t kSlimMinMeanDocLen = 192;
 
 bool Engine::theta_slim(int32_t ui) const {
